@@ -1,0 +1,562 @@
+"""Scene construction: the host-side derivations of ``FluidSim::new`` and the workloads.
+
+Restated from the reference (paths relative to the reference repository):
+
+* regions (src/soft_body/material_region/region.rs:11-272): AABB, AABE, their ``border``
+  differences, Union / Intersection / Difference;
+* the staggered-lattice particle generator and material mass
+  (src/soft_body/material_region/mod.rs:56-155), evaluated in f32 in the same order;
+* material constructors (material.rs:61-121, examples/test_mods/config.rs:330-391);
+* the interaction matrix (soft_body/mod.rs:93-111, interaction.rs:42-50);
+* the grid dimensionality (soft_body/mod.rs:113-117).
+
+The benchmark workloads (2-D dam break, 3-D fluid column, elastic cantilever, fluid-solid
+contact, ensembles) are built here from their stated sizes and parameters.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import wire
+
+f32 = np.float32
+
+
+# ---------------------------------------------------------------------------------------
+# regions
+# ---------------------------------------------------------------------------------------
+class AABB:
+    def __init__(self, mn, dim):
+        self.min = np.zeros(4, f32)
+        self.dim = np.zeros(4, f32)
+        self.min[: len(mn)] = mn
+        self.dim[: len(dim)] = dim
+
+    def bound(self):
+        return self
+
+    def contains(self, p):
+        # p[i] >= min[i] && p[i]-min[i] <= dim[i] on every axis (region.rs:64-71)
+        ok = np.ones(len(p), bool)
+        for i in range(4):
+            ok &= ~(p[:, i] < self.min[i])
+            ok &= ~((p[:, i] - self.min[i]) > self.dim[i])
+        return ok
+
+    def center(self):
+        return np.array([self.min[i] + f32(0.5) * self.dim[i] for i in range(4)], f32)
+
+    def border(self, b):
+        # AABB::border (region.rs:33-43)
+        b = f32(b)
+        c = self.center()
+        inner_min = np.array([min(self.min[i] + b, c[i]) for i in range(4)], f32)
+        inner_dim = np.array([max(self.dim[i] - f32(2.0) * b, f32(0.0)) for i in range(4)], f32)
+        return Difference(self, AABB(inner_min, inner_dim))
+
+
+class AABE:
+    def __init__(self, center, radii):
+        self.c = np.zeros(4, f32)
+        self.r = np.zeros(4, f32)
+        self.c[: len(center)] = center
+        self.r[: len(radii)] = radii
+
+    def bound(self):
+        return AABB(self.c - self.r, f32(2.0) * self.r)
+
+    def contains(self, p):
+        # region.rs:88-106, evaluated in f64 like the reference
+        lhs = np.zeros(len(p))
+        for i in range(4):
+            d = (p[:, i] - self.c[i]).astype(np.float64)
+            r = float(self.r[i])
+            if r > 0.0:
+                coeff = 1.0
+                for j in range(4):
+                    if self.r[j] > 0.0 and j != i:
+                        coeff *= float(self.r[j])
+                lhs = lhs + d * d * coeff * coeff
+        rhs = 1.0
+        for i in range(4):
+            if self.r[i] > 0.0:
+                rhs *= float(self.r[i])
+        return lhs <= rhs * rhs
+
+    def border(self, b):
+        inner = AABE(self.c, np.maximum(self.r - f32(b), f32(0)))
+        return Difference(self, inner)
+
+
+class Difference:
+    def __init__(self, a, b):
+        self.a, self.b = a, b
+
+    def bound(self):
+        return self.a.bound()
+
+    def contains(self, p):
+        return self.a.contains(p) & ~self.b.contains(p)
+
+
+class Union:
+    def __init__(self, a, b):
+        self.a, self.b = a, b
+
+    def bound(self):
+        l, r = self.a.bound(), self.b.bound()
+        mn = np.minimum(l.min, r.min)
+        mx = np.maximum(l.min + l.dim, r.min + r.dim)
+        return AABB(mn, mx - mn)
+
+    def contains(self, p):
+        return self.a.contains(p) | self.b.contains(p)
+
+
+class Intersection:
+    def __init__(self, a, b):
+        self.a, self.b = a, b
+
+    def bound(self):
+        l, r = self.a.bound(), self.b.bound()
+        mn = np.maximum(l.min, r.min)
+        mx = np.minimum(l.min + l.dim, r.min + r.dim)
+        return AABB(mn, mx - mn)
+
+    def contains(self, p):
+        return self.a.contains(p) & self.b.contains(p)
+
+
+def _axis_run(start, step, origin, extent):
+    """f32 run start, start+step, ... while (pos - origin) <= extent, the first always."""
+    out = [start]
+    pos = start
+    while True:
+        pos = f32(pos + step)
+        if (pos - origin) > extent or extent == f32(0.0):
+            break
+        out.append(pos)
+    return out, pos
+
+
+def gen_particles(region, packing, kernel_radius, mat_id, start_den, vel_fn=None):
+    """MaterialRegion::gen_particles (material_region/mod.rs:56-155).
+
+    Returns (particle array, mass).  The lattice is staggered: x step s, y step s/sqrt(2)
+    with alternating x offset s/2, z step s/sqrt(3) with alternating offset s/sqrt(8),
+    accumulated in f32; mass = start_den * bbox volume / lattice points tested (f64)."""
+    h = f32(f32(packing) * f32(kernel_radius))
+    b = region.bound()
+    mn, dm = b.min, b.dim
+    sq2, sq3, sq8 = f32(math.sqrt(2.0)), f32(math.sqrt(3.0)), f32(math.sqrt(8.0))
+    pts = []
+    pos = mn.copy()
+    offset2 = f32(0.0)
+    while True:
+        offset = f32(0.0)
+        pos[1] = f32(mn[1] + offset2)
+        while True:
+            x0 = f32(mn[0] + f32(offset + offset2))
+            xs, _ = _axis_run(x0, h, mn[0], dm[0])
+            row = np.empty((len(xs), 4), f32)
+            row[:, 0] = xs
+            row[:, 1] = pos[1]
+            row[:, 2] = pos[2]
+            row[:, 3] = pos[3]
+            pts.append(row)
+            pos[1] = f32(pos[1] + f32(h / sq2))
+            offset = f32(h / f32(2.0)) if offset == f32(0.0) else f32(0.0)
+            if (pos[1] - mn[1]) > dm[1] or dm[1] == f32(0.0):
+                break
+        pos[2] = f32(pos[2] + f32(h / sq3))
+        offset2 = f32(h / sq8) if offset2 == f32(0.0) else f32(0.0)
+        if (pos[2] - mn[2]) > dm[2] or dm[2] == f32(0.0):
+            break
+    P = np.concatenate(pts) if pts else np.zeros((0, 4), f32)
+    num_in_box = len(P)
+    keep = region.contains(P)
+    P = P[keep]
+    out = wire.particles(len(P))
+    out["pos"] = P
+    if vel_fn is not None:
+        out["vel"] = vel_fn(P)
+    out["den"] = f32(start_den)
+    out["mat"] = mat_id
+    box = float(start_den)
+    vol = 1.0
+    for i in range(4):
+        if dm[i] > 0.0:
+            vol *= float(dm[i])
+    mass = f32((box * vol) / float(max(num_in_box, 1)))
+    return out, mass
+
+
+# ---------------------------------------------------------------------------------------
+# materials and interactions
+# ---------------------------------------------------------------------------------------
+def immobile_material(friction=0.0):
+    """Material::new_immobile (material.rs:113-120)."""
+    return wire.material(visc=friction, start_den=1.0, target_den=1.0, immobile=1)
+
+
+def fluid_material(kernel_radius, alpha, density=1.0, speed_of_sound=None, viscosity=0.0,
+                   start_density=None, target_density=None, state=None, bulk_viscosity=None):
+    """An [objects.*] fluid table as config.rs:354-375 parses it (state inferred when not
+    given: speed_of_sound alone -> Tait, with start/target density -> ideal gas)."""
+    h = f32(kernel_radius)
+    c = f32(speed_of_sound if speed_of_sound is not None else 0.0)
+    m = wire.material()
+    m["start_den"] = start_density if start_density is not None else density
+    m["target_den"] = target_density if target_density is not None else density
+    m["sound_speed"] = c
+    m["visc"] = viscosity
+    if bulk_viscosity is None:
+        bulk_viscosity = float(f32(f32(f32(alpha) * h) * c))
+    m["bulk_visc"] = bulk_viscosity
+    if state is None:
+        if speed_of_sound is not None:
+            state = wire.EQ_IDEAL_GAS if (start_density is not None or target_density is not None) else wire.EQ_TAIT
+        else:
+            state = wire.EQ_ZERO
+    m["state_eq"] = state
+    m["strain_order"] = 2
+    return m
+
+
+def solid_material(density=1.0, normal_stiffness=0.0, shear_stiffness=0.0, normal_damp=0.0,
+                   shear_damp=0.0, strain_order=2, plastic=None, viscosity=0.0):
+    """An elastic/plastic [objects.*] table (config.rs:354-391)."""
+    m = wire.material()
+    m["start_den"] = density
+    m["target_den"] = density
+    m["visc"] = viscosity
+    m["strain_order"] = strain_order
+    m["normal_stiffness"] = normal_stiffness
+    m["shear_stiffness"] = shear_stiffness
+    m["normal_damp"] = normal_damp
+    m["shear_damp"] = shear_damp
+    if plastic is not None:
+        m["plastic"] = 1
+        m["yield_strength"] = plastic.get("yield_strength", 0.0)
+        m["work_hardening"] = plastic.get("work_hardening", 0.0)
+        m["work_hardening_exp"] = plastic.get("work_hardening_exp", 2.0)
+        m["kinematic_hardening"] = plastic.get("kinematic_hardening", 0.0)
+        m["thermal_softening"] = plastic.get("thermal_softening", 0.0)
+        m["relaxation_time"] = plastic.get("relaxation_time", 1.0)
+    return m
+
+
+def lame_from_young(E, nu, sign=-1.0):
+    """(lambda, mu) for Young's modulus E and Poisson ratio nu, with the reference's
+    negative stiffness sign convention (all example solids use negative values)."""
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    return sign * lam, sign * mu
+
+
+def default_interaction(m1, m2, h):
+    """MatInteraction::default_between (interaction.rs:42-50)."""
+    if m1["state_eq"] != wire.EQ_ZERO and m2["state_eq"] != wire.EQ_ZERO:
+        return wire.interaction()
+    return wire.interaction(wire.POT_LINEAR, 10000.0, float(f32(h) * f32(2.0)), 0.0,
+                            float(f32(m1["visc"]) + f32(m2["visc"])))
+
+
+def resolve_interactions(mats, h, configured=None):
+    """FluidSim::new's interaction matrix (soft_body/mod.rs:93-111).
+
+    configured: dict {(i, j): interaction} of explicitly configured pairs."""
+    configured = configured or {}
+    nm = len(mats)
+    inter = np.zeros(nm * nm, wire.INTERACTION)
+    for i in range(nm):
+        for j in range(i, nm):
+            if (i, j) in configured:
+                it = configured[(i, j)]
+            elif (j, i) in configured:
+                it = configured[(j, i)]
+            elif i != j:
+                it = default_interaction(mats[i], mats[j], h)
+            else:
+                it = wire.interaction()
+            inter[i * nm + j] = it
+            inter[j * nm + i] = it
+    return inter
+
+
+def grid_dim(bounds):
+    """dim = number of axes with bounds.dim > 0 (soft_body/mod.rs:113-117)."""
+    return int(sum(1 for i in range(4) if bounds["dim"][i] > 0))
+
+
+# ---------------------------------------------------------------------------------------
+# scene container
+# ---------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class Scene:
+    particles: np.ndarray  # wire.PARTICLE, mobile
+    boundary: np.ndarray  # wire.PARTICLE, immobile
+    materials: np.ndarray  # wire.MATERIAL
+    interactions: np.ndarray  # wire.INTERACTION, nm*nm
+    bounds: np.ndarray  # wire.AABB
+    h: float
+    g: float
+    alpha: float = 0.0
+    solids: np.ndarray = None  # wire.SOLID
+    dt: float = 1e-4
+    subticks: int = 1
+    integrator: str = "rk4"
+    name: str = ""
+
+    def __post_init__(self):
+        if self.solids is None:
+            self.solids = init_solids(self.particles, self.materials)
+
+    @property
+    def n(self):
+        return len(self.particles)
+
+    @property
+    def nb(self):
+        return len(self.boundary)
+
+    @property
+    def ns(self):
+        return len(self.solids)
+
+    @property
+    def dim(self):
+        return grid_dim(self.bounds)
+
+    def copy(self):
+        return dataclasses.replace(
+            self,
+            particles=self.particles.copy(),
+            boundary=self.boundary.copy(),
+            solids=self.solids.copy(),
+            materials=self.materials.copy(),
+            interactions=self.interactions.copy(),
+            bounds=self.bounds.copy(),
+        )
+
+
+def init_solids(particles, materials):
+    """Particles::init_solid_particles (particles.rs:488-506): a SolidParticle for every
+    particle of a stiff material, in particle order; one dummy when there is none."""
+    stiff = (materials["normal_stiffness"] != 0) | (materials["shear_stiffness"] != 0)
+    is_solid = stiff[particles["mat"]] if len(particles) else np.zeros(0, bool)
+    idx = np.nonzero(is_solid)[0]
+    if len(idx) == 0:
+        s = wire.solids(1)
+        s["part_id"] = wire.INVALID
+        return s
+    s = wire.solids(len(idx))
+    s["part_id"] = idx.astype(np.uint32)
+    s["ref_pos"] = particles["pos"][idx]
+    particles["solid_id"][:] = wire.INVALID
+    particles["solid_id"][idx] = np.arange(len(idx), dtype=np.uint32)
+    return s
+
+
+# ---------------------------------------------------------------------------------------
+# deterministic RNG (splitmix64 -> uniform [-1, 1))
+# ---------------------------------------------------------------------------------------
+def splitmix64(seed, n):
+    M = (1 << 64) - 1
+    state = np.uint64(seed)
+    idx = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = state + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    del M
+    return z
+
+
+def uniform_pm1(seed, n):
+    z = splitmix64(seed, n)
+    return ((z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))) * 2.0 - 1.0
+
+
+# ---------------------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------------------
+def dam_break_2d(spacing=0.01, kernel_radius=0.013, rows=64, cols=64, alpha=0.02):
+    """2-D weakly compressible dam break, built like a repo TOML scene would be:
+
+        kernel_radius = 0.013, time_step = 1e-4, gravity = 9.81, integrator = "RK4",
+        artificial_viscocity = 0.02, min = [-0.03,-0.03], dim = [1.06,1.06]
+        [boundaries.wall] spacing = 0.01, region = {min=[-0.03,-0.03], dim=[1.06,1.06],
+                                                     border = 0.03}
+        [objects.water]  density = 1000, speed_of_sound = 20, viscocity = 0,
+                         spacing = 0.01, region = {min = [0,0], dim = [63.75 s, 63.5 s/sqrt2]}
+
+    Materials follow the loader's order: boundaries, then objects, each sorted by name."""
+    h = f32(kernel_radius)
+    s = spacing
+    packing = f32(float(s) / float(h))
+    wall_region = AABB([-0.03, -0.03], [1.06, 1.06]).border(3 * s)
+    fluid_region = AABB([0.0, 0.0], [(cols - 0.25) * s, (rows - 0.5) * s / math.sqrt(2.0)])
+    wall_mat = immobile_material(0.0)
+    water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)
+    bnd, m_b = gen_particles(wall_region, packing, h, 0, wall_mat["start_den"])
+    flu, m_f = gen_particles(fluid_region, packing, h, 1, water["start_den"])
+    wall_mat["mass"] = m_b
+    water["mass"] = m_f
+    mats = np.array([wall_mat, water], wire.MATERIAL)
+    inter = resolve_interactions(mats, h)
+    bounds = wire.aabb([-0.03, -0.03], [1.06, 1.06])
+    return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, alpha, dt=1e-4, subticks=1,
+                 integrator="rk4", name="dam_break_2d")
+
+
+def _cubic_block(n, spacing, origin, jitter, seed):
+    i = np.arange(n[0])
+    j = np.arange(n[1])
+    k = np.arange(n[2])
+    I, J, K = np.meshgrid(i, j, k, indexing="ij")
+    P = np.stack([I.ravel(), J.ravel(), K.ravel()], 1).astype(np.float64)
+    P = (P + 0.5) * spacing + np.asarray(origin)
+    if jitter:
+        u = uniform_pm1(seed, P.size).reshape(P.shape)
+        P = P + jitter * spacing * u
+    return P
+
+
+def _walls(box, spacing, layers, open_top=True):
+    """Boundary lattice: cell centres of the box grown by `layers` that lie outside it."""
+    lo = -layers
+    hi = [int(round(box[a] / spacing)) + layers for a in range(3)]
+    if open_top:
+        hi[1] = int(round(box[1] / spacing))
+    ax = [np.arange(lo, hi[a]) for a in range(3)]
+    I, J, K = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+    P = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    inner = [int(round(box[a] / spacing)) for a in range(3)]
+    inside = np.ones(len(P), bool)
+    for a in range(3):
+        inside &= (P[:, a] >= 0) & (P[:, a] < inner[a])
+    P = P[~inside]
+    return (P.astype(np.float64) + 0.5) * spacing
+
+
+def fluid_column_3d(n_side=128, spacing=0.005, jitter=0.05, seed=0, box=(1.5, 1.0, 1.5),
+                    layers=3, alpha=0.02, h_factor=1.3, rk="rk2", dt=5e-5):
+    """3-D fluid column collapse: n_side^3 particles on a cubic lattice with seeded uniform
+    jitter of +-jitter*spacing in the corner of an open-top box (x, y up, z) with `layers`
+    of fixed boundary particles; poly6 support h = h_factor*spacing; Tait (rho0 1000,
+    gamma 7, c0 20); artificial viscosity alpha -> bulk_visc = alpha*h*c0."""
+    h = f32(h_factor * spacing)
+    P = _cubic_block((n_side,) * 3, spacing, (0.0, 0.0, 0.0), jitter, seed)
+    B = _walls(box, spacing, layers)
+    water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)
+    wall = immobile_material(0.0)
+    wall["mass"] = f32(1.0 * spacing**3)
+    water["mass"] = f32(1000.0 * spacing**3)
+    mats = np.array([wall, water], wire.MATERIAL)
+    flu = wire.particles(len(P))
+    flu["pos"][:, :3] = P
+    flu["den"] = 1000.0
+    flu["mat"] = 1
+    bnd = wire.particles(len(B))
+    bnd["pos"][:, :3] = B
+    bnd["den"] = 1.0
+    bnd["mat"] = 0
+    margin = (layers + 1) * spacing
+    bounds = wire.aabb([-margin] * 3, [box[0] + 2 * margin, box[1] + 2 * margin, box[2] + 2 * margin])
+    inter = resolve_interactions(mats, h)
+    return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, alpha, dt=dt, subticks=1,
+                 integrator=rk, name="fluid_column_3d")
+
+
+def small_fluid_3d(n_side=12, spacing=0.005, seed=1, jitter=0.05, box=(0.12, 0.1, 0.12),
+                   rk="rk2", dt=5e-5):
+    """A small 3-D column of the same parameters, for tests that run the oracle."""
+    return fluid_column_3d(n_side=n_side, spacing=spacing, jitter=jitter, seed=seed, box=box,
+                           rk=rk, dt=dt)
+
+
+def cantilever_3d(nx=40, ny=40, nz=400, spacing=0.0025, E=1e6, nu=0.3, density=1000.0,
+                  h_factor=1.3, clamp_layers=3, dt=1e-5):
+    """3-D elastic cantilever: an nx*ny*nz block of solid particles along z; one end held by
+    clamp_layers of fixed boundary particles; linear elastic (St Venant-Kirchhoff on the
+    Green-Lagrange strain, strain_order 2) with lambda, mu from E and nu in the reference's
+    negative sign convention; gravity loading."""
+    h = f32(h_factor * spacing)
+    P = _cubic_block((nx, ny, nz), spacing, (0.0, 0.0, 0.0), 0.0, 0)
+    lam, mu = lame_from_young(E, nu)
+    solid = solid_material(density=density, normal_stiffness=lam, shear_stiffness=mu)
+    solid["mass"] = f32(density * spacing**3)
+    wall = immobile_material(0.0)
+    wall["mass"] = f32(1.0 * spacing**3)
+    mats = np.array([wall, solid], wire.MATERIAL)
+    # clamp: boundary block behind z < 0 covering the beam cross-section
+    C = _cubic_block((nx, ny, clamp_layers), spacing, (0.0, 0.0, -clamp_layers * spacing), 0.0, 0)
+    flu = wire.particles(len(P))
+    flu["pos"][:, :3] = P
+    flu["den"] = density
+    flu["mat"] = 1
+    bnd = wire.particles(len(C))
+    bnd["pos"][:, :3] = C
+    bnd["den"] = 1.0
+    bnd["mat"] = 0
+    margin = (clamp_layers + 2) * spacing
+    ext = (nx * spacing, ny * spacing, nz * spacing)
+    bounds = wire.aabb([-margin, -margin - 0.5 * ext[2], -margin],
+                       [ext[0] + 2 * margin, ext[1] + 2 * margin + 0.5 * ext[2], ext[2] + 2 * margin])
+    inter = resolve_interactions(mats, h)
+    return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, 0.0, dt=dt, subticks=1,
+                 integrator="rk4", name="cantilever_3d")
+
+
+def contact_3d(n_fluid_side=64, n_sphere=65536, spacing=None, E=5e5, nu=0.4, density=1000.0,
+               v0=(0.0, -2.0, 0.0), contact_strength=1e4, alpha=0.02, h_factor=1.3, dt=2e-5,
+               seed=0):
+    """Fluid-solid contact: an n_fluid_side^3 jittered fluid block and an elastic sphere of
+    about n_sphere particles above it moving at v0; linear contact potential strength
+    contact_strength (radius 2h, the loader's defaults)."""
+    if spacing is None:
+        spacing = 0.005
+    h = f32(h_factor * spacing)
+    Pf = _cubic_block((n_fluid_side,) * 3, spacing, (0.0, 0.0, 0.0), 0.05, seed)
+    L = n_fluid_side * spacing
+    r = (3.0 * n_sphere / (4.0 * math.pi)) ** (1.0 / 3.0) * spacing
+    c = np.array([0.5 * L, L + r + 2 * spacing, 0.5 * L])
+    m = int(math.ceil(r / spacing)) + 1
+    g1 = np.arange(-m, m + 1)
+    I, J, K = np.meshgrid(g1, g1, g1, indexing="ij")
+    Q = np.stack([I.ravel(), J.ravel(), K.ravel()], 1).astype(np.float64) * spacing
+    Q = Q[(Q**2).sum(1) <= r * r] + c
+    box = (L, L + 2 * r + 4 * spacing, L)
+    B = _walls(box, spacing, 3)
+    water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)
+    water["mass"] = f32(1000.0 * spacing**3)
+    lam, mu = lame_from_young(E, nu)
+    ball = solid_material(density=density, normal_stiffness=lam, shear_stiffness=mu)
+    ball["mass"] = f32(density * spacing**3)
+    wall = immobile_material(0.0)
+    wall["mass"] = f32(spacing**3)
+    mats = np.array([wall, ball, water], wire.MATERIAL)
+    n = len(Pf) + len(Q)
+    flu = wire.particles(n)
+    flu["pos"][: len(Pf), :3] = Pf
+    flu["den"][: len(Pf)] = 1000.0
+    flu["mat"][: len(Pf)] = 2
+    flu["pos"][len(Pf):, :3] = Q
+    flu["vel"][len(Pf):, :3] = v0
+    flu["den"][len(Pf):] = density
+    flu["mat"][len(Pf):] = 1
+    bnd = wire.particles(len(B))
+    bnd["pos"][:, :3] = B
+    bnd["den"] = 1.0
+    bnd["mat"] = 0
+    configured = {(1, 2): wire.interaction(wire.POT_LINEAR, contact_strength, float(f32(2.0) * h), 0.0, 0.0)}
+    inter = resolve_interactions(mats, h, configured)
+    margin = 4 * spacing
+    bounds = wire.aabb([-margin] * 3, [box[0] + 2 * margin, box[1] + 2 * margin, box[2] + 2 * margin])
+    return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, alpha, dt=dt, subticks=1,
+                 integrator="rk2", name="contact_3d")

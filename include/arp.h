@@ -1,0 +1,193 @@
+/*
+ * arp.h -- C ABI of the MI355X particle step (SPH fluid, contact, elastic/plastic solids,
+ * explicit Runge-Kutta stages) that drops in for the GLSL compute dispatch of
+ * jsmith628/ar-physics.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 = ARP_OK, negative = error class (ARP_E_*);
+ *     arp_last_error(ctx) gives the message.  No C++ exception crosses this boundary.
+ *   - Host pointers stay owned by the caller and are copied before the call returns.
+ *   - Device pointers (the d_* arguments) are raw HIP device addresses of std430 arrays,
+ *     e.g. torch tensors on the same device; the call is ordered on the context's stream.
+ *   - A context is used by one thread at a time (the reference's GL-context contract);
+ *     separate contexts may live on separate threads.
+ *   - Only dim <= 3 is supported: the 4th component of every vec4 must be 0.
+ *
+ * Wire structs are byte-identical to the reference's std430 GLSL structs, so the Rust host
+ * can pass its existing soft_body types unchanged.
+ */
+#ifndef ARP_H
+#define ARP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARP_OK 0
+#define ARP_E_ARG (-1)      /* invalid argument / unsupported configuration        */
+#define ARP_E_HIP (-2)      /* HIP runtime or module error                         */
+#define ARP_E_STATE (-3)    /* call made in the wrong order (e.g. no scene)        */
+#define ARP_E_NOMEM (-4)    /* device or host allocation failed                    */
+#define ARP_E_LIMIT (-5)    /* a documented size limit was exceeded                */
+
+#define ARP_INVALID_INDEX 0xFFFFFFFFu
+
+/* Particle: src/soft_body/particle_state/mod.rs:24-29 (std430, 48 B) */
+typedef struct ArpParticle {
+    float den;
+    uint32_t mat;
+    uint32_t solid_id;
+    uint32_t _pad0;
+    float pos[4];
+    float vel[4];
+} ArpParticle;
+
+/* SolidParticle: src/soft_body/particle_state/mod.rs:31-35 (std430, 96 B);
+ * stress is a column-major mat4 (stress[4*col + row]); it holds the plastic strain. */
+typedef struct ArpSolidParticle {
+    uint32_t part_id;
+    uint32_t _pad0[3];
+    float ref_pos[4];
+    float stress[16];
+} ArpSolidParticle;
+
+/* Material: src/soft_body/material_region/material.rs:11-28 (std430, 80 B; bool = 4 B) */
+typedef struct ArpMaterial {
+    uint32_t immobile;
+    float mass, start_den;
+    uint32_t state_eq; /* 0 zero, 1 constant, 2 ideal gas, 3 Tait (material.rs:53-59) */
+    float sound_speed, target_den, visc, bulk_visc;
+    int32_t strain_order;
+    float normal_stiffness, shear_stiffness, normal_damp, shear_damp;
+    uint32_t plastic;
+    float yield_strength, work_hardening, work_hardening_exp, kinematic_hardening,
+        thermal_softening;
+    float relaxation_time;
+} ArpMaterial;
+
+/* MatInteraction: src/soft_body/material_region/interaction.rs:10-15 (std430, 20 B);
+ * potential: 0 zero, 1 constant, 2 linear, 3 Tait, 4 Lennard-Jones, 5 Hertz */
+typedef struct ArpMatInteraction {
+    uint32_t potential;
+    float strength, radius;
+    float dampening, friction;
+} ArpMatInteraction;
+
+/* AABB: src/soft_body/material_region/region.rs:11-14 (std430, 32 B) */
+typedef struct ArpAABB {
+    float min[4];
+    float dim[4];
+} ArpAABB;
+
+/* Explicit Runge-Kutta tableau (the role of the numerical-integration crate's
+ * VelIntegrates implementors, examples/test.rs:165-180).  a is strictly lower triangular. */
+#define ARP_MAX_STAGES 6
+typedef struct ArpTableau {
+    int32_t stages;
+    float a[ARP_MAX_STAGES][ARP_MAX_STAGES];
+    float b[ARP_MAX_STAGES];
+    float c[ARP_MAX_STAGES];
+} ArpTableau;
+
+/* Neighbour-structure statistics of the last grid build (per kind: boundary, reference,
+ * position), as stored by the reference's Bucket.count[3] (neighbor_list.rs:12-16). */
+typedef struct ArpGridInfo {
+    uint32_t sub[4];          /* subdivisions, neighbor_list.rs:201-206 */
+    uint64_t ncells;
+    uint32_t n_valid[3];      /* particles that got a bucket slot, per kind */
+    uint32_t n_overflow[3];   /* particles refused by the 128-slot bucket cap, per kind */
+    uint32_t n_nonfinite[3];  /* NaN/inf positions dropped from the grid, per kind */
+    uint32_t occupied_rows;   /* (y,z) cell rows holding at least one mobile particle */
+} ArpGridInfo;
+
+typedef struct arp_ctx arp_ctx;
+
+/* ---- context ------------------------------------------------------------------------- */
+/* Replaces the GL context + program compilation (soft_body/mod.rs:134-137). */
+int arp_create(int device, uint32_t flags, arp_ctx** out);
+void arp_destroy(arp_ctx* ctx);
+const char* arp_last_error(const arp_ctx* ctx);
+/* Order all work of this context on an existing hipStream_t (NULL: the context's own). */
+int arp_set_stream(arp_ctx* ctx, void* hip_stream);
+int arp_synchronize(arp_ctx* ctx);
+
+/* ---- host-side derivations the reference performs in FluidSim::new -------------------- */
+/* poly6 normalisation, computed on the host in f32 (kernel.rs:5-11, 60-63). */
+float arp_norm_const(uint32_t dim, float h);
+/* MatInteraction::default_between (interaction.rs:42-50). */
+void arp_default_interaction(const ArpMaterial* a, const ArpMaterial* b, float h,
+                             ArpMatInteraction* out);
+/* Interaction-matrix resolution of FluidSim::new (soft_body/mod.rs:93-111): cfg[i*nm+j]
+ * is used when present[i*nm+j] != 0, else cfg[j*nm+i] when present, else default_between
+ * for i != j, else the zero interaction.  out has nm*nm entries. */
+void arp_resolve_interactions(uint32_t nm, const ArpMaterial* mats,
+                              const ArpMatInteraction* cfg, const uint8_t* present, float h,
+                              ArpMatInteraction* out);
+/* Named integrators of examples/test.rs:165-180 ("euler", "rk1", "midpoint", "rk2",
+ * "heun2", "heun", "ralston", "rk3", "heun3", "rk4", "rk_3_8"); case-insensitive. */
+int arp_tableau_by_name(const char* name, ArpTableau* out);
+
+/* ---- scene (FluidSim::new, soft_body/mod.rs:58-166) ------------------------------------ */
+/* particles/solids/boundary in the reference's wire format; inter is the resolved nm x nm
+ * matrix; bounds is the neighbour-grid AABB.  Builds the static boundary and reference
+ * grids, renumbers nothing visible to the caller. */
+int arp_scene_upload(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
+                     const ArpSolidParticle* solids, uint32_t ns, const ArpParticle* boundary,
+                     uint32_t nb, const ArpMaterial* mats, uint32_t nm,
+                     const ArpMatInteraction* inter, const ArpAABB* bounds, float kernel_radius,
+                     float gravity, float artificial_viscosity);
+/* Integrator + FluidSim's timestep/subticks (soft_body/mod.rs:47-66, 240). */
+int arp_set_integrator(arp_ctx* ctx, const ArpTableau* tableau, float timestep,
+                       uint32_t subticks);
+/* FluidSim::update (soft_body/mod.rs:235-274), called nframes times: each frame advances
+ * subticks steps of timestep/subticks.  The fused fast path (sort, gather, RK stages). */
+int arp_update(arp_ctx* ctx, uint32_t nframes);
+/* Advance nsteps single RK steps of length dt (no subtick division). */
+int arp_step(arp_ctx* ctx, float dt, uint32_t nsteps);
+/* Simulation time accumulated by arp_update/arp_step (FluidSim::time). */
+double arp_time(const arp_ctx* ctx);
+/* State transfer in the caller's original particle/solid order (FluidSim::particles). */
+int arp_download(arp_ctx* ctx, ArpParticle* particles, uint32_t n, ArpSolidParticle* solids,
+                 uint32_t ns);
+int arp_upload_state(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
+                     const ArpSolidParticle* solids, uint32_t ns);
+
+/* ---- per-stage drop-ins on device std430 buffers ------------------------------------- */
+/* compute_forces (shaders/mod.rs:11-86): neighbour update at the input positions, fluid
+ * forces, and (when the scene has more than one solid) strain + solid forces, summed. */
+int arp_compute_forces(arp_ctx* ctx, const ArpParticle* d_state,
+                       const ArpSolidParticle* d_solids, ArpParticle* d_deriv,
+                       ArpSolidParticle* d_solid_deriv);
+/* ArithShaders::linear_combination (particles.rs:381-436, GLSL lin_comb_N/s_lin_comb_N):
+ * out = sum_t coef[t] * term[t] over den/pos/vel (+ ref_pos/stress); mat and solid_id come
+ * from the first term.  d_solid_terms/d_solid_out may be NULL when the scene has no solids. */
+int arp_lincomb(arp_ctx* ctx, uint32_t nterms, const float* coefs,
+                const ArpParticle* const* d_terms, const ArpSolidParticle* const* d_solid_terms,
+                ArpParticle* d_out, ArpSolidParticle* d_solid_out);
+/* ArithShaders::velocity (particles.rs:438-446, GLSL vel/vel_solid). */
+int arp_velocity(arp_ctx* ctx, float r, const ArpParticle* d_in,
+                 const ArpSolidParticle* d_solid_in, ArpParticle* d_out,
+                 ArpSolidParticle* d_solid_out);
+/* NeighborList::update_contents (neighbor_list.rs:224-294) at the given device state. */
+int arp_neighbors_update(arp_ctx* ctx, const ArpParticle* d_state);
+
+/* ---- diagnostics ------------------------------------------------------------------------ */
+int arp_grid_info(arp_ctx* ctx, ArpGridInfo* out);
+/* Neighbour structure of the last build, in original particle ids:
+ *   cell[i]       bucket id of particle i (ARP_INVALID_INDEX if dropped or refused);
+ *   cand_count[i] number of boundary + position candidates in its 27-cell stencil;
+ *   cand (optional, size >= sum of cand_count): the candidates of particle i, boundary ids
+ *   tagged with bit 31, in ascending (tag, id) order, concatenated over i. */
+int arp_neighbor_dump(arp_ctx* ctx, uint32_t* cell, uint32_t* cand_count, uint32_t* cand,
+                      uint64_t cand_capacity);
+/* Per-cell counts (boundary, reference, position) of the last build for cells
+ * [first, first+count): out has 3*count entries (Bucket.count semantics incl. the cap). */
+int arp_cell_counts(arp_ctx* ctx, uint64_t first, uint64_t count, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARP_H */
