@@ -1,0 +1,76 @@
+"""ctypes binding of libarp.so (the C ABI declared in include/arp.h).
+
+The library must have been built (``python ar-physics_amd/build.py``); loading fails
+loudly otherwise -- there is no fallback implementation."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libarp.so")
+
+SYMBOLS = [
+    "arp_create", "arp_destroy", "arp_last_error", "arp_set_stream", "arp_synchronize",
+    "arp_norm_const", "arp_default_interaction", "arp_resolve_interactions", "arp_tableau_by_name",
+    "arp_scene_upload", "arp_set_integrator", "arp_update", "arp_step", "arp_time", "arp_download",
+    "arp_upload_state", "arp_compute_forces", "arp_lincomb", "arp_velocity", "arp_neighbors_update",
+    "arp_grid_info", "arp_neighbor_dump", "arp_cell_counts", "arp_profile", "arp_profile_query",
+    "arp_profile_reset",
+]
+
+
+class ArpError(RuntimeError):
+    pass
+
+
+_lib = None
+vp = C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ArpError("libarp.so is not built: run python ar-physics_amd/build.py (%s)" % LIBPATH)
+    L = C.CDLL(LIBPATH)
+    L.arp_create.argtypes = [C.c_int, C.c_uint32, C.POINTER(vp)]
+    L.arp_destroy.argtypes = [vp]
+    L.arp_destroy.restype = None
+    L.arp_last_error.argtypes = [vp]
+    L.arp_last_error.restype = C.c_char_p
+    L.arp_set_stream.argtypes = [vp, vp]
+    L.arp_synchronize.argtypes = [vp]
+    L.arp_norm_const.argtypes = [C.c_uint32, C.c_float]
+    L.arp_norm_const.restype = C.c_float
+    L.arp_default_interaction.argtypes = [vp, vp, C.c_float, vp]
+    L.arp_default_interaction.restype = None
+    L.arp_resolve_interactions.argtypes = [C.c_uint32, vp, vp, vp, C.c_float, vp]
+    L.arp_resolve_interactions.restype = None
+    L.arp_tableau_by_name.argtypes = [C.c_char_p, vp]
+    L.arp_scene_upload.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32,
+                                   vp, vp, C.c_float, C.c_float, C.c_float]
+    L.arp_set_integrator.argtypes = [vp, vp, C.c_float, C.c_uint32]
+    L.arp_update.argtypes = [vp, C.c_uint32]
+    L.arp_step.argtypes = [vp, C.c_float, C.c_uint32]
+    L.arp_time.argtypes = [vp]
+    L.arp_time.restype = C.c_double
+    L.arp_download.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32]
+    L.arp_upload_state.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32]
+    L.arp_compute_forces.argtypes = [vp, vp, vp, vp, vp]
+    L.arp_lincomb.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp]
+    L.arp_velocity.argtypes = [vp, C.c_float, vp, vp, vp, vp]
+    L.arp_neighbors_update.argtypes = [vp, vp]
+    L.arp_grid_info.argtypes = [vp, vp]
+    L.arp_neighbor_dump.argtypes = [vp, vp, vp, vp, C.c_uint64]
+    L.arp_cell_counts.argtypes = [vp, C.c_uint64, C.c_uint64, vp]
+    L.arp_profile.argtypes = [vp, C.c_int]
+    L.arp_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    L.arp_profile_reset.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def check(ctx, rc):
+    if rc != 0:
+        msg = lib().arp_last_error(ctx)
+        raise ArpError("arp error %d: %s" % (rc, msg.decode() if msg else ""))

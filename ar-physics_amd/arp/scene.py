@@ -444,14 +444,17 @@ def _walls(box, spacing, layers, open_top=True):
     return (P.astype(np.float64) + 0.5) * spacing
 
 
-def fluid_column_3d(n_side=128, spacing=0.005, jitter=0.05, seed=0, box=(1.5, 1.0, 1.5),
-                    layers=3, alpha=0.02, h_factor=1.3, rk="rk2", dt=5e-5):
-    """3-D fluid column collapse: n_side^3 particles on a cubic lattice with seeded uniform
-    jitter of +-jitter*spacing in the corner of an open-top box (x, y up, z) with `layers`
-    of fixed boundary particles; poly6 support h = h_factor*spacing; Tait (rho0 1000,
-    gamma 7, c0 20); artificial viscosity alpha -> bulk_visc = alpha*h*c0."""
+def fluid_column_3d(shape=(128, 128, 64), spacing=0.005, jitter=0.05, seed=0, box=(1.5, 1.0, 1.5),
+                    layers=3, alpha=0.02, h_factor=1.3, rk="rk2", dt=5e-5, n_side=None):
+    """3-D fluid column collapse: a shape[0] x shape[1] x shape[2] (x, y up, z) cubic lattice
+    of 1,048,576 particles by default, with seeded uniform jitter of +-jitter*spacing, in the
+    corner of an open-top box with `layers` of fixed boundary particles; poly6 support
+    h = h_factor*spacing; Tait (rho0 1000, gamma 7, c0 20); artificial viscosity alpha ->
+    bulk_visc = alpha*h*c0.  n_side gives an n_side^3 block instead."""
+    if n_side is not None:
+        shape = (n_side,) * 3
     h = f32(h_factor * spacing)
-    P = _cubic_block((n_side,) * 3, spacing, (0.0, 0.0, 0.0), jitter, seed)
+    P = _cubic_block(tuple(shape), spacing, (0.0, 0.0, 0.0), jitter, seed)
     B = _walls(box, spacing, layers)
     water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)
     wall = immobile_material(0.0)

@@ -1,0 +1,172 @@
+"""Python mirror of the reference's per-step call surface (``FluidSim``,
+src/soft_body/mod.rs:23-275) on top of the C ABI.
+
+    sim = FluidSim(scene)          # FluidSim::new  (scene upload, static grids)
+    sim.init()                     # FluidSim::init (explicit RK keeps no history)
+    sim.update()                   # FluidSim::update: subticks x RK step
+    p = sim.particles()            # FluidSim::particles, std430 Particle records
+
+Per-stage drop-ins (``compute_forces``, ``lincomb``, ``velocity``, ``neighbors_update``)
+take torch tensors holding std430 records on the context's device.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import tableau as _tab
+from . import wire
+from ._lib import ArpError, check, lib
+
+__all__ = ["FluidSim", "ArpError"]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class FluidSim:
+    def __init__(self, scene, device=0, integrator=None, timestep=None, subticks=None):
+        L = lib()
+        self.scene = scene
+        self.ctx = C.c_void_p()
+        rc = L.arp_create(device, 0, C.byref(self.ctx))
+        if rc != 0:
+            raise ArpError("arp_create failed (%d)" % rc)
+        s = scene
+        self._keep = (np.ascontiguousarray(s.particles), np.ascontiguousarray(s.solids),
+                      np.ascontiguousarray(s.boundary), np.ascontiguousarray(s.materials),
+                      np.ascontiguousarray(s.interactions), np.ascontiguousarray(s.bounds))
+        p, so, b, m, it, bd = self._keep
+        check(self.ctx, L.arp_scene_upload(self.ctx, _p(p), len(p), _p(so), len(so), _p(b), len(b), _p(m),
+                                           len(m), _p(it), _p(bd), s.h, s.g, s.alpha))
+        self.set_integrator(integrator or s.integrator, timestep if timestep is not None else s.dt,
+                            subticks if subticks is not None else s.subticks)
+
+    # ---- integration ----------------------------------------------------------------
+    def set_integrator(self, integrator, timestep, subticks=1):
+        t = _tab.tableau(integrator) if isinstance(integrator, str) else integrator
+        self.tableau = t
+        self.timestep = float(timestep)
+        self.subticks = int(subticks)
+        check(self.ctx, lib().arp_set_integrator(self.ctx, _p(np.ascontiguousarray(t)), self.timestep, self.subticks))
+
+    def init(self):
+        """FluidSim::init: explicit RK methods need no state history."""
+        return self
+
+    def update(self, nframes=1):
+        check(self.ctx, lib().arp_update(self.ctx, nframes))
+
+    def step(self, dt, nsteps=1):
+        check(self.ctx, lib().arp_step(self.ctx, float(dt), int(nsteps)))
+
+    @property
+    def time(self):
+        return lib().arp_time(self.ctx)
+
+    def synchronize(self):
+        check(self.ctx, lib().arp_synchronize(self.ctx))
+
+    def set_stream(self, stream_handle):
+        check(self.ctx, lib().arp_set_stream(self.ctx, C.c_void_p(stream_handle)))
+
+    # ---- state ----------------------------------------------------------------------
+    def particles(self):
+        out = wire.particles(self.scene.n)
+        check(self.ctx, lib().arp_download(self.ctx, _p(out), len(out), None, 0))
+        return out
+
+    def solids(self):
+        out = wire.solids(self.scene.ns)
+        check(self.ctx, lib().arp_download(self.ctx, None, 0, _p(out), len(out)))
+        return out
+
+    def upload_state(self, particles, solids=None):
+        particles = np.ascontiguousarray(particles)
+        so = np.ascontiguousarray(solids) if solids is not None else None
+        check(self.ctx, lib().arp_upload_state(self.ctx, _p(particles), len(particles), _p(so),
+                                               len(so) if so is not None else 0))
+
+    # ---- per-stage drop-ins (device std430 buffers as torch tensors) -------------------
+    @staticmethod
+    def _dp(t):
+        return C.c_void_p(t.data_ptr()) if t is not None else None
+
+    def compute_forces(self, d_state, d_solids=None, d_deriv=None, d_solid_deriv=None):
+        check(self.ctx, lib().arp_compute_forces(self.ctx, self._dp(d_state), self._dp(d_solids),
+                                                 self._dp(d_deriv), self._dp(d_solid_deriv)))
+
+    def neighbors_update(self, d_state):
+        check(self.ctx, lib().arp_neighbors_update(self.ctx, self._dp(d_state)))
+
+    def lincomb(self, coefs, terms, out, solid_terms=None, solid_out=None):
+        n = len(terms)
+        co = np.asarray(coefs, np.float32)
+        tp = (C.c_void_p * n)(*[t.data_ptr() for t in terms])
+        sp = (C.c_void_p * n)(*[t.data_ptr() for t in solid_terms]) if solid_terms else None
+        check(self.ctx, lib().arp_lincomb(self.ctx, n, _p(co), C.cast(tp, C.c_void_p),
+                                          C.cast(sp, C.c_void_p) if sp is not None else None,
+                                          self._dp(out), self._dp(solid_out)))
+
+    def velocity(self, r, d_in, d_out, d_solid_in=None, d_solid_out=None):
+        check(self.ctx, lib().arp_velocity(self.ctx, float(r), self._dp(d_in), self._dp(d_solid_in),
+                                           self._dp(d_out), self._dp(d_solid_out)))
+
+    # ---- diagnostics ------------------------------------------------------------------
+    def grid_info(self):
+        g = np.zeros((), wire.GRID_INFO)
+        check(self.ctx, lib().arp_grid_info(self.ctx, _p(g)))
+        return g
+
+    def neighbor_dump(self):
+        n = self.scene.n
+        cell = np.zeros(n, np.uint32)
+        cc = np.zeros(n, np.uint32)
+        check(self.ctx, lib().arp_neighbor_dump(self.ctx, _p(cell), _p(cc), None, 0))
+        tot = int(cc.sum(dtype=np.uint64))
+        cand = np.zeros(max(tot, 1), np.uint32)
+        check(self.ctx, lib().arp_neighbor_dump(self.ctx, None, _p(cc), _p(cand), tot))
+        return dict(cell=cell, cand_count=cc, cand=cand[:tot])
+
+    def cell_counts(self, first=0, count=None):
+        gi = self.grid_info()
+        if count is None:
+            count = int(gi["ncells"]) - first
+        out = np.zeros(3 * count, np.uint32)
+        check(self.ctx, lib().arp_cell_counts(self.ctx, first, count, _p(out)))
+        return out
+
+    # ---- measurement -------------------------------------------------------------------
+    def profile(self, enable=True):
+        check(self.ctx, lib().arp_profile(self.ctx, 1 if enable else 0))
+
+    def profile_reset(self):
+        check(self.ctx, lib().arp_profile_reset(self.ctx))
+
+    def profile_query(self, kernel):
+        ms = C.c_double()
+        cnt = C.c_uint64()
+        check(self.ctx, lib().arp_profile_query(self.ctx, kernel.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def close(self):
+        if self.ctx:
+            lib().arp_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_device(arr, device="cuda"):
+    """std430 record array -> uint8 torch tensor on the GPU (for the per-stage API)."""
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).to(device)
+
+
+def from_device(t, dtype):
+    return t.cpu().numpy().view(dtype)

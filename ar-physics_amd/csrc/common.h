@@ -1,0 +1,223 @@
+// common.h -- POD structures shared by the host runtime and the gfx950 kernels.
+// Kernels take one argument struct by value (hipModuleLaunchKernel kernelParams).
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define ARP_HD __host__ __device__
+#else
+#define ARP_HD
+struct float4_ {
+    float x, y, z, w;
+};
+#endif
+
+namespace arp {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kFlag = 0x80000000u;    // perm entry: particle has no neighbours
+constexpr uint32_t kBnd = 0x80000000u;     // candidate tag: boundary particle
+constexpr uint32_t kBucketCap = 128u;      // slots per reference bucket (neighbor_list.rs:45)
+constexpr int kMaxSX = 2048;               // cells along x supported by the row sort
+constexpr int kRowCap = 2048;              // row elements sorted in LDS (larger rows: HBM)
+constexpr int kMaxK = 4;                   // stored stage derivatives for non-subdiagonal RK
+
+// mk word of a particle: material id | flags
+constexpr uint32_t kMatMask = 0xFFFFu;
+constexpr uint32_t kMkElastic = 1u << 16;  // solid_id valid && stiff material (fluid.rs:106)
+constexpr uint32_t kMkSolid = 1u << 17;    // solid_id valid
+
+// Material constants as the kernels read them (material.rs:11-28)
+struct DMat {
+    float mass, start_den, c, target_den, visc, bulk_visc;
+    float lambda, mu, lambda_d, mu_d;
+    float yield, hard, hard_exp, kin_hard, relax;
+    uint32_t eq, plastic, elastic;
+    int32_t strain_order;
+    uint32_t _pad;
+};
+static_assert(sizeof(DMat) == 80, "DMat layout");
+
+struct DInter {
+    uint32_t potential;
+    float strength, radius, dampening, friction;
+};
+
+struct GridP {
+    float min[3], dim[3];
+    uint32_t sub[3];
+    uint32_t nrows;      // sub[1]*sub[2]
+    uint32_t rowstride;  // sub[0]+1
+    uint32_t dim_n;      // dimensionality (axes with dim > 0)
+};
+
+struct PhysP {
+    float h, k, g, eps_hh;  // kernel radius, poly6 norm, gravity, EPSILON*h*h
+    uint32_t nm;
+};
+
+// ---------------- kernel argument blocks ---------------------------------------------
+struct KeyArgs {
+    GridP grid;
+    uint32_t n;
+    const float* posd;  // float4 per particle
+    uint32_t* keys;
+    uint32_t* ranks;
+    uint32_t* binCount;
+};
+
+struct ScanArgs {
+    uint32_t nbins;  // rows + 1 (the last bin holds non-finite positions)
+    uint32_t* binCount;
+    uint32_t* binStart;  // nbins + 1
+    uint32_t* activeRows;
+    uint32_t* numActive;
+};
+
+struct ScatterArgs {
+    GridP grid;
+    uint32_t n;
+    const uint32_t* keys;
+    const uint32_t* ranks;
+    const uint32_t* binStart;
+    uint32_t* tmpKey;
+    uint32_t* tmpSrc;
+    // non-finite particles are placed here directly (they never enter a row)
+    int full;  // 1: reorder state into sorted storage; 0: only write perm
+    uint32_t* perm;
+    const float* srcPosd;
+    const float* srcVelq;
+    const uint32_t* srcMk;
+    const uint32_t* srcPid;
+    float* dstPosd;
+    float* dstVelq;
+    uint32_t* dstMk;
+    uint32_t* dstPid;
+    uint32_t* inv;  // pid -> storage index (full mode), may be null
+};
+
+struct RowSortArgs {
+    GridP grid;
+    uint32_t n;
+    const uint32_t* binStart;
+    const uint32_t* activeRows;
+    const uint32_t* numActive;
+    const uint32_t* tmpKey;
+    const uint32_t* tmpSrc;
+    uint32_t* gPid;  // scratch for rows larger than kRowCap
+    uint32_t* gGrp;
+    uint32_t* rowOff;       // nrows * rowstride
+    const uint32_t* bP;     // boundary prefix per cell (ncells+1)
+    const uint32_t* rP;     // reference prefix per cell (ncells+1)
+    int full;
+    uint32_t* perm;
+    const float* srcPosd;
+    const float* srcVelq;
+    const uint32_t* srcMk;
+    const uint32_t* srcPid;
+    float* dstPosd;
+    float* dstVelq;
+    uint32_t* dstMk;
+    uint32_t* dstPid;
+    uint32_t* inv;
+    uint32_t* stats;  // [0] overflowed particles (atomic), may be null
+};
+
+struct FluidArgs {
+    GridP grid;
+    PhysP phys;
+    uint32_t n;
+    const DMat* mats;
+    const DInter* inter;
+    const uint32_t* perm;
+    const uint32_t* binStart;
+    const uint32_t* rowOff;
+    const uint32_t* bP;
+    const float* bposd;  // boundary (static, cell-sorted, capped)
+    const float* bvelq;
+    const uint32_t* bmk;
+    // stage input (storage order)
+    const float* Xposd;
+    const float* Xvelq;
+    const uint32_t* mk;
+    const uint32_t* pid;
+    int mode;  // 0: fused RK update, 1: write AoS derivative at pid, 2: write k to kOut only
+    // RK epilogue
+    const float* Yposd;
+    const float* Yvelq;
+    const float* AccP;  // may be null
+    const float* AccV;
+    const float* Kp[kMaxK];
+    const float* Kv[kMaxK];
+    float kc[kMaxK];
+    int nk;
+    float a_next;  // dt * a(s+1, s)
+    float b_cur;   // dt * b(s)
+    int last;
+    float* Oposd;  // X_{s+1} or y_{n+1}
+    float* Ovelq;
+    float* AccPOut;  // may be null
+    float* AccVOut;
+    float* KpOut;  // may be null
+    float* KvOut;
+    int skip_elastic;  // elastic particles finish in the solid kernel (write k to kOut)
+    // next-stage keys
+    uint32_t* keys;
+    uint32_t* ranks;
+    uint32_t* binCount;
+    // mode 1 output
+    void* aosOut;  // ArpParticle*
+};
+
+struct AosInArgs {
+    uint32_t n;
+    const void* aos;  // ArpParticle*
+    const DMat* mats;
+    const uint32_t* mkStatic;  // per pid
+    float* posd;
+    float* velq;
+    uint32_t* mk;
+    uint32_t* pid;
+};
+
+struct AosOutArgs {
+    uint32_t n;
+    const float* posd;
+    const float* velq;
+    const uint32_t* pid;
+    const uint32_t* sidByPid;
+    const uint32_t* matByPid;
+    void* aos;
+};
+
+struct LincombArgs {
+    uint32_t n, ns;
+    uint32_t nterms;
+    float coef[8];
+    const void* terms[8];   // ArpParticle*
+    const void* sterms[8];  // ArpSolidParticle*
+    void* out;
+    void* sout;
+};
+
+struct FillArgs {
+    uint32_t* p;
+    uint32_t n, v;
+};
+
+struct ClearSolidsArgs {
+    uint32_t ns;
+    const void* in;  // ArpSolidParticle*
+    void* out;
+};
+
+struct VelocityArgs {
+    uint32_t n, ns;
+    float r;
+    const void* in;
+    const void* sin;
+    void* out;
+    void* sout;
+};
+
+}  // namespace arp

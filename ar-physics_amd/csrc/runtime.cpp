@@ -1,0 +1,1126 @@
+// runtime.cpp -- host runtime behind include/arp.h: context, scene upload (FluidSim::new),
+// static boundary/reference grids, the per-stage launch chain and the fused RK step.
+// Kernels come from the embedded gfx950 code object (hipModuleLoadData +
+// hipModuleLaunchKernel); no CUDA-compat layer.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/arp.h"
+#include "common.h"
+
+using namespace arp;
+
+extern "C" const unsigned char arp_kernels_co[];
+
+namespace {
+
+enum Kern {
+    K_KEY,
+    K_SCAN,
+    K_SCATTER,
+    K_ROWSORT,
+    K_FLUID,
+    K_AOS_IN,
+    K_AOS_OUT,
+    K_LINCOMB,
+    K_VELOCITY,
+    K_FILL,
+    K_CLEAR_SOLIDS,
+    K_COUNT
+};
+const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
+                                   "k_fluid",   "k_aos_in",  "k_aos_out",  "k_lincomb",
+                                   "k_velocity", "k_fill",   "k_clear_solids"};
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+template <class T>
+T* P(DBuf& b) {
+    return reinterpret_cast<T*>(b.p);
+}
+
+}  // namespace
+
+struct arp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipModule_t mod = nullptr;
+    hipFunction_t fn[K_COUNT] = {};
+    std::string err;
+    std::vector<DBuf*> bufs;
+
+    // scene
+    bool have_scene = false;
+    uint32_t n = 0, ns = 0, nb = 0, nm = 0;
+    uint64_t ncells = 0;
+    GridP grid{};
+    PhysP phys{};
+    std::vector<ArpMaterial> mats_h;
+    std::vector<uint32_t> mat_by_pid, sid_by_pid, mk_by_pid;
+    std::vector<ArpSolidParticle> solids_h;
+    std::vector<uint32_t> b_orig;  // sorted boundary slot -> original boundary id
+    std::vector<uint32_t> bP_h, rP_h;
+
+    DBuf mats, inter, bposd, bvelq, bmk, bP, rP, mkStatic, matByPid, sidByPid;
+    // state
+    DBuf Ysort[2], Yland[2];  // [0] posd, [1] velq
+    DBuf meta_mk[2], meta_pid[2];
+    int cur = 0;
+    DBuf Xb[2][2];  // [buffer][posd/velq]
+    DBuf Acc[2];
+    DBuf K[kMaxK][2];
+    DBuf inv;
+    // per-stage API buffers
+    DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys;
+    // grid
+    DBuf keys, ranks, binCount, binStart, activeRows, numActive, tmpKey, tmpSrc, perm, rowOff,
+        gPid, gGrp, stats;
+    bool keys_valid = false;
+    // what the last build described (for the dump)
+    const DBuf* last_pid = nullptr;
+    const DBuf* last_posd = nullptr;
+    bool have_build = false;
+
+    // profiling: event pairs around launches
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_marks;  // (kernel, index of first event)
+
+    // integrator
+    ArpTableau tab{};
+    float timestep = 0.01f;
+    uint32_t subticks = 1;
+    double time = 0.0;
+
+    arp_ctx() {
+        bufs = {&mats,     &inter,     &bposd,      &bvelq,    &bmk,    &bP,       &rP,
+                &mkStatic, &matByPid,  &sidByPid,   &Ysort[0], &Ysort[1], &Yland[0], &Yland[1],
+                &meta_mk[0], &meta_mk[1], &meta_pid[0], &meta_pid[1], &Xb[0][0], &Xb[0][1],
+                &Xb[1][0], &Xb[1][1],  &Acc[0],     &Acc[1],   &inv,    &ps_mk[0], &ps_mk[1],
+                &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
+                &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
+                &gPid,     &gGrp,      &stats};
+        for (int k = 0; k < kMaxK; k++) {
+            bufs.push_back(&K[k][0]);
+            bufs.push_back(&K[k][1]);
+        }
+    }
+};
+
+namespace {
+
+int fail(arp_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(c, ARP_E_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+int dalloc(arp_ctx* c, DBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.p && b.bytes >= bytes) return ARP_OK;
+    if (b.p) {
+        hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(c, ARP_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    b.bytes = bytes;
+    return ARP_OK;
+}
+
+int upload(arp_ctx* c, DBuf& b, const void* h, size_t bytes) {
+    int rc = dalloc(c, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+    return ARP_OK;
+}
+
+hipEvent_t next_event(arp_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+template <class A>
+int launch(arp_ctx* c, Kern k, uint32_t blocks, uint32_t threads, const A& args) {
+    if (blocks == 0) return ARP_OK;
+    void* params[] = {const_cast<A*>(&args)};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        size_t first = c->ev_used;
+        e0 = next_event(c);
+        e1 = next_event(c);
+        if (e0 && e1) {
+            hipEventRecord(e0, c->stream);
+            c->ev_marks.push_back({(int)k, first});
+        }
+    }
+    hipError_t e = hipModuleLaunchKernel(c->fn[k], blocks, 1, 1, threads, 1, 1, 0, c->stream,
+                                         params, nullptr);
+    if (e != hipSuccess)
+        return fail(c, ARP_E_HIP, "launch %s failed: %s", kKernNames[k], hipGetErrorString(e));
+    if (e0 && e1) hipEventRecord(e1, c->stream);
+    return ARP_OK;
+}
+
+inline uint32_t nblk(uint64_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
+
+int fill_u32(arp_ctx* c, DBuf& b, uint32_t n, uint32_t v) {
+    FillArgs a{P<uint32_t>(b), n, v};
+    return launch(c, K_FILL, nblk(n), 256, a);
+}
+
+// ---- host restatements of FluidSim::new derivations --------------------------------
+float vol_nball(uint32_t n, float r) {
+    if (n == 0) return 1.0f;
+    if (n == 1) return 2.0f * r;
+    const float pi = 3.14159265358979323846f;
+    return 2.0f * pi * r * r * vol_nball(n - 2, r) / (float)n;
+}
+
+// host copy of the kernel's cell function (identical IEEE float operations)
+bool host_cell(const GridP& g, const float* p, uint32_t& cx, uint32_t& cy, uint32_t& cz) {
+    for (int i = 0; i < 3; i++)
+        if (!std::isfinite(p[i])) return false;
+    uint32_t c[3];
+    for (int i = 0; i < 3; i++) {
+        float t = (p[i] - g.min[i]) / g.dim[i];
+        t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+        float f = std::floor(t * (float)g.sub[i]);
+        uint32_t v = (uint32_t)f;
+        c[i] = v >= g.sub[i] ? g.sub[i] - 1 : v;
+    }
+    cx = c[0];
+    cy = c[1];
+    cz = c[2];
+    return true;
+}
+
+float host_pressure(const ArpMaterial& m, float den) {
+    float c = m.sound_speed, d0 = m.target_den, p;
+    switch (m.state_eq) {
+        case 1: p = c * c; break;
+        case 2: p = c * c * (den - d0); break;
+        case 3: p = (c * c * d0 / 7.0f) * (std::pow(den / d0, 7.0f) - 1.0f); break;
+        default: p = 0.0f;
+    }
+    return p / (den * den);
+}
+
+bool is_stiff(const ArpMaterial& m) { return m.normal_stiffness != 0.0f || m.shear_stiffness != 0.0f; }
+
+// stable counting sort of items by cell with a per-cell cap; returns the prefix over cells
+// of kept items and the kept items in (cell, id) order
+void cell_sort(uint64_t ncells, const std::vector<uint64_t>& key, const std::vector<uint32_t>& cap_room,
+               std::vector<uint32_t>& prefix, std::vector<uint32_t>& kept) {
+    std::vector<uint32_t> cnt(ncells + 1, 0);
+    for (uint64_t k : key)
+        if (k != UINT64_MAX) cnt[k + 1]++;
+    std::vector<uint32_t> start(ncells + 1, 0);
+    for (uint64_t c = 0; c < ncells; c++) start[c + 1] = start[c] + cnt[c + 1];
+    std::vector<uint32_t> order(start[ncells]);
+    std::vector<uint32_t> fillc(ncells, 0);
+    for (size_t i = 0; i < key.size(); i++)
+        if (key[i] != UINT64_MAX) order[start[key[i]] + fillc[key[i]]++] = (uint32_t)i;
+    prefix.assign(ncells + 1, 0);
+    kept.clear();
+    for (uint64_t c = 0; c < ncells; c++) {
+        uint32_t n = cnt[c + 1];
+        uint32_t room = cap_room.empty() ? kBucketCap : cap_room[c];
+        uint32_t k = std::min(n, room);
+        for (uint32_t j = 0; j < k; j++) kept.push_back(order[start[c] + j]);
+        prefix[c + 1] = prefix[c] + k;
+    }
+}
+
+// ---- the sort chain ---------------------------------------------------------------
+struct SortIO {
+    int full;
+    const float* srcP;
+    const float* srcV;
+    const uint32_t* srcMk;
+    const uint32_t* srcPid;
+    float* dstP;
+    float* dstV;
+    uint32_t* dstMk;
+    uint32_t* dstPid;
+};
+
+int run_key(arp_ctx* c, const float* posd) {
+    KeyArgs k{c->grid, c->n, posd, P<uint32_t>(c->keys), P<uint32_t>(c->ranks), P<uint32_t>(c->binCount)};
+    return launch(c, K_KEY, nblk(c->n), 256, k);
+}
+
+int run_sort(arp_ctx* c, const SortIO& io) {
+    int rc;
+    ScanArgs s{c->grid.nrows + 1, P<uint32_t>(c->binCount), P<uint32_t>(c->binStart),
+               P<uint32_t>(c->activeRows), P<uint32_t>(c->numActive)};
+    if ((rc = launch(c, K_SCAN, 1, 1024, s))) return rc;
+    ScatterArgs sc{};
+    sc.grid = c->grid;
+    sc.n = c->n;
+    sc.keys = P<uint32_t>(c->keys);
+    sc.ranks = P<uint32_t>(c->ranks);
+    sc.binStart = P<uint32_t>(c->binStart);
+    sc.tmpKey = P<uint32_t>(c->tmpKey);
+    sc.tmpSrc = P<uint32_t>(c->tmpSrc);
+    sc.full = io.full;
+    sc.perm = P<uint32_t>(c->perm);
+    sc.srcPosd = io.srcP;
+    sc.srcVelq = io.srcV;
+    sc.srcMk = io.srcMk;
+    sc.srcPid = io.srcPid;
+    sc.dstPosd = io.dstP;
+    sc.dstVelq = io.dstV;
+    sc.dstMk = io.dstMk;
+    sc.dstPid = io.dstPid;
+    sc.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
+    if ((rc = launch(c, K_SCATTER, nblk(c->n), 256, sc))) return rc;
+    RowSortArgs r{};
+    r.grid = c->grid;
+    r.n = c->n;
+    r.binStart = P<uint32_t>(c->binStart);
+    r.activeRows = P<uint32_t>(c->activeRows);
+    r.numActive = P<uint32_t>(c->numActive);
+    r.tmpKey = P<uint32_t>(c->tmpKey);
+    r.tmpSrc = P<uint32_t>(c->tmpSrc);
+    r.gPid = P<uint32_t>(c->gPid);
+    r.gGrp = P<uint32_t>(c->gGrp);
+    r.rowOff = P<uint32_t>(c->rowOff);
+    r.bP = P<uint32_t>(c->bP);
+    r.rP = P<uint32_t>(c->rP);
+    r.full = io.full;
+    r.perm = P<uint32_t>(c->perm);
+    r.srcPosd = io.srcP;
+    r.srcVelq = io.srcV;
+    r.srcMk = io.srcMk;
+    r.srcPid = io.srcPid;
+    r.dstPosd = io.dstP;
+    r.dstVelq = io.dstV;
+    r.dstMk = io.dstMk;
+    r.dstPid = io.dstPid;
+    r.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
+    r.stats = P<uint32_t>(c->stats);
+    uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(c->grid.nrows, 1), 4096);
+    return launch(c, K_ROWSORT, blocks, 256, r);
+}
+
+FluidArgs fluid_base(arp_ctx* c) {
+    FluidArgs f{};
+    f.grid = c->grid;
+    f.phys = c->phys;
+    f.n = c->n;
+    f.mats = P<DMat>(c->mats);
+    f.inter = P<DInter>(c->inter);
+    f.perm = P<uint32_t>(c->perm);
+    f.binStart = P<uint32_t>(c->binStart);
+    f.rowOff = P<uint32_t>(c->rowOff);
+    f.bP = P<uint32_t>(c->bP);
+    f.bposd = P<float>(c->bposd);
+    f.bvelq = P<float>(c->bvelq);
+    f.bmk = P<uint32_t>(c->bmk);
+    return f;
+}
+
+int alloc_state(arp_ctx* c) {
+    size_t n = std::max<uint32_t>(c->n, 1);
+    int rc = 0;
+    for (int k = 0; k < 2; k++) {
+        if ((rc = dalloc(c, c->Ysort[k], 16 * n))) return rc;
+        if ((rc = dalloc(c, c->Yland[k], 16 * n))) return rc;
+        if ((rc = dalloc(c, c->meta_mk[k], 4 * n))) return rc;
+        if ((rc = dalloc(c, c->meta_pid[k], 4 * n))) return rc;
+        if ((rc = dalloc(c, c->ps_mk[k], 4 * n))) return rc;
+        if ((rc = dalloc(c, c->ps_pid[k], 4 * n))) return rc;
+        if ((rc = dalloc(c, c->Xb[k][0], 16 * n))) return rc;
+        if ((rc = dalloc(c, c->Xb[k][1], 16 * n))) return rc;
+        if ((rc = dalloc(c, c->K[0][k], 16 * n))) return rc;
+    }
+    if ((rc = dalloc(c, c->aos_tmp, sizeof(ArpParticle) * n))) return rc;
+    if ((rc = dalloc(c, c->inv, 4 * n))) return rc;
+    for (DBuf* b : {&c->keys, &c->ranks, &c->tmpKey, &c->tmpSrc, &c->perm, &c->gPid, &c->gGrp})
+        if ((rc = dalloc(c, *b, 4 * n))) return rc;
+    uint32_t nb = c->grid.nrows + 1;
+    if ((rc = dalloc(c, c->binCount, 4 * (size_t)nb))) return rc;
+    if ((rc = dalloc(c, c->binStart, 4 * ((size_t)nb + 1)))) return rc;
+    if ((rc = dalloc(c, c->activeRows, 4 * (size_t)nb))) return rc;
+    if ((rc = dalloc(c, c->numActive, 16))) return rc;
+    if ((rc = dalloc(c, c->stats, 16))) return rc;
+    if ((rc = dalloc(c, c->rowOff, 4 * (size_t)c->grid.nrows * c->grid.rowstride))) return rc;
+    return ARP_OK;
+}
+
+int ensure_k(arp_ctx* c, int slots, bool acc) {
+    size_t n = std::max<uint32_t>(c->n, 1);
+    int rc;
+    for (int k = 0; k < slots; k++)
+        for (int j = 0; j < 2; j++)
+            if ((rc = dalloc(c, c->K[k][j], 16 * n))) return rc;
+    if (acc)
+        for (int j = 0; j < 2; j++)
+            if ((rc = dalloc(c, c->Acc[j], 16 * n))) return rc;
+    return ARP_OK;
+}
+
+// AoS (device) -> SoA landing buffers in original order
+int aos_to_soa(arp_ctx* c, const void* d_aos, float* posd, float* velq, uint32_t* mk, uint32_t* pid) {
+    AosInArgs a{c->n, d_aos, P<DMat>(c->mats), P<uint32_t>(c->mkStatic), posd, velq, mk, pid};
+    return launch(c, K_AOS_IN, nblk(c->n), 256, a);
+}
+
+int soa_to_aos(arp_ctx* c, const float* posd, const float* velq, const uint32_t* pid, void* d_aos) {
+    AosOutArgs a{c->n, posd, velq, pid, P<uint32_t>(c->sidByPid), P<uint32_t>(c->matByPid), d_aos};
+    return launch(c, K_AOS_OUT, nblk(c->n), 256, a);
+}
+
+// One RK step of length dt on the internal state (Yland/meta[cur] on entry and exit).
+int step_once(arp_ctx* c, float dt) {
+    const ArpTableau& T = c->tab;
+    const int S = T.stages;
+    int rc;
+    // which stage derivatives must be kept for later (non-subdiagonal) stage inputs
+    int kslot[ARP_MAX_STAGES];
+    int nslots = 0;
+    for (int j = 0; j < S; j++) {
+        kslot[j] = -1;
+        for (int t = j + 2; t < S; t++)
+            if (T.a[t][j] != 0.0f) kslot[j] = 0;
+        if (kslot[j] == 0) kslot[j] = nslots++;
+    }
+    if (nslots > kMaxK) return fail(c, ARP_E_LIMIT, "tableau needs %d stored stages (max %d)", nslots, kMaxK);
+    bool acc = false;
+    for (int j = 0; j + 1 < S; j++)
+        if (T.b[j] != 0.0f) acc = true;
+    if ((rc = ensure_k(c, nslots, acc))) return rc;
+
+    // stage-1 sort: full reorder of y_n into cell-major storage
+    if (!c->keys_valid) {
+        if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
+    }
+    const int nx = c->cur ^ 1;
+    SortIO io{1,
+              P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_mk[c->cur]),
+              P<uint32_t>(c->meta_pid[c->cur]),
+              P<float>(c->Ysort[0]), P<float>(c->Ysort[1]), P<uint32_t>(c->meta_mk[nx]),
+              P<uint32_t>(c->meta_pid[nx])};
+    if ((rc = run_sort(c, io))) return rc;
+    c->cur = nx;
+    c->keys_valid = false;
+
+    const float* Xp = P<float>(c->Ysort[0]);
+    const float* Xv = P<float>(c->Ysort[1]);
+    for (int s = 0; s < S; s++) {
+        if (s > 0) {
+            SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
+                       nullptr, nullptr, nullptr, nullptr};
+            if ((rc = run_sort(c, pio))) return rc;
+        }
+        FluidArgs f = fluid_base(c);
+        f.Xposd = Xp;
+        f.Xvelq = Xv;
+        f.mk = P<uint32_t>(c->meta_mk[c->cur]);
+        f.pid = P<uint32_t>(c->meta_pid[c->cur]);
+        f.mode = 0;
+        f.Yposd = P<float>(c->Ysort[0]);
+        f.Yvelq = P<float>(c->Ysort[1]);
+        f.b_cur = dt * T.b[s];
+        const bool last = (s == S - 1);
+        f.last = last ? 1 : 0;
+        if (acc && s > 0) {
+            f.AccP = P<float>(c->Acc[0]);
+            f.AccV = P<float>(c->Acc[1]);
+        }
+        if (acc && !last) {
+            f.AccPOut = P<float>(c->Acc[0]);
+            f.AccVOut = P<float>(c->Acc[1]);
+        }
+        if (!last) {
+            f.a_next = dt * T.a[s + 1][s];
+            f.nk = 0;
+            for (int j = 0; j < s; j++)
+                if (T.a[s + 1][j] != 0.0f) {
+                    f.Kp[f.nk] = P<float>(c->K[kslot[j]][0]);
+                    f.Kv[f.nk] = P<float>(c->K[kslot[j]][1]);
+                    f.kc[f.nk] = dt * T.a[s + 1][j];
+                    f.nk++;
+                }
+            f.Oposd = P<float>(c->Xb[s % 2][0]);
+            f.Ovelq = P<float>(c->Xb[s % 2][1]);
+        } else {
+            f.Oposd = P<float>(c->Yland[0]);
+            f.Ovelq = P<float>(c->Yland[1]);
+        }
+        if (kslot[s] >= 0) {
+            f.KpOut = P<float>(c->K[kslot[s]][0]);
+            f.KvOut = P<float>(c->K[kslot[s]][1]);
+        }
+        f.keys = P<uint32_t>(c->keys);
+        f.ranks = P<uint32_t>(c->ranks);
+        f.binCount = P<uint32_t>(c->binCount);
+        if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
+        if (!last) {
+            Xp = P<float>(c->Xb[s % 2][0]);
+            Xv = P<float>(c->Xb[s % 2][1]);
+        }
+    }
+    c->keys_valid = true;  // the last epilogue binned y_{n+1}
+    c->have_build = false;
+    return ARP_OK;
+}
+
+int need_scene(arp_ctx* c) {
+    if (!c) return ARP_E_ARG;
+    if (!c->have_scene) return fail(c, ARP_E_STATE, "no scene uploaded");
+    return ARP_OK;
+}
+
+// Per-stage path: device AoS state -> sorted SoA in the ps buffers (full reorder).
+int stage_sort_aos(arp_ctx* c, const void* d_state) {
+    int rc;
+    if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
+    float* lp = P<float>(c->Xb[0][0]);
+    float* lv = P<float>(c->Xb[0][1]);
+    if ((rc = aos_to_soa(c, d_state, lp, lv, P<uint32_t>(c->ps_mk[0]), P<uint32_t>(c->ps_pid[0])))) return rc;
+    if ((rc = run_key(c, lp))) return rc;
+    SortIO io{1, lp, lv, P<uint32_t>(c->ps_mk[0]), P<uint32_t>(c->ps_pid[0]),
+              P<float>(c->Xb[1][0]), P<float>(c->Xb[1][1]), P<uint32_t>(c->ps_mk[1]),
+              P<uint32_t>(c->ps_pid[1])};
+    if ((rc = run_sort(c, io))) return rc;
+    c->keys_valid = false;
+    c->last_pid = &c->ps_pid[1];
+    c->last_posd = &c->Xb[1][0];
+    c->have_build = true;
+    return ARP_OK;
+}
+
+}  // namespace
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+extern "C" {
+
+int arp_create(int device, uint32_t flags, arp_ctx** out) {
+    (void)flags;
+    if (!out) return ARP_E_ARG;
+    *out = nullptr;
+    arp_ctx* c = new (std::nothrow) arp_ctx();
+    if (!c) return ARP_E_NOMEM;
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->own_stream);
+    if (e == hipSuccess) e = hipModuleLoadData(&c->mod, arp_kernels_co);
+    for (int k = 0; e == hipSuccess && k < K_COUNT; k++) e = hipModuleGetFunction(&c->fn[k], c->mod, kKernNames[k]);
+    if (e != hipSuccess) {
+        fprintf(stderr, "arp_create: %s\n", hipGetErrorString(e));
+        if (c->own_stream) hipStreamDestroy(c->own_stream);
+        delete c;
+        return ARP_E_HIP;
+    }
+    c->stream = c->own_stream;
+    arp_tableau_by_name("rk4", &c->tab);
+    *out = c;
+    return ARP_OK;
+}
+
+void arp_destroy(arp_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (DBuf* b : c->bufs)
+        if (b->p) hipFree(b->p);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->mod) hipModuleUnload(c->mod);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char* arp_last_error(const arp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int arp_set_stream(arp_ctx* c, void* s) {
+    if (!c) return ARP_E_ARG;
+    c->stream = s ? reinterpret_cast<hipStream_t>(s) : c->own_stream;
+    return ARP_OK;
+}
+
+int arp_synchronize(arp_ctx* c) {
+    if (!c) return ARP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ARP_OK;
+}
+
+float arp_norm_const(uint32_t dim, float h) {
+    const float pi = 3.14159265358979323846f;
+    return (pi * pi * pi) / (6.0f * vol_nball(dim + 6, h));
+}
+
+void arp_default_interaction(const ArpMaterial* a, const ArpMaterial* b, float h, ArpMatInteraction* out) {
+    if (a->state_eq != 0 && b->state_eq != 0) {
+        *out = ArpMatInteraction{0, 0.0f, 0.0f, 0.0f, 0.0f};
+    } else {
+        *out = ArpMatInteraction{2, 10000.0f, h * 2.0f, 0.0f, a->visc + b->visc};
+    }
+}
+
+void arp_resolve_interactions(uint32_t nm, const ArpMaterial* mats, const ArpMatInteraction* cfg,
+                              const uint8_t* present, float h, ArpMatInteraction* out) {
+    for (uint32_t i = 0; i < nm; i++)
+        for (uint32_t j = i; j < nm; j++) {
+            ArpMatInteraction it;
+            if (present && present[i * nm + j])
+                it = cfg[i * nm + j];
+            else if (present && present[j * nm + i])
+                it = cfg[j * nm + i];
+            else if (i != j)
+                arp_default_interaction(&mats[i], &mats[j], h, &it);
+            else
+                it = ArpMatInteraction{0, 0.0f, 0.0f, 0.0f, 0.0f};
+            out[i * nm + j] = it;
+            out[j * nm + i] = it;
+        }
+}
+
+int arp_tableau_by_name(const char* name, ArpTableau* out) {
+    if (!name || !out) return ARP_E_ARG;
+    std::string s(name);
+    for (char& ch : s) ch = (char)tolower((unsigned char)ch);
+    ArpTableau t{};
+    auto set = [&](int S, std::initializer_list<float> a, std::initializer_list<float> b) {
+        t.stages = S;
+        int k = 0;
+        for (float v : a) {
+            t.a[k / S][k % S] = v;
+            k++;
+        }
+        k = 0;
+        for (float v : b) t.b[k++] = v;
+        for (int i = 0; i < S; i++) {
+            float cs = 0;
+            for (int j = 0; j < S; j++) cs += t.a[i][j];
+            t.c[i] = cs;
+        }
+    };
+    if (s == "euler" || s == "rk1")
+        set(1, {0.0f}, {1.0f});
+    else if (s == "midpoint" || s == "rk2")
+        set(2, {0, 0, 0.5f, 0}, {0.0f, 1.0f});
+    else if (s == "heun2" || s == "heun")
+        set(2, {0, 0, 1.0f, 0}, {0.5f, 0.5f});
+    else if (s == "ralston")
+        set(2, {0, 0, 2.0f / 3.0f, 0}, {0.25f, 0.75f});
+    else if (s == "rk3")
+        set(3, {0, 0, 0, 0.5f, 0, 0, -1.0f, 2.0f, 0}, {1.0f / 6.0f, 2.0f / 3.0f, 1.0f / 6.0f});
+    else if (s == "heun3")
+        set(3, {0, 0, 0, 1.0f / 3.0f, 0, 0, 0, 2.0f / 3.0f, 0}, {0.25f, 0.0f, 0.75f});
+    else if (s == "rk4")
+        set(4, {0, 0, 0, 0, 0.5f, 0, 0, 0, 0, 0.5f, 0, 0, 0, 0, 1.0f, 0},
+            {1.0f / 6.0f, 1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 6.0f});
+    else if (s == "rk_3_8")
+        set(4, {0, 0, 0, 0, 1.0f / 3.0f, 0, 0, 0, -1.0f / 3.0f, 1.0f, 0, 0, 1.0f, -1.0f, 1.0f, 0},
+            {0.125f, 0.375f, 0.375f, 0.125f});
+    else
+        return ARP_E_ARG;
+    *out = t;
+    return ARP_OK;
+}
+
+int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const ArpSolidParticle* solids,
+                     uint32_t ns, const ArpParticle* boundary, uint32_t nb, const ArpMaterial* mats,
+                     uint32_t nm, const ArpMatInteraction* inter, const ArpAABB* bounds, float h,
+                     float gravity, float artificial_viscosity) {
+    (void)artificial_viscosity;  // uniform f of the reference shaders is never read (fluid.rs:43)
+    if (!c || !bounds || !mats || !inter || nm == 0 || (n && !particles) || (nb && !boundary))
+        return fail(c, ARP_E_ARG, "arp_scene_upload: missing argument");
+    if (!(h > 0.0f)) return fail(c, ARP_E_ARG, "kernel radius must be > 0");
+    if (nm > 0xFFFF) return fail(c, ARP_E_LIMIT, "at most 65535 materials");
+    if (bounds->dim[3] > 0.0f) return fail(c, ARP_E_ARG, "4-D scenes (bounds.dim[3] > 0) are not supported");
+    HIPCHK(c, hipSetDevice(c->device));
+    GridP g{};
+    uint32_t sub4[4];
+    for (int i = 0; i < 4; i++) {
+        float q = bounds->dim[i] / h;
+        float m = 1.0f > q ? 1.0f : q;
+        if (!(m == m)) m = 1.0f;
+        sub4[i] = m >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)m;
+    }
+    for (int i = 0; i < 3; i++) {
+        g.min[i] = bounds->min[i];
+        g.dim[i] = bounds->dim[i];
+        g.sub[i] = sub4[i];
+    }
+    uint64_t ncells = (uint64_t)g.sub[0] * g.sub[1] * g.sub[2];
+    if (g.sub[0] > (uint32_t)kMaxSX)
+        return fail(c, ARP_E_LIMIT, "grid has %u cells along x (max %d)", g.sub[0], kMaxSX);
+    if (ncells >= 0x7FFFFFFFull) return fail(c, ARP_E_LIMIT, "grid has %llu cells (max 2^31-1)", (unsigned long long)ncells);
+    g.nrows = g.sub[1] * g.sub[2];
+    g.rowstride = g.sub[0] + 1;
+    uint32_t dim = 0;
+    for (int i = 0; i < 4; i++)
+        if (bounds->dim[i] > 0.0f) dim++;
+    g.dim_n = dim;
+    c->grid = g;
+    c->ncells = ncells;
+    c->phys.h = h;
+    c->phys.k = arp_norm_const(dim, h);
+    c->phys.g = gravity;
+    c->phys.eps_hh = 0.1f * h * h;
+    c->phys.nm = nm;
+    c->n = n;
+    c->ns = ns;
+    c->nb = nb;
+    c->nm = nm;
+    c->mats_h.assign(mats, mats + nm);
+    c->solids_h.assign(solids, solids + ns);
+
+    std::vector<DMat> dm(nm);
+    for (uint32_t i = 0; i < nm; i++) {
+        const ArpMaterial& m = mats[i];
+        DMat d{};
+        d.mass = m.mass;
+        d.start_den = m.start_den;
+        d.c = m.sound_speed;
+        d.target_den = m.target_den;
+        d.visc = m.visc;
+        d.bulk_visc = m.bulk_visc;
+        d.lambda = m.normal_stiffness;
+        d.mu = m.shear_stiffness;
+        d.lambda_d = m.normal_damp;
+        d.mu_d = m.shear_damp;
+        d.yield = m.yield_strength;
+        d.hard = m.work_hardening;
+        d.hard_exp = m.work_hardening_exp;
+        d.kin_hard = m.kinematic_hardening;
+        d.relax = m.relaxation_time;
+        d.eq = m.state_eq;
+        d.plastic = m.plastic;
+        d.elastic = is_stiff(m) ? 1u : 0u;
+        d.strain_order = m.strain_order;
+        dm[i] = d;
+    }
+    std::vector<DInter> di((size_t)nm * nm);
+    for (size_t i = 0; i < di.size(); i++)
+        di[i] = DInter{inter[i].potential, inter[i].strength, inter[i].radius, inter[i].dampening, inter[i].friction};
+    int rc;
+    if ((rc = upload(c, c->mats, dm.data(), dm.size() * sizeof(DMat)))) return rc;
+    if ((rc = upload(c, c->inter, di.data(), di.size() * sizeof(DInter)))) return rc;
+
+    // per-particle static words
+    c->mat_by_pid.resize(n);
+    c->sid_by_pid.resize(n);
+    c->mk_by_pid.resize(n);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t m = particles[i].mat;
+        if (m >= nm) return fail(c, ARP_E_ARG, "particle %u has material %u >= %u", i, m, nm);
+        uint32_t sid = particles[i].solid_id;
+        c->mat_by_pid[i] = m;
+        c->sid_by_pid[i] = sid;
+        uint32_t mk = m;
+        if (sid != ARP_INVALID_INDEX) {
+            mk |= kMkSolid;
+            if (is_stiff(mats[m])) mk |= kMkElastic;
+        }
+        c->mk_by_pid[i] = mk;
+    }
+    if ((rc = upload(c, c->mkStatic, c->mk_by_pid.data(), 4 * (size_t)n))) return rc;
+    if ((rc = upload(c, c->matByPid, c->mat_by_pid.data(), 4 * (size_t)n))) return rc;
+    if ((rc = upload(c, c->sidByPid, c->sid_by_pid.data(), 4 * (size_t)n))) return rc;
+
+    // static boundary grid (mode 0 fill, once: neighbor_list.rs:247-264)
+    {
+        std::vector<uint64_t> key(nb);
+        for (uint32_t i = 0; i < nb; i++) {
+            uint32_t cx, cy, cz;
+            const float p4[4] = {boundary[i].pos[0], boundary[i].pos[1], boundary[i].pos[2], boundary[i].pos[3]};
+            bool fin = std::isfinite(p4[3]) && host_cell(g, p4, cx, cy, cz);
+            key[i] = fin ? (uint64_t)cx + (uint64_t)g.sub[0] * (cy + (uint64_t)g.sub[1] * cz) : UINT64_MAX;
+            if (boundary[i].mat >= nm) return fail(c, ARP_E_ARG, "boundary %u has material %u >= %u", i, boundary[i].mat, nm);
+        }
+        std::vector<uint32_t> kept;
+        cell_sort(ncells, key, {}, c->bP_h, kept);
+        c->b_orig = kept;
+        std::vector<float> bp(4 * kept.size() + 4), bv(4 * kept.size() + 4);
+        std::vector<uint32_t> bm(kept.size() + 1);
+        for (size_t s = 0; s < kept.size(); s++) {
+            const ArpParticle& b = boundary[kept[s]];
+            const ArpMaterial& m = mats[b.mat];
+            bp[4 * s + 0] = b.pos[0];
+            bp[4 * s + 1] = b.pos[1];
+            bp[4 * s + 2] = b.pos[2];
+            bp[4 * s + 3] = b.den;
+            bv[4 * s + 0] = b.vel[0];
+            bv[4 * s + 1] = b.vel[1];
+            bv[4 * s + 2] = b.vel[2];
+            bv[4 * s + 3] = host_pressure(m, b.den);
+            uint32_t mk = b.mat;
+            // elastic2 of a boundary neighbour reads particles[boundary id].solid_id
+            // (fluid.rs:141, 174): reproduced for the boundary's own material
+            uint32_t id = kept[s];
+            if (id < n && particles[id].solid_id != ARP_INVALID_INDEX && is_stiff(m)) mk |= kMkElastic;
+            bm[s] = mk;
+        }
+        if ((rc = upload(c, c->bposd, bp.data(), bp.size() * 4))) return rc;
+        if ((rc = upload(c, c->bvelq, bv.data(), bv.size() * 4))) return rc;
+        if ((rc = upload(c, c->bmk, bm.data(), bm.size() * 4))) return rc;
+        if ((rc = upload(c, c->bP, c->bP_h.data(), c->bP_h.size() * 4))) return rc;
+    }
+    // static reference grid (mode 1 fill: neighbor_list.rs:266-275), capped at 128 - count[0]
+    {
+        std::vector<uint64_t> key(n, UINT64_MAX);
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t sid = particles[i].solid_id;
+            if (sid == ARP_INVALID_INDEX || sid >= ns) continue;
+            uint32_t cx, cy, cz;
+            const float* p4 = solids[sid].ref_pos;
+            if (std::isfinite(p4[3]) && host_cell(g, p4, cx, cy, cz))
+                key[i] = (uint64_t)cx + (uint64_t)g.sub[0] * (cy + (uint64_t)g.sub[1] * cz);
+        }
+        std::vector<uint32_t> room(ncells);
+        for (uint64_t cc = 0; cc < ncells; cc++) room[cc] = kBucketCap - (c->bP_h[cc + 1] - c->bP_h[cc]);
+        std::vector<uint32_t> kept;
+        cell_sort(ncells, key, room, c->rP_h, kept);
+        if ((rc = upload(c, c->rP, c->rP_h.data(), c->rP_h.size() * 4))) return rc;
+    }
+    if ((rc = alloc_state(c))) return rc;
+    c->have_scene = true;
+    c->cur = 0;
+    c->time = 0.0;
+    c->keys_valid = false;
+    c->have_build = false;
+    if ((rc = fill_u32(c, c->binCount, g.nrows + 1, 0))) return rc;
+    if ((rc = arp_upload_state(c, particles, n, solids, ns))) return rc;
+    return arp_synchronize(c);
+}
+
+int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const ArpSolidParticle* solids, uint32_t ns) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (n != c->n || (n && !particles)) return fail(c, ARP_E_ARG, "state has %u particles, scene %u", n, c->n);
+    (void)solids;
+    (void)ns;
+    HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    if (c->keys_valid)
+        if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
+    if ((rc = aos_to_soa(c, c->aos_tmp.p, P<float>(c->Yland[0]), P<float>(c->Yland[1]),
+                         P<uint32_t>(c->meta_mk[c->cur]), P<uint32_t>(c->meta_pid[c->cur]))))
+        return rc;
+    c->keys_valid = false;
+    return ARP_OK;
+}
+
+int arp_set_integrator(arp_ctx* c, const ArpTableau* t, float timestep, uint32_t subticks) {
+    if (!c || !t) return ARP_E_ARG;
+    if (t->stages < 1 || t->stages > ARP_MAX_STAGES) return fail(c, ARP_E_ARG, "tableau has %d stages", t->stages);
+    for (int i = 0; i < t->stages; i++)
+        for (int j = i; j < t->stages; j++)
+            if (t->a[i][j] != 0.0f) return fail(c, ARP_E_ARG, "tableau is not explicit (a[%d][%d] != 0)", i, j);
+    c->tab = *t;
+    c->timestep = timestep;
+    c->subticks = subticks ? subticks : 1;
+    return ARP_OK;
+}
+
+int arp_step(arp_ctx* c, float dt, uint32_t nsteps) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (uint32_t s = 0; s < nsteps; s++) {
+        if ((rc = step_once(c, dt))) return rc;
+        c->time += dt;
+    }
+    return ARP_OK;
+}
+
+int arp_update(arp_ctx* c, uint32_t nframes) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    const float dt = c->timestep / (float)c->subticks;
+    for (uint32_t f = 0; f < nframes; f++)
+        for (uint32_t s = 0; s < c->subticks; s++) {
+            if ((rc = step_once(c, dt))) return rc;
+            c->time += dt;
+        }
+    return ARP_OK;
+}
+
+double arp_time(const arp_ctx* c) { return c ? c->time : 0.0; }
+
+int arp_download(arp_ctx* c, ArpParticle* particles, uint32_t n, ArpSolidParticle* solids, uint32_t ns) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (particles) {
+        if (n != c->n) return fail(c, ARP_E_ARG, "download of %u particles, scene has %u", n, c->n);
+        if ((rc = soa_to_aos(c, P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_pid[c->cur]), c->aos_tmp.p)))
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(particles, c->aos_tmp.p, sizeof(ArpParticle) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (solids) {
+        if (ns != c->ns) return fail(c, ARP_E_ARG, "download of %u solids, scene has %u", ns, c->ns);
+        std::copy(c->solids_h.begin(), c->solids_h.end(), solids);
+    }
+    return arp_synchronize(c);
+}
+
+int arp_neighbors_update(arp_ctx* c, const ArpParticle* d_state) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!d_state) return fail(c, ARP_E_ARG, "null state");
+    return stage_sort_aos(c, d_state);
+}
+
+int arp_compute_forces(arp_ctx* c, const ArpParticle* d_state, const ArpSolidParticle* d_solids,
+                       ArpParticle* d_deriv, ArpSolidParticle* d_solid_deriv) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!d_state || !d_deriv) return fail(c, ARP_E_ARG, "null state/derivative");
+    if (c->ns > 1) return fail(c, ARP_E_STATE, "solid path not enabled in this build");
+    if ((rc = stage_sort_aos(c, d_state))) return rc;
+    FluidArgs f = fluid_base(c);
+    f.Xposd = P<float>(c->Xb[1][0]);
+    f.Xvelq = P<float>(c->Xb[1][1]);
+    f.mk = P<uint32_t>(c->ps_mk[1]);
+    f.pid = P<uint32_t>(c->ps_pid[1]);
+    f.mode = 2;
+    f.KpOut = P<float>(c->K[0][0]);
+    f.KvOut = P<float>(c->K[0][1]);
+    if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
+    if ((rc = soa_to_aos(c, P<float>(c->K[0][0]), P<float>(c->K[0][1]), P<uint32_t>(c->ps_pid[1]), d_deriv))) return rc;
+    if (d_solid_deriv && d_solids) {
+        ClearSolidsArgs a{c->ns, d_solids, d_solid_deriv};
+        if ((rc = launch(c, K_CLEAR_SOLIDS, nblk(c->ns), 256, a))) return rc;
+    }
+    return ARP_OK;
+}
+
+int arp_lincomb(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
+                const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out, ArpSolidParticle* d_solid_out) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (nterms == 0 || !coefs || !d_terms || !d_out) return fail(c, ARP_E_ARG, "lincomb: bad arguments");
+    // reduce_owned/reduce_ref: chunks of 5, combined again with coefficient 1
+    // (particles.rs:381-417)
+    if (nterms > 5) {
+        uint32_t nch = (nterms + 4) / 5;
+        if (nch > 5) return fail(c, ARP_E_LIMIT, "lincomb of more than 25 terms");
+        std::vector<DBuf> tmp(nch), stmp(nch);
+        std::vector<const ArpParticle*> tp(nch);
+        std::vector<const ArpSolidParticle*> sp(nch);
+        std::vector<float> ones(nch, 1.0f);
+        for (uint32_t k = 0; k < nch; k++) {
+            if ((rc = dalloc(c, tmp[k], sizeof(ArpParticle) * (size_t)std::max(c->n, 1u)))) return rc;
+            if (d_solid_terms && d_solid_out)
+                if ((rc = dalloc(c, stmp[k], sizeof(ArpSolidParticle) * (size_t)std::max(c->ns, 1u)))) return rc;
+            uint32_t k0 = 5 * k, nk = std::min(5u, nterms - k0);
+            rc = arp_lincomb(c, nk, coefs + k0, d_terms + k0, d_solid_terms ? d_solid_terms + k0 : nullptr,
+                             P<ArpParticle>(tmp[k]), d_solid_out ? P<ArpSolidParticle>(stmp[k]) : nullptr);
+            if (rc) return rc;
+            tp[k] = P<ArpParticle>(tmp[k]);
+            sp[k] = d_solid_out ? P<ArpSolidParticle>(stmp[k]) : nullptr;
+        }
+        rc = arp_lincomb(c, nch, ones.data(), tp.data(), d_solid_out ? sp.data() : nullptr, d_out, d_solid_out);
+        hipStreamSynchronize(c->stream);
+        for (uint32_t k = 0; k < nch; k++) {
+            if (tmp[k].p) hipFree(tmp[k].p);
+            if (stmp[k].p) hipFree(stmp[k].p);
+        }
+        return rc;
+    }
+    LincombArgs a{};
+    a.n = c->n;
+    a.ns = (d_solid_terms && d_solid_out) ? c->ns : 0;
+    a.nterms = nterms;
+    for (uint32_t k = 0; k < nterms; k++) {
+        a.coef[k] = coefs[k];
+        a.terms[k] = d_terms[k];
+        a.sterms[k] = d_solid_terms ? d_solid_terms[k] : nullptr;
+    }
+    a.out = d_out;
+    a.sout = a.ns ? d_solid_out : nullptr;
+    return launch(c, K_LINCOMB, nblk(std::max(c->n, a.ns)), 256, a);
+}
+
+int arp_velocity(arp_ctx* c, float r, const ArpParticle* d_in, const ArpSolidParticle* d_sin, ArpParticle* d_out,
+                 ArpSolidParticle* d_sout) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    VelocityArgs a{c->n, (d_sin && d_sout) ? c->ns : 0, r, d_in, d_sin, d_out, d_sout};
+    return launch(c, K_VELOCITY, nblk(std::max(c->n, a.ns)), 256, a);
+}
+
+int arp_profile(arp_ctx* c, int enable) {
+    if (!c) return ARP_E_ARG;
+    c->prof = enable != 0;
+    return ARP_OK;
+}
+
+int arp_profile_reset(arp_ctx* c) {
+    if (!c) return ARP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ev_used = 0;
+    c->ev_marks.clear();
+    return ARP_OK;
+}
+
+int arp_profile_query(arp_ctx* c, const char* kernel, double* total_ms, uint64_t* launches) {
+    if (!c || !kernel) return ARP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    uint64_t cnt = 0;
+    for (auto& m : c->ev_marks) {
+        if (strcmp(kKernNames[m.first], kernel) != 0) continue;
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+        tot += ms;
+        cnt++;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = cnt;
+    return ARP_OK;
+}
+
+int arp_grid_info(arp_ctx* c, ArpGridInfo* out) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!out) return ARP_E_ARG;
+    memset(out, 0, sizeof *out);
+    for (int i = 0; i < 3; i++) out->sub[i] = c->grid.sub[i];
+    out->sub[3] = 1;
+    out->ncells = c->ncells;
+    out->n_valid[0] = c->bP_h.empty() ? 0 : c->bP_h.back();
+    out->n_valid[1] = c->rP_h.empty() ? 0 : c->rP_h.back();
+    if (!c->have_build) return ARP_OK;
+    uint32_t nr = c->grid.nrows;
+    std::vector<uint32_t> bs(nr + 2);
+    uint32_t st[2] = {0, 0}, na = 0;
+    HIPCHK(c, hipMemcpyAsync(bs.data(), c->binStart.p, 4 * (nr + 2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&na, c->numActive.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(st, c->stats.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->n_nonfinite[2] = bs[nr + 1] - bs[nr];
+    out->n_overflow[2] = st[0];
+    out->n_valid[2] = bs[nr] - st[0];
+    out->occupied_rows = na;
+    return ARP_OK;
+}
+
+int arp_neighbor_dump(arp_ctx* c, uint32_t* cell, uint32_t* cand_count, uint32_t* cand, uint64_t cap) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!c->have_build) return fail(c, ARP_E_STATE, "no neighbour build to dump (call arp_neighbors_update)");
+    const uint32_t n = c->n, nr = c->grid.nrows, sx = c->grid.sub[0], sy = c->grid.sub[1], sz = c->grid.sub[2];
+    std::vector<uint32_t> perm(n), pid(n), bs(nr + 2), ro((size_t)nr * c->grid.rowstride);
+    std::vector<float> pos(4 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(perm.data(), c->perm.p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pid.data(), c->last_pid->p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pos.data(), c->last_posd->p, 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(bs.data(), c->binStart.p, 4 * (nr + 2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ro.data(), c->rowOff.p, 4 * ro.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // sorted position of each particle (perm is the identity in a full build)
+    std::vector<uint32_t> slot_of(n, ARP_INVALID_INDEX);
+    for (uint32_t t = 0; t < n; t++) slot_of[pid[perm[t] & ~kFlag]] = t;
+    std::vector<std::vector<uint32_t>> lists(n);
+    for (uint32_t p = 0; p < n; p++) {
+        uint32_t t = slot_of[p];
+        uint32_t pe = perm[t];
+        uint32_t i = pe & ~kFlag;
+        if (pe & kFlag) {
+            if (cell) cell[p] = ARP_INVALID_INDEX;
+            continue;
+        }
+        uint32_t cx, cy, cz;
+        host_cell(c->grid, &pos[4 * (size_t)i], cx, cy, cz);
+        if (cell) cell[p] = cx + sx * (cy + sy * cz);
+        uint32_t xl = cx > 0 ? cx - 1 : 0, xh = std::min(cx + 1, sx - 1);
+        auto& L = lists[p];
+        for (int dz = -1; dz <= 1; dz++) {
+            int nz = (int)cz + dz;
+            if (nz < 0 || nz >= (int)sz) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                int ny = (int)cy + dy;
+                if (ny < 0 || ny >= (int)sy) continue;
+                uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                uint32_t s0 = bs[row], s1 = bs[row + 1];
+                if (s1 > s0) {
+                    const uint32_t* r = &ro[(size_t)row * c->grid.rowstride];
+                    for (uint32_t q = s0 + r[xl]; q < s0 + r[xh + 1]; q++)
+                        if (!(perm[q] & kFlag)) L.push_back(pid[perm[q] & ~kFlag]);
+                }
+                size_t cb = (size_t)row * sx;
+                for (uint32_t q = c->bP_h[cb + xl]; q < c->bP_h[cb + xh + 1]; q++) L.push_back(c->b_orig[q] | kBnd);
+            }
+        }
+        std::sort(L.begin(), L.end());
+    }
+    uint64_t off = 0;
+    for (uint32_t p = 0; p < n; p++) {
+        if (cand_count) cand_count[p] = (uint32_t)lists[p].size();
+        for (uint32_t v : lists[p]) {
+            if (cand && off < cap) cand[off] = v;
+            off++;
+        }
+    }
+    if (cand && off > cap) return fail(c, ARP_E_LIMIT, "candidate buffer too small (%llu needed)", (unsigned long long)off);
+    return ARP_OK;
+}
+
+int arp_cell_counts(arp_ctx* c, uint64_t first, uint64_t count, uint32_t* out) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!out || first + count > c->ncells) return fail(c, ARP_E_ARG, "cell range out of bounds");
+    const uint32_t nr = c->grid.nrows, sx = c->grid.sub[0];
+    std::vector<uint32_t> bs(nr + 2), ro;
+    if (c->have_build) {
+        ro.resize((size_t)nr * c->grid.rowstride);
+        HIPCHK(c, hipMemcpyAsync(bs.data(), c->binStart.p, 4 * (nr + 2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ro.data(), c->rowOff.p, 4 * ro.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    for (uint64_t k = 0; k < count; k++) {
+        uint64_t cc = first + k;
+        uint32_t b = c->bP_h[cc + 1] - c->bP_h[cc];
+        uint32_t r = c->rP_h[cc + 1] - c->rP_h[cc];
+        uint32_t p = 0;
+        if (c->have_build) {
+            uint32_t row = (uint32_t)(cc / sx), x = (uint32_t)(cc % sx);
+            if (bs[row + 1] > bs[row]) {
+                uint32_t m = ro[(size_t)row * c->grid.rowstride + x + 1] - ro[(size_t)row * c->grid.rowstride + x];
+                p = std::min(m, kBucketCap - b - r);
+            }
+        }
+        out[3 * k] = b;
+        out[3 * k + 1] = r;
+        out[3 * k + 2] = p;
+    }
+    return ARP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Benchmark: SPH particle-steps/s of the 3-D fluid column (1,048,576 particles, RK2,
+dt 5e-5) on one MI355X, with the dominant kernel's HBM roofline and the CPU oracle timed on
+the host cores beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ar-physics_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# algorithmic bytes per particle per RK2 step of the whole pipeline (key pass, cell table,
+# reorder, gather, fused RK; cell-key counting sort design)
+B_ALG_STEP = {"rk2": 352.0, "rk4": 872.0}
+# algorithmic bytes per particle of one k_fluid launch: read the stage state (pos+den,
+# vel+q: 32 B), read y_n (28 B), write the next stage state (28 B), write the cell key (4 B)
+B_ALG_FLUID = 92.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (committed
+    summary under profiles/), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (kernel, workload))
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    for line in open(p):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    return vals.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(scene, integ, dt, steps, threads):
+    from arp import tableau
+    from oracle import oracle as O
+
+    if not os.path.exists(O.LIB):
+        O.build()
+    o = O.Oracle(scene, threads=threads)
+    st = o.state("f")
+    t0 = time.perf_counter()
+    o.step(st, tableau.tableau(integ), dt, steps)
+    sec = time.perf_counter() - t0
+    return dict(value=scene.n * steps / sec, unit="particle-steps/s", cores=threads, kind="port",
+                sample="%d %s step(s) of the full %d-particle column by the FP32 C oracle (OpenMP, %d threads); "
+                       "the reference GLSL cannot run here (no GL context)" % (steps, integ.upper(), scene.n, threads))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--shape", default="128,128,64", help="fluid lattice nx,ny,nz (y up)")
+    ap.add_argument("--integrator", default="rk2")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np  # noqa: F401
+
+    from arp import scene as S
+    from arp.sim import FluidSim
+
+    dt = 5e-5
+    shape = tuple(int(x) for x in args.shape.split(","))
+    log("building scene: %s fluid column" % (shape,))
+    sc = S.fluid_column_3d(shape=shape, rk=args.integrator, dt=dt)
+    log("particles %d boundary %d" % (sc.n, sc.nb))
+    sim = FluidSim(sc, integrator=args.integrator, timestep=dt)
+    log("warmup %d steps" % args.warmup)
+    sim.update(args.warmup)
+    sim.synchronize()
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    sim.profile(True)
+    sim.profile_reset()
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    sim.update(args.steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    kern = {}
+    for k in ("k_fluid", "k_rowsort", "k_scatter", "k_scan", "k_key"):
+        ms, cnt = sim.profile_query(k)
+        if cnt:
+            kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
+    sim.profile(False)
+    p = sim.particles()
+    finite = bool(np.isfinite(p["pos"]).all())
+    value = sc.n * args.steps / sec
+    f = kern.get("k_fluid")
+    bytes_launch = B_ALG_FLUID * sc.n
+    achieved = bytes_launch / (f["avg_us"] * 1e-6) if f else None
+    traffic = read_traffic("k_fluid", "fluid_column_3d")
+    out = {
+        "metric": "SPH particle-steps/sec at 1M particles (fwd)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "fluid_column_3d",
+            "particles": sc.n,
+            "boundary_particles": sc.nb,
+            "lattice": "%dx%dx%d (x,y,z) jittered +-0.05*spacing" % shape,
+            "spacing": 0.005,
+            "h": sc.h,
+            "integrator": args.integrator,
+            "dt": dt,
+            "eos": "tait rho0=1000 c0=20",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_fluid",
+            "achieved": achieved / 1e9 if achieved else None,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK if achieved else None,
+            "traffic": traffic,
+            "alg_bytes_per_launch": bytes_launch,
+        },
+        "step_roofline_frac": B_ALG_STEP.get(args.integrator, 0.0) * value / HBM_PEAK,
+        "kernels": kern,
+        "state_finite": finite,
+        "fwd_adjoint": None,
+    }
+    if not args.no_cpu:
+        log("cpu baseline: %d step(s), %d threads" % (args.cpu_steps, args.cpu_threads))
+        out["cpu_baseline"] = cpu_baseline(sc, args.integrator, dt, args.cpu_steps, args.cpu_threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

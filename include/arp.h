@@ -187,6 +187,15 @@ int arp_neighbor_dump(arp_ctx* ctx, uint32_t* cell, uint32_t* cand_count, uint32
  * [first, first+count): out has 3*count entries (Bucket.count semantics incl. the cap). */
 int arp_cell_counts(arp_ctx* ctx, uint64_t first, uint64_t count, uint32_t* out);
 
+/* ---- measurement ----------------------------------------------------------------------- */
+/* No counterpart in the reference (its Profiler times host enqueue, src/profiler.rs:10-81).
+ * enable != 0: every kernel launch of the context is bracketed by hipEvents recorded on the
+ * launch stream; arp_profile_query synchronises and returns, per kernel name, the summed
+ * event time and the number of launches since the last reset. */
+int arp_profile(arp_ctx* ctx, int enable);
+int arp_profile_query(arp_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+int arp_profile_reset(arp_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

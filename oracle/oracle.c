@@ -154,7 +154,8 @@ int orc_grid_dump(const orc_scene* sc, const float* pos, const float* ref, uint3
         uint32_t b0 = g.pos_index[i];
         uint64_t start = off;
         if (b0 != ARP_INVALID_INDEX) {
-            for (int lane = 0; lane < 81; lane += 3) { /* one lane per stencil cell */
+            for (int lane = 0; lane < 81; lane++) {
+                if ((lane % 9) / 3 != 0) continue; /* one lane per stencil cell */
                 uint64_t b;
                 if (!lane_cell_f(&g, b0, lane, &b)) continue;
                 uint32_t bc = g.cnt[3 * b], rcn = g.cnt[3 * b + 1], pc = g.cnt[3 * b + 2];
