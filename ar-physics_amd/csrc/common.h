@@ -66,12 +66,20 @@ struct KeyArgs {
     uint32_t* binCount;
 };
 
+constexpr int kScanTile = 1024;  // bins per workgroup of the row-count scan
+
 struct ScanArgs {
     uint32_t nbins;  // rows + 1 (the last bin holds non-finite positions)
+    uint32_t ntiles;
     uint32_t* binCount;
     uint32_t* binStart;  // nbins + 1
     uint32_t* activeRows;
     uint32_t* numActive;
+    // decoupled look-back state: two alternating sets selected by *epoch
+    unsigned long long* tileState;  // 2 * ntiles
+    uint32_t* ticket;               // 2
+    uint32_t* epoch;                // 1
+    uint32_t* stats;                // zeroed for the following row sort
 };
 
 struct ScatterArgs {
@@ -94,6 +102,7 @@ struct ScatterArgs {
     uint32_t* dstMk;
     uint32_t* dstPid;
     uint32_t* inv;  // pid -> storage index (full mode), may be null
+    uint32_t* sflag;
 };
 
 struct RowSortArgs {
@@ -121,6 +130,41 @@ struct RowSortArgs {
     uint32_t* dstPid;
     uint32_t* inv;
     uint32_t* stats;  // [0] overflowed particles (atomic), may be null
+    uint32_t* sflag;  // per storage index: cap/non-finite flag (may be null)
+};
+
+// RK stage epilogue: k -> next stage input / y_{n+1}, the b-weighted accumulator and the
+// stored derivatives a non-subdiagonal tableau needs; particle registers in storage order,
+// plastic-strain registers (16 floats) in solid-id order
+struct RkArgs {
+    const float* Yposd;
+    const float* Yvelq;
+    const float* AccP;  // may be null
+    const float* AccV;
+    const float* Kp[kMaxK];
+    const float* Kv[kMaxK];
+    float kc[kMaxK];
+    int nk;
+    float a_next;  // dt * a(s+1, s)
+    float b_cur;   // dt * b(s)
+    int last;
+    float* Oposd;  // X_{s+1} or y_{n+1}
+    float* Ovelq;
+    float* AccPOut;  // may be null
+    float* AccVOut;
+    float* KpOut;  // may be null
+    float* KvOut;
+    // next-stage keys
+    uint32_t* keys;
+    uint32_t* ranks;
+    uint32_t* binCount;
+    // plastic strain registers
+    const float* Yeps;
+    const float* AccE;
+    const float* Ke[kMaxK];
+    float* Oeps;
+    float* AccEOut;
+    float* KeOut;
 };
 
 struct FluidArgs {
@@ -140,33 +184,53 @@ struct FluidArgs {
     const float* Xposd;
     const float* Xvelq;
     const uint32_t* mk;
-    const uint32_t* pid;
-    int mode;  // 0: fused RK update, 1: write AoS derivative at pid, 2: write k to kOut only
-    // RK epilogue
-    const float* Yposd;
-    const float* Yvelq;
-    const float* AccP;  // may be null
-    const float* AccV;
-    const float* Kp[kMaxK];
-    const float* Kv[kMaxK];
-    float kc[kMaxK];
-    int nk;
-    float a_next;  // dt * a(s+1, s)
-    float b_cur;   // dt * b(s)
-    int last;
-    float* Oposd;  // X_{s+1} or y_{n+1}
-    float* Ovelq;
-    float* AccPOut;  // may be null
-    float* AccVOut;
-    float* KpOut;  // may be null
-    float* KvOut;
-    int skip_elastic;  // elastic particles finish in the solid kernel (write k to kOut)
-    // next-stage keys
-    uint32_t* keys;
-    uint32_t* ranks;
-    uint32_t* binCount;
-    // mode 1 output
-    void* aosOut;  // ArpParticle*
+    // 0: fused RK update; 2: write k to KdP/KdV only.  Elastic particles (when the scene
+    // runs the solid path) always write their fluid part to KdP/KdV.
+    int mode;
+    int skip_elastic;
+    float* KdP;
+    float* KdV;
+    RkArgs rk;
+};
+
+// Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists
+struct SolidArgs {
+    GridP grid;
+    PhysP phys;
+    uint32_t ns;
+    const DMat* mats;
+    const uint32_t* part;     // particle id per solid
+    const uint32_t* inv;      // particle id -> storage index
+    const float* ref;         // float4 reference position per solid
+    const float* Linv;        // corrected-gradient L^-1 per solid, 16 (column-major)
+    const uint32_t* sOff;     // strain list: ns+1 offsets
+    const uint32_t* sPid;     //   neighbour particle id
+    const float* sG;          //   float4 (L^-1 gradW(dX) xyz, V2)
+    const uint32_t* eOff;     // elastic-force list (same material): ns+1 offsets
+    const uint32_t* ePid;     //   neighbour particle id
+    const uint32_t* eSid;     //   neighbour solid id
+    const float* eG;          //   float4 (gradW(dX) xyz, W(dX))
+    const float* Xposd;       // stage input, storage order
+    const float* Xvelq;
+    const uint32_t* mk;
+    const float* Xeps;        // stage input plastic strain per solid (16)
+    float* A;                 // per solid: (P L^-1) rows, 3 float4
+    float* Fm;                // per solid: F rows, 3 float4
+    float* keps;              // per solid: plastic strain rate (16)
+    // current grid, for the hourglass stencil test and self contact
+    const uint32_t* perm;
+    const uint32_t* binStart;
+    const uint32_t* rowOff;
+    const uint32_t* sflag;    // per storage index: 1 when refused by the cap / non-finite
+    const uint32_t* pid;      // storage -> particle id
+    const uint32_t* sidByPid;
+    // fluid part of elastic particles (written by k_fluid)
+    const float* KFp;
+    const float* KFv;
+    int mode;                 // 0: RK update; 2: write k to KdP/KdV and keps only
+    float* KdP;
+    float* KdV;
+    RkArgs rk;
 };
 
 struct AosInArgs {
@@ -203,6 +267,20 @@ struct LincombArgs {
 struct FillArgs {
     uint32_t* p;
     uint32_t n, v;
+};
+
+struct EpsInArgs {  // AoS solids -> plastic strain registers
+    uint32_t ns;
+    const void* aos;
+    float* eps;
+};
+
+struct EpsOutArgs {  // plastic strain (or its rate) -> AoS solids {part_id, ref_pos, stress}
+    uint32_t ns;
+    const float* eps;
+    const float* ref;  // null: write ref_pos = 0 (derivative)
+    const uint32_t* part;
+    void* aos;
 };
 
 struct ClearSolidsArgs {

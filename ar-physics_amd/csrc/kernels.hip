@@ -148,41 +148,100 @@ extern "C" __global__ void __launch_bounds__(256) k_key(KeyArgs a) {
     }
 }
 
-// Exclusive scan of the row counts (one 1024-thread block); lists the occupied rows and
-// clears the counters for the next build.
-extern "C" __global__ void __launch_bounds__(1024) k_scan(ScanArgs a) {
-    __shared__ uint32_t sc[1024];
-    __shared__ uint32_t sf[1024];
-    const uint32_t T = 1024, t = threadIdx.x, nb = a.nbins;
-    const uint32_t per = (nb + T - 1) / T;
-    const uint32_t b0 = min(t * per, nb), b1 = min(b0 + per, nb);
-    uint32_t s = 0, f = 0;
-    for (uint32_t b = b0; b < b1; b++) {
-        uint32_t c = a.binCount[b];
-        s += c;
-        f += (c > 0 && b + 1 < nb) ? 1u : 0u;
+// Exclusive scan of the row counts and of the occupied-row flags, one 1024-bin tile per
+// workgroup with decoupled look-back; lists the occupied rows in ascending order and clears
+// the counters for the next build.  A workgroup only waits on tiles whose ticket was taken
+// before its own, and every tile publishes its aggregate before waiting, so each wait ends.
+__device__ __forceinline__ unsigned long long ld_state(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_state(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+constexpr unsigned long long kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStMask = 3ull << 62;
+
+extern "C" __global__ void __launch_bounds__(256) k_scan(ScanArgs a) {
+    __shared__ uint32_t sTile, sEpoch;
+    __shared__ uint32_t wsumC[4], wsumF[4];
+    __shared__ unsigned long long sPrefix;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        const uint32_t ep = __hip_atomic_load(a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sEpoch = ep;
+        sTile = atomicAdd(&a.ticket[ep & 1u], 1u);
     }
-    sc[t] = s;
-    sf[t] = f;
     __syncthreads();
-    for (uint32_t o = 1; o < T; o <<= 1) {
-        uint32_t vs = t >= o ? sc[t - o] : 0, vf = t >= o ? sf[t - o] : 0;
-        __syncthreads();
-        sc[t] += vs;
-        sf[t] += vf;
-        __syncthreads();
+    const uint32_t tile = sTile, par = sEpoch & 1u;
+    unsigned long long* st = a.tileState + (size_t)par * a.ntiles;
+    unsigned long long* stNext = a.tileState + (size_t)(par ^ 1u) * a.ntiles;
+    const uint32_t nb = a.nbins;
+    const uint32_t b0 = tile * kScanTile + tid * 4;
+    uint32_t c[4], f[4], sc = 0, sf = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t b = b0 + k;
+        c[k] = b < nb ? a.binCount[b] : 0u;
+        f[k] = (c[k] > 0 && b + 1 < nb) ? 1u : 0u;
+        sc += c[k];
+        sf += f[k];
     }
-    uint32_t es = sc[t] - s, ef = sf[t] - f;
-    for (uint32_t b = b0; b < b1; b++) {
-        uint32_t c = a.binCount[b];
-        a.binStart[b] = es;
-        if (c > 0 && b + 1 < nb) a.activeRows[ef++] = b;
-        es += c;
-        a.binCount[b] = 0;
+    // block scan of (sc, sf)
+    const uint32_t lane = __lane_id(), wid = tid >> 6;
+    const uint32_t ic = wave_incl_scan(sc), iff = wave_incl_scan(sf);
+    if (lane == 63) {
+        wsumC[wid] = ic;
+        wsumF[wid] = iff;
     }
-    if (t == T - 1) {
-        a.binStart[nb] = sc[T - 1];
-        *a.numActive = sf[T - 1];
+    __syncthreads();
+    uint32_t preC = 0, preF = 0;
+    for (uint32_t w = 0; w < wid; w++) {
+        preC += wsumC[w];
+        preF += wsumF[w];
+    }
+    const uint32_t aggC = wsumC[0] + wsumC[1] + wsumC[2] + wsumC[3];
+    const uint32_t aggF = wsumF[0] + wsumF[1] + wsumF[2] + wsumF[3];
+    if (tid == 0) {
+        const unsigned long long agg = ((unsigned long long)aggF << 32) | aggC;
+        unsigned long long prefix = 0;
+        if (tile == 0) {
+            st_state(&st[0], kStIncl | agg);
+        } else {
+            st_state(&st[tile], kStAgg | agg);
+            for (int64_t j = (int64_t)tile - 1; j >= 0;) {
+                const unsigned long long v = ld_state(&st[j]);
+                const unsigned long long s2 = v & kStMask;
+                if (s2 == 0) continue;  // predecessor has not published yet
+                prefix += v & ~kStMask;
+                if (s2 == kStIncl) break;
+                j--;
+            }
+            st_state(&st[tile], kStIncl | (prefix + agg));
+        }
+        sPrefix = prefix;
+        stNext[tile] = 0;
+        if (tile == 0) {
+            a.ticket[par ^ 1u] = 0;
+            if (a.stats) a.stats[0] = 0;
+        }
+        if (tile == a.ntiles - 1) {
+            const unsigned long long tot = prefix + agg;
+            a.binStart[nb] = (uint32_t)(tot & 0xFFFFFFFFull);
+            *a.numActive = (uint32_t)(tot >> 32);
+            __hip_atomic_store(a.epoch, sEpoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    uint32_t es = (uint32_t)(sPrefix & 0xFFFFFFFFull) + preC + ic - sc;
+    uint32_t ef = (uint32_t)(sPrefix >> 32) + preF + iff - sf;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t b = b0 + k;
+        if (b < nb) {
+            a.binStart[b] = es;
+            if (f[k]) a.activeRows[ef++] = b;
+            es += c[k];
+            a.binCount[b] = 0;
+        }
     }
 }
 
@@ -214,6 +273,7 @@ extern "C" __global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) {
         } else {
             a.perm[dst] = i | kFlag;
         }
+        if (a.sflag) a.sflag[a.full ? dst : i] = 1u;
     }
 }
 
@@ -310,6 +370,7 @@ extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
             } else {
                 a.perm[dst] = src | flag;
             }
+            if (a.sflag) a.sflag[a.full ? dst : src] = flag ? 1u : 0u;
             if (flag && a.stats) atomicAdd(a.stats, 1u);
         }
         __syncthreads();
@@ -325,10 +386,101 @@ struct PairAcc {
     float den, fx, fy, fz;
 };
 
+// Stage epilogue of one particle: y_n + dt*sum a k -> next stage input (or y_{n+1}), the
+// b-weighted accumulator, stored derivatives, the state's p/den^2 and the next cell key.
+__device__ __forceinline__ void rk_update_particle(const RkArgs& r, const GridP& g, const DMat& M,
+                                                   uint32_t i, float4 kP, float4 kV, bool& haveKey,
+                                                   uint32_t& nextKey, uint32_t& nextBin) {
+    const float4 yp = reinterpret_cast<const float4*>(r.Yposd)[i];
+    const float4 yv = reinterpret_cast<const float4*>(r.Yvelq)[i];
+    float4 ap = make_float4(0.f, 0.f, 0.f, 0.f), av = ap;
+    if (r.AccP) {
+        ap = reinterpret_cast<const float4*>(r.AccP)[i];
+        av = reinterpret_cast<const float4*>(r.AccV)[i];
+    }
+    const float b = r.b_cur;
+    ap.x += b * kP.x; ap.y += b * kP.y; ap.z += b * kP.z; ap.w += b * kP.w;
+    av.x += b * kV.x; av.y += b * kV.y; av.z += b * kV.z;
+    float4 op, ov;
+    if (r.last) {
+        op = make_float4(yp.x + ap.x, yp.y + ap.y, yp.z + ap.z, yp.w + ap.w);
+        ov = make_float4(yv.x + av.x, yv.y + av.y, yv.z + av.z, 0.f);
+    } else {
+        const float c = r.a_next;
+        op = make_float4(yp.x + c * kP.x, yp.y + c * kP.y, yp.z + c * kP.z, yp.w + c * kP.w);
+        ov = make_float4(yv.x + c * kV.x, yv.y + c * kV.y, yv.z + c * kV.z, 0.f);
+        for (int kk = 0; kk < r.nk; kk++) {
+            const float4 p2 = reinterpret_cast<const float4*>(r.Kp[kk])[i];
+            const float4 v2 = reinterpret_cast<const float4*>(r.Kv[kk])[i];
+            const float ck = r.kc[kk];
+            op.x += ck * p2.x; op.y += ck * p2.y; op.z += ck * p2.z; op.w += ck * p2.w;
+            ov.x += ck * v2.x; ov.y += ck * v2.y; ov.z += ck * v2.z;
+        }
+        if (r.AccPOut) {
+            reinterpret_cast<float4*>(r.AccPOut)[i] = ap;
+            reinterpret_cast<float4*>(r.AccVOut)[i] = av;
+        }
+    }
+    if (r.KpOut) {
+        reinterpret_cast<float4*>(r.KpOut)[i] = kP;
+        reinterpret_cast<float4*>(r.KvOut)[i] = make_float4(kV.x, kV.y, kV.z, 0.0f);
+    }
+    ov.w = pressure_q(M, op.w);
+    reinterpret_cast<float4*>(r.Oposd)[i] = op;
+    reinterpret_cast<float4*>(r.Ovelq)[i] = ov;
+    if (r.keys) {
+        uint32_t cx, cy, cz;
+        haveKey = true;
+        if (cell_of(g, op.x, op.y, op.z, cx, cy, cz)) {
+            const uint32_t row = cy + g.sub[1] * cz;
+            nextKey = cx + g.sub[0] * row;
+            nextBin = row;
+        } else {
+            nextKey = kInvalid;
+            nextBin = g.nrows;
+        }
+    }
+}
+
+// The same epilogue on a solid's plastic strain (16 floats, solid-id order).
+__device__ __forceinline__ void rk_update_eps(const RkArgs& r, uint32_t s, const float* ke) {
+    const float* y = r.Yeps + 16 * (size_t)s;
+    float out[16], acc[16];
+    for (int q = 0; q < 16; q++) acc[q] = r.AccE ? r.AccE[16 * (size_t)s + q] : 0.0f;
+    for (int q = 0; q < 16; q++) acc[q] += r.b_cur * ke[q];
+    if (r.last) {
+        for (int q = 0; q < 16; q++) out[q] = y[q] + acc[q];
+    } else {
+        for (int q = 0; q < 16; q++) out[q] = y[q] + r.a_next * ke[q];
+        for (int kk = 0; kk < r.nk; kk++)
+            for (int q = 0; q < 16; q++) out[q] += r.kc[kk] * r.Ke[kk][16 * (size_t)s + q];
+        if (r.AccEOut)
+            for (int q = 0; q < 16; q++) r.AccEOut[16 * (size_t)s + q] = acc[q];
+    }
+    if (r.KeOut)
+        for (int q = 0; q < 16; q++) r.KeOut[16 * (size_t)s + q] = ke[q];
+    for (int q = 0; q < 16; q++) r.Oeps[16 * (size_t)s + q] = out[q];
+}
+
+// Logical block id that gives each XCD a contiguous share of the grid: workgroups are
+// dispatched round-robin over the 8 XCDs, each with its own L2, and neighbouring particles
+// then share an L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+struct PairIn {
+    float4 xj, vj;
+    uint32_t mk2;
+    bool bnd;
+};
+
 extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
     __shared__ uint32_t list[kList * 256];
     const uint32_t tid = threadIdx.x;
-    const uint32_t t = blockIdx.x * 256u + tid;
+    const uint32_t t = xcd_block() * 256u + tid;
     const bool act = t < a.n;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
     const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
@@ -336,16 +488,15 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
     const float4* BV = reinterpret_cast<const float4*>(a.bvelq);
     const float h = a.phys.h, hh = h * h, kn = a.phys.k;
 
-    uint32_t i = 0, mk1 = 0, nextBin = 0;
-    bool haveKey = false, doKey = false;
-    uint32_t nextKey = kInvalid;
+    uint32_t i = 0, nextBin = 0, nextKey = kInvalid;
+    bool haveKey = false;
     if (act) {
         const uint32_t pe = a.perm[t];
         i = pe & ~kFlag;
         const bool noNb = (pe & kFlag) != 0;
         const float4 xi = X[i];
         const float4 vi = XV[i];
-        mk1 = a.mk[i];
+        const uint32_t mk1 = a.mk[i];
         const uint32_t mat1 = mk1 & kMatMask;
         const bool elastic1 = (mk1 & kMkElastic) != 0;
         const DMat& M1 = a.mats[mat1];
@@ -353,22 +504,27 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
         const float f1 = M1.visc, bf1 = M1.bulk_visc;
         PairAcc acc = {0.f, 0.f, 0.f, 0.f};
 
-        auto pair = [&](uint32_t e) {
-            const bool bnd = (e & kBnd) != 0;
+        auto load = [&](uint32_t e) {
+            PairIn in;
+            in.bnd = (e & kBnd) != 0;
             const uint32_t j = e & ~kBnd;
-            float4 xj, vj;
-            uint32_t mk2;
-            if (bnd) {
-                xj = B[j];
-                vj = BV[j];
-                mk2 = a.bmk[j];
+            if (in.bnd) {
+                in.xj = B[j];
+                in.vj = BV[j];
+                in.mk2 = a.bmk[j];
             } else {
-                xj = X[j];
-                vj = XV[j];
-                mk2 = a.mk[j];
+                in.xj = X[j];
+                in.vj = XV[j];
+                in.mk2 = a.mk[j];
             }
-            const uint32_t mat2 = mk2 & kMatMask;
-            const bool elastic2 = (mk2 & kMkElastic) != 0;
+            return in;
+        };
+        // one neighbour's terms (fluid.rs:153-214)
+        auto pair = [&](const PairIn& in) {
+            const bool bnd = in.bnd;
+            const float4 xj = in.xj, vj = in.vj;
+            const uint32_t mat2 = in.mk2 & kMatMask;
+            const bool elastic2 = (in.mk2 & kMkElastic) != 0;
             const DMat& M2 = a.mats[mat2];
             const float m2 = M2.mass, d2 = xj.w, q2 = vj.w;
             const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
@@ -379,31 +535,32 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             const float gx = gs * rx, gy = gs * ry, gz = gs * rz;
             if (!bnd) acc.den += m2 * (vx * gx + vy * gy + vz * gz);
             const DInter it = a.inter[mat1 * a.phys.nm + mat2];
-            const float cp = contact_potential(it.potential, sqrtf(rr), it.radius, it.strength);
+            const float cp = it.potential ? contact_potential(it.potential, sqrtf(rr), it.radius, it.strength) : 0.0f;
             const float pf = m1 * m2 * (q1 + q2);
             acc.fx += pf * gx;
             acc.fy += pf * gy;
             acc.fz += pf * gz;
             if (cp != 0.0f) {
-                const float cs = m2 * cp / d2;
+                const float cs = __fdividef(m2 * cp, d2);
                 const float cx = cs * gx, cy = cs * gy, cz = cs * gz;
                 if ((elastic1 || elastic2 || bnd) && mat1 != mat2) {
                     // friction along the tangential relative velocity (fluid.rs:191-199)
-                    const float ix = rx / rr, iy = ry / rr, iz = rz / rr;
+                    const float irr = __fdividef(1.0f, rr);
+                    const float ix = rx * irr, iy = ry * irr, iz = rz * irr;
                     const float cfr = cx * rx + cy * ry + cz * rz;
                     const float vr = vx * rx + vy * ry + vz * rz;
                     const float nx = ix * cfr, ny = iy * cfr, nz = iz * cfr;
                     float tx = vx - ix * vr, ty = vy - iy * vr, tz = vz - iz * vr;
-                    const float tl = sqrtf(tx * tx + ty * ty + tz * tz);
-                    tx = tx / tl;
-                    ty = ty / tl;
-                    tz = tz / tl;
+                    const float itl = __fdividef(1.0f, sqrtf(tx * tx + ty * ty + tz * tz));
+                    tx *= itl;
+                    ty *= itl;
+                    tz *= itl;
                     if (isfinite(tx) && isfinite(ty) && isfinite(tz)) {
                         const float fl = it.friction * sqrtf(nx * nx + ny * ny + nz * nz);
-                        const float tl2 = sqrtf(tx * tx + ty * ty + tz * tz);
-                        acc.fx += fl * (tx / tl2);
-                        acc.fy += fl * (ty / tl2);
-                        acc.fz += fl * (tz / tl2);
+                        const float itl2 = __fdividef(fl, sqrtf(tx * tx + ty * ty + tz * tz));
+                        acc.fx += itl2 * tx;
+                        acc.fy += itl2 * ty;
+                        acc.fz += itl2 * tz;
                     }
                 }
                 acc.fx += cx;
@@ -413,14 +570,13 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             if (!bnd) {
                 const float den_ = rr + a.phys.eps_hh;
                 if (!elastic1) {
-                    const float s = m1 * m2 * (f1 + M2.visc) * (rx * gx + ry * gy + rz * gz);
-                    const float dd = d1 * d2 * den_;
-                    acc.fx -= s * vx / dd;
-                    acc.fy -= s * vy / dd;
-                    acc.fz -= s * vz / dd;
+                    const float s = __fdividef(m1 * m2 * (f1 + M2.visc) * (rx * gx + ry * gy + rz * gz), d1 * d2 * den_);
+                    acc.fx -= s * vx;
+                    acc.fy -= s * vy;
+                    acc.fz -= s * vz;
                 }
                 const float ff = mat1 == mat2 ? bf1 + M2.bulk_visc : it.dampening;
-                const float s = m1 * m2 * (ff * (vx * rx + vy * ry + vz * rz) / ((d1 + d2) * den_));
+                const float s = m1 * m2 * __fdividef(ff * (vx * rx + vy * ry + vz * rz), (d1 + d2) * den_);
                 acc.fx -= s * gx;
                 acc.fy -= s * gy;
                 acc.fz -= s * gz;
@@ -432,110 +588,92 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
             const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
+            // 18 candidate ranges: (mobile, boundary) x 9 neighbour rows.  Candidates within h
+            // are buffered in LDS and drained at a single site of the pair loop whenever the
+            // buffer could overflow.
             uint32_t nl = 0;
-            auto flush = [&]() {
-                for (uint32_t k = 0; k < nl; k++) pair(list[k * 256 + tid]);
-                nl = 0;
-            };
-            auto push = [&](uint32_t e) {
-                list[nl * 256 + tid] = e;
-                if (++nl == (uint32_t)kList) flush();
-            };
-            for (int dz = -1; dz <= 1; dz++) {
-                const int nz = (int)cz + dz;
-                if (nz < 0 || nz >= (int)sz) continue;
-                for (int dy = -1; dy <= 1; dy++) {
-                    const int ny = (int)cy + dy;
-                    if (ny < 0 || ny >= (int)sy) continue;
-                    const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
-                    const uint32_t s0 = a.binStart[row], s1 = a.binStart[row + 1];
-                    if (s1 > s0) {
-                        const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
-                        const uint32_t qa = s0 + ro[xl], qb = s0 + ro[xh + 1];
-                        for (uint32_t q = qa; q < qb; q++) {
-                            const uint32_t pe2 = a.perm[q];
-                            if (pe2 & kFlag) continue;
-                            const float4 xj = X[pe2];
-                            const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                            if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) push(pe2);
+            int range = -1;
+            uint32_t q = 0, qend = 0;
+            for (;;) {
+                while (nl + 4 <= (uint32_t)kList) {
+                    if (q >= qend) {
+                        if (++range >= 18) break;
+                        const int rw = range >> 1;
+                        const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
+                        q = qend = 0;
+                        if (nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
+                            const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                            if (range & 1) {
+                                const size_t cb = (size_t)row * sx;
+                                q = a.bP[cb + xl];
+                                qend = a.bP[cb + xh + 1];
+                            } else {
+                                const uint32_t s0 = a.binStart[row], s1 = a.binStart[row + 1];
+                                if (s1 > s0) {
+                                    const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                                    q = s0 + ro[xl];
+                                    qend = s0 + ro[xh + 1];
+                                }
+                            }
+                        }
+                        continue;
+                    }
+                    // four candidates at a time: independent loads in flight together
+                    if (range & 1) {
+                        float4 xj[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) xj[u] = B[min(q + u, qend - 1)];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                            if (q + u < qend && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
+                                list[(nl++) * 256 + tid] = (q + u) | kBnd;
+                        }
+                    } else {
+                        uint32_t pe2[4];
+                        float4 xj[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) pe2[u] = (q + u < qend) ? a.perm[q + u] : kFlag;
+#pragma unroll
+                        for (int u = 0; u < 4; u++) xj[u] = X[pe2[u] & ~kFlag];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                            if (!(pe2[u] & kFlag) && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
+                                list[(nl++) * 256 + tid] = pe2[u];
                         }
                     }
-                    const size_t cb = (size_t)row * sx;
-                    const uint32_t ba = a.bP[cb + xl], bb = a.bP[cb + xh + 1];
-                    for (uint32_t q = ba; q < bb; q++) {
-                        const float4 xj = B[q];
-                        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                        if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) push(q | kBnd);
-                    }
+                    q += 4;
                 }
+                uint32_t k = 0;
+                for (; k + 1 < nl; k += 2) {
+                    const PairIn p0 = load(list[k * 256 + tid]);
+                    const PairIn p1 = load(list[(k + 1) * 256 + tid]);
+                    pair(p0);
+                    pair(p1);
+                }
+                if (k < nl) pair(load(list[k * 256 + tid]));
+                nl = 0;
+                if (range >= 18) break;
             }
-            flush();
         }
 
         // derivative (fluid.rs:243-247): den' = sum, vel' = F/den + (0,-g,0), pos' = vel
-        const float kPx = vi.x, kPy = vi.y, kPz = vi.z, kPw = acc.den;
-        const float kVx = acc.fx / d1, kVy = acc.fy / d1 - a.phys.g, kVz = acc.fz / d1;
-
+        const float id1 = __fdividef(1.0f, d1);
+        const float4 kP = make_float4(vi.x, vi.y, vi.z, acc.den);
+        const float4 kV = make_float4(acc.fx * id1, acc.fy * id1 - a.phys.g, acc.fz * id1, 0.0f);
         if (a.mode == 2 || (a.skip_elastic && elastic1)) {
-            reinterpret_cast<float4*>(a.KpOut)[i] = make_float4(kPx, kPy, kPz, kPw);
-            reinterpret_cast<float4*>(a.KvOut)[i] = make_float4(kVx, kVy, kVz, 0.0f);
+            reinterpret_cast<float4*>(a.KdP)[i] = kP;
+            reinterpret_cast<float4*>(a.KdV)[i] = kV;
         } else {
-            const float4 yp = reinterpret_cast<const float4*>(a.Yposd)[i];
-            const float4 yv = reinterpret_cast<const float4*>(a.Yvelq)[i];
-            float4 ap = make_float4(0.f, 0.f, 0.f, 0.f), av = ap;
-            if (a.AccP) {
-                ap = reinterpret_cast<const float4*>(a.AccP)[i];
-                av = reinterpret_cast<const float4*>(a.AccV)[i];
-            }
-            const float b = a.b_cur;
-            ap.x += b * kPx; ap.y += b * kPy; ap.z += b * kPz; ap.w += b * kPw;
-            av.x += b * kVx; av.y += b * kVy; av.z += b * kVz;
-            float4 op, ov;
-            if (a.last) {
-                op = make_float4(yp.x + ap.x, yp.y + ap.y, yp.z + ap.z, yp.w + ap.w);
-                ov = make_float4(yv.x + av.x, yv.y + av.y, yv.z + av.z, 0.f);
-            } else {
-                const float c = a.a_next;
-                op = make_float4(yp.x + c * kPx, yp.y + c * kPy, yp.z + c * kPz, yp.w + c * kPw);
-                ov = make_float4(yv.x + c * kVx, yv.y + c * kVy, yv.z + c * kVz, 0.f);
-                for (int kk = 0; kk < a.nk; kk++) {
-                    const float4 p2 = reinterpret_cast<const float4*>(a.Kp[kk])[i];
-                    const float4 v2 = reinterpret_cast<const float4*>(a.Kv[kk])[i];
-                    const float ck = a.kc[kk];
-                    op.x += ck * p2.x; op.y += ck * p2.y; op.z += ck * p2.z; op.w += ck * p2.w;
-                    ov.x += ck * v2.x; ov.y += ck * v2.y; ov.z += ck * v2.z;
-                }
-                if (a.AccPOut) {
-                    reinterpret_cast<float4*>(a.AccPOut)[i] = ap;
-                    reinterpret_cast<float4*>(a.AccVOut)[i] = av;
-                }
-            }
-            if (a.KpOut) {
-                reinterpret_cast<float4*>(a.KpOut)[i] = make_float4(kPx, kPy, kPz, kPw);
-                reinterpret_cast<float4*>(a.KvOut)[i] = make_float4(kVx, kVy, kVz, 0.0f);
-            }
-            ov.w = pressure_q(M1, op.w);
-            reinterpret_cast<float4*>(a.Oposd)[i] = op;
-            reinterpret_cast<float4*>(a.Ovelq)[i] = ov;
-            doKey = a.keys != nullptr;
-            if (doKey) {
-                uint32_t cx, cy, cz;
-                haveKey = true;
-                if (cell_of(a.grid, op.x, op.y, op.z, cx, cy, cz)) {
-                    const uint32_t row = cy + a.grid.sub[1] * cz;
-                    nextKey = cx + a.grid.sub[0] * row;
-                    nextBin = row;
-                } else {
-                    nextBin = a.grid.nrows;
-                }
-            }
+            rk_update_particle(a.rk, a.grid, M1, i, kP, kV, haveKey, nextKey, nextBin);
         }
     }
-    if (a.keys) {
-        const uint32_t rank = agg_rank(a.binCount, nextBin, haveKey);
+    if (a.rk.keys && a.mode == 0) {
+        const uint32_t rank = agg_rank(a.rk.binCount, nextBin, haveKey);
         if (haveKey) {
-            a.keys[i] = nextKey;
-            a.ranks[i] = rank;
+            a.rk.keys[i] = nextKey;
+            a.rk.ranks[i] = rank;
         }
     }
 }
@@ -662,4 +800,418 @@ extern "C" __global__ void __launch_bounds__(256) k_clear_solids(ClearSolidsArgs
     o.ref_pos = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int q = 0; q < 4; q++) o.stress[q] = make_float4(0.f, 0.f, 0.f, 0.f);
     reinterpret_cast<AosSolid*>(a.out)[i] = o;
+}
+
+// ------------------------------------------------------------------------------------
+// solids: corrected-gradient strain (strain.rs:71-181) and solid forces
+// (solid.rs:276-524).  Matrices are GLSL mat4, column-major m[4*col + row].
+// ------------------------------------------------------------------------------------
+struct M4 {
+    float m[16];
+};
+
+__device__ __forceinline__ M4 m4_zero() {
+    M4 r;
+#pragma unroll
+    for (int q = 0; q < 16; q++) r.m[q] = 0.0f;
+    return r;
+}
+__device__ __forceinline__ M4 m4_ident() {
+    M4 r = m4_zero();
+    r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f;
+    return r;
+}
+__device__ __forceinline__ M4 m4_mul(const M4& A, const M4& B) {
+    M4 t;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            t.m[4 * c + r] = A.m[r] * B.m[4 * c] + A.m[4 + r] * B.m[4 * c + 1] + A.m[8 + r] * B.m[4 * c + 2] +
+                             A.m[12 + r] * B.m[4 * c + 3];
+    return t;
+}
+__device__ __forceinline__ M4 m4_trans(const M4& A) {
+    M4 t;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) t.m[4 * c + r] = A.m[4 * r + c];
+    return t;
+}
+__device__ __forceinline__ M4 m4_add(const M4& A, const M4& B) {
+    M4 t;
+#pragma unroll
+    for (int q = 0; q < 16; q++) t.m[q] = A.m[q] + B.m[q];
+    return t;
+}
+__device__ __forceinline__ M4 m4_scale(float s, const M4& A) {
+    M4 t;
+#pragma unroll
+    for (int q = 0; q < 16; q++) t.m[q] = s * A.m[q];
+    return t;
+}
+__device__ M4 m4_inv(const M4& M) {
+    const float* m = M.m;
+    float a00 = m[0], a01 = m[1], a02 = m[2], a03 = m[3];
+    float a10 = m[4], a11 = m[5], a12 = m[6], a13 = m[7];
+    float a20 = m[8], a21 = m[9], a22 = m[10], a23 = m[11];
+    float a30 = m[12], a31 = m[13], a32 = m[14], a33 = m[15];
+    float b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10;
+    float b02 = a00 * a13 - a03 * a10, b03 = a01 * a12 - a02 * a11;
+    float b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+    float b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30;
+    float b08 = a20 * a33 - a23 * a30, b09 = a21 * a32 - a22 * a31;
+    float b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+    float det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+    float id = 1.0f / det;
+    M4 t;
+    t.m[0] = (a11 * b11 - a12 * b10 + a13 * b09) * id;
+    t.m[1] = (a02 * b10 - a01 * b11 - a03 * b09) * id;
+    t.m[2] = (a31 * b05 - a32 * b04 + a33 * b03) * id;
+    t.m[3] = (a22 * b04 - a21 * b05 - a23 * b03) * id;
+    t.m[4] = (a12 * b08 - a10 * b11 - a13 * b07) * id;
+    t.m[5] = (a00 * b11 - a02 * b08 + a03 * b07) * id;
+    t.m[6] = (a32 * b02 - a30 * b05 - a33 * b01) * id;
+    t.m[7] = (a20 * b05 - a22 * b02 + a23 * b01) * id;
+    t.m[8] = (a10 * b10 - a11 * b08 + a13 * b06) * id;
+    t.m[9] = (a01 * b08 - a00 * b10 - a03 * b06) * id;
+    t.m[10] = (a30 * b04 - a31 * b02 + a33 * b00) * id;
+    t.m[11] = (a21 * b02 - a20 * b04 - a23 * b00) * id;
+    t.m[12] = (a11 * b07 - a10 * b09 - a12 * b06) * id;
+    t.m[13] = (a00 * b09 - a01 * b07 + a02 * b06) * id;
+    t.m[14] = (a31 * b01 - a30 * b03 - a32 * b00) * id;
+    t.m[15] = (a20 * b03 - a21 * b01 + a22 * b00) * id;
+    return t;
+}
+__device__ __forceinline__ float m4_trace(const M4& A, uint32_t dim) {
+    float t = A.m[0];
+    for (uint32_t i = 1; i < dim; i++) t += A.m[5 * i];
+    return t;
+}
+__device__ __forceinline__ float m4_dot(const M4& x, const M4& y) {
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        s += x.m[4 * c] * y.m[4 * c] + x.m[4 * c + 1] * y.m[4 * c + 1] + x.m[4 * c + 2] * y.m[4 * c + 2] +
+             x.m[4 * c + 3] * y.m[4 * c + 3];
+    return s;
+}
+// hooke: normal*trace(strain)*I + 2*shear*strain (solid.rs:139-141)
+__device__ __forceinline__ M4 m4_hooke(const M4& S, float lambda, float mu, uint32_t dim) {
+    const float tr = lambda * m4_trace(S, dim), s2 = 2.0f * mu;
+    M4 o;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) o.m[4 * c + r] = (c == r ? tr : 0.0f) + s2 * S.m[4 * c + r];
+    return o;
+}
+__device__ M4 m4_cauchy(const M4& F, uint32_t dim) {
+    M4 C = m4_mul(m4_trans(F), F);
+    for (uint32_t i = dim; i < 4; i++) C.m[5 * i] = 1.0f;
+    return C;
+}
+// Seth-Hill strain of order two_m/2 (solid.rs:105-168): log series, Denman-Beavers root
+__device__ M4 m4_seth_hill(const M4& F, int two_m, uint32_t dim) {
+    const M4 C = m4_cauchy(F, dim);
+    const M4 I = m4_ident();
+    if (two_m == 0) {
+        M4 sum = m4_zero(), term = m4_scale(-1.0f, I), AmI;
+        for (int q = 0; q < 16; q++) AmI.m[q] = -1.0f * (C.m[q] - I.m[q]);
+        for (int i = 1; i <= 5; i++) {
+            term = m4_mul(term, AmI);
+            for (int q = 0; q < 16; q++) sum.m[q] += term.m[q] / (float)i;
+        }
+        return m4_scale(0.5f, sum);
+    }
+    const uint32_t m = (uint32_t)abs(two_m) >> 1;
+    M4 Cm = I;
+    for (uint32_t i = 0; i < m; i++) Cm = m4_mul(Cm, C);
+    if ((abs(two_m) & 1) == 1) {
+        M4 Y = C, Z = I;
+        for (int i = 0; i < 15; i++) {
+            const M4 iZ = m4_inv(Z), iY = m4_inv(Y);
+            M4 Yn, Zn;
+            for (int q = 0; q < 16; q++) {
+                Yn.m[q] = 0.5f * (Y.m[q] + iZ.m[q]);
+                Zn.m[q] = 0.5f * (Z.m[q] + iY.m[q]);
+            }
+            Y = Yn;
+            Z = Zn;
+        }
+        Cm = m4_mul(Cm, Y);
+    }
+    if (two_m < 0) Cm = m4_inv(Cm);
+    M4 d;
+    for (int q = 0; q < 16; q++) d.m[q] = Cm.m[q] - I.m[q];
+    return m4_scale(1.0f / (float)two_m, d);
+}
+__device__ M4 m4_seth_hill_rate(const M4& F, const M4& Fd, int two_m, uint32_t dim) {
+    M4 dC = m4_mul(m4_trans(Fd), F);
+    dC = m4_add(dC, m4_trans(dC));
+    const M4 C = m4_cauchy(F, dim);
+    if (two_m == 0) return m4_mul(m4_scale(0.5f, dC), m4_inv(C));
+    const uint32_t m = (uint32_t)abs(two_m) >> 1;
+    M4 sr = m4_zero();
+    for (uint32_t i = 0; i < m; i++) {
+        M4 term = m4_ident();
+        for (uint32_t j = 0; j < m; j++) term = m4_mul(term, j == i ? dC : C);
+        sr = m4_add(sr, term);
+    }
+    sr = m4_scale(1.0f / (float)two_m, sr);
+    if (two_m < 0) {
+        const M4 iC = m4_inv(C);
+        return m4_mul(m4_mul(m4_scale(-1.0f, iC), sr), iC);
+    }
+    return sr;
+}
+
+__device__ __forceinline__ void store_rows3(float* dst, const M4& A) {
+    float4* d = reinterpret_cast<float4*>(dst);
+    d[0] = make_float4(A.m[0], A.m[4], A.m[8], 0.f);
+    d[1] = make_float4(A.m[1], A.m[5], A.m[9], 0.f);
+    d[2] = make_float4(A.m[2], A.m[6], A.m[10], 0.f);
+}
+
+// One thread per solid: F and F' from the static reference neighbours, the PK2 stress
+// hooke(E(F)+eps_p) + hooke(E'), A = (F S^T) L^-1 for the elastic force, and the plastic
+// flow rate (solid.rs:334-341, 497-518).
+extern "C" __global__ void __launch_bounds__(256) k_strain(SolidArgs a) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= a.ns) return;
+    float* ke = a.keps + 16 * (size_t)s;
+    const uint32_t p = a.part[s];
+    if (p == kInvalid) {
+        for (int q = 0; q < 16; q++) ke[q] = 0.0f;
+        return;
+    }
+    const uint32_t i = a.inv[p];
+    const uint32_t mk = a.mk[i];
+    if (!(mk & kMkElastic)) {
+        for (int q = 0; q < 16; q++) ke[q] = 0.0f;
+        return;
+    }
+    const DMat& M = a.mats[mk & kMatMask];
+    const float4* X = reinterpret_cast<const float4*>(a.Xposd);
+    const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
+    const float4 xi = X[i], vi = XV[i];
+    float F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [row*3+col]
+    const float4* G = reinterpret_cast<const float4*>(a.sG);
+    for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
+        const uint32_t j = a.inv[a.sPid[q]];
+        const float4 g = G[q];
+        const float4 xj = X[j], vj = XV[j];
+        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+        const float dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+        const float gg[3] = {g.x, g.y, g.z};
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                F[3 * r + c] += g.w * (dx[r] * gg[c]);
+                Fd[3 * r + c] += g.w * (dv[r] * gg[c]);
+            }
+    }
+    M4 F4 = m4_zero(), Fd4 = m4_zero();
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            F4.m[4 * c + r] = F[3 * r + c];
+            Fd4.m[4 * c + r] = Fd[3 * r + c];
+        }
+    M4 eps;
+    const float* ep = a.Xeps + 16 * (size_t)s;
+    for (int q = 0; q < 16; q++) eps.m[q] = ep[q];
+    const uint32_t dim = a.grid.dim_n;
+    const M4 E = m4_add(m4_seth_hill(F4, M.strain_order, dim), eps);
+    const M4 Ed = m4_seth_hill_rate(F4, Fd4, M.strain_order, dim);
+    const M4 S = m4_add(m4_hooke(E, M.lambda, M.mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
+    const M4 P = m4_mul(F4, m4_trans(S));
+    M4 Li;
+    for (int q = 0; q < 16; q++) Li.m[q] = a.Linv[16 * (size_t)s + q];
+    const M4 A = m4_mul(P, Li);
+    store_rows3(a.A + 12 * (size_t)s, A);
+    store_rows3(a.Fm + 12 * (size_t)s, F4);
+    for (int q = 0; q < 16; q++) ke[q] = 0.0f;
+    if (M.plastic) {
+        // eq_plastic_strain_sq carries the integer factor (dim-1)/dim == 0 (solid.rs:212-215)
+        M4 dev = eps;
+        const float tr = m4_trace(eps, dim);
+        for (int c = 0; c < 4; c++) dev.m[5 * c] = eps.m[5 * c] - tr / (float)dim;
+        const float eq = (float)((dim - 1) / dim) * m4_dot(dev, dev);
+        float sn = m4_dot(S, S);
+        const float ys = (M.yield + M.hard * powf(eq, M.hard_exp / 2.0f)) * (1.0f + M.kin_hard * logf(1.0f));
+        if (ys * ys <= sn) {
+            sn = sqrtf(sn);
+            const float sc = (sn - ys) / M.relax;
+            for (int q = 0; q < 16; q++) ke[q] = sc * (S.m[q] / sn);
+        }
+    }
+}
+
+__device__ __forceinline__ float3 rows3_mul(const float4* R, float x, float y, float z) {
+    return make_float3(R[0].x * x + R[0].y * y + R[0].z * z, R[1].x * x + R[1].y * y + R[1].z * z,
+                       R[2].x * x + R[2].y * y + R[2].z * z);
+}
+
+// One thread per solid: elastic force over same-material reference neighbours, hourglass
+// restoring force for those inside the current 27-cell stencil, self contact of
+// reference-distant pairs (solid.rs:343-451); then dest + dest2 (shaders/mod.rs:79) and
+// the stage epilogue for the particle and its plastic strain.
+extern "C" __global__ void __launch_bounds__(256) k_solid(SolidArgs a) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const float4* X = reinterpret_cast<const float4*>(a.Xposd);
+    const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
+    const float4* REF = reinterpret_cast<const float4*>(a.ref);
+    bool haveKey = false;
+    uint32_t i = 0, nextKey = kInvalid, nextBin = 0;
+    const float h = a.phys.h, EPS = 0.1f;
+    uint32_t p = kInvalid, mk = 0;
+    if (s < a.ns) {
+        p = a.part[s];
+        if (p != kInvalid) {
+            i = a.inv[p];
+            mk = a.mk[i];
+        }
+    }
+    if (s < a.ns && p != kInvalid && (mk & kMkElastic)) {
+        const uint32_t mat = mk & kMatMask;
+        const DMat& M = a.mats[mat];
+        const float4 xi = X[i], vi = XV[i];
+        const float m1 = M.mass, d1 = xi.w, V1 = m1 / d1;
+        const float4 Xs = REF[s];
+        const float4* A1 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s;
+        const float4* F1 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s;
+        const bool valid_i = a.sflag[i] == 0u;
+        uint32_t ci[3] = {0, 0, 0};
+        cell_of(a.grid, xi.x, xi.y, xi.z, ci[0], ci[1], ci[2]);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        const float4* EG = reinterpret_cast<const float4*>(a.eG);
+        for (uint32_t q = a.eOff[s]; q < a.eOff[s + 1]; q++) {
+            const uint32_t s2 = a.eSid[q];
+            const uint32_t j = a.inv[a.ePid[q]];
+            const float4 g = EG[q];
+            const float4 xj = X[j];
+            const float V2 = m1 / xj.w;
+            const float4* A2 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s2;
+            const float c = -V1 * V2;
+            float4 Asum[3];
+            for (int r = 0; r < 3; r++)
+                Asum[r] = make_float4(c * (A1[r].x + A2[r].x), c * (A1[r].y + A2[r].y), c * (A1[r].z + A2[r].z), 0.f);
+            const float3 el = rows3_mul(Asum, g.x, g.y, g.z);
+            fx += el.x;
+            fy += el.y;
+            if (a.grid.dim_n > 2) fz += el.z;
+            // hourglass (solid.rs:432-443): same material, W(dX) != 0, inside the stencil
+            if (g.w != 0.0f && valid_i && a.sflag[j] == 0u) {
+                uint32_t cj[3] = {0, 0, 0};
+                cell_of(a.grid, xj.x, xj.y, xj.z, cj[0], cj[1], cj[2]);
+                bool adj = true;
+                for (int d = 0; d < 3; d++) adj &= (ci[d] + 1 >= cj[d]) && (cj[d] + 1 >= ci[d]);
+                if (adj) {
+                    const float4 X2 = REF[s2];
+                    const float dX[3] = {X2.x - Xs.x, X2.y - Xs.y, X2.z - Xs.z};
+                    const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+                    const float l = sqrtf(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+                    const float4* F2 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s2;
+                    float err[3];
+                    for (int r = 0; r < 3; r++) {
+                        const float4 fa = make_float4(0.5f * (F1[r].x + F2[r].x), 0.5f * (F1[r].y + F2[r].y),
+                                                      0.5f * (F1[r].z + F2[r].z), 0.f);
+                        err[r] = fa.x * dX[0] + fa.y * dX[1] + fa.z * dX[2] - dx[r];
+                    }
+                    const float m2 = m1, d2 = xj.w;
+                    const float sc = (100000000.0f * m1 * m2 / (d1 * d2)) *
+                                     (0.5f * (err[0] * dx[0] + err[1] * dx[1] + err[2] * dx[2]) / (l + 10.0f * EPS * h)) *
+                                     (1.0f / (l * l + EPS * h * h)) * g.w;
+                    fx -= sc * dx[0] / (l + EPS * h);
+                    fy -= sc * dx[1] / (l + EPS * h);
+                    fz -= sc * dx[2] / (l + EPS * h);
+                }
+            }
+        }
+        // self contact: same-material pairs further than 8h apart in the reference state but
+        // inside the current stencil and closer than 8h (solid.rs:426-450)
+        if (!M.plastic && valid_i) {
+            const float r_cut = 8.0f * h, r_geom = h * h / r_cut;
+            const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
+            const uint32_t xl = ci[0] > 0 ? ci[0] - 1 : 0, xh = min(ci[0] + 1, sx - 1);
+            for (int dz = -1; dz <= 1; dz++) {
+                const int nz = (int)ci[2] + dz;
+                if (nz < 0 || nz >= (int)sz) continue;
+                for (int dy = -1; dy <= 1; dy++) {
+                    const int ny = (int)ci[1] + dy;
+                    if (ny < 0 || ny >= (int)sy) continue;
+                    const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                    const uint32_t s0 = a.binStart[row], s1 = a.binStart[row + 1];
+                    if (s1 <= s0) continue;
+                    const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                    for (uint32_t q = s0 + ro[xl]; q < s0 + ro[xh + 1]; q++) {
+                        const uint32_t pe = a.perm[q];
+                        if (pe & kFlag) continue;
+                        const uint32_t mk2 = a.mk[pe];
+                        if ((mk2 & kMatMask) != mat || !(mk2 & kMkSolid)) continue;
+                        const uint32_t s2 = a.sidByPid[a.pid[pe]];
+                        const float4 X2 = REF[s2];
+                        const float dX0 = X2.x - Xs.x, dX1 = X2.y - Xs.y, dX2 = X2.z - Xs.z;
+                        if (!(dX0 * dX0 + dX1 * dX1 + dX2 * dX2 > r_cut * r_cut)) continue;
+                        const float4 xj = X[pe];
+                        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                        const float rr = rx * rx + ry * ry + rz * rz;
+                        if (!(rr < r_cut * r_cut)) continue;
+                        const float l = sqrtf(rr);
+                        const float dr = fmaxf(r_cut - l, 0.0f);
+                        const float cs = -5000.0f * sqrtf(dr * r_geom);
+                        fx += cs * rx / l;
+                        fy += cs * ry / l;
+                        fz += cs * rz / l;
+                    }
+                }
+            }
+        }
+        // dest2 (solid.rs:492-494) + dest (fluid part)
+        const float4 kf = reinterpret_cast<const float4*>(a.KFp)[i];
+        const float4 kfv = reinterpret_cast<const float4*>(a.KFv)[i];
+        const float4 kP = make_float4(kf.x + vi.x, kf.y + vi.y, kf.z + vi.z, kf.w + 0.0f);
+        const float4 kV = make_float4(kfv.x + fx / d1, kfv.y + (fy / d1 - a.phys.g), kfv.z + fz / d1, 0.0f);
+        if (a.mode == 2) {
+            reinterpret_cast<float4*>(a.KdP)[i] = kP;
+            reinterpret_cast<float4*>(a.KdV)[i] = kV;
+        } else {
+            rk_update_particle(a.rk, a.grid, M, i, kP, kV, haveKey, nextKey, nextBin);
+            rk_update_eps(a.rk, s, a.keps + 16 * (size_t)s);
+        }
+    } else if (s < a.ns && a.mode == 0) {
+        // solids of non-elastic materials keep their plastic strain (zero rate)
+        float z[16];
+        for (int q = 0; q < 16; q++) z[q] = 0.0f;
+        rk_update_eps(a.rk, s, z);
+    }
+    if (a.rk.keys && a.mode == 0) {
+        const uint32_t rank = agg_rank(a.rk.binCount, nextBin, haveKey);
+        if (haveKey) {
+            a.rk.keys[i] = nextKey;
+            a.rk.ranks[i] = rank;
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_eps_in(EpsInArgs a) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= a.ns) return;
+    const AosSolid so = reinterpret_cast<const AosSolid*>(a.aos)[s];
+    float4* e = reinterpret_cast<float4*>(a.eps + 16 * (size_t)s);
+    for (int q = 0; q < 4; q++) e[q] = so.stress[q];
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_eps_out(EpsOutArgs a) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= a.ns) return;
+    AosSolid o;
+    o.part_id = a.part[s];
+    o.pad[0] = o.pad[1] = o.pad[2] = 0;
+    o.ref_pos = a.ref ? reinterpret_cast<const float4*>(a.ref)[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4* e = reinterpret_cast<const float4*>(a.eps + 16 * (size_t)s);
+    for (int q = 0; q < 4; q++) o.stress[q] = e[q];
+    reinterpret_cast<AosSolid*>(a.aos)[s] = o;
 }

@@ -34,11 +34,16 @@ enum Kern {
     K_VELOCITY,
     K_FILL,
     K_CLEAR_SOLIDS,
+    K_STRAIN,
+    K_SOLID,
+    K_EPS_IN,
+    K_EPS_OUT,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
                                    "k_fluid",   "k_aos_in",  "k_aos_out",  "k_lincomb",
-                                   "k_velocity", "k_fill",   "k_clear_solids"};
+                                   "k_velocity", "k_fill",   "k_clear_solids", "k_strain",
+                                   "k_solid",   "k_eps_in",  "k_eps_out"};
 
 struct DBuf {
     void* p = nullptr;
@@ -81,12 +86,16 @@ struct arp_ctx {
     DBuf Xb[2][2];  // [buffer][posd/velq]
     DBuf Acc[2];
     DBuf K[kMaxK][2];
-    DBuf inv;
+    DBuf inv, sflag, KF[2];
+    // solids: static reference-neighbour data and plastic-strain registers (solid-id order)
+    bool solids_active = false;
+    DBuf part_d, ref_d, Linv_d, sOff_d, sPid_d, sG_d, eOff_d, ePid_d, eSid_d, eG_d;
+    DBuf Yeps, Xeps[2], AccE, Ke[kMaxK], keps, A_d, Fm_d, ps_eps;
     // per-stage API buffers
-    DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys;
+    DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys, saos_tmp;
     // grid
     DBuf keys, ranks, binCount, binStart, activeRows, numActive, tmpKey, tmpSrc, perm, rowOff,
-        gPid, gGrp, stats;
+        gPid, gGrp, stats, scanState, scanTicket, scanEpoch;
     bool keys_valid = false;
     // what the last build described (for the dump)
     const DBuf* last_pid = nullptr;
@@ -112,11 +121,16 @@ struct arp_ctx {
                 &Xb[1][0], &Xb[1][1],  &Acc[0],     &Acc[1],   &inv,    &ps_mk[0], &ps_mk[1],
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
-                &gPid,     &gGrp,      &stats};
+                &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
             bufs.push_back(&K[k][1]);
+            bufs.push_back(&Ke[k]);
         }
+        for (DBuf* b : {&sflag, &KF[0], &KF[1], &part_d, &ref_d, &Linv_d, &sOff_d, &sPid_d, &sG_d, &eOff_d,
+                        &ePid_d, &eSid_d, &eG_d, &Yeps, &Xeps[0], &Xeps[1], &AccE, &keps, &A_d, &Fm_d, &ps_eps,
+                        &saos_tmp})
+            bufs.push_back(b);
     }
 };
 
@@ -262,6 +276,135 @@ void cell_sort(uint64_t ncells, const std::vector<uint64_t>& key, const std::vec
     }
 }
 
+// inverse(mat4) by cofactors, column-major
+void invert4(const float* m, float* t) {
+    float a00 = m[0], a01 = m[1], a02 = m[2], a03 = m[3];
+    float a10 = m[4], a11 = m[5], a12 = m[6], a13 = m[7];
+    float a20 = m[8], a21 = m[9], a22 = m[10], a23 = m[11];
+    float a30 = m[12], a31 = m[13], a32 = m[14], a33 = m[15];
+    float b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10;
+    float b02 = a00 * a13 - a03 * a10, b03 = a01 * a12 - a02 * a11;
+    float b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+    float b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30;
+    float b08 = a20 * a33 - a23 * a30, b09 = a21 * a32 - a22 * a31;
+    float b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+    float det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+    float id = 1.0f / det;
+    t[0] = (a11 * b11 - a12 * b10 + a13 * b09) * id;
+    t[1] = (a02 * b10 - a01 * b11 - a03 * b09) * id;
+    t[2] = (a31 * b05 - a32 * b04 + a33 * b03) * id;
+    t[3] = (a22 * b04 - a21 * b05 - a23 * b03) * id;
+    t[4] = (a12 * b08 - a10 * b11 - a13 * b07) * id;
+    t[5] = (a00 * b11 - a02 * b08 + a03 * b07) * id;
+    t[6] = (a32 * b02 - a30 * b05 - a33 * b01) * id;
+    t[7] = (a20 * b05 - a22 * b02 + a23 * b01) * id;
+    t[8] = (a10 * b10 - a11 * b08 + a13 * b06) * id;
+    t[9] = (a01 * b08 - a00 * b10 - a03 * b06) * id;
+    t[10] = (a30 * b04 - a31 * b02 + a33 * b00) * id;
+    t[11] = (a21 * b02 - a20 * b04 - a23 * b00) * id;
+    t[12] = (a11 * b07 - a10 * b09 - a12 * b06) * id;
+    t[13] = (a00 * b09 - a01 * b07 + a02 * b06) * id;
+    t[14] = (a31 * b01 - a30 * b03 - a32 * b00) * id;
+    t[15] = (a20 * b03 - a21 * b01 + a22 * b00) * id;
+}
+
+// Static reference-neighbour data of the solid path.  The reference configuration never
+// changes (ref_pos' = 0, solid.rs:30, 295), so the reference buckets, the corrected-gradient
+// matrix L (strain.rs:101-136) and the per-pair kernel gradients are fixed at upload.
+int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, const ArpSolidParticle* solids,
+                      uint32_t ns, const ArpMaterial* mats, const std::vector<uint32_t>& kept) {
+    const GridP& g = c->grid;
+    const float h = c->phys.h, k = c->phys.k;
+    const uint32_t dim = g.dim_n;
+    std::vector<uint32_t> refcell(n, ARP_INVALID_INDEX);
+    for (uint64_t cc = 0; cc < c->ncells; cc++)
+        for (uint32_t q = c->rP_h[cc]; q < c->rP_h[cc + 1]; q++) refcell[kept[q]] = (uint32_t)cc;
+    std::vector<uint32_t> part(ns), sOff(ns + 1, 0), eOff(ns + 1, 0), sPid, ePid, eSid;
+    std::vector<float> ref(4 * (size_t)ns), Linv(16 * (size_t)ns, 0.0f), sG, eG;
+    for (uint32_t s = 0; s < ns; s++) {
+        part[s] = solids[s].part_id;
+        for (int q = 0; q < 4; q++) ref[4 * s + q] = solids[s].ref_pos[q];
+    }
+    auto gradw = [&](const float* r, float* out) -> float {
+        float rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3];
+        float diff = std::fmax(h * h - rr, 0.0f);
+        float gs = -6.0f * k * diff * diff;
+        for (int q = 0; q < 4; q++) out[q] = gs * r[q];
+        return diff;
+    };
+    for (uint32_t s = 0; s < ns; s++) {
+        sOff[s] = (uint32_t)sPid.size();
+        eOff[s] = (uint32_t)ePid.size();
+        const uint32_t p = part[s];
+        if (p == ARP_INVALID_INDEX || p >= n) continue;
+        const uint32_t mat = particles[p].mat;
+        if (!is_stiff(mats[mat])) continue;
+        float L[16] = {0};
+        std::vector<float> gws;
+        size_t s_first = sPid.size();
+        const uint32_t rc = refcell[p];
+        if (rc != ARP_INVALID_INDEX) {
+            const uint32_t cx = rc % g.sub[0], cy = (rc / g.sub[0]) % g.sub[1], cz = rc / (g.sub[0] * g.sub[1]);
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        int64_t x = (int64_t)cx + dx, y = (int64_t)cy + dy, z = (int64_t)cz + dz;
+                        if (x < 0 || y < 0 || z < 0 || x >= g.sub[0] || y >= g.sub[1] || z >= g.sub[2]) continue;
+                        uint64_t cell = (uint64_t)x + g.sub[0] * ((uint64_t)y + (uint64_t)g.sub[1] * z);
+                        for (uint32_t q = c->rP_h[cell]; q < c->rP_h[cell + 1]; q++) {
+                            const uint32_t p2 = kept[q];
+                            const uint32_t s2 = particles[p2].solid_id;
+                            const uint32_t mat2 = particles[p2].mat;
+                            float dX[4], gw[4];
+                            for (int r = 0; r < 4; r++) dX[r] = ref[4 * s2 + r] - ref[4 * s + r];
+                            const float diff = gradw(dX, gw);
+                            if (is_stiff(mats[mat2])) {
+                                const float V2 = mats[mat2].mass / mats[mat2].start_den;
+                                for (int j = 0; j < 4; j++)
+                                    for (int i = 0; i < 4; i++) L[4 * j + i] += V2 * (gw[i] * dX[j]);
+                                if (diff > 0.0f) {
+                                    sPid.push_back(p2);
+                                    for (int r = 0; r < 3; r++) gws.push_back(gw[r]);
+                                    sG.push_back(0.f), sG.push_back(0.f), sG.push_back(0.f), sG.push_back(V2);
+                                }
+                            }
+                            if (mat2 == mat && diff > 0.0f) {
+                                ePid.push_back(p2);
+                                eSid.push_back(s2);
+                                eG.push_back(gw[0]);
+                                eG.push_back(gw[1]);
+                                eG.push_back(gw[2]);
+                                eG.push_back(k * diff * diff * diff);
+                            }
+                        }
+                    }
+        }
+        for (uint32_t i = dim; i < 4; i++) L[5 * i] = 1.0f;
+        float Li[16];
+        invert4(L, Li);
+        for (int q = 0; q < 16; q++) Linv[16 * (size_t)s + q] = Li[q];
+        // strain pairs carry L^-1 gradW(dX) (strain.rs:156)
+        for (size_t q = s_first, t = 0; q < sPid.size(); q++, t++) {
+            const float* gw = &gws[3 * t];
+            for (int r = 0; r < 3; r++) sG[4 * q + r] = Li[r] * gw[0] + Li[4 + r] * gw[1] + Li[8 + r] * gw[2];
+        }
+    }
+    sOff[ns] = (uint32_t)sPid.size();
+    eOff[ns] = (uint32_t)ePid.size();
+    int rc;
+    if ((rc = upload(c, c->part_d, part.data(), 4 * part.size()))) return rc;
+    if ((rc = upload(c, c->ref_d, ref.data(), 4 * ref.size()))) return rc;
+    if ((rc = upload(c, c->Linv_d, Linv.data(), 4 * Linv.size()))) return rc;
+    if ((rc = upload(c, c->sOff_d, sOff.data(), 4 * sOff.size()))) return rc;
+    if ((rc = upload(c, c->sPid_d, sPid.data(), 4 * sPid.size()))) return rc;
+    if ((rc = upload(c, c->sG_d, sG.data(), 4 * sG.size()))) return rc;
+    if ((rc = upload(c, c->eOff_d, eOff.data(), 4 * eOff.size()))) return rc;
+    if ((rc = upload(c, c->ePid_d, ePid.data(), 4 * ePid.size()))) return rc;
+    if ((rc = upload(c, c->eSid_d, eSid.data(), 4 * eSid.size()))) return rc;
+    if ((rc = upload(c, c->eG_d, eG.data(), 4 * eG.size()))) return rc;
+    return ARP_OK;
+}
+
 // ---- the sort chain ---------------------------------------------------------------
 struct SortIO {
     int full;
@@ -282,9 +425,13 @@ int run_key(arp_ctx* c, const float* posd) {
 
 int run_sort(arp_ctx* c, const SortIO& io) {
     int rc;
-    ScanArgs s{c->grid.nrows + 1, P<uint32_t>(c->binCount), P<uint32_t>(c->binStart),
-               P<uint32_t>(c->activeRows), P<uint32_t>(c->numActive)};
-    if ((rc = launch(c, K_SCAN, 1, 1024, s))) return rc;
+    const uint32_t nbins = c->grid.nrows + 1;
+    const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
+    ScanArgs s{nbins, ntiles, P<uint32_t>(c->binCount), P<uint32_t>(c->binStart),
+               P<uint32_t>(c->activeRows), P<uint32_t>(c->numActive),
+               P<unsigned long long>(c->scanState), P<uint32_t>(c->scanTicket), P<uint32_t>(c->scanEpoch),
+               P<uint32_t>(c->stats)};
+    if ((rc = launch(c, K_SCAN, ntiles, 256, s))) return rc;
     ScatterArgs sc{};
     sc.grid = c->grid;
     sc.n = c->n;
@@ -304,6 +451,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.dstMk = io.dstMk;
     sc.dstPid = io.dstPid;
     sc.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
+    sc.sflag = P<uint32_t>(c->sflag);
     if ((rc = launch(c, K_SCATTER, nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
     r.grid = c->grid;
@@ -330,6 +478,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.dstPid = io.dstPid;
     r.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
     r.stats = P<uint32_t>(c->stats);
+    r.sflag = P<uint32_t>(c->sflag);
     uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(c->grid.nrows, 1), 4096);
     return launch(c, K_ROWSORT, blocks, 256, r);
 }
@@ -367,6 +516,18 @@ int alloc_state(arp_ctx* c) {
     }
     if ((rc = dalloc(c, c->aos_tmp, sizeof(ArpParticle) * n))) return rc;
     if ((rc = dalloc(c, c->inv, 4 * n))) return rc;
+    if ((rc = dalloc(c, c->sflag, 4 * n))) return rc;
+    if (c->solids_active) {
+        size_t ns = std::max<uint32_t>(c->ns, 1);
+        for (int k = 0; k < 2; k++) {
+            if ((rc = dalloc(c, c->KF[k], 16 * n))) return rc;
+            if ((rc = dalloc(c, c->Xeps[k], 64 * ns))) return rc;
+        }
+        for (DBuf* b : {&c->Yeps, &c->keps, &c->ps_eps})
+            if ((rc = dalloc(c, *b, 64 * ns))) return rc;
+        for (DBuf* b : {&c->A_d, &c->Fm_d})
+            if ((rc = dalloc(c, *b, 48 * ns))) return rc;
+    }
     for (DBuf* b : {&c->keys, &c->ranks, &c->tmpKey, &c->tmpSrc, &c->perm, &c->gPid, &c->gGrp})
         if ((rc = dalloc(c, *b, 4 * n))) return rc;
     uint32_t nb = c->grid.nrows + 1;
@@ -375,6 +536,15 @@ int alloc_state(arp_ctx* c) {
     if ((rc = dalloc(c, c->activeRows, 4 * (size_t)nb))) return rc;
     if ((rc = dalloc(c, c->numActive, 16))) return rc;
     if ((rc = dalloc(c, c->stats, 16))) return rc;
+    {
+        const uint32_t ntiles = (nb + kScanTile - 1) / kScanTile;
+        if ((rc = dalloc(c, c->scanState, 16 * (size_t)ntiles))) return rc;
+        if ((rc = dalloc(c, c->scanTicket, 16))) return rc;
+        if ((rc = dalloc(c, c->scanEpoch, 16))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->scanState.p, 0, 16 * (size_t)ntiles, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->scanTicket.p, 0, 16, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->scanEpoch.p, 0, 16, c->stream));
+    }
     if ((rc = dalloc(c, c->rowOff, 4 * (size_t)c->grid.nrows * c->grid.rowstride))) return rc;
     return ARP_OK;
 }
@@ -388,6 +558,12 @@ int ensure_k(arp_ctx* c, int slots, bool acc) {
     if (acc)
         for (int j = 0; j < 2; j++)
             if ((rc = dalloc(c, c->Acc[j], 16 * n))) return rc;
+    if (c->solids_active) {
+        size_t ns = std::max<uint32_t>(c->ns, 1);
+        for (int k = 0; k < slots; k++)
+            if ((rc = dalloc(c, c->Ke[k], 64 * ns))) return rc;
+        if (acc && (rc = dalloc(c, c->AccE, 64 * ns))) return rc;
+    }
     return ARP_OK;
 }
 
@@ -400,6 +576,36 @@ int aos_to_soa(arp_ctx* c, const void* d_aos, float* posd, float* velq, uint32_t
 int soa_to_aos(arp_ctx* c, const float* posd, const float* velq, const uint32_t* pid, void* d_aos) {
     AosOutArgs a{c->n, posd, velq, pid, P<uint32_t>(c->sidByPid), P<uint32_t>(c->matByPid), d_aos};
     return launch(c, K_AOS_OUT, nblk(c->n), 256, a);
+}
+
+SolidArgs solid_base(arp_ctx* c) {
+    SolidArgs a{};
+    a.grid = c->grid;
+    a.phys = c->phys;
+    a.ns = c->ns;
+    a.mats = P<DMat>(c->mats);
+    a.part = P<uint32_t>(c->part_d);
+    a.inv = P<uint32_t>(c->inv);
+    a.ref = P<float>(c->ref_d);
+    a.Linv = P<float>(c->Linv_d);
+    a.sOff = P<uint32_t>(c->sOff_d);
+    a.sPid = P<uint32_t>(c->sPid_d);
+    a.sG = P<float>(c->sG_d);
+    a.eOff = P<uint32_t>(c->eOff_d);
+    a.ePid = P<uint32_t>(c->ePid_d);
+    a.eSid = P<uint32_t>(c->eSid_d);
+    a.eG = P<float>(c->eG_d);
+    a.A = P<float>(c->A_d);
+    a.Fm = P<float>(c->Fm_d);
+    a.keps = P<float>(c->keps);
+    a.perm = P<uint32_t>(c->perm);
+    a.binStart = P<uint32_t>(c->binStart);
+    a.rowOff = P<uint32_t>(c->rowOff);
+    a.sflag = P<uint32_t>(c->sflag);
+    a.sidByPid = P<uint32_t>(c->sidByPid);
+    a.KFp = P<float>(c->KF[0]);
+    a.KFv = P<float>(c->KF[1]);
+    return a;
 }
 
 // One RK step of length dt on the internal state (Yland/meta[cur] on entry and exit).
@@ -438,58 +644,84 @@ int step_once(arp_ctx* c, float dt) {
 
     const float* Xp = P<float>(c->Ysort[0]);
     const float* Xv = P<float>(c->Ysort[1]);
+    const float* Xe = P<float>(c->Yeps);
     for (int s = 0; s < S; s++) {
         if (s > 0) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
                        nullptr, nullptr, nullptr, nullptr};
             if ((rc = run_sort(c, pio))) return rc;
         }
+        const bool last = (s == S - 1);
+        RkArgs r{};
+        r.Yposd = P<float>(c->Ysort[0]);
+        r.Yvelq = P<float>(c->Ysort[1]);
+        r.b_cur = dt * T.b[s];
+        r.last = last ? 1 : 0;
+        if (acc && s > 0) {
+            r.AccP = P<float>(c->Acc[0]);
+            r.AccV = P<float>(c->Acc[1]);
+            r.AccE = c->solids_active ? P<float>(c->AccE) : nullptr;
+        }
+        if (acc && !last) {
+            r.AccPOut = P<float>(c->Acc[0]);
+            r.AccVOut = P<float>(c->Acc[1]);
+            r.AccEOut = c->solids_active ? P<float>(c->AccE) : nullptr;
+        }
+        if (!last) {
+            r.a_next = dt * T.a[s + 1][s];
+            r.nk = 0;
+            for (int j = 0; j < s; j++)
+                if (T.a[s + 1][j] != 0.0f) {
+                    r.Kp[r.nk] = P<float>(c->K[kslot[j]][0]);
+                    r.Kv[r.nk] = P<float>(c->K[kslot[j]][1]);
+                    r.Ke[r.nk] = c->solids_active ? P<float>(c->Ke[kslot[j]]) : nullptr;
+                    r.kc[r.nk] = dt * T.a[s + 1][j];
+                    r.nk++;
+                }
+            r.Oposd = P<float>(c->Xb[s % 2][0]);
+            r.Ovelq = P<float>(c->Xb[s % 2][1]);
+            r.Oeps = P<float>(c->Xeps[s % 2]);
+        } else {
+            r.Oposd = P<float>(c->Yland[0]);
+            r.Ovelq = P<float>(c->Yland[1]);
+            r.Oeps = P<float>(c->Yeps);  // in place: each solid's thread reads then writes
+        }
+        r.Yeps = P<float>(c->Yeps);
+        if (kslot[s] >= 0) {
+            r.KpOut = P<float>(c->K[kslot[s]][0]);
+            r.KvOut = P<float>(c->K[kslot[s]][1]);
+            r.KeOut = c->solids_active ? P<float>(c->Ke[kslot[s]]) : nullptr;
+        }
+        r.keys = P<uint32_t>(c->keys);
+        r.ranks = P<uint32_t>(c->ranks);
+        r.binCount = P<uint32_t>(c->binCount);
+
         FluidArgs f = fluid_base(c);
         f.Xposd = Xp;
         f.Xvelq = Xv;
         f.mk = P<uint32_t>(c->meta_mk[c->cur]);
-        f.pid = P<uint32_t>(c->meta_pid[c->cur]);
         f.mode = 0;
-        f.Yposd = P<float>(c->Ysort[0]);
-        f.Yvelq = P<float>(c->Ysort[1]);
-        f.b_cur = dt * T.b[s];
-        const bool last = (s == S - 1);
-        f.last = last ? 1 : 0;
-        if (acc && s > 0) {
-            f.AccP = P<float>(c->Acc[0]);
-            f.AccV = P<float>(c->Acc[1]);
-        }
-        if (acc && !last) {
-            f.AccPOut = P<float>(c->Acc[0]);
-            f.AccVOut = P<float>(c->Acc[1]);
-        }
-        if (!last) {
-            f.a_next = dt * T.a[s + 1][s];
-            f.nk = 0;
-            for (int j = 0; j < s; j++)
-                if (T.a[s + 1][j] != 0.0f) {
-                    f.Kp[f.nk] = P<float>(c->K[kslot[j]][0]);
-                    f.Kv[f.nk] = P<float>(c->K[kslot[j]][1]);
-                    f.kc[f.nk] = dt * T.a[s + 1][j];
-                    f.nk++;
-                }
-            f.Oposd = P<float>(c->Xb[s % 2][0]);
-            f.Ovelq = P<float>(c->Xb[s % 2][1]);
-        } else {
-            f.Oposd = P<float>(c->Yland[0]);
-            f.Ovelq = P<float>(c->Yland[1]);
-        }
-        if (kslot[s] >= 0) {
-            f.KpOut = P<float>(c->K[kslot[s]][0]);
-            f.KvOut = P<float>(c->K[kslot[s]][1]);
-        }
-        f.keys = P<uint32_t>(c->keys);
-        f.ranks = P<uint32_t>(c->ranks);
-        f.binCount = P<uint32_t>(c->binCount);
+        f.skip_elastic = c->solids_active ? 1 : 0;
+        f.KdP = P<float>(c->KF[0]);
+        f.KdV = P<float>(c->KF[1]);
+        f.rk = r;
         if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
+        if (c->solids_active) {
+            SolidArgs sa = solid_base(c);
+            sa.Xposd = Xp;
+            sa.Xvelq = Xv;
+            sa.mk = P<uint32_t>(c->meta_mk[c->cur]);
+            sa.pid = P<uint32_t>(c->meta_pid[c->cur]);
+            sa.Xeps = Xe;
+            sa.mode = 0;
+            sa.rk = r;
+            if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
+            if ((rc = launch(c, K_SOLID, nblk(c->ns), 256, sa))) return rc;
+        }
         if (!last) {
             Xp = P<float>(c->Xb[s % 2][0]);
             Xv = P<float>(c->Xb[s % 2][1]);
+            Xe = P<float>(c->Xeps[s % 2]);
         }
     }
     c->keys_valid = true;  // the last epilogue binned y_{n+1}
@@ -808,6 +1040,10 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
         std::vector<uint32_t> kept;
         cell_sort(ncells, key, room, c->rP_h, kept);
         if ((rc = upload(c, c->rP, c->rP_h.data(), c->rP_h.size() * 4))) return rc;
+        // the solid path runs only when there is more than one solid (shaders/mod.rs:51)
+        c->solids_active = ns > 1;
+        if (c->solids_active)
+            if ((rc = build_solid_lists(c, particles, n, solids, ns, mats, kept))) return rc;
     }
     if ((rc = alloc_state(c))) return rc;
     c->have_scene = true;
@@ -824,9 +1060,14 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     int rc;
     if ((rc = need_scene(c))) return rc;
     if (n != c->n || (n && !particles)) return fail(c, ARP_E_ARG, "state has %u particles, scene %u", n, c->n);
-    (void)solids;
-    (void)ns;
     HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    if (c->solids_active) {
+        const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
+        if (solids && ns != c->ns) return fail(c, ARP_E_ARG, "state has %u solids, scene %u", ns, c->ns);
+        if ((rc = upload(c, c->saos_tmp, src, sizeof(ArpSolidParticle) * (size_t)c->ns))) return rc;
+        EpsInArgs e{c->ns, c->saos_tmp.p, P<float>(c->Yeps)};
+        if ((rc = launch(c, K_EPS_IN, nblk(c->ns), 256, e))) return rc;
+    }
     if (c->keys_valid)
         if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
     if ((rc = aos_to_soa(c, c->aos_tmp.p, P<float>(c->Yland[0]), P<float>(c->Yland[1]),
@@ -884,7 +1125,15 @@ int arp_download(arp_ctx* c, ArpParticle* particles, uint32_t n, ArpSolidParticl
     }
     if (solids) {
         if (ns != c->ns) return fail(c, ARP_E_ARG, "download of %u solids, scene has %u", ns, c->ns);
-        std::copy(c->solids_h.begin(), c->solids_h.end(), solids);
+        if (c->solids_active) {
+            if ((rc = dalloc(c, c->saos_tmp, sizeof(ArpSolidParticle) * (size_t)ns))) return rc;
+            EpsOutArgs e{ns, P<float>(c->Yeps), P<float>(c->ref_d), P<uint32_t>(c->part_d), c->saos_tmp.p};
+            if ((rc = launch(c, K_EPS_OUT, nblk(ns), 256, e))) return rc;
+            HIPCHK(c, hipMemcpyAsync(solids, c->saos_tmp.p, sizeof(ArpSolidParticle) * (size_t)ns,
+                                     hipMemcpyDeviceToHost, c->stream));
+        } else {
+            std::copy(c->solids_h.begin(), c->solids_h.end(), solids);
+        }
     }
     return arp_synchronize(c);
 }
@@ -901,21 +1150,42 @@ int arp_compute_forces(arp_ctx* c, const ArpParticle* d_state, const ArpSolidPar
     int rc;
     if ((rc = need_scene(c))) return rc;
     if (!d_state || !d_deriv) return fail(c, ARP_E_ARG, "null state/derivative");
-    if (c->ns > 1) return fail(c, ARP_E_STATE, "solid path not enabled in this build");
+    if (c->solids_active && !d_solids) return fail(c, ARP_E_ARG, "the scene has solids: d_solids is required");
     if ((rc = stage_sort_aos(c, d_state))) return rc;
+    DBuf* kd = c->solids_active ? c->KF : c->K[0];
     FluidArgs f = fluid_base(c);
     f.Xposd = P<float>(c->Xb[1][0]);
     f.Xvelq = P<float>(c->Xb[1][1]);
     f.mk = P<uint32_t>(c->ps_mk[1]);
-    f.pid = P<uint32_t>(c->ps_pid[1]);
     f.mode = 2;
-    f.KpOut = P<float>(c->K[0][0]);
-    f.KvOut = P<float>(c->K[0][1]);
+    f.KdP = P<float>(kd[0]);
+    f.KdV = P<float>(kd[1]);
     if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
-    if ((rc = soa_to_aos(c, P<float>(c->K[0][0]), P<float>(c->K[0][1]), P<uint32_t>(c->ps_pid[1]), d_deriv))) return rc;
+    if (c->solids_active) {
+        EpsInArgs e{c->ns, d_solids, P<float>(c->ps_eps)};
+        if ((rc = launch(c, K_EPS_IN, nblk(c->ns), 256, e))) return rc;
+        SolidArgs sa = solid_base(c);
+        sa.Xposd = P<float>(c->Xb[1][0]);
+        sa.Xvelq = P<float>(c->Xb[1][1]);
+        sa.mk = P<uint32_t>(c->ps_mk[1]);
+        sa.pid = P<uint32_t>(c->ps_pid[1]);
+        sa.Xeps = P<float>(c->ps_eps);
+        sa.mode = 2;
+        sa.KdP = P<float>(kd[0]);
+        sa.KdV = P<float>(kd[1]);
+        if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
+        if ((rc = launch(c, K_SOLID, nblk(c->ns), 256, sa))) return rc;
+    }
+    if ((rc = soa_to_aos(c, P<float>(kd[0]), P<float>(kd[1]), P<uint32_t>(c->ps_pid[1]), d_deriv))) return rc;
     if (d_solid_deriv && d_solids) {
-        ClearSolidsArgs a{c->ns, d_solids, d_solid_deriv};
-        if ((rc = launch(c, K_CLEAR_SOLIDS, nblk(c->ns), 256, a))) return rc;
+        if (c->solids_active) {
+            // dest.solids (cleared) + dest2.solids {part_id, 0, plastic rate}
+            EpsOutArgs e{c->ns, P<float>(c->keps), nullptr, P<uint32_t>(c->part_d), d_solid_deriv};
+            if ((rc = launch(c, K_EPS_OUT, nblk(c->ns), 256, e))) return rc;
+        } else {
+            ClearSolidsArgs a{c->ns, d_solids, d_solid_deriv};
+            if ((rc = launch(c, K_CLEAR_SOLIDS, nblk(c->ns), 256, a))) return rc;
+        }
     }
     return ARP_OK;
 }

@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--shape", default="128,128,64", help="fluid lattice nx,ny,nz (y up)")
     ap.add_argument("--integrator", default="rk2")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
