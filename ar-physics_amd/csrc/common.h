@@ -115,7 +115,9 @@ struct RowSortArgs {
     const uint32_t* tmpSrc;
     uint32_t* gPid;  // scratch for rows larger than kRowCap
     uint32_t* gGrp;
-    uint32_t* rowOff;       // nrows * rowstride
+    uint32_t* rowOff;       // nrows * rowstride: global start of every cell of an occupied row
+    uint32_t* rowStamp;     // nrows: build stamp of the last build that found the row occupied
+    uint32_t stamp;
     const uint32_t* bP;     // boundary prefix per cell (ncells+1)
     const uint32_t* rP;     // reference prefix per cell (ncells+1)
     int full;
@@ -174,8 +176,11 @@ struct FluidArgs {
     const DMat* mats;
     const DInter* inter;
     const uint32_t* perm;
-    const uint32_t* binStart;
     const uint32_t* rowOff;
+    const uint32_t* rowStamp;
+    uint32_t stamp;
+    int identity;           // the build reordered the state: sorted slot == storage index
+    const uint32_t* sflag;  // per storage index: no bucket slot
     const uint32_t* bP;
     const float* bposd;  // boundary (static, cell-sorted, capped)
     const float* bvelq;
@@ -219,8 +224,9 @@ struct SolidArgs {
     float* keps;              // per solid: plastic strain rate (16)
     // current grid, for the hourglass stencil test and self contact
     const uint32_t* perm;
-    const uint32_t* binStart;
     const uint32_t* rowOff;
+    const uint32_t* rowStamp;
+    uint32_t stamp;
     const uint32_t* sflag;    // per storage index: 1 when refused by the cap / non-finite
     const uint32_t* pid;      // storage -> particle id
     const uint32_t* sidByPid;

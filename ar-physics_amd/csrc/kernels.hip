@@ -330,9 +330,10 @@ extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
         __syncthreads();
         uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
         for (uint32_t x = tid; x <= sx; x += 256) {
-            ro[x] = off[x];
+            ro[x] = s0 + off[x];
             if (x < sx) cnt[x] = 0;  // reused as fill counters
         }
+        if (tid == 0) a.rowStamp[row] = a.stamp;
         __syncthreads();
         for (uint32_t e = tid; e < ne; e += 256) {
             uint32_t x = big ? a.tmpKey[s0 + e] - row * sx : eX[e];
@@ -380,7 +381,7 @@ extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
 // ------------------------------------------------------------------------------------
 // fluid_forces (fluid.rs:95-251) with the RK stage epilogue fused in
 // ------------------------------------------------------------------------------------
-constexpr int kList = 16;  // within-h neighbours buffered per thread before a flush
+constexpr int kList = 24;  // within-h neighbours buffered per thread before a drain
 
 struct PairAcc {
     float den, fx, fy, fz;
@@ -477,9 +478,24 @@ struct PairIn {
     bool bnd;
 };
 
+#ifndef ARP_BATCH
+#define ARP_BATCH 8  // candidates whose loads are issued together
+#endif
+constexpr int kMatLds = 16;  // material / interaction tables cached in LDS up to this many
+
 extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
     __shared__ uint32_t list[kList * 256];
+    __shared__ float4 smat[kMatLds];                // mass, visc, bulk_visc
+    __shared__ DInter sint[kMatLds * kMatLds];
     const uint32_t tid = threadIdx.x;
+    const uint32_t nm = a.phys.nm;
+    const bool ldsTab = nm <= (uint32_t)kMatLds;
+    if (ldsTab) {
+        for (uint32_t m = tid; m < nm; m += 256)
+            smat[m] = make_float4(a.mats[m].mass, a.mats[m].visc, a.mats[m].bulk_visc, 0.f);
+        for (uint32_t m = tid; m < nm * nm; m += 256) sint[m] = a.inter[m];
+        __syncthreads();
+    }
     const uint32_t t = xcd_block() * 256u + tid;
     const bool act = t < a.n;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
@@ -491,9 +507,15 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
     uint32_t i = 0, nextBin = 0, nextKey = kInvalid;
     bool haveKey = false;
     if (act) {
-        const uint32_t pe = a.perm[t];
-        i = pe & ~kFlag;
-        const bool noNb = (pe & kFlag) != 0;
+        bool noNb;
+        if (a.identity) {
+            i = t;
+            noNb = a.sflag[t] != 0u;
+        } else {
+            const uint32_t pe = a.perm[t];
+            i = pe & ~kFlag;
+            noNb = (pe & kFlag) != 0;
+        }
         const float4 xi = X[i];
         const float4 vi = XV[i];
         const uint32_t mk1 = a.mk[i];
@@ -525,8 +547,21 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             const float4 xj = in.xj, vj = in.vj;
             const uint32_t mat2 = in.mk2 & kMatMask;
             const bool elastic2 = (in.mk2 & kMkElastic) != 0;
-            const DMat& M2 = a.mats[mat2];
-            const float m2 = M2.mass, d2 = xj.w, q2 = vj.w;
+            float m2, visc2, bulk2;
+            DInter it;
+            if (ldsTab) {
+                const float4 mm = smat[mat2];
+                m2 = mm.x;
+                visc2 = mm.y;
+                bulk2 = mm.z;
+                it = sint[mat1 * nm + mat2];
+            } else {
+                m2 = a.mats[mat2].mass;
+                visc2 = a.mats[mat2].visc;
+                bulk2 = a.mats[mat2].bulk_visc;
+                it = a.inter[mat1 * nm + mat2];
+            }
+            const float d2 = xj.w, q2 = vj.w;
             const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
             const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
             const float rr = rx * rx + ry * ry + rz * rz;
@@ -534,7 +569,6 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             const float gs = -6.0f * kn * diff * diff;
             const float gx = gs * rx, gy = gs * ry, gz = gs * rz;
             if (!bnd) acc.den += m2 * (vx * gx + vy * gy + vz * gz);
-            const DInter it = a.inter[mat1 * a.phys.nm + mat2];
             const float cp = it.potential ? contact_potential(it.potential, sqrtf(rr), it.radius, it.strength) : 0.0f;
             const float pf = m1 * m2 * (q1 + q2);
             acc.fx += pf * gx;
@@ -570,12 +604,12 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             if (!bnd) {
                 const float den_ = rr + a.phys.eps_hh;
                 if (!elastic1) {
-                    const float s = __fdividef(m1 * m2 * (f1 + M2.visc) * (rx * gx + ry * gy + rz * gz), d1 * d2 * den_);
+                    const float s = __fdividef(m1 * m2 * (f1 + visc2) * (rx * gx + ry * gy + rz * gz), d1 * d2 * den_);
                     acc.fx -= s * vx;
                     acc.fy -= s * vy;
                     acc.fz -= s * vz;
                 }
-                const float ff = mat1 == mat2 ? bf1 + M2.bulk_visc : it.dampening;
+                const float ff = mat1 == mat2 ? bf1 + bulk2 : it.dampening;
                 const float s = m1 * m2 * __fdividef(ff * (vx * rx + vy * ry + vz * rz), (d1 + d2) * den_);
                 acc.fx -= s * gx;
                 acc.fy -= s * gy;
@@ -588,73 +622,102 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
             const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
-            // 18 candidate ranges: (mobile, boundary) x 9 neighbour rows.  Candidates within h
-            // are buffered in LDS and drained at a single site of the pair loop whenever the
-            // buffer could overflow.
+            // metadata of one neighbour row: [mobile lo, hi), [boundary lo, hi); all four
+            // loads are independent, and the next row's are issued before this row is scanned
+            auto row_meta = [&](int rw, uint4& m) {
+                m = make_uint4(0, 0, 0, 0);
+                const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
+                if (rw < 9 && nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
+                    const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                    const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                    const size_t cb = (size_t)row * sx;
+                    const uint32_t st = a.rowStamp[row];
+                    const uint32_t lo = ro[xl], hi = ro[xh + 1];
+                    m.z = a.bP[cb + xl];
+                    m.w = a.bP[cb + xh + 1];
+                    if (st == a.stamp) {
+                        m.x = lo;
+                        m.y = hi;
+                    }
+                }
+            };
+            // Candidates within h are buffered in LDS and drained at a single site of the pair
+            // loop whenever the buffer could overflow.
             uint32_t nl = 0;
-            int range = -1;
-            uint32_t q = 0, qend = 0;
+            int rw = 0;
+            uint4 cur, nxtm;
+            row_meta(0, cur);
+            row_meta(1, nxtm);
+            int part = 0;  // 0: mobile segment, 1: boundary segment
+            uint32_t q = cur.x, qend = cur.y;
             for (;;) {
-                while (nl + 4 <= (uint32_t)kList) {
+                while (nl + ARP_BATCH <= (uint32_t)kList) {
                     if (q >= qend) {
-                        if (++range >= 18) break;
-                        const int rw = range >> 1;
-                        const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
-                        q = qend = 0;
-                        if (nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
-                            const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
-                            if (range & 1) {
-                                const size_t cb = (size_t)row * sx;
-                                q = a.bP[cb + xl];
-                                qend = a.bP[cb + xh + 1];
-                            } else {
-                                const uint32_t s0 = a.binStart[row], s1 = a.binStart[row + 1];
-                                if (s1 > s0) {
-                                    const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
-                                    q = s0 + ro[xl];
-                                    qend = s0 + ro[xh + 1];
-                                }
-                            }
+                        if (part == 0) {
+                            part = 1;
+                            q = cur.z;
+                            qend = cur.w;
+                            continue;
                         }
+                        if (++rw >= 9) break;
+                        cur = nxtm;
+                        row_meta(rw + 1, nxtm);
+                        part = 0;
+                        q = cur.x;
+                        qend = cur.y;
                         continue;
                     }
-                    // four candidates at a time: independent loads in flight together
-                    if (range & 1) {
-                        float4 xj[4];
+                    if (part) {
+                        float4 xj[ARP_BATCH];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) xj[u] = B[min(q + u, qend - 1)];
+                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = B[min(q + u, qend - 1)];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
+                        for (int u = 0; u < ARP_BATCH; u++) {
                             const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
                             if (q + u < qend && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
                                 list[(nl++) * 256 + tid] = (q + u) | kBnd;
                         }
+                    } else if (a.identity) {
+                        uint32_t fl[ARP_BATCH];
+                        float4 xj[ARP_BATCH];
+#pragma unroll
+                        for (int u = 0; u < ARP_BATCH; u++) {
+                            const uint32_t qq = min(q + u, qend - 1);
+                            fl[u] = a.sflag[qq];
+                            xj[u] = X[qq];
+                        }
+#pragma unroll
+                        for (int u = 0; u < ARP_BATCH; u++) {
+                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                            if (q + u < qend && !fl[u] && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
+                                list[(nl++) * 256 + tid] = q + u;
+                        }
                     } else {
-                        uint32_t pe2[4];
-                        float4 xj[4];
+                        uint32_t pe2[ARP_BATCH];
+                        float4 xj[ARP_BATCH];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) pe2[u] = (q + u < qend) ? a.perm[q + u] : kFlag;
+                        for (int u = 0; u < ARP_BATCH; u++) pe2[u] = (q + u < qend) ? a.perm[q + u] : kFlag;
 #pragma unroll
-                        for (int u = 0; u < 4; u++) xj[u] = X[pe2[u] & ~kFlag];
+                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = X[pe2[u] & ~kFlag];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
+                        for (int u = 0; u < ARP_BATCH; u++) {
                             const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
                             if (!(pe2[u] & kFlag) && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
                                 list[(nl++) * 256 + tid] = pe2[u];
                         }
                     }
-                    q += 4;
+                    q += ARP_BATCH;
                 }
-                uint32_t k = 0;
-                for (; k + 1 < nl; k += 2) {
-                    const PairIn p0 = load(list[k * 256 + tid]);
-                    const PairIn p1 = load(list[(k + 1) * 256 + tid]);
-                    pair(p0);
-                    pair(p1);
+                // next neighbour's loads are issued before the current pair's arithmetic
+                PairIn nxt;
+                if (nl) nxt = load(list[tid]);
+                for (uint32_t k = 0; k < nl; k++) {
+                    const PairIn cur2 = nxt;
+                    if (k + 1 < nl) nxt = load(list[(k + 1) * 256 + tid]);
+                    pair(cur2);
                 }
-                if (k < nl) pair(load(list[k * 256 + tid]));
                 nl = 0;
-                if (range >= 18) break;
+                if (rw >= 9) break;
             }
         }
 
@@ -1143,10 +1206,9 @@ extern "C" __global__ void __launch_bounds__(256) k_solid(SolidArgs a) {
                     const int ny = (int)ci[1] + dy;
                     if (ny < 0 || ny >= (int)sy) continue;
                     const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
-                    const uint32_t s0 = a.binStart[row], s1 = a.binStart[row + 1];
-                    if (s1 <= s0) continue;
+                    if (a.rowStamp[row] != a.stamp) continue;
                     const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
-                    for (uint32_t q = s0 + ro[xl]; q < s0 + ro[xh + 1]; q++) {
+                    for (uint32_t q = ro[xl]; q < ro[xh + 1]; q++) {
                         const uint32_t pe = a.perm[q];
                         if (pe & kFlag) continue;
                         const uint32_t mk2 = a.mk[pe];

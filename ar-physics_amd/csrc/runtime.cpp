@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -95,7 +96,9 @@ struct arp_ctx {
     DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys, saos_tmp;
     // grid
     DBuf keys, ranks, binCount, binStart, activeRows, numActive, tmpKey, tmpSrc, perm, rowOff,
-        gPid, gGrp, stats, scanState, scanTicket, scanEpoch;
+        gPid, gGrp, stats, scanState, scanTicket, scanEpoch, rowStamp;
+    uint32_t stamp = 0;     // build counter written into rowStamp by the row sort
+    bool last_full = false;  // the last build reordered the state (perm is the identity)
     bool keys_valid = false;
     // what the last build described (for the dump)
     const DBuf* last_pid = nullptr;
@@ -121,7 +124,7 @@ struct arp_ctx {
                 &Xb[1][0], &Xb[1][1],  &Acc[0],     &Acc[1],   &inv,    &ps_mk[0], &ps_mk[1],
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
-                &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch};
+                &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
             bufs.push_back(&K[k][1]);
@@ -464,6 +467,9 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.gPid = P<uint32_t>(c->gPid);
     r.gGrp = P<uint32_t>(c->gGrp);
     r.rowOff = P<uint32_t>(c->rowOff);
+    r.rowStamp = P<uint32_t>(c->rowStamp);
+    r.stamp = ++c->stamp;
+    c->last_full = io.full != 0;
     r.bP = P<uint32_t>(c->bP);
     r.rP = P<uint32_t>(c->rP);
     r.full = io.full;
@@ -491,8 +497,11 @@ FluidArgs fluid_base(arp_ctx* c) {
     f.mats = P<DMat>(c->mats);
     f.inter = P<DInter>(c->inter);
     f.perm = P<uint32_t>(c->perm);
-    f.binStart = P<uint32_t>(c->binStart);
     f.rowOff = P<uint32_t>(c->rowOff);
+    f.rowStamp = P<uint32_t>(c->rowStamp);
+    f.stamp = c->stamp;
+    f.identity = c->last_full ? 1 : 0;
+    f.sflag = P<uint32_t>(c->sflag);
     f.bP = P<uint32_t>(c->bP);
     f.bposd = P<float>(c->bposd);
     f.bvelq = P<float>(c->bvelq);
@@ -546,6 +555,9 @@ int alloc_state(arp_ctx* c) {
         HIPCHK(c, hipMemsetAsync(c->scanEpoch.p, 0, 16, c->stream));
     }
     if ((rc = dalloc(c, c->rowOff, 4 * (size_t)c->grid.nrows * c->grid.rowstride))) return rc;
+    if ((rc = dalloc(c, c->rowStamp, 4 * (size_t)c->grid.nrows))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->rowStamp.p, 0, 4 * (size_t)c->grid.nrows, c->stream));
+    c->stamp = 0;
     return ARP_OK;
 }
 
@@ -599,8 +611,9 @@ SolidArgs solid_base(arp_ctx* c) {
     a.Fm = P<float>(c->Fm_d);
     a.keps = P<float>(c->keps);
     a.perm = P<uint32_t>(c->perm);
-    a.binStart = P<uint32_t>(c->binStart);
     a.rowOff = P<uint32_t>(c->rowOff);
+    a.rowStamp = P<uint32_t>(c->rowStamp);
+    a.stamp = c->stamp;
     a.sflag = P<uint32_t>(c->sflag);
     a.sidByPid = P<uint32_t>(c->sidByPid);
     a.KFp = P<float>(c->KF[0]);
@@ -770,7 +783,26 @@ int arp_create(int device, uint32_t flags, arp_ctx** out) {
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreate(&c->own_stream);
-    if (e == hipSuccess) e = hipModuleLoadData(&c->mod, arp_kernels_co);
+    // ARP_CODE_OBJECT: load a separately built gfx950 code object of the same kernels
+    // (tuning variants, e.g. another ARP_BATCH) instead of the embedded one
+    std::vector<char> co;
+    if (const char* path = getenv("ARP_CODE_OBJECT")) {
+        FILE* f = fopen(path, "rb");
+        if (f) {
+            fseek(f, 0, SEEK_END);
+            long sz = ftell(f);
+            fseek(f, 0, SEEK_SET);
+            co.resize(sz > 0 ? (size_t)sz : 0);
+            if (sz > 0 && fread(co.data(), 1, (size_t)sz, f) != (size_t)sz) co.clear();
+            fclose(f);
+        }
+        if (co.empty()) {
+            fprintf(stderr, "arp_create: cannot read ARP_CODE_OBJECT=%s\n", path);
+            delete c;
+            return ARP_E_ARG;
+        }
+    }
+    if (e == hipSuccess) e = hipModuleLoadData(&c->mod, co.empty() ? (const void*)arp_kernels_co : (const void*)co.data());
     for (int k = 0; e == hipSuccess && k < K_COUNT; k++) e = hipModuleGetFunction(&c->fn[k], c->mod, kKernNames[k]);
     if (e != hipSuccess) {
         fprintf(stderr, "arp_create: %s\n", hipGetErrorString(e));
@@ -1341,7 +1373,7 @@ int arp_neighbor_dump(arp_ctx* c, uint32_t* cell, uint32_t* cand_count, uint32_t
                 uint32_t s0 = bs[row], s1 = bs[row + 1];
                 if (s1 > s0) {
                     const uint32_t* r = &ro[(size_t)row * c->grid.rowstride];
-                    for (uint32_t q = s0 + r[xl]; q < s0 + r[xh + 1]; q++)
+                    for (uint32_t q = r[xl]; q < r[xh + 1]; q++)
                         if (!(perm[q] & kFlag)) L.push_back(pid[perm[q] & ~kFlag]);
                 }
                 size_t cb = (size_t)row * sx;
