@@ -479,7 +479,7 @@ struct PairIn {
 };
 
 #ifndef ARP_BATCH
-#define ARP_BATCH 8  // candidates whose loads are issued together
+#define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
 constexpr int kMatLds = 16;  // material / interaction tables cached in LDS up to this many
 
@@ -603,7 +603,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             }
             if (!bnd) {
                 const float den_ = rr + a.phys.eps_hh;
-                if (!elastic1) {
+                if (!elastic1 && f1 + visc2 != 0.0f) {
                     const float s = __fdividef(m1 * m2 * (f1 + visc2) * (rx * gx + ry * gy + rz * gz), d1 * d2 * den_);
                     acc.fx -= s * vx;
                     acc.fy -= s * vy;
@@ -670,7 +670,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
                     if (part) {
                         float4 xj[ARP_BATCH];
 #pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = B[min(q + u, qend - 1)];
+                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = B[q + u];
 #pragma unroll
                         for (int u = 0; u < ARP_BATCH; u++) {
                             const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
@@ -682,14 +682,14 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
                         float4 xj[ARP_BATCH];
 #pragma unroll
                         for (int u = 0; u < ARP_BATCH; u++) {
-                            const uint32_t qq = min(q + u, qend - 1);
-                            fl[u] = a.sflag[qq];
-                            xj[u] = X[qq];
+                            fl[u] = a.sflag[q + u];
+                            xj[u] = X[q + u];
                         }
 #pragma unroll
                         for (int u = 0; u < ARP_BATCH; u++) {
                             const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                            if (q + u < qend && !fl[u] && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
+                            // the self pair contributes exactly zero (grad W(0) = 0)
+                            if (q + u < qend && !fl[u] && q + u != i && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
                                 list[(nl++) * 256 + tid] = q + u;
                         }
                     } else {
@@ -702,7 +702,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
 #pragma unroll
                         for (int u = 0; u < ARP_BATCH; u++) {
                             const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                            if (!(pe2[u] & kFlag) && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
+                            if (!(pe2[u] & kFlag) && pe2[u] != i && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
                                 list[(nl++) * 256 + tid] = pe2[u];
                         }
                     }

@@ -46,6 +46,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_velocity", "k_fill",   "k_clear_solids", "k_strain",
                                    "k_solid",   "k_eps_in",  "k_eps_out"};
 
+constexpr size_t kPad = 16;  // elements of slack after every per-particle array
+
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -510,7 +512,8 @@ FluidArgs fluid_base(arp_ctx* c) {
 }
 
 int alloc_state(arp_ctx* c) {
-    size_t n = std::max<uint32_t>(c->n, 1);
+    // + kPad: k_fluid reads candidate batches past the end of a range (values unused)
+    size_t n = std::max<uint32_t>(c->n, 1) + kPad;
     int rc = 0;
     for (int k = 0; k < 2; k++) {
         if ((rc = dalloc(c, c->Ysort[k], 16 * n))) return rc;
@@ -562,7 +565,7 @@ int alloc_state(arp_ctx* c) {
 }
 
 int ensure_k(arp_ctx* c, int slots, bool acc) {
-    size_t n = std::max<uint32_t>(c->n, 1);
+    size_t n = std::max<uint32_t>(c->n, 1) + kPad;
     int rc;
     for (int k = 0; k < slots; k++)
         for (int j = 0; j < 2; j++)
@@ -1031,8 +1034,8 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
         std::vector<uint32_t> kept;
         cell_sort(ncells, key, {}, c->bP_h, kept);
         c->b_orig = kept;
-        std::vector<float> bp(4 * kept.size() + 4), bv(4 * kept.size() + 4);
-        std::vector<uint32_t> bm(kept.size() + 1);
+        std::vector<float> bp(4 * (kept.size() + kPad)), bv(4 * (kept.size() + kPad));
+        std::vector<uint32_t> bm(kept.size() + kPad);
         for (size_t s = 0; s < kept.size(); s++) {
             const ArpParticle& b = boundary[kept[s]];
             const ArpMaterial& m = mats[b.mat];
