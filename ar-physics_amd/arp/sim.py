@@ -112,6 +112,22 @@ class FluidSim:
         check(self.ctx, lib().arp_velocity(self.ctx, float(r), self._dp(d_in), self._dp(d_solid_in),
                                            self._dp(d_out), self._dp(d_solid_out)))
 
+    # ---- reverse mode (device std430 buffers as torch tensors) ---------------------------
+    def adjoint_record(self, dt, nsteps, checkpoint_every=None):
+        """Advance nsteps steps of length dt, recording for adjoint_backward (a checkpoint
+        every checkpoint_every steps; default: the whole trajectory in one segment)."""
+        k = nsteps if checkpoint_every is None else checkpoint_every
+        check(self.ctx, lib().arp_adjoint_record(self.ctx, float(dt), int(nsteps), int(k)))
+
+    def adjoint_backward(self, d_grad_out, d_grad_in, want_params=True):
+        """d_grad_out / d_grad_in: device tensors of n Particle records (dL/d den, pos, vel
+        after / before the recorded steps).  Returns an (nmat, 2) array of dL/d
+        (bulk_viscosity, viscosity) per material, or None."""
+        nm = len(self.scene.materials)
+        gp = np.zeros(2 * nm, np.float32) if want_params else None
+        check(self.ctx, lib().arp_adjoint_backward(self.ctx, self._dp(d_grad_out), self._dp(d_grad_in), _p(gp)))
+        return gp.reshape(nm, 2) if want_params else None
+
     # ---- diagnostics ------------------------------------------------------------------
     def grid_info(self):
         g = np.zeros((), wire.GRID_INFO)

@@ -66,6 +66,7 @@ struct KeyArgs {
     uint32_t* binCount;
 };
 
+constexpr int kMatLds = 16;  // material / interaction tables cached in LDS up to this many
 constexpr int kScanTile = 1024;  // bins per workgroup of the row-count scan
 
 struct ScanArgs {
@@ -103,6 +104,7 @@ struct ScatterArgs {
     uint32_t* dstPid;
     uint32_t* inv;  // pid -> storage index (full mode), may be null
     uint32_t* sflag;
+    uint32_t* srcOut;  // full mode: previous storage index of every new slot (may be null)
 };
 
 struct RowSortArgs {
@@ -133,6 +135,7 @@ struct RowSortArgs {
     uint32_t* inv;
     uint32_t* stats;  // [0] overflowed particles (atomic), may be null
     uint32_t* sflag;  // per storage index: cap/non-finite flag (may be null)
+    uint32_t* srcOut;
 };
 
 // RK stage epilogue: k -> next stage input / y_{n+1}, the b-weighted accumulator and the
@@ -237,6 +240,76 @@ struct SolidArgs {
     float* KdP;
     float* KdV;
     RkArgs rk;
+};
+
+// Adjoint of one fluid stage: dL/dX_s = J_f(X_s)^T w_s (gather over the stage's neighbour
+// structure), accumulated into the step cotangent, plus per-material parameter partials.
+struct AdjArgs {
+    GridP grid;
+    PhysP phys;
+    uint32_t n;
+    const DMat* mats;
+    const DInter* inter;
+    const uint32_t* perm;
+    const uint32_t* rowOff;
+    const uint32_t* rowStamp;
+    uint32_t stamp;
+    int identity;
+    const uint32_t* sflag;
+    const uint32_t* bP;
+    const float* bposd;
+    const float* bvelq;
+    const uint32_t* bmk;
+    const float* Xposd;
+    const float* Xvelq;
+    const uint32_t* mk;
+    const float* Wp;   // cotangent of k: (pos'.xyz, den')
+    const float* Wv;   // (vel'.xyz, 0)
+    float* Gp;         // dL/dX: (x.xyz, rho)
+    float* Gv;         // (v.xyz, 0)
+    float* LamP;       // step cotangent accumulator (+= G)
+    float* LamV;
+    float* pgrad;      // per block: nm * 2 (d bulk_visc, d visc)
+};
+
+// W = cl * lambda + sum_t c_t * G_t (stage cotangent of k)
+struct AdjWArgs {
+    uint32_t n;
+    float cl;
+    const float* LP;
+    const float* LV;
+    int ng;
+    const float* GP[kMaxK + 2];
+    const float* GV[kMaxK + 2];
+    float cg[kMaxK + 2];
+    float* WP;
+    float* WV;
+};
+
+// out[src[t]] = in[t] (cotangent back to the previous storage order), or out = in
+struct AdjPermArgs {
+    uint32_t n;
+    const uint32_t* src;
+    const float* inP;
+    const float* inV;
+    float* outP;
+    float* outV;
+};
+
+// cotangent AoS (ArpParticle records holding dL/d(den, pos, vel)) <-> SoA in storage order
+struct GradAosArgs {
+    uint32_t n;
+    int to_aos;
+    const uint32_t* pid;
+    void* aos;
+    float* P;
+    float* V;
+};
+
+struct ReduceArgs {
+    uint32_t nblocks, width;
+    const float* part;  // nblocks * width
+    float* out;         // width, accumulated (+=)
 };
 
 struct AosInArgs {

@@ -270,6 +270,7 @@ extern "C" __global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) {
         if (a.full) {
             a.perm[dst] = dst | kFlag;
             move_particle(a, i, dst);
+            if (a.srcOut) a.srcOut[dst] = i;
         } else {
             a.perm[dst] = i | kFlag;
         }
@@ -368,6 +369,7 @@ extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
                 a.dstMk[dst] = a.srcMk[src];
                 a.dstPid[dst] = p;
                 if (a.inv) a.inv[p] = dst;
+                if (a.srcOut) a.srcOut[dst] = src;
             } else {
                 a.perm[dst] = src | flag;
             }
@@ -481,7 +483,6 @@ struct PairIn {
 #ifndef ARP_BATCH
 #define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
-constexpr int kMatLds = 16;  // material / interaction tables cached in LDS up to this many
 
 extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
     __shared__ uint32_t list[kList * 256];
@@ -1276,4 +1277,420 @@ extern "C" __global__ void __launch_bounds__(256) k_eps_out(EpsOutArgs a) {
     const float4* e = reinterpret_cast<const float4*>(a.eps + 16 * (size_t)s);
     for (int q = 0; q < 4; q++) o.stress[q] = e[q];
     reinterpret_cast<AosSolid*>(a.aos)[s] = o;
+}
+
+// ------------------------------------------------------------------------------------
+// adjoint of the fluid stage (reverse mode of fluid.rs:153-247)
+// ------------------------------------------------------------------------------------
+
+// d/dL of contact_potential(eq, L, strength, r0) (fluid.rs:54-67)
+__device__ __forceinline__ float contact_potential_d(uint32_t eq, float r, float strength, float r0) {
+    switch (eq) {
+        case 2: return -strength;
+        case 3: {
+            const float f = powf(r0 / r, 7.0f);
+            return (strength / 7.0f) * (f - 1.0f) - strength * f;
+        }
+        case 4: {
+            float f = r0 / r;
+            f *= f * f;
+            f *= f;
+            return strength * (2.0f * f - 2.0f) * (-6.0f * f / r);
+        }
+        case 5: return strength * powf(r / r0, 6.0f);
+        default: return 0.0f;
+    }
+}
+
+// d q / d rho with q = p(rho) / rho^2
+__device__ __forceinline__ float pressure_q_d(const DMat& m, float den) {
+    float p = eos_pressure(m.eq, den, m.c, m.target_den), dp = 0.0f;
+    if (m.eq == 2) dp = m.c * m.c;
+    if (m.eq == 3) dp = m.c * m.c * powf(den / m.target_den, 6.0f);
+    return dp / (den * den) - 2.0f * p / (den * den * den);
+}
+
+struct PairSide {
+    float mi, mj, rhoi, rhoj, qi, qj, dqi, dqj, fi, fj, bfi, bfj;
+    bool eli, elj, bnd, same;
+    uint32_t mati, matj;
+};
+
+struct PairVjp {
+    float gr[3], gv[3], gri, grj;
+    float F[3];
+    float dbulk, dvisc;
+};
+
+// Gradient of a.(pair force) + wd*(pair density rate) for target i, neighbour j with
+// r = x_j - x_i, v = v_j - v_i.  Zero outside the kernel support.
+__device__ void pair_vjp(const PairSide& P, const DInter& it, float rx, float ry, float rz, float vx, float vy,
+                         float vz, float ax, float ay, float az, float wd, float hh, float kn, float eps_hh,
+                         PairVjp& o) {
+    for (int c = 0; c < 3; c++) o.gr[c] = o.gv[c] = o.F[c] = 0.0f;
+    o.gri = o.grj = o.dbulk = o.dvisc = 0.0f;
+    const float rr = rx * rx + ry * ry + rz * rz;
+    const float d = hh - rr;
+    if (!(d > 0.0f)) return;
+    const float gs = -6.0f * kn * d * d, dgs = 12.0f * kn * d;  // dgs = d gs / d rr
+    const float r[3] = {rx, ry, rz}, v[3] = {vx, vy, vz}, A[3] = {ax, ay, az};
+    const float ar = ax * rx + ay * ry + az * rz, vr = vx * rx + vy * ry + vz * rz, av = ax * vx + ay * vy + az * vz;
+    const float C = P.mi * P.mj;
+    if (!P.bnd) {  // density rate: wd * mj * gs * (v.r)
+        const float c1 = wd * P.mj;
+        for (int c = 0; c < 3; c++) {
+            o.gv[c] += c1 * gs * r[c];
+            o.gr[c] += c1 * (gs * v[c] + 2.0f * dgs * vr * r[c]);
+        }
+    }
+    {  // pressure: C (qi + qj) grad W
+        const float Q = P.qi + P.qj;
+        for (int c = 0; c < 3; c++) {
+            o.F[c] += C * Q * gs * r[c];
+            o.gr[c] += C * Q * (gs * A[c] + 2.0f * dgs * ar * r[c]);
+        }
+        o.gri += C * P.dqi * gs * ar;
+        o.grj += C * P.dqj * gs * ar;
+    }
+    if (it.potential) {  // contact (radius, strength swapped as in fluid.rs:177) + friction
+        const float L = sqrtf(rr);
+        const float cp = contact_potential(it.potential, L, it.radius, it.strength);
+        if (cp != 0.0f) {
+            const float dcp = contact_potential_d(it.potential, L, it.radius, it.strength);
+            const float cs = P.mj * cp / P.rhoj;
+            const float iL = 1.0f / L;
+            for (int c = 0; c < 3; c++) {
+                o.F[c] += cs * gs * r[c];
+                o.gr[c] += (P.mj / P.rhoj) * (dcp * r[c] * iL * gs * ar + cp * (gs * A[c] + 2.0f * dgs * ar * r[c]));
+            }
+            o.grj += -cs * gs * ar / P.rhoj;
+            if ((P.eli || P.elj || P.bnd) && P.mati != P.matj && it.friction != 0.0f) {
+                float t[3];
+                for (int c = 0; c < 3; c++) t[c] = v[c] - r[c] * vr / rr;
+                const float tl = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                if (tl > 0.0f && isfinite(tl)) {
+                    const float mu = it.friction;
+                    const float c0 = cs * gs;
+                    const float N = fabsf(c0) * L;
+                    float th[3];
+                    for (int c = 0; c < 3; c++) th[c] = t[c] / tl;
+                    const float at = A[0] * th[0] + A[1] * th[1] + A[2] * th[2];
+                    float ut[3];
+                    for (int c = 0; c < 3; c++) {
+                        o.F[c] += mu * N * th[c];
+                        ut[c] = mu * N * (A[c] - th[c] * at) / tl;
+                    }
+                    const float rut = r[0] * ut[0] + r[1] * ut[1] + r[2] * ut[2];
+                    const float sg = c0 >= 0.0f ? 1.0f : -1.0f;
+                    for (int c = 0; c < 3; c++) {
+                        o.gv[c] += ut[c] - r[c] * rut / rr;
+                        o.gr[c] += -(vr / rr * ut[c] + v[c] * rut / rr - 2.0f * vr * rut * r[c] / (rr * rr));
+                        const float dc0 = (P.mj / P.rhoj) * (dcp * r[c] * iL * gs + cp * 2.0f * dgs * r[c]);
+                        o.gr[c] += mu * at * sg * (L * dc0 + c0 * r[c] * iL);
+                    }
+                    o.grj += mu * at * (-N / P.rhoj);
+                }
+            }
+        }
+    }
+    if (!P.bnd) {
+        const float D = rr + eps_hh;
+        if (!P.eli && P.fi + P.fj != 0.0f) {  // viscosity: -C (fi+fj) gs rr / (rhoi rhoj D) v
+            const float u = gs * rr / D;
+            const float du = (dgs * rr + gs) / D - gs * rr / (D * D);
+            const float K4 = C * (P.fi + P.fj) / (P.rhoi * P.rhoj);
+            const float s4 = K4 * u;
+            for (int c = 0; c < 3; c++) {
+                o.F[c] -= s4 * v[c];
+                o.gv[c] -= s4 * A[c];
+                o.gr[c] -= av * K4 * du * 2.0f * r[c];
+            }
+            o.gri += av * s4 / P.rhoi;
+            o.grj += av * s4 / P.rhoj;
+        }
+        if (!P.eli) o.dvisc += -av * C * (gs * rr / D) / (P.rhoi * P.rhoj);
+        // artificial viscosity: -C ff (v.r)/((rhoi+rhoj) D) grad W
+        const float ff = P.same ? P.bfi + P.bfj : it.dampening;
+        const float E0 = -C / (P.rhoi + P.rhoj);
+        const float E = E0 * ff;
+        const float h5 = gs / D, dh5 = dgs / D - gs / (D * D);
+        const float phi = E * vr * ar * h5;
+        for (int c = 0; c < 3; c++) {
+            o.F[c] += E * vr * h5 * r[c];
+            o.gv[c] += E * ar * h5 * r[c];
+            o.gr[c] += E * (ar * h5 * v[c] + vr * h5 * A[c] + vr * ar * dh5 * 2.0f * r[c]);
+        }
+        o.gri += -phi / (P.rhoi + P.rhoj);
+        o.grj += -phi / (P.rhoi + P.rhoj);
+        if (P.same) o.dbulk += E0 * vr * ar * h5;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
+    __shared__ uint32_t list[kList * 256];
+    __shared__ float spg[2 * kMatLds];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nm = a.phys.nm;
+    for (uint32_t m = tid; m < 2 * kMatLds; m += 256) spg[m] = 0.0f;
+    __syncthreads();
+    const uint32_t t = xcd_block() * 256u + tid;
+    const float4* X = reinterpret_cast<const float4*>(a.Xposd);
+    const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
+    const float4* B = reinterpret_cast<const float4*>(a.bposd);
+    const float4* BV = reinterpret_cast<const float4*>(a.bvelq);
+    const float4* WP = reinterpret_cast<const float4*>(a.Wp);
+    const float4* WV = reinterpret_cast<const float4*>(a.Wv);
+    const float h = a.phys.h, hh = h * h, kn = a.phys.k;
+    float pb[kMatLds], pv[kMatLds];  // per-thread parameter partials (nm <= kMatLds)
+    for (int m = 0; m < kMatLds; m++) pb[m] = pv[m] = 0.0f;
+    if (t < a.n) {
+        uint32_t i;
+        bool noNb;
+        if (a.identity) {
+            i = t;
+            noNb = a.sflag[t] != 0u;
+        } else {
+            const uint32_t pe = a.perm[t];
+            i = pe & ~kFlag;
+            noNb = (pe & kFlag) != 0;
+        }
+        const float4 xi = X[i], vi = XV[i];
+        const uint32_t mk1 = a.mk[i];
+        const uint32_t mat1 = mk1 & kMatMask;
+        const DMat& M1 = a.mats[mat1];
+        const float rho1 = xi.w;
+        const float4 wp = WP[i], wv = WV[i];
+        const float ir1 = 1.0f / rho1;
+        const float a1x = wv.x * ir1, a1y = wv.y * ir1, a1z = wv.z * ir1;
+        const float dq1 = pressure_q_d(M1, rho1);
+        float G[3] = {0, 0, 0}, Gv[3] = {0, 0, 0}, Gr = 0.0f, F[3] = {0, 0, 0};
+        if (!noNb) {
+            uint32_t cx, cy, cz;
+            cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
+            const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
+            const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
+            uint32_t nl = 0;
+            int rw = -1, part = 1;
+            uint32_t q = 0, qend = 0;
+            uint4 cur = make_uint4(0, 0, 0, 0);
+            for (;;) {
+                while (nl < (uint32_t)kList) {
+                    if (q >= qend) {
+                        if (part == 0) {
+                            part = 1;
+                            q = cur.z;
+                            qend = cur.w;
+                            continue;
+                        }
+                        if (++rw >= 9) break;
+                        part = 0;
+                        cur = make_uint4(0, 0, 0, 0);
+                        const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
+                        if (nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
+                            const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                            const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                            const size_t cb = (size_t)row * sx;
+                            if (a.rowStamp[row] == a.stamp) {
+                                cur.x = ro[xl];
+                                cur.y = ro[xh + 1];
+                            }
+                            cur.z = a.bP[cb + xl];
+                            cur.w = a.bP[cb + xh + 1];
+                        }
+                        q = cur.x;
+                        qend = cur.y;
+                        continue;
+                    }
+                    if (part) {
+                        const float4 xj = B[q];
+                        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                        if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) list[(nl++) * 256 + tid] = q | kBnd;
+                    } else {
+                        const uint32_t pe2 = a.identity ? (a.sflag[q] ? (q | kFlag) : q) : a.perm[q];
+                        if (!(pe2 & kFlag) && pe2 != i) {
+                            const float4 xj = X[pe2];
+                            const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                            if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) list[(nl++) * 256 + tid] = pe2;
+                        }
+                    }
+                    q++;
+                }
+                for (uint32_t k = 0; k < nl; k++) {
+                    const uint32_t e = list[k * 256 + tid];
+                    const bool bnd = (e & kBnd) != 0;
+                    const uint32_t j = e & ~kBnd;
+                    const float4 xj = bnd ? B[j] : X[j];
+                    const float4 vj = bnd ? BV[j] : XV[j];
+                    const uint32_t mk2 = bnd ? a.bmk[j] : a.mk[j];
+                    const uint32_t mat2 = mk2 & kMatMask;
+                    const DMat& M2 = a.mats[mat2];
+                    PairSide P;
+                    P.mi = M1.mass;
+                    P.mj = M2.mass;
+                    P.rhoi = rho1;
+                    P.rhoj = xj.w;
+                    P.qi = vi.w;
+                    P.qj = vj.w;
+                    P.dqi = dq1;
+                    P.dqj = 0.0f;
+                    P.fi = M1.visc;
+                    P.fj = M2.visc;
+                    P.bfi = M1.bulk_visc;
+                    P.bfj = M2.bulk_visc;
+                    P.eli = (mk1 & kMkElastic) != 0;
+                    P.elj = (mk2 & kMkElastic) != 0;
+                    P.bnd = bnd;
+                    P.same = mat1 == mat2;
+                    P.mati = mat1;
+                    P.matj = mat2;
+                    PairVjp o;
+                    // side a: this particle's own terms
+                    pair_vjp(P, a.inter[mat1 * nm + mat2], xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x,
+                             vj.y - vi.y, vj.z - vi.z, a1x, a1y, a1z, wp.w, hh, kn, a.phys.eps_hh, o);
+                    for (int c = 0; c < 3; c++) {
+                        G[c] -= o.gr[c];
+                        Gv[c] -= o.gv[c];
+                        F[c] += o.F[c];
+                    }
+                    Gr += o.gri;
+                    if (nm <= (uint32_t)kMatLds) {
+                        pb[mat1] += o.dbulk;
+                        pb[mat2] += o.dbulk;
+                        pv[mat1] += o.dvisc;
+                        pv[mat2] += o.dvisc;
+                    }
+                    if (!bnd) {
+                        // side b: the neighbour's terms as seen from this particle
+                        const float4 wpj = WP[j], wvj = WV[j];
+                        const float irj = 1.0f / xj.w;
+                        PairSide Q;
+                        Q.mi = M2.mass;
+                        Q.mj = M1.mass;
+                        Q.rhoi = xj.w;
+                        Q.rhoj = rho1;
+                        Q.qi = vj.w;
+                        Q.qj = vi.w;
+                        Q.dqi = 0.0f;
+                        Q.dqj = dq1;
+                        Q.fi = M2.visc;
+                        Q.fj = M1.visc;
+                        Q.bfi = M2.bulk_visc;
+                        Q.bfj = M1.bulk_visc;
+                        Q.eli = P.elj;
+                        Q.elj = P.eli;
+                        Q.bnd = false;
+                        Q.same = P.same;
+                        Q.mati = mat2;
+                        Q.matj = mat1;
+                        pair_vjp(Q, a.inter[mat2 * nm + mat1], xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, vi.x - vj.x,
+                                 vi.y - vj.y, vi.z - vj.z, wvj.x * irj, wvj.y * irj, wvj.z * irj, wpj.w, hh, kn,
+                                 a.phys.eps_hh, o);
+                        for (int c = 0; c < 3; c++) {
+                            G[c] += o.gr[c];
+                            Gv[c] += o.gv[c];
+                        }
+                        Gr += o.grj;
+                    }
+                }
+                nl = 0;
+                if (rw >= 9) break;
+            }
+        }
+        // pos' = v; vel' = F/rho + g
+        Gv[0] += wp.x;
+        Gv[1] += wp.y;
+        Gv[2] += wp.z;
+        Gr += -(wv.x * F[0] + wv.y * F[1] + wv.z * F[2]) * ir1 * ir1;
+        const float4 gp = make_float4(G[0], G[1], G[2], Gr), gv = make_float4(Gv[0], Gv[1], Gv[2], 0.0f);
+        reinterpret_cast<float4*>(a.Gp)[i] = gp;
+        reinterpret_cast<float4*>(a.Gv)[i] = gv;
+        float4* LP = reinterpret_cast<float4*>(a.LamP) + i;
+        float4* LV = reinterpret_cast<float4*>(a.LamV) + i;
+        float4 lp = *LP, lv = *LV;
+        lp.x += gp.x; lp.y += gp.y; lp.z += gp.z; lp.w += gp.w;
+        lv.x += gv.x; lv.y += gv.y; lv.z += gv.z;
+        *LP = lp;
+        *LV = lv;
+    }
+    // parameter partials: block sum in LDS, one row of partials per block
+    if (nm <= (uint32_t)kMatLds) {
+        for (uint32_t m = 0; m < nm; m++) {
+            float b = pb[m], v = pv[m];
+            for (int o = 32; o >= 1; o >>= 1) {
+                b += __shfl_xor(b, o);
+                v += __shfl_xor(v, o);
+            }
+            if (__lane_id() == 0) {
+                atomicAdd(&spg[2 * m], b);
+                atomicAdd(&spg[2 * m + 1], v);
+            }
+        }
+        __syncthreads();
+        for (uint32_t m = tid; m < 2 * nm; m += 256) a.pgrad[(size_t)blockIdx.x * 2 * nm + m] = spg[m];
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_adj_w(AdjWArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f), v = p;
+    if (a.cl != 0.0f) {
+        const float4 lp = reinterpret_cast<const float4*>(a.LP)[i], lv = reinterpret_cast<const float4*>(a.LV)[i];
+        p = make_float4(a.cl * lp.x, a.cl * lp.y, a.cl * lp.z, a.cl * lp.w);
+        v = make_float4(a.cl * lv.x, a.cl * lv.y, a.cl * lv.z, 0.f);
+    }
+    for (int k = 0; k < a.ng; k++) {
+        const float4 gp = reinterpret_cast<const float4*>(a.GP[k])[i], gv = reinterpret_cast<const float4*>(a.GV[k])[i];
+        const float c = a.cg[k];
+        p.x += c * gp.x; p.y += c * gp.y; p.z += c * gp.z; p.w += c * gp.w;
+        v.x += c * gv.x; v.y += c * gv.y; v.z += c * gv.z;
+    }
+    reinterpret_cast<float4*>(a.WP)[i] = p;
+    reinterpret_cast<float4*>(a.WV)[i] = v;
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_adj_perm(AdjPermArgs a) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= a.n) return;
+    const uint32_t d = a.src ? a.src[t] : t;
+    reinterpret_cast<float4*>(a.outP)[d] = reinterpret_cast<const float4*>(a.inP)[t];
+    reinterpret_cast<float4*>(a.outV)[d] = reinterpret_cast<const float4*>(a.inV)[t];
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_grad_aos(GradAosArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    AosParticle* A = reinterpret_cast<AosParticle*>(a.aos) + a.pid[i];
+    float4* P = reinterpret_cast<float4*>(a.P) + i;
+    float4* V = reinterpret_cast<float4*>(a.V) + i;
+    if (a.to_aos) {
+        const float4 p = *P, v = *V;
+        AosParticle o;
+        o.den = p.w;
+        o.mat = 0;
+        o.solid_id = 0;
+        o.pad = 0;
+        o.pos = make_float4(p.x, p.y, p.z, 0.f);
+        o.vel = make_float4(v.x, v.y, v.z, 0.f);
+        *A = o;
+    } else {
+        const AosParticle o = *A;
+        *P = make_float4(o.pos.x, o.pos.y, o.pos.z, o.den);
+        *V = make_float4(o.vel.x, o.vel.y, o.vel.z, 0.f);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
+    // one workgroup per output entry; fixed summation order
+    __shared__ float s[256];
+    const uint32_t w = blockIdx.x, tid = threadIdx.x;
+    float acc = 0.0f;
+    for (uint32_t b = tid; b < a.nblocks; b += 256) acc += a.part[(size_t)b * a.width + w];
+    s[tid] = acc;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (tid < o) s[tid] += s[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) a.out[w] += s[0];
 }

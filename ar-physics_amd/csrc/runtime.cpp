@@ -39,12 +39,18 @@ enum Kern {
     K_SOLID,
     K_EPS_IN,
     K_EPS_OUT,
+    K_FLUID_ADJ,
+    K_ADJ_W,
+    K_ADJ_PERM,
+    K_GRAD_AOS,
+    K_REDUCE,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
                                    "k_fluid",   "k_aos_in",  "k_aos_out",  "k_lincomb",
                                    "k_velocity", "k_fill",   "k_clear_solids", "k_strain",
-                                   "k_solid",   "k_eps_in",  "k_eps_out"};
+                                   "k_solid",   "k_eps_in",  "k_eps_out", "k_fluid_adj",
+                                   "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -57,6 +63,29 @@ template <class T>
 T* P(DBuf& b) {
     return reinterpret_cast<T*>(b.p);
 }
+
+// Where a neighbour build writes its structure (the context's own buffers, or a recording
+// slot kept for the adjoint of that stage).
+struct GridSlot {
+    uint32_t* perm = nullptr;
+    uint32_t* sflag = nullptr;
+    uint32_t* rowOff = nullptr;
+    uint32_t* rowStamp = nullptr;
+    uint32_t stamp = 0;
+    bool full = false;
+};
+
+// One recorded step of a checkpoint segment: everything the stage adjoints read.
+struct RecStep {
+    DBuf mk, pid, src;
+    DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
+    DBuf perm[ARP_MAX_STAGES], sflag[ARP_MAX_STAGES], rowOff[ARP_MAX_STAGES], rowStamp[ARP_MAX_STAGES];
+    GridSlot slot[ARP_MAX_STAGES];
+};
+
+struct Checkpoint {
+    DBuf P, V, mk, pid;
+};
 
 }  // namespace
 
@@ -100,7 +129,15 @@ struct arp_ctx {
     DBuf keys, ranks, binCount, binStart, activeRows, numActive, tmpKey, tmpSrc, perm, rowOff,
         gPid, gGrp, stats, scanState, scanTicket, scanEpoch, rowStamp;
     uint32_t stamp = 0;     // build counter written into rowStamp by the row sort
-    bool last_full = false;  // the last build reordered the state (perm is the identity)
+    GridSlot slot;          // where builds go (the buffers above unless recording)
+    // adjoint recording
+    bool rec_valid = false;
+    float rec_dt = 0.0f;
+    uint32_t rec_nsteps = 0, rec_K = 0, rec_seg = 0;
+    ArpTableau rec_tab{};
+    std::vector<Checkpoint> ck;
+    std::vector<RecStep> rec;
+    DBuf LamIn[2], LamAcc[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
     bool keys_valid = false;
     // what the last build described (for the dump)
     const DBuf* last_pid = nullptr;
@@ -421,6 +458,7 @@ struct SortIO {
     float* dstV;
     uint32_t* dstMk;
     uint32_t* dstPid;
+    uint32_t* srcOut = nullptr;  // full reorder: sorted slot -> source index
 };
 
 int run_key(arp_ctx* c, const float* posd) {
@@ -446,7 +484,8 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.tmpKey = P<uint32_t>(c->tmpKey);
     sc.tmpSrc = P<uint32_t>(c->tmpSrc);
     sc.full = io.full;
-    sc.perm = P<uint32_t>(c->perm);
+    GridSlot& g = c->slot;
+    sc.perm = g.perm;
     sc.srcPosd = io.srcP;
     sc.srcVelq = io.srcV;
     sc.srcMk = io.srcMk;
@@ -456,7 +495,8 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.dstMk = io.dstMk;
     sc.dstPid = io.dstPid;
     sc.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
-    sc.sflag = P<uint32_t>(c->sflag);
+    sc.sflag = g.sflag;
+    sc.srcOut = io.full ? io.srcOut : nullptr;
     if ((rc = launch(c, K_SCATTER, nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
     r.grid = c->grid;
@@ -468,14 +508,14 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.tmpSrc = P<uint32_t>(c->tmpSrc);
     r.gPid = P<uint32_t>(c->gPid);
     r.gGrp = P<uint32_t>(c->gGrp);
-    r.rowOff = P<uint32_t>(c->rowOff);
-    r.rowStamp = P<uint32_t>(c->rowStamp);
-    r.stamp = ++c->stamp;
-    c->last_full = io.full != 0;
+    r.rowOff = g.rowOff;
+    r.rowStamp = g.rowStamp;
+    r.stamp = g.stamp = ++c->stamp;
+    g.full = io.full != 0;
     r.bP = P<uint32_t>(c->bP);
     r.rP = P<uint32_t>(c->rP);
     r.full = io.full;
-    r.perm = P<uint32_t>(c->perm);
+    r.perm = g.perm;
     r.srcPosd = io.srcP;
     r.srcVelq = io.srcV;
     r.srcMk = io.srcMk;
@@ -486,7 +526,8 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.dstPid = io.dstPid;
     r.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
     r.stats = P<uint32_t>(c->stats);
-    r.sflag = P<uint32_t>(c->sflag);
+    r.sflag = g.sflag;
+    r.srcOut = io.full ? io.srcOut : nullptr;
     uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(c->grid.nrows, 1), 4096);
     return launch(c, K_ROWSORT, blocks, 256, r);
 }
@@ -498,17 +539,26 @@ FluidArgs fluid_base(arp_ctx* c) {
     f.n = c->n;
     f.mats = P<DMat>(c->mats);
     f.inter = P<DInter>(c->inter);
-    f.perm = P<uint32_t>(c->perm);
-    f.rowOff = P<uint32_t>(c->rowOff);
-    f.rowStamp = P<uint32_t>(c->rowStamp);
-    f.stamp = c->stamp;
-    f.identity = c->last_full ? 1 : 0;
-    f.sflag = P<uint32_t>(c->sflag);
+    f.perm = c->slot.perm;
+    f.rowOff = c->slot.rowOff;
+    f.rowStamp = c->slot.rowStamp;
+    f.stamp = c->slot.stamp;
+    f.identity = c->slot.full ? 1 : 0;
+    f.sflag = c->slot.sflag;
     f.bP = P<uint32_t>(c->bP);
     f.bposd = P<float>(c->bposd);
     f.bvelq = P<float>(c->bvelq);
     f.bmk = P<uint32_t>(c->bmk);
     return f;
+}
+
+GridSlot ctx_slot(arp_ctx* c) {
+    GridSlot g;
+    g.perm = P<uint32_t>(c->perm);
+    g.sflag = P<uint32_t>(c->sflag);
+    g.rowOff = P<uint32_t>(c->rowOff);
+    g.rowStamp = P<uint32_t>(c->rowStamp);
+    return g;
 }
 
 int alloc_state(arp_ctx* c) {
@@ -561,6 +611,8 @@ int alloc_state(arp_ctx* c) {
     if ((rc = dalloc(c, c->rowStamp, 4 * (size_t)c->grid.nrows))) return rc;
     HIPCHK(c, hipMemsetAsync(c->rowStamp.p, 0, 4 * (size_t)c->grid.nrows, c->stream));
     c->stamp = 0;
+    c->slot = ctx_slot(c);
+    c->rec_valid = false;
     return ARP_OK;
 }
 
@@ -613,11 +665,11 @@ SolidArgs solid_base(arp_ctx* c) {
     a.A = P<float>(c->A_d);
     a.Fm = P<float>(c->Fm_d);
     a.keps = P<float>(c->keps);
-    a.perm = P<uint32_t>(c->perm);
-    a.rowOff = P<uint32_t>(c->rowOff);
-    a.rowStamp = P<uint32_t>(c->rowStamp);
-    a.stamp = c->stamp;
-    a.sflag = P<uint32_t>(c->sflag);
+    a.perm = c->slot.perm;
+    a.rowOff = c->slot.rowOff;
+    a.rowStamp = c->slot.rowStamp;
+    a.stamp = c->slot.stamp;
+    a.sflag = c->slot.sflag;
     a.sidByPid = P<uint32_t>(c->sidByPid);
     a.KFp = P<float>(c->KF[0]);
     a.KFv = P<float>(c->KF[1]);
@@ -625,7 +677,9 @@ SolidArgs solid_base(arp_ctx* c) {
 }
 
 // One RK step of length dt on the internal state (Yland/meta[cur] on entry and exit).
-int step_once(arp_ctx* c, float dt) {
+// With rec, the step also keeps what its adjoint reads: the sorted y_n and every stage
+// input, the sorted meta, the reorder map and each stage's neighbour structure.
+int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     const ArpTableau& T = c->tab;
     const int S = T.stages;
     int rc;
@@ -639,6 +693,7 @@ int step_once(arp_ctx* c, float dt) {
         if (kslot[j] == 0) kslot[j] = nslots++;
     }
     if (nslots > kMaxK) return fail(c, ARP_E_LIMIT, "tableau needs %d stored stages (max %d)", nslots, kMaxK);
+    if (!rec) c->rec_valid = false;  // the live state leaves the recorded trajectory
     bool acc = false;
     for (int j = 0; j + 1 < S; j++)
         if (T.b[j] != 0.0f) acc = true;
@@ -649,28 +704,39 @@ int step_once(arp_ctx* c, float dt) {
         if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
     }
     const int nx = c->cur ^ 1;
+    float* Y0p = rec ? P<float>(rec->Xp[0]) : P<float>(c->Ysort[0]);
+    float* Y0v = rec ? P<float>(rec->Xv[0]) : P<float>(c->Ysort[1]);
+    const GridSlot own = c->slot;
+    if (rec) c->slot = rec->slot[0];
     SortIO io{1,
               P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_mk[c->cur]),
               P<uint32_t>(c->meta_pid[c->cur]),
-              P<float>(c->Ysort[0]), P<float>(c->Ysort[1]), P<uint32_t>(c->meta_mk[nx]),
-              P<uint32_t>(c->meta_pid[nx])};
+              Y0p, Y0v, P<uint32_t>(c->meta_mk[nx]),
+              P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr};
     if ((rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
+    if (rec) {
+        rec->slot[0] = c->slot;
+        HIPCHK(c, hipMemcpyAsync(rec->mk.p, c->meta_mk[nx].p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rec->pid.p, c->meta_pid[nx].p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+    }
 
-    const float* Xp = P<float>(c->Ysort[0]);
-    const float* Xv = P<float>(c->Ysort[1]);
+    const float* Xp = Y0p;
+    const float* Xv = Y0v;
     const float* Xe = P<float>(c->Yeps);
     for (int s = 0; s < S; s++) {
         if (s > 0) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
                        nullptr, nullptr, nullptr, nullptr};
+            if (rec) c->slot = rec->slot[s];
             if ((rc = run_sort(c, pio))) return rc;
+            if (rec) rec->slot[s] = c->slot;
         }
         const bool last = (s == S - 1);
         RkArgs r{};
-        r.Yposd = P<float>(c->Ysort[0]);
-        r.Yvelq = P<float>(c->Ysort[1]);
+        r.Yposd = Y0p;
+        r.Yvelq = Y0v;
         r.b_cur = dt * T.b[s];
         r.last = last ? 1 : 0;
         if (acc && s > 0) {
@@ -694,8 +760,8 @@ int step_once(arp_ctx* c, float dt) {
                     r.kc[r.nk] = dt * T.a[s + 1][j];
                     r.nk++;
                 }
-            r.Oposd = P<float>(c->Xb[s % 2][0]);
-            r.Ovelq = P<float>(c->Xb[s % 2][1]);
+            r.Oposd = rec ? P<float>(rec->Xp[s + 1]) : P<float>(c->Xb[s % 2][0]);
+            r.Ovelq = rec ? P<float>(rec->Xv[s + 1]) : P<float>(c->Xb[s % 2][1]);
             r.Oeps = P<float>(c->Xeps[s % 2]);
         } else {
             r.Oposd = P<float>(c->Yland[0]);
@@ -735,11 +801,12 @@ int step_once(arp_ctx* c, float dt) {
             if ((rc = launch(c, K_SOLID, nblk(c->ns), 256, sa))) return rc;
         }
         if (!last) {
-            Xp = P<float>(c->Xb[s % 2][0]);
-            Xv = P<float>(c->Xb[s % 2][1]);
+            Xp = r.Oposd;
+            Xv = r.Ovelq;
             Xe = P<float>(c->Xeps[s % 2]);
         }
     }
+    if (rec) c->slot = own;
     c->keys_valid = true;  // the last epilogue binned y_{n+1}
     c->have_build = false;
     return ARP_OK;
@@ -749,6 +816,148 @@ int need_scene(arp_ctx* c) {
     if (!c) return ARP_E_ARG;
     if (!c->have_scene) return fail(c, ARP_E_STATE, "no scene uploaded");
     return ARP_OK;
+}
+
+// ---- reverse mode ---------------------------------------------------------------------
+int d2d(arp_ctx* c, DBuf& dst, const DBuf& src, size_t bytes) {
+    int rc = dalloc(c, dst, bytes);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return ARP_OK;
+}
+
+int alloc_rec(arp_ctx* c, RecStep& R, int S) {
+    const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
+    const size_t nr = std::max<uint32_t>(c->grid.nrows, 1);
+    int rc;
+    for (DBuf* b : {&R.mk, &R.pid, &R.src})
+        if ((rc = dalloc(c, *b, 4 * n))) return rc;
+    for (int s = 0; s < S; s++) {
+        if ((rc = dalloc(c, R.Xp[s], 16 * n)) || (rc = dalloc(c, R.Xv[s], 16 * n)) ||
+            (rc = dalloc(c, R.perm[s], 4 * n)) || (rc = dalloc(c, R.sflag[s], 4 * n)) ||
+            (rc = dalloc(c, R.rowOff[s], 4 * nr * c->grid.rowstride)) || (rc = dalloc(c, R.rowStamp[s], 4 * nr)))
+            return rc;
+        // stamps are unique per build, so rows left from an older build never match
+        HIPCHK(c, hipMemsetAsync(R.rowStamp[s].p, 0, 4 * nr, c->stream));
+        GridSlot& g = R.slot[s];
+        g.perm = P<uint32_t>(R.perm[s]);
+        g.sflag = P<uint32_t>(R.sflag[s]);
+        g.rowOff = P<uint32_t>(R.rowOff[s]);
+        g.rowStamp = P<uint32_t>(R.rowStamp[s]);
+        g.stamp = 0;
+        g.full = false;
+    }
+    return ARP_OK;
+}
+
+void free_buf(DBuf& b) {
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void free_rec(arp_ctx* c) {
+    for (RecStep& R : c->rec) {
+        for (DBuf* b : {&R.mk, &R.pid, &R.src}) free_buf(*b);
+        for (int s = 0; s < ARP_MAX_STAGES; s++)
+            for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
+    }
+    for (Checkpoint& k : c->ck)
+        for (DBuf* b : {&k.P, &k.V, &k.mk, &k.pid}) free_buf(*b);
+    c->rec.clear();
+    c->ck.clear();
+    c->rec_valid = false;
+}
+
+// the live state (y in its storage order and the meta of that order)
+int ck_save(arp_ctx* c, Checkpoint& k) {
+    const size_t n = c->n;
+    int rc;
+    if ((rc = d2d(c, k.P, c->Yland[0], 16 * n)) || (rc = d2d(c, k.V, c->Yland[1], 16 * n)) ||
+        (rc = d2d(c, k.mk, c->meta_mk[c->cur], 4 * n)) || (rc = d2d(c, k.pid, c->meta_pid[c->cur], 4 * n)))
+        return rc;
+    return ARP_OK;
+}
+
+// Restore a saved state.  The next step bins it again from its positions; the step is then
+// the same computation as from the original state (the cell-major order is canonical: row,
+// cell, particle id), so replays reproduce the recorded trajectory bit for bit.
+int ck_restore(arp_ctx* c, Checkpoint& k) {
+    const size_t n = c->n;
+    HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, k.P.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, k.V.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->meta_mk[c->cur].p, k.mk.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->meta_pid[c->cur].p, k.pid.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+    int rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0);
+    if (rc) return rc;
+    c->keys_valid = false;
+    c->have_build = false;
+    return ARP_OK;
+}
+
+// Reverse sweep over one recorded step: lam (P,V) holds dL/dy_{n+1} in the step's sorted
+// order on entry and dL/dy_n in the order of the state before the step on exit.
+//   w_s = dt b_s lam + sum_{t>s} dt a_ts G_t,   G_s = J_f(X_s)^T w_s,   dL/dy_n = lam + sum_s G_s
+int reverse_step(arp_ctx* c, RecStep& R, float dt) {
+    const ArpTableau& T = c->tab;
+    const int S = T.stages;
+    const size_t n = c->n;
+    const uint32_t nm = c->nm;
+    int rc;
+    HIPCHK(c, hipMemcpyAsync(c->LamAcc[0].p, c->LamIn[0].p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->LamAcc[1].p, c->LamIn[1].p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    for (int s = S - 1; s >= 0; s--) {
+        AdjWArgs w{};
+        w.n = c->n;
+        w.cl = dt * T.b[s];
+        w.LP = P<float>(c->LamIn[0]);
+        w.LV = P<float>(c->LamIn[1]);
+        for (int t = s + 1; t < S; t++)
+            if (T.a[t][s] != 0.0f) {
+                w.GP[w.ng] = P<float>(c->Gb[t][0]);
+                w.GV[w.ng] = P<float>(c->Gb[t][1]);
+                w.cg[w.ng] = dt * T.a[t][s];
+                w.ng++;
+            }
+        w.WP = P<float>(c->Wb[0]);
+        w.WV = P<float>(c->Wb[1]);
+        if ((rc = launch(c, K_ADJ_W, nblk(c->n), 256, w))) return rc;
+        const GridSlot& g = R.slot[s];
+        AdjArgs a{};
+        a.grid = c->grid;
+        a.phys = c->phys;
+        a.n = c->n;
+        a.mats = P<DMat>(c->mats);
+        a.inter = P<DInter>(c->inter);
+        a.perm = g.perm;
+        a.rowOff = g.rowOff;
+        a.rowStamp = g.rowStamp;
+        a.stamp = g.stamp;
+        a.identity = g.full ? 1 : 0;
+        a.sflag = g.sflag;
+        a.bP = P<uint32_t>(c->bP);
+        a.bposd = P<float>(c->bposd);
+        a.bvelq = P<float>(c->bvelq);
+        a.bmk = P<uint32_t>(c->bmk);
+        a.Xposd = P<float>(R.Xp[s]);
+        a.Xvelq = P<float>(R.Xv[s]);
+        a.mk = P<uint32_t>(R.mk);
+        a.Wp = P<float>(c->Wb[0]);
+        a.Wv = P<float>(c->Wb[1]);
+        a.Gp = P<float>(c->Gb[s][0]);
+        a.Gv = P<float>(c->Gb[s][1]);
+        a.LamP = P<float>(c->LamAcc[0]);
+        a.LamV = P<float>(c->LamAcc[1]);
+        a.pgrad = P<float>(c->pgradPart);
+        if ((rc = launch(c, K_FLUID_ADJ, nblk(c->n), 256, a))) return rc;
+        if (nm <= (uint32_t)kMatLds) {
+            ReduceArgs r{nblk(c->n), 2 * nm, P<float>(c->pgradPart), P<float>(c->pgradSum)};
+            if ((rc = launch(c, K_REDUCE, 2 * nm, 256, r))) return rc;
+        }
+    }
+    AdjPermArgs pa{c->n, P<uint32_t>(R.src), P<float>(c->LamAcc[0]), P<float>(c->LamAcc[1]),
+                   P<float>(c->LamIn[0]), P<float>(c->LamIn[1])};
+    return launch(c, K_ADJ_PERM, nblk(c->n), 256, pa);
 }
 
 // Per-stage path: device AoS state -> sorted SoA in the ps buffers (full reorder).
@@ -825,6 +1034,11 @@ void arp_destroy(arp_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (DBuf* b : c->bufs)
         if (b->p) hipFree(b->p);
+    free_rec(c);
+    for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->Wb[0], &c->Wb[1], &c->pgradPart,
+                    &c->pgradSum})
+        free_buf(*b);
+    for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(c->Gb[s][0]), free_buf(c->Gb[s][1]);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->mod) hipModuleUnload(c->mod);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -1095,6 +1309,7 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     int rc;
     if ((rc = need_scene(c))) return rc;
     if (n != c->n || (n && !particles)) return fail(c, ARP_E_ARG, "state has %u particles, scene %u", n, c->n);
+    c->rec_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     if (c->solids_active) {
         const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
@@ -1148,6 +1363,88 @@ int arp_update(arp_ctx* c, uint32_t nframes) {
 }
 
 double arp_time(const arp_ctx* c) { return c ? c->time : 0.0; }
+
+int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoint_every) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (c->solids_active) return fail(c, ARP_E_STATE, "the adjoint covers fluid scenes (this scene has solids)");
+    if (nsteps == 0 || checkpoint_every == 0 || !(dt > 0.0f) || !std::isfinite(dt))
+        return fail(c, ARP_E_ARG, "adjoint_record needs nsteps >= 1, checkpoint_every >= 1 and dt > 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t K = std::min(checkpoint_every, nsteps);
+    const uint32_t nseg = (nsteps + K - 1) / K;
+    const int S = c->tab.stages;
+    free_rec(c);
+    c->rec.resize(K);
+    c->ck.resize(nseg + 1);  // segment starts, then the end state
+    for (RecStep& R : c->rec)
+        if ((rc = alloc_rec(c, R, S))) return rc;
+    for (uint32_t j = 0; j < nsteps; j++) {
+        if (j % K == 0 && (rc = ck_save(c, c->ck[j / K]))) return rc;
+        if ((rc = step_once(c, dt, &c->rec[j % K]))) return rc;
+        c->time += dt;
+    }
+    if ((rc = ck_save(c, c->ck[nseg]))) return rc;
+    c->rec_dt = dt;
+    c->rec_nsteps = nsteps;
+    c->rec_K = K;
+    c->rec_seg = nseg - 1;
+    c->rec_tab = c->tab;
+    c->rec_valid = true;
+    return ARP_OK;
+}
+
+int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, float* grad_params) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!c->rec_valid) return fail(c, ARP_E_STATE, "no recorded trajectory (call arp_adjoint_record)");
+    if (!d_grad_out || !d_grad_in) return fail(c, ARP_E_ARG, "null cotangent buffer");
+    if (grad_params && c->nm > (uint32_t)kMatLds)
+        return fail(c, ARP_E_LIMIT, "material parameter gradients need at most %d materials", kMatLds);
+    if (memcmp(&c->rec_tab, &c->tab, sizeof(ArpTableau)) != 0)
+        return fail(c, ARP_E_STATE, "the integrator changed after the trajectory was recorded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
+    const int S = c->tab.stages;
+    for (int j = 0; j < 2; j++) {
+        if ((rc = dalloc(c, c->LamIn[j], 16 * n)) || (rc = dalloc(c, c->LamAcc[j], 16 * n)) ||
+            (rc = dalloc(c, c->Wb[j], 16 * n)))
+            return rc;
+        for (int s = 0; s < S; s++)
+            if ((rc = dalloc(c, c->Gb[s][j], 16 * n))) return rc;
+    }
+    if ((rc = dalloc(c, c->pgradPart, 4 * (size_t)nblk(c->n) * 2 * kMatLds))) return rc;
+    if ((rc = dalloc(c, c->pgradSum, 4 * 2 * kMatLds))) return rc;
+    if ((rc = fill_u32(c, c->pgradSum, 2 * kMatLds, 0))) return rc;
+    const uint32_t K = c->rec_K, nsteps = c->rec_nsteps, nseg = (nsteps + K - 1) / K;
+    Checkpoint& fin = c->ck[nseg];
+    // dL/dy_N (records by particle id) -> the final storage order
+    GradAosArgs in{c->n, 0, P<uint32_t>(fin.pid), const_cast<void*>(d_grad_out), P<float>(c->LamIn[0]),
+                   P<float>(c->LamIn[1])};
+    if ((rc = launch(c, K_GRAD_AOS, nblk(c->n), 256, in))) return rc;
+    for (int seg = (int)nseg - 1; seg >= 0; seg--) {
+        const uint32_t j0 = (uint32_t)seg * K, j1 = std::min(nsteps, j0 + K);
+        if ((uint32_t)seg != c->rec_seg) {
+            // replay the segment from its checkpoint, recording
+            if ((rc = ck_restore(c, c->ck[seg]))) return rc;
+            for (uint32_t j = j0; j < j1; j++)
+                if ((rc = step_once(c, c->rec_dt, &c->rec[j - j0]))) return rc;
+            c->rec_seg = (uint32_t)seg;
+        }
+        for (uint32_t j = j1; j-- > j0;)
+            if ((rc = reverse_step(c, c->rec[j - j0], c->rec_dt))) return rc;
+    }
+    // dL/dy_0 is now in the storage order of the recorded start state
+    GradAosArgs out{c->n, 1, P<uint32_t>(c->ck[0].pid), d_grad_in, P<float>(c->LamIn[0]), P<float>(c->LamIn[1])};
+    if ((rc = launch(c, K_GRAD_AOS, nblk(c->n), 256, out))) return rc;
+    // the live state goes back to the end of the recorded trajectory
+    if ((rc = ck_restore(c, fin))) return rc;
+    if (grad_params) {
+        HIPCHK(c, hipMemcpyAsync(grad_params, c->pgradSum.p, 4 * 2 * (size_t)c->nm, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ARP_OK;
+}
 
 int arp_download(arp_ctx* c, ArpParticle* particles, uint32_t n, ArpSolidParticle* solids, uint32_t ns) {
     int rc;

@@ -196,6 +196,22 @@ int arp_profile(arp_ctx* ctx, int enable);
 int arp_profile_query(arp_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
 int arp_profile_reset(arp_ctx* ctx);
 
+/* ---- reverse mode (no counterpart in the reference, which has no adjoint) ------------- */
+/* Advance nsteps fused steps of length dt from the current state (fluid scenes: no solid
+ * path), keeping a checkpoint of the state every checkpoint_every steps and the stage
+ * inputs and neighbour structures of the last segment for arp_adjoint_backward.  Memory:
+ * one checkpoint (40 B/particle) per segment plus checkpoint_every recorded steps.  A later
+ * arp_step/arp_update/arp_upload_state drops the recording. */
+int arp_adjoint_record(arp_ctx* ctx, float dt, uint32_t nsteps, uint32_t checkpoint_every);
+/* Reverse-mode sweep over the recorded trajectory (segments are replayed from their
+ * checkpoints as needed).  d_grad_out: n device ArpParticle records holding dL/d(den,
+ * pos.xyz, vel.xyz) of the state after the recorded steps, indexed like arp_download's
+ * output; d_grad_in receives dL/d of the state before them (mat/solid_id fields zero).
+ * grad_params (host, may be NULL): 2*nmat floats, [2m] = dL/d bulk_viscosity and
+ * [2m+1] = dL/d viscosity of material m (at most 16 materials).  The live state is left at
+ * the end of the recorded trajectory. */
+int arp_adjoint_backward(arp_ctx* ctx, const void* d_grad_out, void* d_grad_in, float* grad_params);
+
 #ifdef __cplusplus
 }
 #endif
