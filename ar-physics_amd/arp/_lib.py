@@ -32,6 +32,13 @@ def lib():
         return _lib
     if not os.path.exists(LIBPATH):
         raise ArpError("libarp.so is not built: run python ar-physics_amd/build.py (%s)" % LIBPATH)
+    # One HIP runtime per process: torch ships its own libamdhip64, and a second copy loaded
+    # first (the one libarp.so links) leaves torch without devices.  Loading torch first makes
+    # libarp.so bind to torch's copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     L.arp_create.argtypes = [C.c_int, C.c_uint32, C.POINTER(vp)]
     L.arp_destroy.argtypes = [vp]

@@ -856,18 +856,23 @@ void free_buf(DBuf& b) {
     b.bytes = 0;
 }
 
-void free_rec(arp_ctx* c) {
-    for (RecStep& R : c->rec) {
+// Keep the first nrec recorded steps and nck checkpoints (their buffers are reused by the
+// next recording), free the rest.
+void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
+    for (size_t i = nrec; i < c->rec.size(); i++) {
+        RecStep& R = c->rec[i];
         for (DBuf* b : {&R.mk, &R.pid, &R.src}) free_buf(*b);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
     }
-    for (Checkpoint& k : c->ck)
-        for (DBuf* b : {&k.P, &k.V, &k.mk, &k.pid}) free_buf(*b);
-    c->rec.clear();
-    c->ck.clear();
+    for (size_t i = nck; i < c->ck.size(); i++)
+        for (DBuf* b : {&c->ck[i].P, &c->ck[i].V, &c->ck[i].mk, &c->ck[i].pid}) free_buf(*b);
+    if (c->rec.size() > nrec) c->rec.resize(nrec);
+    if (c->ck.size() > nck) c->ck.resize(nck);
     c->rec_valid = false;
 }
+
+void free_rec(arp_ctx* c) { trim_rec(c, 0, 0); }
 
 // the live state (y in its storage order and the meta of that order)
 int ck_save(arp_ctx* c, Checkpoint& k) {
@@ -1374,7 +1379,7 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     const uint32_t K = std::min(checkpoint_every, nsteps);
     const uint32_t nseg = (nsteps + K - 1) / K;
     const int S = c->tab.stages;
-    free_rec(c);
+    trim_rec(c, K, nseg + 1);
     c->rec.resize(K);
     c->ck.resize(nseg + 1);  // segment starts, then the end state
     for (RecStep& R : c->rec)

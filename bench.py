@@ -62,6 +62,41 @@ def cpu_baseline(scene, integ, dt, steps, threads):
                        "the reference GLSL cannot run here (no GL context)" % (steps, integ.upper(), scene.n, threads))
 
 
+def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
+    """Forward with recording + reverse sweep of nsteps steps (one segment, no replay) on the
+    same workload, against nsteps plain steps."""
+    import numpy as np
+    import torch
+
+    log("adjoint: record %d steps + reverse sweep" % nsteps)
+    rng = np.random.default_rng(0)
+    g = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
+    g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(rng.standard_normal((sc.n, 7)).astype(np.float32)).cuda()
+    gin = torch.zeros_like(g)
+    torch.cuda.synchronize()
+    # one untimed round (allocates the recording and the reverse-mode buffers)
+    sim.adjoint_record(dt, nsteps, nsteps)
+    sim.adjoint_backward(g, gin, want_params=False)
+    sim.synchronize()
+    sim.profile(True)
+    sim.profile_reset()
+    t0 = time.perf_counter()
+    sim.adjoint_record(dt, nsteps, nsteps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.adjoint_backward(g, gin, want_params=True)
+    sim.synchronize()
+    t2 = time.perf_counter()
+    ms, cnt = sim.profile_query("k_fluid_adj")
+    sim.profile(False)
+    finite = bool(torch.isfinite(gin).all().item())
+    rec_ms, bwd_ms = 1e3 * (t1 - t0), 1e3 * (t2 - t1)
+    return dict(steps=nsteps, checkpoint_every=nsteps, record_ms=rec_ms, backward_ms=bwd_ms,
+                forward_ms=fwd_ms_per_step * nsteps,
+                ratio_fwd_plus_adjoint_to_fwd=(rec_ms + bwd_ms) / (fwd_ms_per_step * nsteps),
+                k_fluid_adj_avg_us=1e3 * ms / cnt if cnt else None, grad_finite=finite)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,7 +107,22 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--adjoint-steps", type=int, default=10,
+                    help="steps of the recorded rollout whose reverse sweep is timed (0: skip)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = 64 * 1024 * 1024
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy kernel: %d bytes read and %d bytes written per launch" % (4 * n, 4 * n))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding); "
               "run with --gpus 1", file=sys.stderr)
@@ -113,6 +163,9 @@ def main():
         if cnt:
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
     sim.profile(False)
+    fwd_adj = None
+    if args.adjoint_steps > 0:
+        fwd_adj = time_adjoint(sim, sc, dt, args.adjoint_steps, 1e3 * sec / args.steps)
     p = sim.particles()
     finite = bool(np.isfinite(p["pos"]).all())
     value = sc.n * args.steps / sec
@@ -157,7 +210,7 @@ def main():
         "step_roofline_frac": B_ALG_STEP.get(args.integrator, 0.0) * value / HBM_PEAK,
         "kernels": kern,
         "state_finite": finite,
-        "fwd_adjoint": None,
+        "fwd_adjoint": fwd_adj,
     }
     if not args.no_cpu:
         log("cpu baseline: %d step(s), %d threads" % (args.cpu_steps, args.cpu_threads))

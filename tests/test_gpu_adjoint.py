@@ -61,6 +61,12 @@ def _oracle_L(O, sc, x, g, tab, dt, nsteps, mats=None):
     return float(np.sum(g * (yv - x)))
 
 
+def _same_state(a, b):
+    """Bitwise equality of every field of two Particle record arrays (padding ignored)."""
+    return all(np.array_equal(np.ascontiguousarray(a[f]).view(np.uint32), np.ascontiguousarray(b[f]).view(np.uint32))
+               for f in ("den", "mat", "solid_id", "pos", "vel"))
+
+
 def _records(torch, arr7):
     r = np.zeros((len(arr7), 12), np.float32)
     r[:, [0, 4, 5, 6, 8, 9, 10]] = arr7
@@ -168,7 +174,7 @@ def test_adjoint_leaves_state_and_replays_exactly(torch_cuda):
         gp = sim.adjoint_backward(d_out, d_in)
         sim.synchronize()
         outs.append((d_in.cpu().numpy().copy(), gp.copy()))
-        assert np.array_equal(sim.particles().view(np.uint8), end.view(np.uint8))
+        assert _same_state(sim.particles(), end)
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     # one segment (no replay) gives the same gradient as checkpoints every 2 steps
@@ -184,7 +190,7 @@ def test_adjoint_leaves_state_and_replays_exactly(torch_cuda):
     sim3 = FluidSim(sc, integrator="rk2", timestep=dt)
     sim3.upload_state(sc.particles)
     sim3.step(dt, 8)
-    assert np.array_equal(sim.particles().view(np.uint8), sim3.particles().view(np.uint8))
+    assert _same_state(sim.particles(), sim3.particles())
     for s in (sim, sim2, sim3):
         s.close()
 
