@@ -15,6 +15,7 @@ SYMBOLS = [
     "arp_upload_state", "arp_compute_forces", "arp_lincomb", "arp_velocity", "arp_neighbors_update",
     "arp_grid_info", "arp_neighbor_dump", "arp_cell_counts", "arp_profile", "arp_profile_query",
     "arp_profile_reset", "arp_adjoint_record", "arp_adjoint_backward",
+    "arp_profile_select",
 ]
 
 
@@ -73,6 +74,7 @@ def lib():
     L.arp_profile.argtypes = [vp, C.c_int]
     L.arp_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     L.arp_profile_reset.argtypes = [vp]
+    L.arp_profile_select.argtypes = [vp, C.c_char_p]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_adjoint_backward.argtypes = [vp, vp, vp, vp]
     _lib = L

@@ -156,6 +156,11 @@ class FluidSim:
     def profile(self, enable=True):
         check(self.ctx, lib().arp_profile(self.ctx, 1 if enable else 0))
 
+    def profile_select(self, kernels=None):
+        """Time only these kernels (list of names; None: all)."""
+        names = ",".join(kernels) if kernels else ""
+        check(self.ctx, lib().arp_profile_select(self.ctx, names.encode()))
+
     def profile_reset(self):
         check(self.ctx, lib().arp_profile_reset(self.ctx))
 

@@ -184,6 +184,7 @@ struct FluidArgs {
     uint32_t stamp;
     int identity;           // the build reordered the state: sorted slot == storage index
     const uint32_t* sflag;  // per storage index: no bucket slot
+    const uint32_t* stats;  // [0] particles refused by the cap in this build (may be null)
     const uint32_t* bP;
     const float* bposd;  // boundary (static, cell-sorted, capped)
     const float* bvelq;

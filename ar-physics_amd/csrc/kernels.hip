@@ -531,15 +531,9 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             PairIn in;
             in.bnd = (e & kBnd) != 0;
             const uint32_t j = e & ~kBnd;
-            if (in.bnd) {
-                in.xj = B[j];
-                in.vj = BV[j];
-                in.mk2 = a.bmk[j];
-            } else {
-                in.xj = X[j];
-                in.vj = XV[j];
-                in.mk2 = a.mk[j];
-            }
+            in.xj = (in.bnd ? B : X)[j];
+            in.vj = (in.bnd ? BV : XV)[j];
+            in.mk2 = (in.bnd ? a.bmk : a.mk)[j];
             return in;
         };
         // one neighbour's terms (fluid.rs:153-214)
@@ -578,7 +572,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             if (cp != 0.0f) {
                 const float cs = __fdividef(m2 * cp, d2);
                 const float cx = cs * gx, cy = cs * gy, cz = cs * gz;
-                if ((elastic1 || elastic2 || bnd) && mat1 != mat2) {
+                if ((elastic1 || elastic2 || bnd) && mat1 != mat2 && it.friction != 0.0f) {
                     // friction along the tangential relative velocity (fluid.rs:191-199)
                     const float irr = __fdividef(1.0f, rr);
                     const float ix = rx * irr, iy = ry * irr, iz = rz * irr;
@@ -643,7 +637,9 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
                 }
             };
             // Candidates within h are buffered in LDS and drained at a single site of the pair
-            // loop whenever the buffer could overflow.
+            // loop whenever the buffer could overflow.  Inside a batch the test and the append
+            // are branch-free: every candidate is written to slot nl, which advances only for
+            // accepted ones.
             uint32_t nl = 0;
             int rw = 0;
             uint4 cur, nxtm;
@@ -651,6 +647,9 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             row_meta(1, nxtm);
             int part = 0;  // 0: mobile segment, 1: boundary segment
             uint32_t q = cur.x, qend = cur.y;
+            // a build without refused particles leaves sflag all zero (non-finite particles
+            // are in no row), so the flag loads of the identity path can be skipped
+            const bool flags = a.identity && (a.stats == nullptr || *a.stats != 0u);
             for (;;) {
                 while (nl + ARP_BATCH <= (uint32_t)kList) {
                     if (q >= qend) {
@@ -668,44 +667,40 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
                         qend = cur.y;
                         continue;
                     }
-                    if (part) {
-                        float4 xj[ARP_BATCH];
+                    const bool bseg = part != 0;
+                    const float4* base = bseg ? B : X;
+                    const uint32_t tag = bseg ? kBnd : 0u;
+                    uint32_t e[ARP_BATCH], ok[ARP_BATCH];
 #pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = B[q + u];
-#pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) {
-                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                            if (q + u < qend && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
-                                list[(nl++) * 256 + tid] = (q + u) | kBnd;
+                    for (int u = 0; u < ARP_BATCH; u++) {
+                        const uint32_t qq = q + u;  // < qend + ARP_BATCH: inside the padding
+                        const uint32_t inr = qq < qend ? 1u : 0u;
+                        uint32_t v = qq, okv = inr;
+                        // uniform branches; the loads are unconditional (index 0 on boundary
+                        // segments, whose indices may exceed the mobile arrays)
+                        if (!a.identity) {
+                            const uint32_t pe = a.perm[bseg ? 0u : qq];  // past qend: padding
+                            okv = inr & ((bseg || !(pe & kFlag)) ? 1u : 0u);
+                            v = bseg ? qq : (inr ? (pe & ~kFlag) : 0u);
+                        } else if (flags) {
+                            const uint32_t fl = a.sflag[bseg ? 0u : qq];
+                            okv = inr & ((bseg || fl == 0u) ? 1u : 0u);
                         }
-                    } else if (a.identity) {
-                        uint32_t fl[ARP_BATCH];
-                        float4 xj[ARP_BATCH];
+                        e[u] = v;
+                        ok[u] = okv;
+                    }
+                    float4 xj[ARP_BATCH];
 #pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) {
-                            fl[u] = a.sflag[q + u];
-                            xj[u] = X[q + u];
-                        }
+                    for (int u = 0; u < ARP_BATCH; u++) xj[u] = base[e[u]];
 #pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) {
-                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                            // the self pair contributes exactly zero (grad W(0) = 0)
-                            if (q + u < qend && !fl[u] && q + u != i && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
-                                list[(nl++) * 256 + tid] = q + u;
-                        }
-                    } else {
-                        uint32_t pe2[ARP_BATCH];
-                        float4 xj[ARP_BATCH];
-#pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) pe2[u] = (q + u < qend) ? a.perm[q + u] : kFlag;
-#pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = X[pe2[u] & ~kFlag];
-#pragma unroll
-                        for (int u = 0; u < ARP_BATCH; u++) {
-                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                            if (!(pe2[u] & kFlag) && pe2[u] != i && hh - (rx * rx + ry * ry + rz * rz) > 0.0f)
-                                list[(nl++) * 256 + tid] = pe2[u];
-                        }
+                    for (int u = 0; u < ARP_BATCH; u++) {
+                        const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                        const uint32_t v = e[u] | tag;
+                        // the self pair contributes exactly zero (grad W(0) = 0)
+                        const uint32_t in = ok[u] & (v != i ? 1u : 0u) &
+                                            (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
+                        list[nl * 256 + tid] = v;
+                        nl += in;
                     }
                     q += ARP_BATCH;
                 }

@@ -21,6 +21,15 @@ using namespace arp;
 
 extern "C" const unsigned char arp_kernels_co[];
 
+// HIP's extended module launch (hip/hip_ext.h; that header needs the device-side headers):
+// the start/stop events are recorded by the dispatch itself.
+extern "C" hipError_t hipExtModuleLaunchKernel(hipFunction_t f, uint32_t globalWorkSizeX, uint32_t globalWorkSizeY,
+                                               uint32_t globalWorkSizeZ, uint32_t localWorkSizeX,
+                                               uint32_t localWorkSizeY, uint32_t localWorkSizeZ,
+                                               size_t sharedMemBytes, hipStream_t hStream, void** kernelParams,
+                                               void** extra, hipEvent_t startEvent, hipEvent_t stopEvent,
+                                               uint32_t flags);
+
 namespace {
 
 enum Kern {
@@ -146,6 +155,7 @@ struct arp_ctx {
 
     // profiling: event pairs around launches
     bool prof = false;
+    bool prof_sel[K_COUNT];  // kernels timed while profiling
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> ev_marks;  // (kernel, index of first event)
@@ -157,6 +167,7 @@ struct arp_ctx {
     double time = 0.0;
 
     arp_ctx() {
+        for (bool& b : prof_sel) b = true;
         bufs = {&mats,     &inter,     &bposd,      &bvelq,    &bmk,    &bP,       &rP,
                 &mkStatic, &matByPid,  &sidByPid,   &Ysort[0], &Ysort[1], &Yland[0], &Yland[1],
                 &meta_mk[0], &meta_mk[1], &meta_pid[0], &meta_pid[1], &Xb[0][0], &Xb[0][1],
@@ -231,20 +242,20 @@ int launch(arp_ctx* c, Kern k, uint32_t blocks, uint32_t threads, const A& args)
     if (blocks == 0) return ARP_OK;
     void* params[] = {const_cast<A*>(&args)};
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->prof) {
+    if (c->prof && c->prof_sel[k]) {
         size_t first = c->ev_used;
         e0 = next_event(c);
         e1 = next_event(c);
-        if (e0 && e1) {
-            hipEventRecord(e0, c->stream);
-            c->ev_marks.push_back({(int)k, first});
-        }
+        if (e0 && e1) c->ev_marks.push_back({(int)k, first});
     }
-    hipError_t e = hipModuleLaunchKernel(c->fn[k], blocks, 1, 1, threads, 1, 1, 0, c->stream,
-                                         params, nullptr);
+    hipError_t e;
+    if (e0 && e1)  // the events time the dispatch itself (no marker packets around it)
+        e = hipExtModuleLaunchKernel(c->fn[k], blocks * threads, 1, 1, threads, 1, 1, 0, c->stream, params,
+                                     nullptr, e0, e1, 0);
+    else
+        e = hipModuleLaunchKernel(c->fn[k], blocks, 1, 1, threads, 1, 1, 0, c->stream, params, nullptr);
     if (e != hipSuccess)
         return fail(c, ARP_E_HIP, "launch %s failed: %s", kKernNames[k], hipGetErrorString(e));
-    if (e0 && e1) hipEventRecord(e1, c->stream);
     return ARP_OK;
 }
 
@@ -545,6 +556,7 @@ FluidArgs fluid_base(arp_ctx* c) {
     f.stamp = c->slot.stamp;
     f.identity = c->slot.full ? 1 : 0;
     f.sflag = c->slot.sflag;
+    f.stats = P<uint32_t>(c->stats);
     f.bP = P<uint32_t>(c->bP);
     f.bposd = P<float>(c->bposd);
     f.bvelq = P<float>(c->bvelq);
@@ -1585,6 +1597,31 @@ int arp_velocity(arp_ctx* c, float r, const ArpParticle* d_in, const ArpSolidPar
 int arp_profile(arp_ctx* c, int enable) {
     if (!c) return ARP_E_ARG;
     c->prof = enable != 0;
+    return ARP_OK;
+}
+
+int arp_profile_select(arp_ctx* c, const char* kernels) {
+    if (!c) return ARP_E_ARG;
+    if (!kernels || !*kernels) {
+        for (bool& b : c->prof_sel) b = true;
+        return ARP_OK;
+    }
+    bool sel[K_COUNT] = {};
+    std::string all(kernels);
+    size_t pos = 0;
+    while (pos <= all.size()) {
+        size_t e = all.find(',', pos);
+        if (e == std::string::npos) e = all.size();
+        std::string name = all.substr(pos, e - pos);
+        if (!name.empty()) {
+            int k = 0;
+            while (k < K_COUNT && name != kKernNames[k]) k++;
+            if (k == K_COUNT) return fail(c, ARP_E_ARG, "unknown kernel '%s'", name.c_str());
+            sel[k] = true;
+        }
+        pos = e + 1;
+    }
+    std::copy(sel, sel + K_COUNT, c->prof_sel);
     return ARP_OK;
 }
 

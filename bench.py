@@ -78,6 +78,7 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
     sim.adjoint_record(dt, nsteps, nsteps)
     sim.adjoint_backward(g, gin, want_params=False)
     sim.synchronize()
+    sim.profile_select(["k_fluid_adj"])
     sim.profile(True)
     sim.profile_reset()
     t0 = time.perf_counter()
@@ -107,6 +108,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--events", default="all", choices=["all", "fluid", "none"],
+                    help="kernels timed with HIP events inside the timed region")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--adjoint-steps", type=int, default=10,
@@ -149,7 +152,8 @@ def main():
             torch.cuda.synchronize()
     except Exception:
         pass
-    sim.profile(True)
+    sim.profile_select(["k_fluid"] if args.events == "fluid" else None)
+    sim.profile(args.events != "none")
     sim.profile_reset()
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()

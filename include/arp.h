@@ -195,6 +195,9 @@ int arp_cell_counts(arp_ctx* ctx, uint64_t first, uint64_t count, uint32_t* out)
 int arp_profile(arp_ctx* ctx, int enable);
 int arp_profile_query(arp_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
 int arp_profile_reset(arp_ctx* ctx);
+/* Time only the named kernels (comma-separated, e.g. "k_fluid,k_fluid_adj"); NULL or ""
+ * selects all.  Untimed kernels launch without events. */
+int arp_profile_select(arp_ctx* ctx, const char* kernels);
 
 /* ---- reverse mode (no counterpart in the reference, which has no adjoint) ------------- */
 /* Advance nsteps fused steps of length dt from the current state (fluid scenes: no solid
