@@ -108,8 +108,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--events", default="all", choices=["all", "fluid", "none"],
-                    help="kernels timed with HIP events inside the timed region")
+    ap.add_argument("--events", default="fluid", choices=["all", "fluid", "none"],
+                    help="kernels timed with HIP events inside the timed region (fluid: the dominant "
+                         "kernel only, which keeps the event overhead off the other launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--adjoint-steps", type=int, default=10,
