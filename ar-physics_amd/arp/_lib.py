@@ -15,7 +15,8 @@ SYMBOLS = [
     "arp_upload_state", "arp_compute_forces", "arp_lincomb", "arp_velocity", "arp_neighbors_update",
     "arp_grid_info", "arp_neighbor_dump", "arp_cell_counts", "arp_profile", "arp_profile_query",
     "arp_profile_reset", "arp_adjoint_record", "arp_adjoint_backward",
-    "arp_profile_select",
+    "arp_profile_select", "arp_batch_create", "arp_batch_destroy", "arp_batch_last_error", "arp_batch_step",
+    "arp_batch_tune",
 ]
 
 
@@ -75,6 +76,13 @@ def lib():
     L.arp_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     L.arp_profile_reset.argtypes = [vp]
     L.arp_profile_select.argtypes = [vp, C.c_char_p]
+    L.arp_batch_create.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
+    L.arp_batch_destroy.argtypes = [vp]
+    L.arp_batch_destroy.restype = None
+    L.arp_batch_last_error.argtypes = [vp]
+    L.arp_batch_last_error.restype = C.c_char_p
+    L.arp_batch_step.argtypes = [vp, C.c_float, C.c_uint32]
+    L.arp_batch_tune.argtypes = [vp, C.c_float, C.c_uint32, C.POINTER(C.c_uint32)]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_adjoint_backward.argtypes = [vp, vp, vp, vp]
     _lib = L

@@ -476,6 +476,50 @@ def fluid_column_3d(shape=(128, 128, 64), spacing=0.005, jitter=0.05, seed=0, bo
                  integrator=rk, name="fluid_column_3d")
 
 
+def fluid_block_3d(n, alpha=0.02, spacing=0.005, jitter=0.05, seed=0, layers=3, h_factor=1.3, rk="rk2",
+                   dt=5e-5, name="fluid_block_3d"):
+    """A 3-D block of exactly n fluid particles (the first n points of a jittered cubic
+    lattice of about n^(1/3) per side, taken in lattice order) in the corner of an
+    open-top box twice the block's width and 1.5 times its height, with `layers` of boundary
+    particles; Tait (rho0 1000, c0 20), bulk_visc = alpha h c0."""
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    shape = (side, side, int(np.ceil(n / (side * side))))
+    h = f32(h_factor * spacing)
+    P = _cubic_block(shape, spacing, (0.0, 0.0, 0.0), jitter, seed)[:n]
+    ext = [shape[a] * spacing for a in range(3)]
+    box = (2.0 * ext[0], 1.5 * ext[1], 2.0 * ext[2])
+    B = _walls(box, spacing, layers)
+    water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)
+    wall = immobile_material(0.0)
+    wall["mass"] = f32(1.0 * spacing**3)
+    water["mass"] = f32(1000.0 * spacing**3)
+    mats = np.array([wall, water], wire.MATERIAL)
+    flu = wire.particles(len(P))
+    flu["pos"][:, :3] = P
+    flu["den"] = 1000.0
+    flu["mat"] = 1
+    bnd = wire.particles(len(B))
+    bnd["pos"][:, :3] = B
+    bnd["den"] = 1.0
+    bnd["mat"] = 0
+    margin = (layers + 1) * spacing
+    bounds = wire.aabb([-margin] * 3, [box[0] + 2 * margin, box[1] + 2 * margin, box[2] + 2 * margin])
+    inter = resolve_interactions(mats, h)
+    return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, alpha, dt=dt, subticks=1, integrator=rk, name=name)
+
+
+def ensemble_3d(count=256, nmin=2048, nmax=65536, alpha=(0.01, 0.1), seed=0, **kw):
+    """Independent fluid scenes for batched runs: particle counts log-uniform in [nmin, nmax]
+    and artificial viscosity alpha uniform in `alpha`, both drawn from the splitmix64 stream
+    of `seed`; scene k uses lattice jitter seed seed + 1 + k."""
+    u = (uniform_pm1(seed, 2 * count).reshape(count, 2) + 1.0) * 0.5  # [0, 1)
+    ns = np.exp(np.log(nmin) + u[:, 0] * (np.log(nmax) - np.log(nmin)))
+    ns = np.clip(np.floor(ns).astype(np.int64), nmin, nmax)
+    al = alpha[0] + u[:, 1] * (alpha[1] - alpha[0])
+    return [fluid_block_3d(int(n), alpha=float(a), seed=seed + 1 + k, name="ensemble_%d" % k, **kw)
+            for k, (n, a) in enumerate(zip(ns, al))]
+
+
 def small_fluid_3d(n_side=12, spacing=0.005, seed=1, jitter=0.05, box=(0.12, 0.1, 0.12),
                    rk="rk2", dt=5e-5):
     """A small 3-D column of the same parameters, for tests that run the oracle."""

@@ -17,7 +17,7 @@ from . import tableau as _tab
 from . import wire
 from ._lib import ArpError, check, lib
 
-__all__ = ["FluidSim", "ArpError"]
+__all__ = ["FluidSim", "Batch", "ArpError"]
 
 
 def _p(a):
@@ -174,6 +174,48 @@ class FluidSim:
         if self.ctx:
             lib().arp_destroy(self.ctx)
             self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent fluid scenes stepped together (arp_batch_*): one launch per kernel and RK
+    stage for all of them, replayed from a hipGraph when graph=True."""
+
+    def __init__(self, sims, graph=True):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.ctx.value for s in self.sims])
+        self.b = C.c_void_p()
+        rc = lib().arp_batch_create(C.cast(arr, C.c_void_p), len(self.sims), 1 if graph else 0, C.byref(self.b))
+        if rc != 0:
+            msg = lib().arp_last_error(self.sims[0].ctx)
+            raise ArpError("arp_batch_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().arp_batch_last_error(self.b)
+            raise ArpError("arp batch error %d: %s" % (rc, msg.decode() if msg else ""))
+
+    def step(self, dt, nsteps=1):
+        self._check(lib().arp_batch_step(self.b, float(dt), int(nsteps)))
+
+    def tune(self, dt, trial_steps=2):
+        d = C.c_uint32()
+        self._check(lib().arp_batch_tune(self.b, float(dt), int(trial_steps), C.byref(d)))
+        return d.value
+
+    def synchronize(self):
+        for s in self.sims:
+            s.synchronize()
+
+    def close(self):
+        if self.b:
+            lib().arp_batch_destroy(self.b)
+            self.b = C.c_void_p()
 
     def __del__(self):
         try:

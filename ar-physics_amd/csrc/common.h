@@ -81,6 +81,7 @@ struct ScanArgs {
     uint32_t* ticket;               // 2
     uint32_t* epoch;                // 1
     uint32_t* stats;                // zeroed for the following row sort
+    uint32_t* stamp;                // build counter of the target grid, advanced once per build
 };
 
 struct ScatterArgs {
@@ -119,7 +120,7 @@ struct RowSortArgs {
     uint32_t* gGrp;
     uint32_t* rowOff;       // nrows * rowstride: global start of every cell of an occupied row
     uint32_t* rowStamp;     // nrows: build stamp of the last build that found the row occupied
-    uint32_t stamp;
+    const uint32_t* stamp;  // current build counter (advanced by the scan of this build)
     const uint32_t* bP;     // boundary prefix per cell (ncells+1)
     const uint32_t* rP;     // reference prefix per cell (ncells+1)
     int full;
@@ -181,7 +182,7 @@ struct FluidArgs {
     const uint32_t* perm;
     const uint32_t* rowOff;
     const uint32_t* rowStamp;
-    uint32_t stamp;
+    const uint32_t* stamp;  // build counter the row stamps are compared with
     int identity;           // the build reordered the state: sorted slot == storage index
     const uint32_t* sflag;  // per storage index: no bucket slot
     const uint32_t* stats;  // [0] particles refused by the cap in this build (may be null)
@@ -230,7 +231,7 @@ struct SolidArgs {
     const uint32_t* perm;
     const uint32_t* rowOff;
     const uint32_t* rowStamp;
-    uint32_t stamp;
+    const uint32_t* stamp;  // build counter the row stamps are compared with
     const uint32_t* sflag;    // per storage index: 1 when refused by the cap / non-finite
     const uint32_t* pid;      // storage -> particle id
     const uint32_t* sidByPid;
@@ -254,7 +255,7 @@ struct AdjArgs {
     const uint32_t* perm;
     const uint32_t* rowOff;
     const uint32_t* rowStamp;
-    uint32_t stamp;
+    const uint32_t* stamp;  // build counter the row stamps are compared with
     int identity;
     const uint32_t* sflag;
     const uint32_t* bP;
@@ -311,6 +312,16 @@ struct ReduceArgs {
     uint32_t nblocks, width;
     const float* part;  // nblocks * width
     float* out;         // width, accumulated (+=)
+};
+
+// A batch of independent scenes in one launch: block L belongs to scene blockScene[L] and
+// is block L - blockFirst[s] of blockFirst[s+1] - blockFirst[s] blocks of that scene; the
+// scene's argument block is args + s * stride.
+struct BatchArgs {
+    const void* args;
+    uint32_t stride;
+    const uint32_t* blockScene;
+    const uint32_t* blockFirst;  // nscenes + 1
 };
 
 struct AosInArgs {

@@ -126,8 +126,8 @@ __device__ __forceinline__ float contact_potential(uint32_t eq, float r, float s
 // ------------------------------------------------------------------------------------
 
 // Cell key of every particle + slot in its (y,z) row, reserved by wave-aggregated atomics.
-extern "C" __global__ void __launch_bounds__(256) k_key(KeyArgs a) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void key_body(const KeyArgs& a, uint32_t bid, uint32_t) {
+    const uint32_t i = bid * 256u + threadIdx.x;
     const bool in = i < a.n;
     uint32_t bin = 0, key = kInvalid;
     if (in) {
@@ -160,7 +160,7 @@ __device__ __forceinline__ void st_state(unsigned long long* p, unsigned long lo
 }
 constexpr unsigned long long kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStMask = 3ull << 62;
 
-extern "C" __global__ void __launch_bounds__(256) k_scan(ScanArgs a) {
+__device__ __forceinline__ void scan_body(const ScanArgs& a, uint32_t, uint32_t) {
     __shared__ uint32_t sTile, sEpoch;
     __shared__ uint32_t wsumC[4], wsumF[4];
     __shared__ unsigned long long sPrefix;
@@ -222,6 +222,7 @@ extern "C" __global__ void __launch_bounds__(256) k_scan(ScanArgs a) {
         if (tile == 0) {
             a.ticket[par ^ 1u] = 0;
             if (a.stats) a.stats[0] = 0;
+            if (a.stamp) *a.stamp += 1u;
         }
         if (tile == a.ntiles - 1) {
             const unsigned long long tot = prefix + agg;
@@ -256,8 +257,8 @@ __device__ __forceinline__ void move_particle(const ScatterArgs& a, uint32_t src
 
 // Scatter (key, source) into the reserved row slots; particles with non-finite positions
 // go straight to their final place after all rows (neighbor_list.rs:86-93 drops them).
-extern "C" __global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t) {
+    const uint32_t i = bid * 256u + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t key = a.keys[i], rank = a.ranks[i];
     if (key != kInvalid) {
@@ -281,7 +282,7 @@ extern "C" __global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) {
 // One workgroup per occupied (y,z) row: LDS counting sort by x, then ascending particle id
 // inside each cell (the order the 128-slot cap keeps), per-row cell offsets, cap flags,
 // and the permutation / reorder of the state into cell-major storage.
-extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
+__device__ __forceinline__ void rowsort_body(const RowSortArgs& a, uint32_t bid, uint32_t nb) {
     __shared__ uint32_t cnt[kMaxSX + 1];
     __shared__ uint32_t off[kMaxSX + 1];
     __shared__ uint32_t ePid[kRowCap];
@@ -291,7 +292,7 @@ extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
     const uint32_t tid = threadIdx.x;
     const uint32_t sx = a.grid.sub[0];
     const uint32_t nact = *a.numActive;
-    for (uint32_t ri = blockIdx.x; ri < nact; ri += gridDim.x) {
+    for (uint32_t ri = bid; ri < nact; ri += nb) {
         const uint32_t row = a.activeRows[ri];
         const uint32_t s0 = a.binStart[row];
         const uint32_t ne = a.binStart[row + 1] - s0;
@@ -334,7 +335,7 @@ extern "C" __global__ void __launch_bounds__(256) k_rowsort(RowSortArgs a) {
             ro[x] = s0 + off[x];
             if (x < sx) cnt[x] = 0;  // reused as fill counters
         }
-        if (tid == 0) a.rowStamp[row] = a.stamp;
+        if (tid == 0) a.rowStamp[row] = *a.stamp;
         __syncthreads();
         for (uint32_t e = tid; e < ne; e += 256) {
             uint32_t x = big ? a.tmpKey[s0 + e] - row * sx : eX[e];
@@ -484,7 +485,7 @@ struct PairIn {
 #define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
 
-extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
+__device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
     __shared__ uint32_t list[kList * 256];
     __shared__ float4 smat[kMatLds];                // mass, visc, bulk_visc
     __shared__ DInter sint[kMatLds * kMatLds];
@@ -497,7 +498,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
         for (uint32_t m = tid; m < nm * nm; m += 256) sint[m] = a.inter[m];
         __syncthreads();
     }
-    const uint32_t t = xcd_block() * 256u + tid;
+    const uint32_t t = bid * 256u + tid;
     const bool act = t < a.n;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
     const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
@@ -616,6 +617,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             uint32_t cx, cy, cz;
             cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
+            const uint32_t stampv = *a.stamp;
             const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
             // metadata of one neighbour row: [mobile lo, hi), [boundary lo, hi); all four
             // loads are independent, and the next row's are issued before this row is scanned
@@ -630,7 +632,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
                     const uint32_t lo = ro[xl], hi = ro[xh + 1];
                     m.z = a.bP[cb + xl];
                     m.w = a.bP[cb + xh + 1];
-                    if (st == a.stamp) {
+                    if (st == stampv) {
                         m.x = lo;
                         m.y = hi;
                     }
@@ -735,6 +737,41 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) {
             a.rk.ranks[i] = rank;
         }
     }
+}
+
+// ------------------------------------------------------------------------------------
+// Entry points of the step kernels: one scene per launch, or a batch of independent scenes
+// in one launch (each block looks up its scene's argument block and its block index within
+// that scene).
+// ------------------------------------------------------------------------------------
+template <class A>
+__device__ __forceinline__ const A& batch_scene(const BatchArgs& b, uint32_t L, uint32_t& bid, uint32_t& nb) {
+    const uint32_t s = b.blockScene[L];
+    bid = L - b.blockFirst[s];
+    nb = b.blockFirst[s + 1] - b.blockFirst[s];
+    return *reinterpret_cast<const A*>(reinterpret_cast<const char*>(b.args) + (size_t)s * b.stride);
+}
+
+#define ARP_ENTRIES(name, Args, body)                                                     \
+    extern "C" __global__ void __launch_bounds__(256) name(Args a) { body(a, blockIdx.x, gridDim.x); } \
+    extern "C" __global__ void __launch_bounds__(256) name##_batch(BatchArgs b) {         \
+        uint32_t bid, nb;                                                                 \
+        const Args& a = batch_scene<Args>(b, blockIdx.x, bid, nb);                        \
+        body(a, bid, nb);                                                                 \
+    }
+
+ARP_ENTRIES(k_key, KeyArgs, key_body)
+ARP_ENTRIES(k_scan, ScanArgs, scan_body)
+ARP_ENTRIES(k_scatter, ScatterArgs, scatter_body)
+ARP_ENTRIES(k_rowsort, RowSortArgs, rowsort_body)
+
+// k_fluid: blocks are remapped so that each XCD processes a contiguous share of the
+// cell-sorted particles (of the whole batch)
+extern "C" __global__ void __launch_bounds__(256) k_fluid(FluidArgs a) { fluid_body(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_batch(BatchArgs b) {
+    uint32_t bid, nb;
+    const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
+    fluid_body(a, bid, nb);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1202,7 +1239,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid(SolidArgs a) {
                     const int ny = (int)ci[1] + dy;
                     if (ny < 0 || ny >= (int)sy) continue;
                     const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
-                    if (a.rowStamp[row] != a.stamp) continue;
+                    if (a.rowStamp[row] != *a.stamp) continue;
                     const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
                     for (uint32_t q = ro[xl]; q < ro[xh + 1]; q++) {
                         const uint32_t pe = a.perm[q];
@@ -1463,6 +1500,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
             uint32_t cx, cy, cz;
             cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
+            const uint32_t stampv = *a.stamp;
             const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
             uint32_t nl = 0;
             int rw = -1, part = 1;
@@ -1485,7 +1523,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
                             const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
                             const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
                             const size_t cb = (size_t)row * sx;
-                            if (a.rowStamp[row] == a.stamp) {
+                            if (a.rowStamp[row] == stampv) {
                                 cur.x = ro[xl];
                                 cur.y = ro[xh + 1];
                             }

@@ -53,13 +53,20 @@ enum Kern {
     K_ADJ_PERM,
     K_GRAD_AOS,
     K_REDUCE,
+    K_KEY_B,
+    K_SCAN_B,
+    K_SCATTER_B,
+    K_ROWSORT_B,
+    K_FLUID_B,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
                                    "k_fluid",   "k_aos_in",  "k_aos_out",  "k_lincomb",
                                    "k_velocity", "k_fill",   "k_clear_solids", "k_strain",
                                    "k_solid",   "k_eps_in",  "k_eps_out", "k_fluid_adj",
-                                   "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce"};
+                                   "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce",
+                                   "k_key_batch", "k_scan_batch", "k_scatter_batch", "k_rowsort_batch",
+                                   "k_fluid_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -80,13 +87,13 @@ struct GridSlot {
     uint32_t* sflag = nullptr;
     uint32_t* rowOff = nullptr;
     uint32_t* rowStamp = nullptr;
-    uint32_t stamp = 0;
+    uint32_t* stamp = nullptr;  // device build counter of this slot (rows stamped with it are current)
     bool full = false;
 };
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
 struct RecStep {
-    DBuf mk, pid, src;
+    DBuf mk, pid, src, stamps;
     DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
     DBuf perm[ARP_MAX_STAGES], sflag[ARP_MAX_STAGES], rowOff[ARP_MAX_STAGES], rowStamp[ARP_MAX_STAGES];
     GridSlot slot[ARP_MAX_STAGES];
@@ -94,6 +101,14 @@ struct RecStep {
 
 struct Checkpoint {
     DBuf P, V, mk, pid;
+};
+
+// A launch recorded instead of issued (the batch path collects one per scene and issues
+// them together).
+struct Launch {
+    int k;
+    uint32_t blocks, threads;
+    std::vector<uint8_t> args;
 };
 
 }  // namespace
@@ -137,7 +152,7 @@ struct arp_ctx {
     // grid
     DBuf keys, ranks, binCount, binStart, activeRows, numActive, tmpKey, tmpSrc, perm, rowOff,
         gPid, gGrp, stats, scanState, scanTicket, scanEpoch, rowStamp;
-    uint32_t stamp = 0;     // build counter written into rowStamp by the row sort
+    DBuf stampDev;          // build counter written into rowStamp by the row sort
     GridSlot slot;          // where builds go (the buffers above unless recording)
     // adjoint recording
     bool rec_valid = false;
@@ -156,6 +171,8 @@ struct arp_ctx {
     // profiling: event pairs around launches
     bool prof = false;
     bool prof_sel[K_COUNT];  // kernels timed while profiling
+    std::vector<Launch>* capture = nullptr;  // record launches here instead of issuing them
+    uint32_t rowsort_div = 32;               // particles per row-sort workgroup (tuning knob)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> ev_marks;  // (kernel, index of first event)
@@ -174,7 +191,8 @@ struct arp_ctx {
                 &Xb[1][0], &Xb[1][1],  &Acc[0],     &Acc[1],   &inv,    &ps_mk[0], &ps_mk[1],
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
-                &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp};
+                &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp,
+                &stampDev};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
             bufs.push_back(&K[k][1]);
@@ -240,6 +258,12 @@ hipEvent_t next_event(arp_ctx* c) {
 template <class A>
 int launch(arp_ctx* c, Kern k, uint32_t blocks, uint32_t threads, const A& args) {
     if (blocks == 0) return ARP_OK;
+    if (c->capture) {
+        Launch L{(int)k, blocks, threads, std::vector<uint8_t>(sizeof(A))};
+        memcpy(L.args.data(), &args, sizeof(A));
+        c->capture->push_back(std::move(L));
+        return ARP_OK;
+    }
     void* params[] = {const_cast<A*>(&args)};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof && c->prof_sel[k]) {
@@ -484,7 +508,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     ScanArgs s{nbins, ntiles, P<uint32_t>(c->binCount), P<uint32_t>(c->binStart),
                P<uint32_t>(c->activeRows), P<uint32_t>(c->numActive),
                P<unsigned long long>(c->scanState), P<uint32_t>(c->scanTicket), P<uint32_t>(c->scanEpoch),
-               P<uint32_t>(c->stats)};
+               P<uint32_t>(c->stats), c->slot.stamp};
     if ((rc = launch(c, K_SCAN, ntiles, 256, s))) return rc;
     ScatterArgs sc{};
     sc.grid = c->grid;
@@ -521,7 +545,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.gGrp = P<uint32_t>(c->gGrp);
     r.rowOff = g.rowOff;
     r.rowStamp = g.rowStamp;
-    r.stamp = g.stamp = ++c->stamp;
+    r.stamp = g.stamp;
     g.full = io.full != 0;
     r.bP = P<uint32_t>(c->bP);
     r.rP = P<uint32_t>(c->rP);
@@ -539,7 +563,10 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.stats = P<uint32_t>(c->stats);
     r.sflag = g.sflag;
     r.srcOut = io.full ? io.srcOut : nullptr;
-    uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(c->grid.nrows, 1), 4096);
+    // one workgroup per occupied row, looping; enough workgroups for about rowsort_div
+    // particles each (a tuning knob), at most 4096
+    const uint32_t want = std::max<uint32_t>(1, (c->n + c->rowsort_div - 1) / c->rowsort_div);
+    uint32_t blocks = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(c->grid.nrows, 1), 4096), want);
     return launch(c, K_ROWSORT, blocks, 256, r);
 }
 
@@ -570,6 +597,7 @@ GridSlot ctx_slot(arp_ctx* c) {
     g.sflag = P<uint32_t>(c->sflag);
     g.rowOff = P<uint32_t>(c->rowOff);
     g.rowStamp = P<uint32_t>(c->rowStamp);
+    g.stamp = P<uint32_t>(c->stampDev);
     return g;
 }
 
@@ -622,7 +650,8 @@ int alloc_state(arp_ctx* c) {
     if ((rc = dalloc(c, c->rowOff, 4 * (size_t)c->grid.nrows * c->grid.rowstride))) return rc;
     if ((rc = dalloc(c, c->rowStamp, 4 * (size_t)c->grid.nrows))) return rc;
     HIPCHK(c, hipMemsetAsync(c->rowStamp.p, 0, 4 * (size_t)c->grid.nrows, c->stream));
-    c->stamp = 0;
+    if ((rc = dalloc(c, c->stampDev, 16))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->stampDev.p, 0, 16, c->stream));
     c->slot = ctx_slot(c);
     c->rec_valid = false;
     return ARP_OK;
@@ -844,19 +873,21 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
     int rc;
     for (DBuf* b : {&R.mk, &R.pid, &R.src})
         if ((rc = dalloc(c, *b, 4 * n))) return rc;
+    if ((rc = dalloc(c, R.stamps, 4 * ARP_MAX_STAGES))) return rc;
+    HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
     for (int s = 0; s < S; s++) {
         if ((rc = dalloc(c, R.Xp[s], 16 * n)) || (rc = dalloc(c, R.Xv[s], 16 * n)) ||
             (rc = dalloc(c, R.perm[s], 4 * n)) || (rc = dalloc(c, R.sflag[s], 4 * n)) ||
             (rc = dalloc(c, R.rowOff[s], 4 * nr * c->grid.rowstride)) || (rc = dalloc(c, R.rowStamp[s], 4 * nr)))
             return rc;
-        // stamps are unique per build, so rows left from an older build never match
+        // the slot's counter only grows, so rows left from an older build never match
         HIPCHK(c, hipMemsetAsync(R.rowStamp[s].p, 0, 4 * nr, c->stream));
         GridSlot& g = R.slot[s];
         g.perm = P<uint32_t>(R.perm[s]);
         g.sflag = P<uint32_t>(R.sflag[s]);
         g.rowOff = P<uint32_t>(R.rowOff[s]);
         g.rowStamp = P<uint32_t>(R.rowStamp[s]);
-        g.stamp = 0;
+        g.stamp = P<uint32_t>(R.stamps) + s;
         g.full = false;
     }
     return ARP_OK;
@@ -873,7 +904,7 @@ void free_buf(DBuf& b) {
 void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
     for (size_t i = nrec; i < c->rec.size(); i++) {
         RecStep& R = c->rec[i];
-        for (DBuf* b : {&R.mk, &R.pid, &R.src}) free_buf(*b);
+        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps}) free_buf(*b);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
     }
@@ -1765,6 +1796,345 @@ int arp_cell_counts(arp_ctx* c, uint64_t first, uint64_t count, uint32_t* out) {
         out[3 * k + 2] = p;
     }
     return ARP_OK;
+}
+
+
+// ====================================================================================
+// Batched ensembles: independent scenes (one context each) stepped together, one launch
+// per kernel and RK stage for the whole batch, replayed from a captured hipGraph.
+// ====================================================================================
+}  // extern "C"
+
+namespace {
+
+// host-side per-step state of a context (what step_once changes besides device memory)
+struct HostState {
+    int cur;
+    bool keys_valid, have_build, rec_valid;
+    GridSlot slot;
+};
+
+HostState save_host(const arp_ctx* c) { return HostState{c->cur, c->keys_valid, c->have_build, c->rec_valid, c->slot}; }
+
+void load_host(arp_ctx* c, const HostState& h) {
+    c->cur = h.cur;
+    c->keys_valid = h.keys_valid;
+    c->have_build = h.have_build;
+    c->rec_valid = h.rec_valid;
+    c->slot = h.slot;
+}
+
+Kern batch_kernel(int k) {
+    switch (k) {
+        case K_KEY: return K_KEY_B;
+        case K_SCAN: return K_SCAN_B;
+        case K_SCATTER: return K_SCATTER_B;
+        case K_ROWSORT: return K_ROWSORT_B;
+        case K_FLUID: return K_FLUID_B;
+        default: return K_COUNT;
+    }
+}
+
+struct BLaunch {
+    Kern k;
+    uint32_t blocks, stride;
+    DBuf args, scene, first;
+    std::vector<uint8_t> blob;  // host copy (plan comparison)
+    std::vector<uint32_t> hfirst, hscene;  // host sources of the uploads
+};
+
+}  // namespace
+
+struct arp_batch {
+    std::vector<arp_ctx*> ctx;
+    arp_ctx* lead = nullptr;
+    uint32_t flags = 0;
+    std::vector<BLaunch> plan[2];
+    std::vector<HostState> after[2];
+    float plan_dt = 0.0f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;  // one per context + one for the batch stream
+    std::string err;
+};
+
+namespace {
+
+int bfail(arp_batch* b, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    b->err = buf;
+    if (b->lead) b->lead->err = buf;
+    return code;
+}
+
+void free_plan(arp_batch* b) {
+    for (auto& P : b->plan)
+        for (BLaunch& L : P)
+            for (DBuf* d : {&L.args, &L.scene, &L.first}) free_buf(*d);
+    b->plan[0].clear();
+    b->plan[1].clear();
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Capture two consecutive steps of every scene (the device buffers alternate with period
+// two) and lay them out as batched launches.  Host state ends as after two steps; the
+// caller issues the steps and then sets the state for the number actually run.
+int build_plan(arp_batch* b, float dt) {
+    const size_t ns = b->ctx.size();
+    arp_ctx* L0 = b->lead;
+    int rc;
+    for (arp_ctx* c : b->ctx)
+        if (!c->keys_valid) {  // bin the current state once so every step has the same launches
+            if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
+            c->keys_valid = true;
+        }
+    std::vector<HostState> start(ns);
+    for (size_t s = 0; s < ns; s++) start[s] = save_host(b->ctx[s]);
+    std::vector<BLaunch> plan[2];
+    for (int p = 0; p < 2; p++) {
+        std::vector<std::vector<Launch>> lists(ns);
+        for (size_t s = 0; s < ns; s++) {
+            arp_ctx* c = b->ctx[s];
+            c->capture = &lists[s];
+            rc = step_once(c, dt);
+            c->capture = nullptr;
+            if (rc) {
+                for (size_t t = 0; t < ns; t++) load_host(b->ctx[t], start[t]);
+                return bfail(b, rc, "scene %zu: %s", s, c->err.c_str());
+            }
+            b->after[p].resize(ns);
+            b->after[p][s] = save_host(c);
+        }
+        const size_t nl = lists[0].size();
+        for (size_t s = 1; s < ns; s++) {
+            bool same = lists[s].size() == nl;
+            for (size_t j = 0; same && j < nl; j++) same = lists[s][j].k == lists[0][j].k;
+            if (!same) {
+                for (size_t t = 0; t < ns; t++) load_host(b->ctx[t], start[t]);
+                return bfail(b, ARP_E_ARG, "scene %zu runs a different launch sequence (integrator or solid path differ)", s);
+            }
+        }
+        for (size_t j = 0; j < nl; j++) {
+            BLaunch B;
+            B.k = batch_kernel(lists[0][j].k);
+            if (B.k == K_COUNT) {
+                for (size_t t = 0; t < ns; t++) load_host(b->ctx[t], start[t]);
+                return bfail(b, ARP_E_ARG, "kernel %s has no batched form (scenes with solids are not batched)",
+                             kKernNames[lists[0][j].k]);
+            }
+            B.stride = (uint32_t)lists[0][j].args.size();
+            std::vector<uint32_t> first(ns + 1, 0), scene;
+            B.blob.resize((size_t)B.stride * ns);
+            for (size_t s = 0; s < ns; s++) {
+                const Launch& l = lists[s][j];
+                memcpy(B.blob.data() + (size_t)s * B.stride, l.args.data(), B.stride);
+                first[s + 1] = first[s] + l.blocks;
+                scene.insert(scene.end(), l.blocks, (uint32_t)s);
+            }
+            B.blocks = first[ns];
+            B.hfirst = std::move(first);
+            B.hscene = std::move(scene);
+            if (B.hscene.empty()) B.hscene.push_back(0);
+            if ((rc = upload(L0, B.args, B.blob.data(), B.blob.size())) ||
+                (rc = upload(L0, B.first, B.hfirst.data(), 4 * B.hfirst.size())) ||
+                (rc = upload(L0, B.scene, B.hscene.data(), 4 * B.hscene.size())))
+                return rc;
+            plan[p].push_back(std::move(B));
+        }
+    }
+    // keep the previous graph when the launches are unchanged
+    bool same = b->plan[0].size() == plan[0].size() && b->plan[1].size() == plan[1].size();
+    for (int p = 0; same && p < 2; p++)
+        for (size_t j = 0; same && j < plan[p].size(); j++)
+            same = plan[p][j].k == b->plan[p][j].k && plan[p][j].blob == b->plan[p][j].blob &&
+                   plan[p][j].blocks == b->plan[p][j].blocks;
+    if (same) {
+        for (auto& P : plan)
+            for (BLaunch& L : P)
+                for (DBuf* d : {&L.args, &L.scene, &L.first}) free_buf(*d);
+    } else {
+        free_plan(b);
+        b->plan[0] = std::move(plan[0]);
+        b->plan[1] = std::move(plan[1]);
+    }
+    HIPCHK(L0, hipStreamSynchronize(L0->stream));  // the uploads read pageable host memory
+    b->plan_dt = dt;
+    return ARP_OK;
+}
+
+int issue(arp_batch* b, int p) {
+    for (BLaunch& L : b->plan[p]) {
+        BatchArgs a{L.args.p, L.stride, P<uint32_t>(L.scene), P<uint32_t>(L.first)};
+        int rc = launch(b->lead, L.k, L.blocks, 256, a);
+        if (rc) return rc;
+    }
+    return ARP_OK;
+}
+
+// order the batch stream after (before) the contexts' own streams
+int join_streams(arp_batch* b, bool into_batch) {
+    arp_ctx* L0 = b->lead;
+    const size_t ns = b->ctx.size();
+    if (b->ev.size() < ns + 1) {
+        for (size_t i = b->ev.size(); i < ns + 1; i++) {
+            hipEvent_t e;
+            HIPCHK(L0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->ev.push_back(e);
+        }
+    }
+    if (into_batch) {
+        for (size_t s = 0; s < ns; s++) {
+            arp_ctx* c = b->ctx[s];
+            if (c->stream == L0->stream) continue;
+            HIPCHK(L0, hipEventRecord(b->ev[s], c->stream));
+            HIPCHK(L0, hipStreamWaitEvent(L0->stream, b->ev[s], 0));
+        }
+    } else {
+        HIPCHK(L0, hipEventRecord(b->ev[ns], L0->stream));
+        for (arp_ctx* c : b->ctx)
+            if (c->stream != L0->stream) HIPCHK(L0, hipStreamWaitEvent(c->stream, b->ev[ns], 0));
+    }
+    return ARP_OK;
+}
+
+int run_batch(arp_batch* b, float dt, uint32_t nsteps) {
+    arp_ctx* L0 = b->lead;
+    int rc;
+    if (nsteps == 0) return ARP_OK;
+    if ((rc = build_plan(b, dt))) return rc;
+    if ((rc = join_streams(b, true))) return rc;
+    const bool graph = (b->flags & ARP_BATCH_GRAPH) && !L0->prof && nsteps >= 2;
+    if (graph) {
+        if (!b->exec) {
+            HIPCHK(L0, hipStreamBeginCapture(L0->stream, hipStreamCaptureModeThreadLocal));
+            int rc2 = issue(b, 0);
+            if (!rc2) rc2 = issue(b, 1);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(L0->stream, &g);
+            if (rc2) return rc2;
+            if (e != hipSuccess) return bfail(b, ARP_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            b->graph = g;
+            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) return bfail(b, ARP_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        for (uint32_t k = 0; k < nsteps / 2; k++) HIPCHK(L0, hipGraphLaunch(b->exec, L0->stream));
+        if (nsteps & 1u)
+            if ((rc = issue(b, 0))) return rc;
+    } else {
+        for (uint32_t k = 0; k < nsteps; k++)
+            if ((rc = issue(b, (int)(k & 1u)))) return rc;
+    }
+    const int last = (nsteps & 1u) ? 0 : 1;
+    for (size_t s = 0; s < b->ctx.size(); s++) {
+        load_host(b->ctx[s], b->after[last][s]);
+        b->ctx[s]->time += (double)dt * nsteps;
+    }
+    return join_streams(b, false);
+}
+
+}  // namespace
+
+extern "C" {
+
+int arp_batch_create(arp_ctx* const* ctxs, uint32_t nscenes, uint32_t flags, arp_batch** out) {
+    if (!out || !ctxs || nscenes == 0) return ARP_E_ARG;
+    *out = nullptr;
+    arp_ctx* L0 = ctxs[0];
+    for (uint32_t s = 0; s < nscenes; s++) {
+        arp_ctx* c = ctxs[s];
+        if (!c) return ARP_E_ARG;
+        if (!c->have_scene) return fail(L0, ARP_E_STATE, "scene %u has no scene uploaded", s);
+        if (c->device != L0->device) return fail(L0, ARP_E_ARG, "scene %u is on another device", s);
+        if (c->solids_active) return fail(L0, ARP_E_ARG, "scene %u runs the solid path (batches hold fluid scenes)", s);
+        if (memcmp(&c->tab, &L0->tab, sizeof(ArpTableau)) != 0)
+            return fail(L0, ARP_E_ARG, "scene %u uses another integrator", s);
+        for (uint32_t t = 0; t < s; t++)
+            if (ctxs[t] == c) return fail(L0, ARP_E_ARG, "scene %u appears twice", s);
+    }
+    arp_batch* b = new (std::nothrow) arp_batch();
+    if (!b) return ARP_E_NOMEM;
+    b->ctx.assign(ctxs, ctxs + nscenes);
+    b->lead = L0;
+    b->flags = flags;
+    *out = b;
+    return ARP_OK;
+}
+
+void arp_batch_destroy(arp_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->lead->device);
+    hipStreamSynchronize(b->lead->stream);
+    free_plan(b);
+    for (hipEvent_t e : b->ev) hipEventDestroy(e);
+    delete b;
+}
+
+const char* arp_batch_last_error(const arp_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int arp_batch_step(arp_batch* b, float dt, uint32_t nsteps) {
+    if (!b) return ARP_E_ARG;
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return bfail(b, ARP_E_ARG, "dt must be positive");
+    HIPCHK(b->lead, hipSetDevice(b->lead->device));
+    return run_batch(b, dt, nsteps);
+}
+
+int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chosen_div) {
+    if (!b) return ARP_E_ARG;
+    arp_ctx* L0 = b->lead;
+    HIPCHK(L0, hipSetDevice(L0->device));
+    if (trial_steps == 0) trial_steps = 2;
+    // states are saved and restored around every trial: tuning leaves the scenes untouched
+    const size_t ns = b->ctx.size();
+    std::vector<Checkpoint> ck(ns);
+    std::vector<HostState> hs(ns);
+    std::vector<double> tm(ns);
+    int rc;
+    for (size_t s = 0; s < ns; s++) {
+        if ((rc = ck_save(b->ctx[s], ck[s]))) return rc;
+        hs[s] = save_host(b->ctx[s]);
+        tm[s] = b->ctx[s]->time;
+    }
+    const uint32_t cand[] = {16, 32, 64, 128, 512};
+    hipEvent_t e0, e1;
+    HIPCHK(L0, hipEventCreate(&e0));
+    HIPCHK(L0, hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t bestDiv = cand[0];
+    for (uint32_t div : cand) {
+        for (arp_ctx* c : b->ctx) c->rowsort_div = div;
+        // one untimed step builds the plan and warms the caches
+        if ((rc = run_batch(b, dt, 1))) break;
+        HIPCHK(L0, hipEventRecord(e0, L0->stream));
+        if ((rc = run_batch(b, dt, trial_steps))) break;
+        HIPCHK(L0, hipEventRecord(e1, L0->stream));
+        HIPCHK(L0, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(L0, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bestDiv = div;
+        }
+        for (size_t s = 0; s < ns; s++) {
+            load_host(b->ctx[s], hs[s]);  // first: the restore writes the meta of this parity
+            if ((rc = ck_restore(b->ctx[s], ck[s]))) break;
+            b->ctx[s]->time = tm[s];
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (arp_ctx* c : b->ctx) c->rowsort_div = bestDiv;
+    for (Checkpoint& k : ck)
+        for (DBuf* d : {&k.P, &k.V, &k.mk, &k.pid}) free_buf(*d);
+    if (chosen_div) *chosen_div = bestDiv;
+    return rc;
 }
 
 }  // extern "C"

@@ -215,6 +215,25 @@ int arp_adjoint_record(arp_ctx* ctx, float dt, uint32_t nsteps, uint32_t checkpo
  * the end of the recorded trajectory. */
 int arp_adjoint_backward(arp_ctx* ctx, const void* d_grad_out, void* d_grad_in, float* grad_params);
 
+/* ---- batched ensembles (no counterpart in the reference: one FluidSim per scene) ------- */
+/* Independent fluid scenes, one context each (uploaded as usual, same device, same
+ * integrator), stepped together: every kernel of an RK stage runs once for the whole batch
+ * (each workgroup looks up its scene), and with ARP_BATCH_GRAPH the two-step launch cycle is
+ * captured in a hipGraph and replayed.  The contexts keep working on their own between
+ * batch calls (download, upload, ...).  The batch runs on the first context's stream and
+ * is ordered after / before the other contexts' streams. */
+typedef struct arp_batch arp_batch;
+#define ARP_BATCH_GRAPH 1u
+int arp_batch_create(arp_ctx* const* ctxs, uint32_t nscenes, uint32_t flags, arp_batch** out);
+void arp_batch_destroy(arp_batch* batch);
+const char* arp_batch_last_error(const arp_batch* batch);
+/* nsteps RK steps of length dt on every scene (the time of each context advances). */
+int arp_batch_step(arp_batch* batch, float dt, uint32_t nsteps);
+/* Autotuner: times trial_steps batch steps for each candidate row-sort workgroup size
+ * (particles per workgroup) and keeps the fastest (*chosen_div); the scenes' states are
+ * restored after every trial. */
+int arp_batch_tune(arp_batch* batch, float dt, uint32_t trial_steps, uint32_t* chosen_div);
+
 #ifdef __cplusplus
 }
 #endif
