@@ -98,6 +98,50 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
                 k_fluid_adj_avg_us=1e3 * ms / cnt if cnt else None, grad_finite=finite)
 
 
+def time_ensemble(nscenes, steps, dt):
+    """Independent fluid scenes (log-uniform 2,048-65,536 particles, alpha uniform in
+    [0.01, 0.1], seeded) stepped as one batch (graph replay, tuned), against the same scenes
+    stepped one after the other."""
+    import numpy as np
+    import torch
+
+    from arp import scene as S
+    from arp.sim import Batch, FluidSim
+
+    log("ensemble: building %d scenes" % nscenes)
+    scenes = S.ensemble_3d(count=nscenes)
+    sims = [FluidSim(sc, integrator="rk2", timestep=dt) for sc in scenes]
+    npart = sum(sc.n for sc in scenes)
+    B = Batch(sims, graph=True)
+    div = B.tune(dt, 2)
+    B.step(dt, 2)
+    B.synchronize()
+    torch.cuda.synchronize()
+    log("ensemble: timing %d batched steps (%d particles)" % (steps, npart))
+    t0 = time.perf_counter()
+    B.step(dt, steps)
+    B.synchronize()
+    sec = time.perf_counter() - t0
+    # the same scenes one launch sequence each, a few steps
+    seq_steps = 5
+    for s in sims:
+        s.synchronize()
+    t1 = time.perf_counter()
+    for s in sims:
+        s.step(dt, seq_steps)
+    for s in sims:
+        s.synchronize()
+    seq = time.perf_counter() - t1
+    finite = all(bool(np.isfinite(s.particles()["pos"]).all()) for s in sims[:8])
+    B.close()
+    for s in sims:
+        s.close()
+    return dict(scenes=nscenes, particles=npart, steps=steps, integrator="rk2", dt=dt,
+                ms_per_step=1e3 * sec / steps, particle_steps_per_s=npart * steps / sec,
+                sequential_particle_steps_per_s=npart * seq_steps / seq, graph=True,
+                tuned_rowsort_particles_per_workgroup=div, state_finite=finite)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +157,9 @@ def main():
                          "kernel only, which keeps the event overhead off the other launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="scenes of the batched-ensemble measurement (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=10,
                     help="steps of the recorded rollout whose reverse sweep is timed (0: skip)")
     args = ap.parse_args()
@@ -132,7 +179,7 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np  # noqa: F401
+    import numpy as np
 
     from arp import scene as S
     from arp.sim import FluidSim
@@ -169,10 +216,14 @@ def main():
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
     sim.profile(False)
     fwd_adj = None
+    ens = None
     if args.adjoint_steps > 0:
         fwd_adj = time_adjoint(sim, sc, dt, args.adjoint_steps, 1e3 * sec / args.steps)
     p = sim.particles()
     finite = bool(np.isfinite(p["pos"]).all())
+    sim.close()
+    if args.ensemble > 0:
+        ens = time_ensemble(args.ensemble, args.ensemble_steps, dt)
     value = sc.n * args.steps / sec
     f = kern.get("k_fluid")
     bytes_launch = B_ALG_FLUID * sc.n
@@ -216,6 +267,7 @@ def main():
         "kernels": kern,
         "state_finite": finite,
         "fwd_adjoint": fwd_adj,
+        "ensemble": ens,
     }
     if not args.no_cpu:
         log("cpu baseline: %d step(s), %d threads" % (args.cpu_steps, args.cpu_threads))
