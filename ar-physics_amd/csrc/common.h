@@ -265,8 +265,9 @@ struct AdjArgs {
     const float* Xposd;
     const float* Xvelq;
     const uint32_t* mk;
-    const float* Wp;   // cotangent of k: (pos'.xyz, den')
-    const float* Wv;   // (vel'.xyz, 0)
+    const float* Wp;   // cotangent of k: wscale * (pos'.xyz, den')
+    const float* Wv;   // wscale * (vel'.xyz, 0)
+    float wscale;
     float* Gp;         // dL/dX: (x.xyz, rho)
     float* Gv;         // (v.xyz, 0)
     float* LamP;       // step cotangent accumulator (+= G)

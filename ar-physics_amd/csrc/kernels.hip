@@ -1458,7 +1458,7 @@ __device__ void pair_vjp(const PairSide& P, const DInter& it, float rx, float ry
     }
 }
 
-extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) {
     __shared__ uint32_t list[kList * 256];
     __shared__ float spg[2 * kMatLds];
     const uint32_t tid = threadIdx.x;
@@ -1491,7 +1491,10 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
         const uint32_t mat1 = mk1 & kMatMask;
         const DMat& M1 = a.mats[mat1];
         const float rho1 = xi.w;
-        const float4 wp = WP[i], wv = WV[i];
+        const float ws = a.wscale;
+        float4 wp = WP[i], wv = WV[i];
+        wp = make_float4(ws * wp.x, ws * wp.y, ws * wp.z, ws * wp.w);
+        wv = make_float4(ws * wv.x, ws * wv.y, ws * wv.z, 0.0f);
         const float ir1 = 1.0f / rho1;
         const float a1x = wv.x * ir1, a1y = wv.y * ir1, a1z = wv.z * ir1;
         const float dq1 = pressure_q_d(M1, rho1);
@@ -1502,12 +1505,32 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
             const uint32_t stampv = *a.stamp;
             const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
+            // the candidate sweep of the forward kernel: branch-free batches, one drain site
+            auto row_meta = [&](int rw, uint4& m) {
+                m = make_uint4(0, 0, 0, 0);
+                const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
+                if (rw < 9 && nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
+                    const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                    const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                    const size_t cb = (size_t)row * sx;
+                    const uint32_t st = a.rowStamp[row];
+                    const uint32_t lo = ro[xl], hi = ro[xh + 1];
+                    m.z = a.bP[cb + xl];
+                    m.w = a.bP[cb + xh + 1];
+                    if (st == stampv) {
+                        m.x = lo;
+                        m.y = hi;
+                    }
+                }
+            };
             uint32_t nl = 0;
-            int rw = -1, part = 1;
-            uint32_t q = 0, qend = 0;
-            uint4 cur = make_uint4(0, 0, 0, 0);
+            int rw = 0, part = 0;
+            uint4 cur, nxtm;
+            row_meta(0, cur);
+            row_meta(1, nxtm);
+            uint32_t q = cur.x, qend = cur.y;
             for (;;) {
-                while (nl < (uint32_t)kList) {
+                while (nl + ARP_BATCH <= (uint32_t)kList) {
                     if (q >= qend) {
                         if (part == 0) {
                             part = 1;
@@ -1516,37 +1539,46 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
                             continue;
                         }
                         if (++rw >= 9) break;
+                        cur = nxtm;
+                        row_meta(rw + 1, nxtm);
                         part = 0;
-                        cur = make_uint4(0, 0, 0, 0);
-                        const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
-                        if (nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
-                            const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
-                            const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
-                            const size_t cb = (size_t)row * sx;
-                            if (a.rowStamp[row] == stampv) {
-                                cur.x = ro[xl];
-                                cur.y = ro[xh + 1];
-                            }
-                            cur.z = a.bP[cb + xl];
-                            cur.w = a.bP[cb + xh + 1];
-                        }
                         q = cur.x;
                         qend = cur.y;
                         continue;
                     }
-                    if (part) {
-                        const float4 xj = B[q];
-                        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                        if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) list[(nl++) * 256 + tid] = q | kBnd;
-                    } else {
-                        const uint32_t pe2 = a.identity ? (a.sflag[q] ? (q | kFlag) : q) : a.perm[q];
-                        if (!(pe2 & kFlag) && pe2 != i) {
-                            const float4 xj = X[pe2];
-                            const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                            if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) list[(nl++) * 256 + tid] = pe2;
+                    const bool bseg = part != 0;
+                    const float4* base = bseg ? B : X;
+                    const uint32_t tag = bseg ? kBnd : 0u;
+                    uint32_t e[ARP_BATCH], ok[ARP_BATCH];
+#pragma unroll
+                    for (int u = 0; u < ARP_BATCH; u++) {
+                        const uint32_t qq = q + u;  // < qend + ARP_BATCH: inside the padding
+                        const uint32_t inr = qq < qend ? 1u : 0u;
+                        uint32_t v = qq, okv = inr;
+                        if (!a.identity) {
+                            const uint32_t pe = a.perm[bseg ? 0u : qq];
+                            okv = inr & ((bseg || !(pe & kFlag)) ? 1u : 0u);
+                            v = bseg ? qq : (inr ? (pe & ~kFlag) : 0u);
+                        } else {
+                            const uint32_t fl = a.sflag[bseg ? 0u : qq];
+                            okv = inr & ((bseg || fl == 0u) ? 1u : 0u);
                         }
+                        e[u] = v;
+                        ok[u] = okv;
                     }
-                    q++;
+                    float4 xj[ARP_BATCH];
+#pragma unroll
+                    for (int u = 0; u < ARP_BATCH; u++) xj[u] = base[e[u]];
+#pragma unroll
+                    for (int u = 0; u < ARP_BATCH; u++) {
+                        const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                        const uint32_t v = e[u] | tag;
+                        const uint32_t in = ok[u] & (v != i ? 1u : 0u) &
+                                            (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
+                        list[nl * 256 + tid] = v;
+                        nl += in;
+                    }
+                    q += ARP_BATCH;
                 }
                 for (uint32_t k = 0; k < nl; k++) {
                     const uint32_t e = list[k * 256 + tid];
@@ -1594,7 +1626,9 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_adj(AdjArgs a) {
                     }
                     if (!bnd) {
                         // side b: the neighbour's terms as seen from this particle
-                        const float4 wpj = WP[j], wvj = WV[j];
+                        float4 wpj = WP[j], wvj = WV[j];
+                        wpj.w *= ws;
+                        wvj = make_float4(ws * wvj.x, ws * wvj.y, ws * wvj.z, 0.0f);
                         const float irj = 1.0f / xj.w;
                         PairSide Q;
                         Q.mi = M2.mass;

@@ -969,7 +969,21 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             }
         w.WP = P<float>(c->Wb[0]);
         w.WV = P<float>(c->Wb[1]);
-        if ((rc = launch(c, K_ADJ_W, nblk(c->n), 256, w))) return rc;
+        // a single scaled term is read in place by the stage kernel; otherwise it is formed
+        const float* wP = P<float>(c->Wb[0]);
+        const float* wV = P<float>(c->Wb[1]);
+        float wscale = 1.0f;
+        if (w.ng == 0) {
+            wP = w.LP;
+            wV = w.LV;
+            wscale = w.cl;
+        } else if (w.ng == 1 && w.cl == 0.0f) {
+            wP = w.GP[0];
+            wV = w.GV[0];
+            wscale = w.cg[0];
+        } else if ((rc = launch(c, K_ADJ_W, nblk(c->n), 256, w))) {
+            return rc;
+        }
         const GridSlot& g = R.slot[s];
         AdjArgs a{};
         a.grid = c->grid;
@@ -990,8 +1004,9 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         a.Xposd = P<float>(R.Xp[s]);
         a.Xvelq = P<float>(R.Xv[s]);
         a.mk = P<uint32_t>(R.mk);
-        a.Wp = P<float>(c->Wb[0]);
-        a.Wv = P<float>(c->Wb[1]);
+        a.Wp = wP;
+        a.Wv = wV;
+        a.wscale = wscale;
         a.Gp = P<float>(c->Gb[s][0]);
         a.Gv = P<float>(c->Gb[s][1]);
         a.LamP = P<float>(c->LamAcc[0]);
