@@ -68,7 +68,7 @@ def _rollout_fn():
             gp = sim.adjoint_backward(g_out, g_in, want_params=ctx.has_params)
             sim.synchronize()
             gx = g_in[:, list(_COLS)].clone()
-            gpar = torch.as_tensor(gp, device=gy.device) if ctx.has_params else None
+            gpar = torch.as_tensor(gp[:, :2].copy(), device=gy.device) if ctx.has_params else None
             return gx, gpar, None, None, None, None
 
     return Rollout

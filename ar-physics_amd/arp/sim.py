@@ -121,12 +121,28 @@ class FluidSim:
 
     def adjoint_backward(self, d_grad_out, d_grad_in, want_params=True):
         """d_grad_out / d_grad_in: device tensors of n Particle records (dL/d den, pos, vel
-        after / before the recorded steps).  Returns an (nmat, 2) array of dL/d
-        (bulk_viscosity, viscosity) per material, or None."""
+        after / before the recorded steps).  Returns an (nmat, 6) array of dL/d
+        (bulk_viscosity, viscosity, normal_stiffness, shear_stiffness, normal_damp,
+        shear_damp) per material, or None."""
         nm = len(self.scene.materials)
-        gp = np.zeros(2 * nm, np.float32) if want_params else None
+        gp = np.zeros(6 * nm, np.float32) if want_params else None
         check(self.ctx, lib().arp_adjoint_backward(self.ctx, self._dp(d_grad_out), self._dp(d_grad_in), _p(gp)))
-        return gp.reshape(nm, 2) if want_params else None
+        return gp.reshape(nm, 6) if want_params else None
+
+    def set_stiffness_field(self, scale):
+        """Per-solid stiffness scale of (normal, shear) stiffness; None: 1 (the reference)."""
+        if scale is None:
+            check(self.ctx, lib().arp_set_stiffness_field(self.ctx, None, 0))
+            return
+        a = np.ascontiguousarray(scale, np.float32)
+        self._kscale = a
+        check(self.ctx, lib().arp_set_stiffness_field(self.ctx, _p(a), len(a)))
+
+    def stiffness_grad(self):
+        """dL/dk_s per solid after adjoint_backward."""
+        out = np.zeros(self.scene.ns, np.float32)
+        check(self.ctx, lib().arp_adjoint_stiffness_grad(self.ctx, _p(out), len(out)))
+        return out
 
     # ---- diagnostics ------------------------------------------------------------------
     def grid_info(self):

@@ -235,6 +235,7 @@ struct SolidArgs {
     const uint32_t* sflag;    // per storage index: 1 when refused by the cap / non-finite
     const uint32_t* pid;      // storage -> particle id
     const uint32_t* sidByPid;
+    const float* kscale;      // per solid stiffness scale of (lambda, mu), or null (1)
     // fluid part of elastic particles (written by k_fluid)
     const float* KFp;
     const float* KFv;
@@ -242,6 +243,29 @@ struct SolidArgs {
     float* KdP;
     float* KdV;
     RkArgs rk;
+};
+
+// Adjoint of the solid part of one stage (strain_order 2, no plastic flow): pass 1 per solid
+// gathers the cotangents of its stress (elastic force), its F (hourglass) and the direct
+// position / density terms, then runs the strain adjoint; pass 2 turns the F, F' cotangents
+// of a solid and its reference neighbours into position / velocity cotangents.
+struct SolidAdjArgs {
+    SolidArgs f;        // the stage's forward arguments (A, Fm recomputed by k_strain)
+    const float* Wp;    // stage cotangent of k, times wscale
+    const float* Wv;
+    float wscale;
+    float* Gp;          // += solid part of dL/dX
+    float* Gv;
+    float* LamP;
+    float* LamV;
+    float* Fbar;        // per solid 18: cotangent rows of F (3x3) then of F' (3x3)
+    float* spar;        // per solid 4 (+=): dL/d (lambda, mu, lambda_d, mu_d) as the solid sees them
+};
+
+struct InvArgs {
+    uint32_t n;
+    const uint32_t* pid;
+    uint32_t* inv;
 };
 
 // Adjoint of one fluid stage: dL/dX_s = J_f(X_s)^T w_s (gather over the stage's neighbour

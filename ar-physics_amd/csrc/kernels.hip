@@ -1093,6 +1093,7 @@ extern "C" __global__ void __launch_bounds__(256) k_strain(SolidArgs a) {
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
     const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
     const float4 xi = X[i], vi = XV[i];
+    const float ks = a.kscale ? a.kscale[s] : 1.0f;
     float F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [row*3+col]
     const float4* G = reinterpret_cast<const float4*>(a.sG);
     for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
@@ -1122,7 +1123,7 @@ extern "C" __global__ void __launch_bounds__(256) k_strain(SolidArgs a) {
     const uint32_t dim = a.grid.dim_n;
     const M4 E = m4_add(m4_seth_hill(F4, M.strain_order, dim), eps);
     const M4 Ed = m4_seth_hill_rate(F4, Fd4, M.strain_order, dim);
-    const M4 S = m4_add(m4_hooke(E, M.lambda, M.mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
+    const M4 S = m4_add(m4_hooke(E, ks * M.lambda, ks * M.mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
     const M4 P = m4_mul(F4, m4_trans(S));
     M4 Li;
     for (int q = 0; q < 16; q++) Li.m[q] = a.Linv[16 * (size_t)s + q];
@@ -1761,4 +1762,339 @@ extern "C" __global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
         __syncthreads();
     }
     if (tid == 0) a.out[w] += s[0];
+}
+
+// ------------------------------------------------------------------------------------
+// adjoint of the solid part (reverse mode of strain.rs:139-160, solid.rs:334-450 for
+// strain_order 2 without plastic flow)
+// ------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(256) k_inv(InvArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < a.n) a.inv[a.pid[i]] = i;
+}
+
+// d(a . f)/dr for the self-contact force f(r) = cs(l) r / l, cs = -5000 sqrt(max(rc - l, 0) rg)
+__device__ __forceinline__ float3 contact_vjp(float ax, float ay, float az, float rx, float ry, float rz, float rc,
+                                             float rg) {
+    const float l = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float dr = fmaxf(rc - l, 0.0f);
+    const float sq = sqrtf(dr * rg);
+    const float cs = -5000.0f * sq;
+    const float dcs = sq > 0.0f ? 2500.0f * rg / sq : 0.0f;  // d cs / d l
+    const float ar = ax * rx + ay * ry + az * rz;
+    const float k1 = cs / l, k2 = ar * (dcs / l - cs / (l * l)) / l;
+    return make_float3(k1 * ax + k2 * rx, k1 * ay + k2 * ry, k1 * az + k2 * rz);
+}
+
+// Hourglass force of target 1 on the pair (1, 2) (solid.rs:432-443):
+//   f = -sc dx/(l + eps h),  sc = K T Q W,  K = 1e8 m1 m2/(d1 d2),
+//   T = (err . dx)/2/(l + 10 eps h), Q = 1/(l^2 + eps h^2), err = ((F1+F2)/2) dX - dx.
+// Given the cotangent a of f: returns f, and adds d(a.f)/d dx to gdx, d/d(fa rows) (the
+// cotangent of (F1+F2)/2) to gfa, and d/dd1, d/dd2.
+struct HgOut {
+    float f[3], gdx[3], gfa[9], gd1, gd2;
+};
+__device__ __forceinline__ void hourglass_vjp(const float4* F1, const float4* F2, const float dX[3], const float dx[3],
+                                              float m1, float m2, float d1, float d2, float W, float h, float ax,
+                                              float ay, float az, HgOut& o) {
+    const float EPS = 0.1f;  // EPSILON of the reference shaders (fluid.rs:9, solid.rs:9)
+    float fa[9];
+    for (int r = 0; r < 3; r++) {
+        const float4 x = F1[r], y = F2[r];
+        fa[3 * r + 0] = 0.5f * (x.x + y.x);
+        fa[3 * r + 1] = 0.5f * (x.y + y.y);
+        fa[3 * r + 2] = 0.5f * (x.z + y.z);
+    }
+    float err[3];
+    for (int r = 0; r < 3; r++) err[r] = fa[3 * r] * dX[0] + fa[3 * r + 1] * dX[1] + fa[3 * r + 2] * dX[2] - dx[r];
+    const float l = sqrtf(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    const float ed = err[0] * dx[0] + err[1] * dx[1] + err[2] * dx[2];
+    const float K = 100000000.0f * m1 * m2 / (d1 * d2);
+    const float l10 = l + 10.0f * EPS * h, l1 = l + EPS * h, qd = l * l + EPS * h * h;
+    const float T = 0.5f * ed / l10, Q = 1.0f / qd;
+    const float sc = K * T * Q * W;
+    const float a[3] = {ax, ay, az};
+    const float adx = ax * dx[0] + ay * dx[1] + az * dx[2];
+    const float au = adx / l1;  // a . u, u = dx / l1
+    for (int r = 0; r < 3; r++) o.f[r] = -sc * dx[r] / l1;
+    // d(a.f) = -(a.u) d sc - sc a.du
+    const float cT = -au * K * Q * W, cQ = -au * K * T * W;
+    const float il = l > 0.0f ? 1.0f / l : 0.0f;
+    for (int r = 0; r < 3; r++) {
+        const float dT = 0.5f * (err[r] - dx[r]) / l10 - 0.5f * ed / (l10 * l10) * dx[r] * il;
+        const float dQ = -Q * Q * 2.0f * dx[r];
+        const float du = a[r] / l1 - adx * dx[r] * il / (l1 * l1);
+        o.gdx[r] += cT * dT + cQ * dQ - sc * du;
+    }
+    // fa enters through err: d ed / d fa[r][k] = dX[k] dx[r]
+    for (int r = 0; r < 3; r++)
+        for (int k = 0; k < 3; k++) o.gfa[3 * r + k] += cT * 0.5f / l10 * dX[k] * dx[r];
+    o.gd1 += au * sc / d1;
+    o.gd2 += au * sc / d2;
+}
+
+__device__ __forceinline__ float4 ld_w(const float* base, uint32_t i, float ws) {
+    const float4 v = reinterpret_cast<const float4*>(base)[i];
+    return make_float4(ws * v.x, ws * v.y, ws * v.z, ws * v.w);
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) {
+    const SolidArgs& a = aa.f;
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= a.ns) return;
+    float* fb = aa.Fbar + 18 * (size_t)s;
+    const uint32_t p = a.part[s];
+    const uint32_t i = p == kInvalid ? 0u : a.inv[p];
+    if (p == kInvalid || !(a.mk[i] & kMkElastic)) {
+        for (int q = 0; q < 18; q++) fb[q] = 0.0f;
+        return;
+    }
+    const uint32_t mk = a.mk[i], mat = mk & kMatMask;
+    const DMat& M = a.mats[mat];
+    const float4* X = reinterpret_cast<const float4*>(a.Xposd);
+    const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
+    const float4* REF = reinterpret_cast<const float4*>(a.ref);
+    const float h = a.phys.h;
+    const uint32_t dim = a.grid.dim_n;
+    const float ws = aa.wscale;
+    const float4 xi = X[i];
+    const float m1 = M.mass, d1 = xi.w, V1 = m1 / d1;
+    const float4 wpi = ld_w(aa.Wp, i, ws), wvi = ld_w(aa.Wv, i, ws);
+    const float ai[3] = {wvi.x / d1, wvi.y / d1, wvi.z / d1};
+    const float zm = dim > 2 ? 1.0f : 0.0f;  // the elastic force drops z below 3-D
+    const float4* A1 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s;
+    const float4* F1 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s;
+    const float4 Xs = REF[s];
+    const bool valid_i = a.sflag[i] == 0u;
+    uint32_t ci[3] = {0, 0, 0};
+    cell_of(a.grid, xi.x, xi.y, xi.z, ci[0], ci[1], ci[2]);
+    float Ab[9] = {0}, Fhb[9] = {0}, xb[3] = {0, 0, 0}, rb = 0.0f, fs[3] = {0, 0, 0};
+    const float4* EG = reinterpret_cast<const float4*>(a.eG);
+    for (uint32_t q = a.eOff[s]; q < a.eOff[s + 1]; q++) {
+        const uint32_t s2 = a.eSid[q];
+        const uint32_t j = a.inv[a.ePid[q]];
+        const float4 g = EG[q];
+        const float4 xj = X[j];
+        const float dj = xj.w, V2 = m1 / dj, c = -V1 * V2;
+        const float4* A2 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s2;
+        const float gg[3] = {g.x, g.y, g.z};
+        float e[3];
+        for (int r = 0; r < 3; r++) {
+            const float4 x = A1[r], y = A2[r];
+            e[r] = (x.x + y.x) * gg[0] + (x.y + y.y) * gg[1] + (x.z + y.z) * gg[2];
+        }
+        fs[0] += c * e[0];
+        fs[1] += c * e[1];
+        fs[2] += zm * c * e[2];
+        const float4 wvj = ld_w(aa.Wv, j, ws);
+        const float aj[3] = {wvj.x / dj, wvj.y / dj, wvj.z / dj};
+        const float aim[3] = {ai[0], ai[1], zm * ai[2]}, ajm[3] = {aj[0], aj[1], zm * aj[2]};
+        // own force c (A1 + A2) g and the partner's c (A2 + A1)(-g): both hold A1
+        for (int r = 0; r < 3; r++)
+            for (int k = 0; k < 3; k++) Ab[3 * r + k] += c * (aim[r] - ajm[r]) * gg[k];
+        const float ae_i = aim[0] * e[0] + aim[1] * e[1] + aim[2] * e[2];
+        const float ae_j = ajm[0] * e[0] + ajm[1] * e[1] + ajm[2] * e[2];
+        rb += (-c / d1) * ae_i + (-c / d1) * (-ae_j);
+        // hourglass, own term and the partner's term on the same pair
+        if (g.w != 0.0f && valid_i && a.sflag[j] == 0u) {
+            uint32_t cj[3] = {0, 0, 0};
+            cell_of(a.grid, xj.x, xj.y, xj.z, cj[0], cj[1], cj[2]);
+            bool adj = true;
+            for (int d = 0; d < 3; d++) adj &= (ci[d] + 1 >= cj[d]) && (cj[d] + 1 >= ci[d]);
+            if (adj) {
+                const float4 X2 = REF[s2];
+                const float4* F2 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s2;
+                const float dX[3] = {X2.x - Xs.x, X2.y - Xs.y, X2.z - Xs.z};
+                const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+                HgOut o = {};
+                hourglass_vjp(F1, F2, dX, dx, m1, m1, d1, dj, g.w, h, ai[0], ai[1], ai[2], o);
+                for (int r = 0; r < 3; r++) {
+                    fs[r] += o.f[r];
+                    xb[r] -= o.gdx[r];
+                }
+                rb += o.gd1;
+                for (int q2 = 0; q2 < 9; q2++) Fhb[q2] += 0.5f * o.gfa[q2];
+                const float mdX[3] = {-dX[0], -dX[1], -dX[2]}, mdx[3] = {-dx[0], -dx[1], -dx[2]};
+                HgOut o2 = {};
+                hourglass_vjp(F2, F1, mdX, mdx, m1, m1, dj, d1, g.w, h, aj[0], aj[1], aj[2], o2);
+                for (int r = 0; r < 3; r++) xb[r] += o2.gdx[r];
+                rb += o2.gd2;
+                for (int q2 = 0; q2 < 9; q2++) Fhb[q2] += 0.5f * o2.gfa[q2];
+            }
+        }
+    }
+    // self contact (solid.rs:426-450): own force and the partners' forces on this particle
+    if (!M.plastic && valid_i) {
+        const float r_cut = 8.0f * h, r_geom = h * h / r_cut;
+        const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
+        const uint32_t xl = ci[0] > 0 ? ci[0] - 1 : 0, xh = min(ci[0] + 1, sx - 1);
+        const uint32_t stampv = *a.stamp;
+        for (int dz = -1; dz <= 1; dz++) {
+            const int nz = (int)ci[2] + dz;
+            if (nz < 0 || nz >= (int)sz) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                const int ny = (int)ci[1] + dy;
+                if (ny < 0 || ny >= (int)sy) continue;
+                const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                if (a.rowStamp[row] != stampv) continue;
+                const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                for (uint32_t q = ro[xl]; q < ro[xh + 1]; q++) {
+                    const uint32_t pe = a.perm[q];
+                    if (pe & kFlag) continue;
+                    const uint32_t mk2 = a.mk[pe];
+                    if ((mk2 & kMatMask) != mat || !(mk2 & kMkSolid)) continue;
+                    const uint32_t s2 = a.sidByPid[a.pid[pe]];
+                    const float4 X2 = REF[s2];
+                    const float dX0 = X2.x - Xs.x, dX1 = X2.y - Xs.y, dX2 = X2.z - Xs.z;
+                    if (!(dX0 * dX0 + dX1 * dX1 + dX2 * dX2 > r_cut * r_cut)) continue;
+                    const float4 xj = X[pe];
+                    const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                    const float rr = rx * rx + ry * ry + rz * rz;
+                    if (!(rr < r_cut * r_cut)) continue;
+                    const float l = sqrtf(rr);
+                    const float cs = -5000.0f * sqrtf(fmaxf(r_cut - l, 0.0f) * r_geom);
+                    fs[0] += cs * rx / l;
+                    fs[1] += cs * ry / l;
+                    fs[2] += cs * rz / l;
+                    const float3 go = contact_vjp(ai[0], ai[1], ai[2], rx, ry, rz, r_cut, r_geom);
+                    const float4 wvj = ld_w(aa.Wv, pe, ws);
+                    const float dj = xj.w;
+                    const float3 gp = contact_vjp(wvj.x / dj, wvj.y / dj, wvj.z / dj, -rx, -ry, -rz, r_cut, r_geom);
+                    xb[0] += gp.x - go.x;
+                    xb[1] += gp.y - go.y;
+                    xb[2] += gp.z - go.z;
+                }
+            }
+        }
+    }
+    // vel' += fs / d1, pos' += v (dest2, shaders/mod.rs:79)
+    rb += -(wvi.x * fs[0] + wvi.y * fs[1] + wvi.z * fs[2]) / (d1 * d1);
+    float4* GP = reinterpret_cast<float4*>(aa.Gp) + i;
+    float4* GV = reinterpret_cast<float4*>(aa.Gv) + i;
+    float4* LP = reinterpret_cast<float4*>(aa.LamP) + i;
+    float4* LV = reinterpret_cast<float4*>(aa.LamV) + i;
+    float4 gp = *GP, gv = *GV, lp = *LP, lv = *LV;
+    gp.x += xb[0]; gp.y += xb[1]; gp.z += xb[2]; gp.w += rb;
+    gv.x += wpi.x; gv.y += wpi.y; gv.z += wpi.z;
+    lp.x += xb[0]; lp.y += xb[1]; lp.z += xb[2]; lp.w += rb;
+    lv.x += wpi.x; lv.y += wpi.y; lv.z += wpi.z;
+    *GP = gp; *GV = gv; *LP = lp; *LV = lv;
+
+    // strain adjoint: A = (F S^T) L^-1, S = hooke(E) + hooke(E'), E = (F^T F - I)/2 + eps,
+    // E' = (F'^T F + F^T F')/2
+    const float4* XVv = XV;
+    const float4 vi = XVv[i];
+    float F[9] = {0}, Fd[9] = {0};
+    const float4* G = reinterpret_cast<const float4*>(a.sG);
+    for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
+        const uint32_t j = a.inv[a.sPid[q]];
+        const float4 g = G[q];
+        const float4 xj = X[j], vj = XVv[j];
+        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+        const float dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+        const float gg[3] = {g.x, g.y, g.z};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                F[3 * r + c] += g.w * (dx[r] * gg[c]);
+                Fd[3 * r + c] += g.w * (dv[r] * gg[c]);
+            }
+    }
+    M4 F4 = m4_zero(), Fd4 = m4_zero(), Ab4 = m4_zero(), Fh4 = m4_zero();
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            F4.m[4 * c + r] = F[3 * r + c];
+            Fd4.m[4 * c + r] = Fd[3 * r + c];
+            Ab4.m[4 * c + r] = Ab[3 * r + c];
+            Fh4.m[4 * c + r] = Fhb[3 * r + c];
+        }
+    M4 eps;
+    const float* ep = a.Xeps + 16 * (size_t)s;
+    for (int q = 0; q < 16; q++) eps.m[q] = ep[q];
+    const float ks = a.kscale ? a.kscale[s] : 1.0f;
+    const float lam = ks * M.lambda, mu = ks * M.mu;
+    const M4 E = m4_add(m4_seth_hill(F4, 2, dim), eps);
+    const M4 Ed = m4_seth_hill_rate(F4, Fd4, 2, dim);
+    const M4 S = m4_add(m4_hooke(E, lam, mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
+    M4 Li;
+    for (int q = 0; q < 16; q++) Li.m[q] = a.Linv[16 * (size_t)s + q];
+    const M4 Pb = m4_mul(Ab4, m4_trans(Li));
+    const M4 Sb = m4_mul(m4_trans(Pb), F4);
+    M4 Fb = m4_add(m4_mul(Pb, S), Fh4);
+    const float trS = Sb.m[0] + Sb.m[5] + Sb.m[10] + Sb.m[15];
+    M4 Eb, Edb;
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) {
+            const float dg = (r == c && (uint32_t)r < dim) ? trS : 0.0f;
+            Eb.m[4 * c + r] = lam * dg + 2.0f * mu * Sb.m[4 * c + r];
+            Edb.m[4 * c + r] = M.lambda_d * dg + 2.0f * M.mu_d * Sb.m[4 * c + r];
+        }
+    const M4 EbS = m4_add(Eb, m4_trans(Eb)), EdbS = m4_add(Edb, m4_trans(Edb));
+    Fb = m4_add(Fb, m4_add(m4_scale(0.5f, m4_mul(F4, EbS)), m4_scale(0.5f, m4_mul(Fd4, EdbS))));
+    const M4 Fdb = m4_scale(0.5f, m4_mul(F4, EdbS));
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            fb[3 * r + c] = Fb.m[4 * c + r];
+            fb[9 + 3 * r + c] = Fdb.m[4 * c + r];
+        }
+    float* sp = aa.spar + 4 * (size_t)s;
+    sp[0] += m4_trace(E, dim) * trS;
+    sp[1] += 2.0f * m4_dot(Sb, E);
+    sp[2] += m4_trace(Ed, dim) * trS;
+    sp[3] += 2.0f * m4_dot(Sb, Ed);
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) {
+    const SolidArgs& a = aa.f;
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= a.ns) return;
+    const uint32_t p = a.part[s];
+    if (p == kInvalid) return;
+    const uint32_t i = a.inv[p];
+    if (!(a.mk[i] & kMkElastic)) return;
+    const DMat& M = a.mats[a.mk[i] & kMatMask];
+    const float4* REF = reinterpret_cast<const float4*>(a.ref);
+    const float h = a.phys.h, hh = h * h, kn = a.phys.k;
+    const float Vp = M.mass / M.start_den;  // this particle's volume in its neighbours' F
+    const float* fb = aa.Fbar + 18 * (size_t)s;
+    const float4 Xs = REF[s];
+    float xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
+    const float4* G = reinterpret_cast<const float4*>(a.sG);
+    for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
+        const uint32_t pid2 = a.sPid[q];
+        const uint32_t s2 = a.sidByPid[pid2];
+        const float4 g = G[q];
+        const float gg[3] = {g.x, g.y, g.z};
+        // own F: dx = x_q - x_s enters with weight V2 L_s^-1 gradW(dX)
+        for (int r = 0; r < 3; r++) {
+            const float t = g.w * (fb[3 * r] * gg[0] + fb[3 * r + 1] * gg[1] + fb[3 * r + 2] * gg[2]);
+            const float td = g.w * (fb[9 + 3 * r] * gg[0] + fb[9 + 3 * r + 1] * gg[1] + fb[9 + 3 * r + 2] * gg[2]);
+            xb[r] -= t;
+            vb[r] -= td;
+        }
+        // the neighbour's F holds this particle with weight Vp L_s2^-1 gradW(X_s - X_s2)
+        const float4 X2 = REF[s2];
+        const float r4[4] = {Xs.x - X2.x, Xs.y - X2.y, Xs.z - X2.z, Xs.w - X2.w};
+        const float rr = r4[0] * r4[0] + r4[1] * r4[1] + r4[2] * r4[2] + r4[3] * r4[3];
+        const float diff = fmaxf(hh - rr, 0.0f);
+        const float gs = -6.0f * kn * diff * diff;
+        const float gw[3] = {gs * r4[0], gs * r4[1], gs * r4[2]};
+        const float* L2 = a.Linv + 16 * (size_t)s2;
+        float g2[3];
+        for (int r = 0; r < 3; r++) g2[r] = L2[r] * gw[0] + L2[4 + r] * gw[1] + L2[8 + r] * gw[2];
+        const float* fb2 = aa.Fbar + 18 * (size_t)s2;
+        for (int r = 0; r < 3; r++) {
+            xb[r] += Vp * (fb2[3 * r] * g2[0] + fb2[3 * r + 1] * g2[1] + fb2[3 * r + 2] * g2[2]);
+            vb[r] += Vp * (fb2[9 + 3 * r] * g2[0] + fb2[9 + 3 * r + 1] * g2[1] + fb2[9 + 3 * r + 2] * g2[2]);
+        }
+    }
+    float4* GP = reinterpret_cast<float4*>(aa.Gp) + i;
+    float4* GV = reinterpret_cast<float4*>(aa.Gv) + i;
+    float4* LP = reinterpret_cast<float4*>(aa.LamP) + i;
+    float4* LV = reinterpret_cast<float4*>(aa.LamV) + i;
+    float4 gp = *GP, gv = *GV, lp = *LP, lv = *LV;
+    gp.x += xb[0]; gp.y += xb[1]; gp.z += xb[2];
+    gv.x += vb[0]; gv.y += vb[1]; gv.z += vb[2];
+    lp.x += xb[0]; lp.y += xb[1]; lp.z += xb[2];
+    lv.x += vb[0]; lv.y += vb[1]; lv.z += vb[2];
+    *GP = gp; *GV = gv; *LP = lp; *LV = lv;
 }

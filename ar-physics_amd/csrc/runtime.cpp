@@ -58,6 +58,9 @@ enum Kern {
     K_SCATTER_B,
     K_ROWSORT_B,
     K_FLUID_B,
+    K_INV,
+    K_SOLID_ADJ1,
+    K_SOLID_ADJ2,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -66,7 +69,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_solid",   "k_eps_in",  "k_eps_out", "k_fluid_adj",
                                    "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce",
                                    "k_key_batch", "k_scan_batch", "k_scatter_batch", "k_rowsort_batch",
-                                   "k_fluid_batch"};
+                                   "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -131,6 +134,7 @@ struct arp_ctx {
     std::vector<ArpMaterial> mats_h;
     std::vector<uint32_t> mat_by_pid, sid_by_pid, mk_by_pid;
     std::vector<ArpSolidParticle> solids_h;
+    std::vector<float> kscale_h;  // per-solid stiffness scale (empty: 1)
     std::vector<uint32_t> b_orig;  // sorted boundary slot -> original boundary id
     std::vector<uint32_t> bP_h, rP_h;
 
@@ -162,6 +166,8 @@ struct arp_ctx {
     std::vector<Checkpoint> ck;
     std::vector<RecStep> rec;
     DBuf LamIn[2], LamAcc[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
+    DBuf Fbar, spar;  // solid reverse mode: per-solid F cotangents, per-solid stiffness partials
+    DBuf kscale;      // per-solid stiffness scale (empty: 1)
     bool keys_valid = false;
     // what the last build described (for the dump)
     const DBuf* last_pid = nullptr;
@@ -714,6 +720,7 @@ SolidArgs solid_base(arp_ctx* c) {
     a.sidByPid = P<uint32_t>(c->sidByPid);
     a.KFp = P<float>(c->KF[0]);
     a.KFv = P<float>(c->KF[1]);
+    a.kscale = c->kscale.p ? P<float>(c->kscale) : nullptr;
     return a;
 }
 
@@ -954,6 +961,10 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
     int rc;
     HIPCHK(c, hipMemcpyAsync(c->LamAcc[0].p, c->LamIn[0].p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->LamAcc[1].p, c->LamIn[1].p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    if (c->solids_active) {  // particle id -> storage index of this step
+        InvArgs ia{c->n, P<uint32_t>(R.pid), P<uint32_t>(c->inv)};
+        if ((rc = launch(c, K_INV, nblk(c->n), 256, ia))) return rc;
+    }
     for (int s = S - 1; s >= 0; s--) {
         AdjWArgs w{};
         w.n = c->n;
@@ -1016,6 +1027,34 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         if (nm <= (uint32_t)kMatLds) {
             ReduceArgs r{nblk(c->n), 2 * nm, P<float>(c->pgradPart), P<float>(c->pgradSum)};
             if ((rc = launch(c, K_REDUCE, 2 * nm, 256, r))) return rc;
+        }
+        if (c->solids_active) {
+            // the stage's stress and F again, then the two solid passes
+            SolidArgs sa = solid_base(c);
+            sa.perm = g.perm;
+            sa.rowOff = g.rowOff;
+            sa.rowStamp = g.rowStamp;
+            sa.stamp = g.stamp;
+            sa.sflag = g.sflag;
+            sa.Xposd = P<float>(R.Xp[s]);
+            sa.Xvelq = P<float>(R.Xv[s]);
+            sa.mk = P<uint32_t>(R.mk);
+            sa.pid = P<uint32_t>(R.pid);
+            sa.Xeps = P<float>(c->Yeps);
+            if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
+            SolidAdjArgs sj{};
+            sj.f = sa;
+            sj.Wp = wP;
+            sj.Wv = wV;
+            sj.wscale = wscale;
+            sj.Gp = P<float>(c->Gb[s][0]);
+            sj.Gv = P<float>(c->Gb[s][1]);
+            sj.LamP = P<float>(c->LamAcc[0]);
+            sj.LamV = P<float>(c->LamAcc[1]);
+            sj.Fbar = P<float>(c->Fbar);
+            sj.spar = P<float>(c->spar);
+            if ((rc = launch(c, K_SOLID_ADJ1, nblk(c->ns), 256, sj))) return rc;
+            if ((rc = launch(c, K_SOLID_ADJ2, nblk(c->ns), 256, sj))) return rc;
         }
     }
     AdjPermArgs pa{c->n, P<uint32_t>(R.src), P<float>(c->LamAcc[0]), P<float>(c->LamAcc[1]),
@@ -1430,7 +1469,13 @@ double arp_time(const arp_ctx* c) { return c ? c->time : 0.0; }
 int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoint_every) {
     int rc;
     if ((rc = need_scene(c))) return rc;
-    if (c->solids_active) return fail(c, ARP_E_STATE, "the adjoint covers fluid scenes (this scene has solids)");
+    if (c->solids_active)
+        for (uint32_t m = 0; m < c->nm; m++) {
+            const ArpMaterial& M = c->mats_h[m];
+            if (is_stiff(M) && (M.strain_order != 2 || M.plastic))
+                return fail(c, ARP_E_STATE,
+                            "the solid adjoint covers strain_order 2 without plastic flow (material %u)", m);
+        }
     if (nsteps == 0 || checkpoint_every == 0 || !(dt > 0.0f) || !std::isfinite(dt))
         return fail(c, ARP_E_ARG, "adjoint_record needs nsteps >= 1, checkpoint_every >= 1 and dt > 0");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1457,6 +1502,41 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     return ARP_OK;
 }
 
+int arp_set_stiffness_field(arp_ctx* c, const float* scale, uint32_t ns) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!scale) {
+        free_buf(c->kscale);
+        c->kscale_h.clear();
+        return ARP_OK;
+    }
+    if (ns != c->ns) return fail(c, ARP_E_ARG, "stiffness field has %u entries, scene %u solids", ns, c->ns);
+    c->kscale_h.assign(scale, scale + ns);
+    c->rec_valid = false;
+    return upload(c, c->kscale, scale, 4 * (size_t)std::max<uint32_t>(ns, 1));
+}
+
+int arp_adjoint_stiffness_grad(arp_ctx* c, float* out, uint32_t ns) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!c->solids_active || !c->spar.p) return fail(c, ARP_E_STATE, "no solid reverse sweep has run");
+    if (!out || ns != c->ns) return fail(c, ARP_E_ARG, "need %u entries", c->ns);
+    std::vector<float> sp(4 * (size_t)ns);
+    HIPCHK(c, hipMemcpyAsync(sp.data(), c->spar.p, 4 * sp.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // lambda_eff = k_s lambda, mu_eff = k_s mu: dL/dk_s = lambda dL/dlambda_eff + mu dL/dmu_eff
+    for (uint32_t s = 0; s < ns; s++) {
+        const uint32_t p = c->solids_h[s].part_id;
+        if (p >= c->n) {
+            out[s] = 0.0f;
+            continue;
+        }
+        const ArpMaterial& M = c->mats_h[c->mat_by_pid[p]];
+        out[s] = M.normal_stiffness * sp[4 * s] + M.shear_stiffness * sp[4 * s + 1];
+    }
+    return ARP_OK;
+}
+
 int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, float* grad_params) {
     int rc;
     if ((rc = need_scene(c))) return rc;
@@ -1477,6 +1557,11 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
             if ((rc = dalloc(c, c->Gb[s][j], 16 * n))) return rc;
     }
     if ((rc = dalloc(c, c->pgradPart, 4 * (size_t)nblk(c->n) * 2 * kMatLds))) return rc;
+    if (c->solids_active) {
+        const size_t ns = std::max<uint32_t>(c->ns, 1);
+        if ((rc = dalloc(c, c->Fbar, 4 * 18 * ns)) || (rc = dalloc(c, c->spar, 4 * 4 * ns))) return rc;
+        if ((rc = fill_u32(c, c->spar, 4 * c->ns, 0))) return rc;
+    }
     if ((rc = dalloc(c, c->pgradSum, 4 * 2 * kMatLds))) return rc;
     if ((rc = fill_u32(c, c->pgradSum, 2 * kMatLds, 0))) return rc;
     const uint32_t K = c->rec_K, nsteps = c->rec_nsteps, nseg = (nsteps + K - 1) / K;
@@ -1503,8 +1588,31 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
     // the live state goes back to the end of the recorded trajectory
     if ((rc = ck_restore(c, fin))) return rc;
     if (grad_params) {
-        HIPCHK(c, hipMemcpyAsync(grad_params, c->pgradSum.p, 4 * 2 * (size_t)c->nm, hipMemcpyDeviceToHost, c->stream));
+        std::vector<float> pg(2 * (size_t)c->nm);
+        HIPCHK(c, hipMemcpyAsync(pg.data(), c->pgradSum.p, 4 * pg.size(), hipMemcpyDeviceToHost, c->stream));
+        std::vector<float> sp;
+        if (c->solids_active) {
+            sp.resize(4 * (size_t)c->ns);
+            HIPCHK(c, hipMemcpyAsync(sp.data(), c->spar.p, 4 * sp.size(), hipMemcpyDeviceToHost, c->stream));
+        }
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        std::vector<double> acc(6 * (size_t)c->nm, 0.0);
+        for (uint32_t m = 0; m < c->nm; m++) {
+            acc[6 * m + 0] = pg[2 * m];
+            acc[6 * m + 1] = pg[2 * m + 1];
+        }
+        // per-solid partials w.r.t. the effective (scaled) constants -> per material
+        for (uint32_t s = 0; s < c->ns && !sp.empty(); s++) {
+            const uint32_t p = c->solids_h[s].part_id;
+            if (p >= c->n) continue;
+            const uint32_t m = c->mat_by_pid[p];
+            const float ks = c->kscale_h.empty() ? 1.0f : c->kscale_h[s];
+            acc[6 * m + 2] += (double)sp[4 * s] * ks;
+            acc[6 * m + 3] += (double)sp[4 * s + 1] * ks;
+            acc[6 * m + 4] += sp[4 * s + 2];
+            acc[6 * m + 5] += sp[4 * s + 3];
+        }
+        for (size_t q = 0; q < acc.size(); q++) grad_params[q] = (float)acc[q];
     }
     return ARP_OK;
 }

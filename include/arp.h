@@ -210,10 +210,17 @@ int arp_adjoint_record(arp_ctx* ctx, float dt, uint32_t nsteps, uint32_t checkpo
  * checkpoints as needed).  d_grad_out: n device ArpParticle records holding dL/d(den,
  * pos.xyz, vel.xyz) of the state after the recorded steps, indexed like arp_download's
  * output; d_grad_in receives dL/d of the state before them (mat/solid_id fields zero).
- * grad_params (host, may be NULL): 2*nmat floats, [2m] = dL/d bulk_viscosity and
- * [2m+1] = dL/d viscosity of material m (at most 16 materials).  The live state is left at
- * the end of the recorded trajectory. */
+ * grad_params (host, may be NULL): 6*nmat floats per material m: [6m] bulk_viscosity,
+ * [6m+1] viscosity, [6m+2] normal_stiffness, [6m+3] shear_stiffness, [6m+4] normal_damp,
+ * [6m+5] shear_damp (at most 16 materials).  Solids: strain_order 2 without plastic flow.
+ * The live state is left at the end of the recorded trajectory. */
 int arp_adjoint_backward(arp_ctx* ctx, const void* d_grad_out, void* d_grad_in, float* grad_params);
+/* Per-solid stiffness scale k_s multiplying the solid's (normal, shear) stiffness in the
+ * stress (no counterpart in the reference, whose stiffness is per material; NULL restores
+ * k_s = 1, the reference behaviour).  ns = number of solids. */
+int arp_set_stiffness_field(arp_ctx* ctx, const float* scale, uint32_t ns);
+/* After arp_adjoint_backward: dL/dk_s of the stiffness scale of every solid (ns floats). */
+int arp_adjoint_stiffness_grad(arp_ctx* ctx, float* out, uint32_t ns);
 
 /* ---- batched ensembles (no counterpart in the reference: one FluidSim per scene) ------- */
 /* Independent fluid scenes, one context each (uploaded as usual, same device, same

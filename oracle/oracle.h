@@ -38,6 +38,8 @@ typedef struct orc_scene {
     ArpAABB bounds;
     float h, g;
     int threads; /* OpenMP threads for particle loops (<=0: library default) */
+    const float* kscale; /* ns per-solid stiffness scale of (lambda, mu), or NULL (an extension
+                            used to check gradients w.r.t. a stiffness field; NULL = reference) */
 } orc_scene;
 
 typedef struct orc_state_f {

@@ -29,6 +29,7 @@ class _Scene(C.Structure):
         ("bden", C.c_void_p), ("bpos", C.c_void_p), ("bvel", C.c_void_p), ("bmat", C.c_void_p),
         ("mats", C.c_void_p), ("inter", C.c_void_p),
         ("bounds", _AABB), ("h", C.c_float), ("g", C.c_float), ("threads", C.c_int),
+        ("kscale", C.c_void_p),
     ]
 
 
@@ -95,7 +96,7 @@ def _p(a):
 class Oracle:
     """The oracle bound to one scene (the static part of FluidSim::new's state)."""
 
-    def __init__(self, scene, threads=None):
+    def __init__(self, scene, threads=None, kscale=None):
         self.scene = scene
         p, b = scene.particles, scene.boundary
         self._keep = dict(
@@ -119,6 +120,10 @@ class Oracle:
         sc.h = scene.h
         sc.g = scene.g
         sc.threads = threads if threads is not None else (os.cpu_count() or 1)
+        # optional per-solid stiffness scale (extension for gradient checks; None = reference)
+        if kscale is not None:
+            self._keep["kscale"] = np.ascontiguousarray(kscale, np.float32)
+            sc.kscale = self._keep["kscale"].ctypes.data
         self.sc = sc
 
     # -- states -------------------------------------------------------------------------

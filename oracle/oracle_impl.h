@@ -660,12 +660,14 @@ static void F(strain_one)(const F(orc_ctx) * cx, const F(orc_grid) * g, const F(
 
 /* PK2 stress of a solid: hooke(E(F)+eps_p) + hooke(E'(F,F')) (solid.rs:334-339) */
 static void F(pk2)(const F(orc_ctx) * cx, const F(orc_mat) * M, const REAL* strains,
-                   const REAL* eps_p, REAL* S) {
+                   const REAL* eps_p, uint32_t sid, REAL* S) {
     REAL E[16], Ed[16], a[16], b[16];
+    /* optional per-solid stiffness scale (an extension of the reference, 1 when absent) */
+    const REAL ks = cx->sc->kscale ? (REAL)cx->sc->kscale[sid] : (REAL)1;
     F(mseth_hill)(strains + 16, M->strain_order, cx->dim, E);
     F(madd)(E, eps_p, E);
     F(mseth_hill_rate)(strains + 16, strains + 32, M->strain_order, cx->dim, Ed);
-    F(mhooke)(E, M->normal_stiffness, M->shear_stiffness, cx->dim, a);
+    F(mhooke)(E, ks * M->normal_stiffness, ks * M->shear_stiffness, cx->dim, a);
     F(mhooke)(Ed, M->normal_damp, M->shear_damp, cx->dim, b);
     F(madd)(a, b, S);
 }
@@ -690,7 +692,7 @@ static void F(solid_one)(const F(orc_ctx) * cx, const F(orc_grid) * g, const F(o
     memset(sden, 0, sizeof sden);
     const REAL* S1 = strains + 48 * (size_t)s_id;
     REAL pk2[16], stress[16], t[16];
-    F(pk2)(cx, M1, S1, in->stress + 16 * (size_t)s_id, pk2);
+    F(pk2)(cx, M1, S1, in->stress + 16 * (size_t)s_id, s_id, pk2);
     F(mtrans)(pk2, t);
     F(mmul)(S1 + 16, t, stress);
     const REAL* X1 = in->ref + 4 * (size_t)s_id;
@@ -713,7 +715,7 @@ static void F(solid_one)(const F(orc_ctx) * cx, const F(orc_grid) * g, const F(o
                 REAL V2 = m1 / in->den[p2];
                 const REAL* S2 = strains + 48 * (size_t)s2;
                 REAL pk2b[16], st2[16], A[16], r[4], gw[4], el[4];
-                F(pk2)(cx, M1, S2, in->stress + 16 * (size_t)s2, pk2b);
+                F(pk2)(cx, M1, S2, in->stress + 16 * (size_t)s2, s2, pk2b);
                 F(mtrans)(pk2b, t);
                 F(mmul)(S2 + 16, t, st2);
                 F(mmul)(st2, S2, A);
