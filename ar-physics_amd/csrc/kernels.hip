@@ -720,6 +720,11 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         }
 
         // derivative (fluid.rs:243-247): den' = sum, vel' = F/den + (0,-g,0), pos' = vel
+        // (i passes through an empty asm so the epilogue's addresses are formed here, not
+        // held in registers across the neighbour loops)
+        uint32_t mte = mat1;
+        asm volatile("" : "+v"(i), "+v"(mte));
+        const DMat& Me = a.mats[mte];
         const float id1 = __fdividef(1.0f, d1);
         const float4 kP = make_float4(vi.x, vi.y, vi.z, acc.den);
         const float4 kV = make_float4(acc.fx * id1, acc.fy * id1 - a.phys.g, acc.fz * id1, 0.0f);
@@ -727,7 +732,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             reinterpret_cast<float4*>(a.KdP)[i] = kP;
             reinterpret_cast<float4*>(a.KdV)[i] = kV;
         } else {
-            rk_update_particle(a.rk, a.grid, M1, i, kP, kV, haveKey, nextKey, nextBin);
+            rk_update_particle(a.rk, a.grid, Me, i, kP, kV, haveKey, nextKey, nextBin);
         }
     }
     if (a.rk.keys && a.mode == 0) {
