@@ -201,6 +201,13 @@ struct FluidArgs {
     float* KdP;
     float* KdV;
     RkArgs rk;
+    // optional record of every particle's neighbours within h (for the reverse sweep):
+    // entry k of storage index i at nbList[k * n + i], count at nbCount[i]; a particle with
+    // more than nbCap neighbours sets *nbOver
+    uint32_t* nbList;
+    uint32_t* nbCount;
+    uint32_t* nbOver;
+    uint32_t nbCap;
 };
 
 // Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists
@@ -297,6 +304,10 @@ struct AdjArgs {
     float* LamP;       // step cotangent accumulator (+= G)
     float* LamV;
     float* pgrad;      // per block: nm * 2 (d bulk_visc, d visc)
+    // neighbours recorded by the forward stage (null, or *nbOver set: search again)
+    const uint32_t* nbList;
+    const uint32_t* nbCount;
+    const uint32_t* nbOver;
 };
 
 // W = cl * lambda + sum_t c_t * G_t (stage cotangent of k)

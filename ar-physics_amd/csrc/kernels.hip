@@ -485,6 +485,9 @@ struct PairIn {
 #define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
 
+// REC: also keep every particle's neighbours within h (the recording of the reverse sweep);
+// a separate instance so the plain step carries none of its registers
+template <bool REC>
 __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
     __shared__ uint32_t list[kList * 256];
     __shared__ float4 smat[kMatLds];                // mass, visc, bulk_visc
@@ -613,6 +616,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             }
         };
 
+        uint32_t nrec = 0;  // neighbours recorded for the reverse sweep
         if (!noNb) {
             uint32_t cx, cy, cz;
             cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
@@ -706,6 +710,11 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     }
                     q += ARP_BATCH;
                 }
+                if (REC) {
+                    for (uint32_t k = 0; k < nl; k++)
+                        if (nrec + k < a.nbCap) a.nbList[(size_t)(nrec + k) * a.n + i] = list[k * 256 + tid];
+                    nrec += nl;
+                }
                 // next neighbour's loads are issued before the current pair's arithmetic
                 PairIn nxt;
                 if (nl) nxt = load(list[tid]);
@@ -717,6 +726,10 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                 nl = 0;
                 if (rw >= 9) break;
             }
+        }
+        if (REC) {
+            a.nbCount[i] = min(nrec, a.nbCap);
+            if (nrec > a.nbCap) atomicOr(a.nbOver, 1u);
         }
 
         // derivative (fluid.rs:243-247): den' = sum, vel' = F/den + (0,-g,0), pos' = vel
@@ -773,12 +786,13 @@ ARP_ENTRIES(k_rowsort, RowSortArgs, rowsort_body)
 // k_fluid: blocks are remapped so that each XCD processes a contiguous share of the
 // cell-sorted particles (of the whole batch)
 // (five waves per SIMD: measured faster than four without the few spilled registers)
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid(FluidArgs a) { fluid_body(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid(FluidArgs a) { fluid_body<false>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_batch(BatchArgs b) {
     uint32_t bid, nb;
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
-    fluid_body(a, bid, nb);
+    fluid_body<false>(a, bid, nb);
 }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { fluid_body<true>(a, xcd_block(), gridDim.x); }
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA
@@ -1465,12 +1479,14 @@ __device__ void pair_vjp(const PairSide& P, const DInter& it, float rx, float ry
     }
 }
 
-extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) {
+// REC: the neighbours come from the forward stage's record instead of a new sweep
+template <bool REC>
+__device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
     __shared__ uint32_t list[kList * 256];
-    __shared__ float spg[2 * kMatLds];
+    __shared__ float spg[4][2 * kMatLds];  // per wave, summed in a fixed order
     const uint32_t tid = threadIdx.x;
     const uint32_t nm = a.phys.nm;
-    for (uint32_t m = tid; m < 2 * kMatLds; m += 256) spg[m] = 0.0f;
+    for (uint32_t m = tid; m < 4 * 2 * kMatLds; m += 256) spg[m / (2 * kMatLds)][m % (2 * kMatLds)] = 0.0f;
     __syncthreads();
     const uint32_t t = xcd_block() * 256u + tid;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
@@ -1532,12 +1548,30 @@ extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) {
             };
             uint32_t nl = 0;
             int rw = 0, part = 0;
-            uint4 cur, nxtm;
-            row_meta(0, cur);
-            row_meta(1, nxtm);
+            // the forward stage's record of this particle's neighbours, when it is complete
+            const bool useRec = REC;
+            uint32_t rpos = 0;
+            const uint32_t rcnt = useRec ? a.nbCount[i] : 0u;
+            uint4 cur = make_uint4(0, 0, 0, 0), nxtm = cur;
+            if (!useRec) {
+                row_meta(0, cur);
+                row_meta(1, nxtm);
+            }
             uint32_t q = cur.x, qend = cur.y;
             for (;;) {
                 while (nl + ARP_BATCH <= (uint32_t)kList) {
+                    if (useRec) {
+                        if (rpos >= rcnt) {
+                            rw = 9;
+                            break;
+                        }
+                        const uint32_t take = min(rcnt - rpos, (uint32_t)kList - nl);
+                        for (uint32_t k = 0; k < take; k++)
+                            list[(nl + k) * 256 + tid] = a.nbList[(size_t)(rpos + k) * a.n + i];
+                        nl += take;
+                        rpos += take;
+                        continue;
+                    }
                     if (q >= qend) {
                         if (part == 0) {
                             part = 1;
@@ -1695,14 +1729,18 @@ extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) {
                 v += __shfl_xor(v, o);
             }
             if (__lane_id() == 0) {
-                atomicAdd(&spg[2 * m], b);
-                atomicAdd(&spg[2 * m + 1], v);
+                spg[tid >> 6][2 * m] = b;
+                spg[tid >> 6][2 * m + 1] = v;
             }
         }
         __syncthreads();
-        for (uint32_t m = tid; m < 2 * nm; m += 256) a.pgrad[(size_t)blockIdx.x * 2 * nm + m] = spg[m];
+        for (uint32_t m = tid; m < 2 * nm; m += 256)
+            a.pgrad[(size_t)blockIdx.x * 2 * nm + m] = ((spg[0][m] + spg[1][m]) + spg[2][m]) + spg[3][m];
     }
 }
+
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) { fluid_adj_body<false>(a); }
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec(AdjArgs a) { fluid_adj_body<true>(a); }
 
 extern "C" __global__ void __launch_bounds__(256) k_adj_w(AdjWArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;

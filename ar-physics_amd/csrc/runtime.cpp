@@ -61,6 +61,8 @@ enum Kern {
     K_INV,
     K_SOLID_ADJ1,
     K_SOLID_ADJ2,
+    K_FLUID_REC,
+    K_FLUID_ADJ_REC,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -69,9 +71,11 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_solid",   "k_eps_in",  "k_eps_out", "k_fluid_adj",
                                    "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce",
                                    "k_key_batch", "k_scan_batch", "k_scatter_batch", "k_rowsort_batch",
-                                   "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2"};
+                                   "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2",
+                                   "k_fluid_rec", "k_fluid_adj_rec"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
+constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
 
 struct DBuf {
     void* p = nullptr;
@@ -96,7 +100,9 @@ struct GridSlot {
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
 struct RecStep {
-    DBuf mk, pid, src, stamps;
+    DBuf mk, pid, src, stamps, nbOver;
+    uint32_t over[ARP_MAX_STAGES] = {};  // host copy of nbOver after the recording
+    DBuf nbList[ARP_MAX_STAGES], nbCount[ARP_MAX_STAGES];
     DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
     DBuf perm[ARP_MAX_STAGES], sflag[ARP_MAX_STAGES], rowOff[ARP_MAX_STAGES], rowStamp[ARP_MAX_STAGES];
     GridSlot slot[ARP_MAX_STAGES];
@@ -765,6 +771,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     c->cur = nx;
     c->keys_valid = false;
     if (rec) {
+        if ((rc = fill_u32(c, rec->nbOver, ARP_MAX_STAGES, 0))) return rc;
         rec->slot[0] = c->slot;
         HIPCHK(c, hipMemcpyAsync(rec->mk.p, c->meta_mk[nx].p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(rec->pid.p, c->meta_pid[nx].p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
@@ -835,7 +842,13 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         f.KdP = P<float>(c->KF[0]);
         f.KdV = P<float>(c->KF[1]);
         f.rk = r;
-        if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
+        if (rec) {  // keep the stage's neighbour lists for the reverse sweep
+            f.nbList = P<uint32_t>(rec->nbList[s]);
+            f.nbCount = P<uint32_t>(rec->nbCount[s]);
+            f.nbOver = P<uint32_t>(rec->nbOver) + s;
+            f.nbCap = kRecNb;
+        }
+        if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) return rc;
         if (c->solids_active) {
             SolidArgs sa = solid_base(c);
             sa.Xposd = Xp;
@@ -882,6 +895,9 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
         if ((rc = dalloc(c, *b, 4 * n))) return rc;
     if ((rc = dalloc(c, R.stamps, 4 * ARP_MAX_STAGES))) return rc;
     HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
+    if ((rc = dalloc(c, R.nbOver, 4 * ARP_MAX_STAGES))) return rc;
+    for (int s = 0; s < S; s++)
+        if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)kRecNb * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
     for (int s = 0; s < S; s++) {
         if ((rc = dalloc(c, R.Xp[s], 16 * n)) || (rc = dalloc(c, R.Xv[s], 16 * n)) ||
             (rc = dalloc(c, R.perm[s], 4 * n)) || (rc = dalloc(c, R.sflag[s], 4 * n)) ||
@@ -911,7 +927,8 @@ void free_buf(DBuf& b) {
 void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
     for (size_t i = nrec; i < c->rec.size(); i++) {
         RecStep& R = c->rec[i];
-        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps}) free_buf(*b);
+        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps, &R.nbOver}) free_buf(*b);
+        for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
     }
@@ -947,6 +964,15 @@ int ck_restore(arp_ctx* c, Checkpoint& k) {
     if (rc) return rc;
     c->keys_valid = false;
     c->have_build = false;
+    return ARP_OK;
+}
+
+// host copies of the neighbour-record overflow flags of the first nrec recorded steps
+int fetch_over(arp_ctx* c, uint32_t nrec) {
+    for (uint32_t j = 0; j < nrec; j++)
+        HIPCHK(c, hipMemcpyAsync(c->rec[j].over, c->rec[j].nbOver.p, sizeof(c->rec[j].over), hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ARP_OK;
 }
 
@@ -1023,7 +1049,11 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         a.LamP = P<float>(c->LamAcc[0]);
         a.LamV = P<float>(c->LamAcc[1]);
         a.pgrad = P<float>(c->pgradPart);
-        if ((rc = launch(c, K_FLUID_ADJ, nblk(c->n), 256, a))) return rc;
+        a.nbList = P<uint32_t>(R.nbList[s]);
+        a.nbCount = P<uint32_t>(R.nbCount[s]);
+        a.nbOver = P<uint32_t>(R.nbOver) + s;
+        // the recorded neighbours, unless a particle had more than the record holds
+        if ((rc = launch(c, R.over[s] ? K_FLUID_ADJ : K_FLUID_ADJ_REC, nblk(c->n), 256, a))) return rc;
         if (nm <= (uint32_t)kMatLds) {
             ReduceArgs r{nblk(c->n), 2 * nm, P<float>(c->pgradPart), P<float>(c->pgradSum)};
             if ((rc = launch(c, K_REDUCE, 2 * nm, 256, r))) return rc;
@@ -1493,6 +1523,7 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
         c->time += dt;
     }
     if ((rc = ck_save(c, c->ck[nseg]))) return rc;
+    if ((rc = fetch_over(c, (nsteps - 1) % K + 1))) return rc;
     c->rec_dt = dt;
     c->rec_nsteps = nsteps;
     c->rec_K = K;
@@ -1577,6 +1608,7 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
             if ((rc = ck_restore(c, c->ck[seg]))) return rc;
             for (uint32_t j = j0; j < j1; j++)
                 if ((rc = step_once(c, c->rec_dt, &c->rec[j - j0]))) return rc;
+            if ((rc = fetch_over(c, j1 - j0))) return rc;
             c->rec_seg = (uint32_t)seg;
         }
         for (uint32_t j = j1; j-- > j0;)
