@@ -144,6 +144,19 @@ class FluidSim:
         check(self.ctx, lib().arp_adjoint_stiffness_grad(self.ctx, _p(out), len(out)))
         return out
 
+    # ---- neighbour lists kept across stages --------------------------------------------
+    def set_neighbor_skin(self, skin=0.2, cap=0, always_build=False):
+        """Lists of the candidates within h*(1+skin) kept across RK stages while every
+        particle stays within skin*h/2 of its position at the search (0 disables; cap 0:
+        48 entries per particle).  always_build rebuilds at every stage (testing)."""
+        check(self.ctx, lib().arp_set_neighbor_skin(self.ctx, float(skin), int(cap), 1 if always_build else 0))
+
+    def neighbor_stats(self):
+        """(stages that searched and built lists, stages that reused them)."""
+        b, r = C.c_uint32(0), C.c_uint32(0)
+        check(self.ctx, lib().arp_neighbor_stats(self.ctx, C.byref(b), C.byref(r)))
+        return int(b.value), int(r.value)
+
     # ---- diagnostics ------------------------------------------------------------------
     def grid_info(self):
         g = np.zeros((), wire.GRID_INFO)

@@ -106,6 +106,10 @@ struct ScatterArgs {
     uint32_t* inv;  // pid -> storage index (full mode), may be null
     uint32_t* sflag;
     uint32_t* srcOut;  // full mode: previous storage index of every new slot (may be null)
+    // neighbour-list reuse words (may be null): when the stage reuses its lists the scatter
+    // copies the state in place of the full reorder and places nothing
+    const uint32_t* nlw;
+    const uint32_t* stamp;
 };
 
 struct RowSortArgs {
@@ -137,6 +141,7 @@ struct RowSortArgs {
     uint32_t* stats;  // [0] overflowed particles (atomic), may be null
     uint32_t* sflag;  // per storage index: cap/non-finite flag (may be null)
     uint32_t* srcOut;
+    const uint32_t* nlw;  // neighbour-list reuse words (may be null): no row sort when reused
 };
 
 // RK stage epilogue: k -> next stage input / y_{n+1}, the b-weighted accumulator and the
@@ -171,6 +176,32 @@ struct RkArgs {
     float* Oeps;
     float* AccEOut;
     float* KeOut;
+};
+
+// Neighbour lists kept across RK stages (fluid-only scenes).  A stage whose lists are stale
+// searches the grid with radius h + skin and writes, per particle, every candidate within it
+// (entry k of storage index i at list[k * n + i]) and its own position; a later stage uses
+// the lists while every particle stays within skin/2 of that position and no reference
+// bucket would exceed its 128 slots.  The decision is taken on the device from the build
+// counter g of the stage (the grid stamp):
+//   words[g & 1]     = g  when the stage at counter g ran with lists (built or reused)
+//   words[2 + (g&1)] = g  when that stage found a stale entry for the next stage
+//   words[4], [5]        number of list builds, number of stages run with lists
+// Stage g reuses iff words[(g-1)&1] == g-1 and words[2+((g-1)&1)] != g-1.
+// cellCnt holds two arrays of per-bucket counts: stage g counts the buckets of its outputs
+// into array (g+1)&1 and clears array g&1 at the buckets of its inputs.
+struct NlArgs {
+    uint32_t* words;
+    uint32_t* list;
+    uint32_t* count;
+    float* ref;         // float4 per storage index: position at the build
+    uint32_t* cellCnt;  // 2 * ncells
+    const uint32_t* rP; // reference prefix per cell (ncells+1), with FluidArgs::bP the static entries
+    uint32_t ncells;
+    uint32_t cap;        // entries per particle
+    uint32_t capThresh;  // counts above this are checked against the bucket's static entries
+    float rad, rad2;     // h + skin and its square (build radius)
+    float lim2;          // (skin/2)^2, slightly reduced: allowed squared displacement
 };
 
 struct FluidArgs {
@@ -208,6 +239,7 @@ struct FluidArgs {
     uint32_t* nbCount;
     uint32_t* nbOver;
     uint32_t nbCap;
+    NlArgs nl;  // neighbour-list reuse (k_fluid_nl only)
 };
 
 // Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists

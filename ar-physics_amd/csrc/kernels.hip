@@ -255,11 +255,30 @@ __device__ __forceinline__ void move_particle(const ScatterArgs& a, uint32_t src
     if (a.inv) a.inv[p] = dst;
 }
 
+// Whether the stage about to run reuses the neighbour lists (see NlArgs).  Read after the
+// stage's row-count scan advanced the counter; the words it reads are written only by the
+// stage before.
+__device__ __forceinline__ bool nl_reuse(const uint32_t* w, const uint32_t* stamp) {
+    const uint32_t g = *stamp, p = (g - 1u) & 1u;
+    return w[p] == g - 1u && w[2 + p] != g - 1u;
+}
+
 // Scatter (key, source) into the reserved row slots; particles with non-finite positions
 // go straight to their final place after all rows (neighbor_list.rs:86-93 drops them).
+// A stage that reuses its neighbour lists keeps the storage order: a full build then only
+// copies the state to the sorted buffers.
 __device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t) {
     const uint32_t i = bid * 256u + threadIdx.x;
     if (i >= a.n) return;
+    if (a.nlw && nl_reuse(a.nlw, a.stamp)) {
+        if (a.full) {
+            reinterpret_cast<float4*>(a.dstPosd)[i] = reinterpret_cast<const float4*>(a.srcPosd)[i];
+            reinterpret_cast<float4*>(a.dstVelq)[i] = reinterpret_cast<const float4*>(a.srcVelq)[i];
+            a.dstMk[i] = a.srcMk[i];
+            a.dstPid[i] = a.srcPid[i];
+        }
+        return;
+    }
     const uint32_t key = a.keys[i], rank = a.ranks[i];
     if (key != kInvalid) {
         uint32_t row = key / a.grid.sub[0];
@@ -290,6 +309,7 @@ __device__ __forceinline__ void rowsort_body(const RowSortArgs& a, uint32_t bid,
     __shared__ uint16_t eX[kRowCap];
     __shared__ uint32_t wsum[4];
     const uint32_t tid = threadIdx.x;
+    if (a.nlw && nl_reuse(a.nlw, a.stamp)) return;  // the stage reuses its neighbour lists
     const uint32_t sx = a.grid.sub[0];
     const uint32_t nact = *a.numActive;
     for (uint32_t ri = bid; ri < nact; ri += nb) {
@@ -394,7 +414,7 @@ struct PairAcc {
 // b-weighted accumulator, stored derivatives, the state's p/den^2 and the next cell key.
 __device__ __forceinline__ void rk_update_particle(const RkArgs& r, const GridP& g, const DMat& M,
                                                    uint32_t i, float4 kP, float4 kV, bool& haveKey,
-                                                   uint32_t& nextKey, uint32_t& nextBin) {
+                                                   uint32_t& nextKey, uint32_t& nextBin, float4& opOut) {
     const float4 yp = reinterpret_cast<const float4*>(r.Yposd)[i];
     const float4 yv = reinterpret_cast<const float4*>(r.Yvelq)[i];
     float4 ap = make_float4(0.f, 0.f, 0.f, 0.f), av = ap;
@@ -430,6 +450,7 @@ __device__ __forceinline__ void rk_update_particle(const RkArgs& r, const GridP&
         reinterpret_cast<float4*>(r.KvOut)[i] = make_float4(kV.x, kV.y, kV.z, 0.0f);
     }
     ov.w = pressure_q(M, op.w);
+    opOut = op;
     reinterpret_cast<float4*>(r.Oposd)[i] = op;
     reinterpret_cast<float4*>(r.Ovelq)[i] = ov;
     if (r.keys) {
@@ -485,10 +506,14 @@ struct PairIn {
 #define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
 
-// REC: also keep every particle's neighbours within h (the recording of the reverse sweep);
-// a separate instance so the plain step carries none of its registers
-template <bool REC>
+// MODE 0: plain step.  MODE 1: also keep every particle's neighbours within h (the recording
+// of the reverse sweep).  MODE 2: neighbour lists kept across stages (NlArgs): the stage
+// either walks its particles' lists or searches the grid with radius h + skin and writes new
+// lists.  Separate instances, so the plain step carries none of the other modes' registers.
+template <int MODE>
 __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
+    constexpr bool REC = MODE == 1;
+    constexpr bool NL = MODE == 2;
     __shared__ uint32_t list[kList * 256];
     __shared__ float4 smat[kMatLds];                // mass, visc, bulk_visc
     __shared__ DInter sint[kMatLds * kMatLds];
@@ -508,12 +533,30 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
     const float4* B = reinterpret_cast<const float4*>(a.bposd);
     const float4* BV = reinterpret_cast<const float4*>(a.bvelq);
     const float h = a.phys.h, hh = h * h, kn = a.phys.k;
+    // MODE 2: the stage's build counter and whether it walks the kept lists
+    uint32_t gen = 0;
+    bool use = false;
+    if (NL) {
+        gen = *a.stamp;
+        use = nl_reuse(a.nl.words, a.stamp);
+        if (bid == 0 && tid == 0) {
+            a.nl.words[gen & 1u] = gen;
+            atomicAdd(&a.nl.words[use ? 5 : 4], 1u);
+            // a build over a sort that refused particles (bucket cap) is not kept
+            if (!use && a.stats && *a.stats != 0u) a.nl.words[2 + (gen & 1u)] = gen;
+        }
+    }
+    // candidates are tested against this radius (MODE 2 builds: h + skin)
+    const float testR2 = (NL && !use) ? a.nl.rad2 : hh;
 
     uint32_t i = 0, nextBin = 0, nextKey = kInvalid;
     bool haveKey = false;
     if (act) {
         bool noNb;
-        if (a.identity) {
+        if (NL && use) {
+            i = t;  // the lists index storage order, which a reused stage keeps
+            noNb = false;
+        } else if (a.identity) {
             i = t;
             noNb = a.sflag[t] != 0u;
         } else {
@@ -564,6 +607,8 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
             const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
             const float rr = rx * rx + ry * ry + rz * rz;
+            // a build's list also holds candidates between h and h + skin: no terms
+            if (NL && !(hh - rr > 0.0f)) return;
             const float diff = fmaxf(hh - rr, 0.0f);
             const float gs = -6.0f * kn * diff * diff;
             const float gx = gs * rx, gy = gs * ry, gz = gs * rz;
@@ -615,20 +660,80 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                 acc.fz -= s * gz;
             }
         };
+        // the buffered candidates' terms; the next neighbour's loads are issued before the
+        // current pair's arithmetic
+        auto drain = [&](uint32_t nl) {
+            PairIn nxt;
+            if (nl) nxt = load(list[tid]);
+            for (uint32_t k = 0; k < nl; k++) {
+                const PairIn cur2 = nxt;
+                if (k + 1 < nl) nxt = load(list[(k + 1) * 256 + tid]);
+                pair(cur2);
+            }
+        };
 
-        uint32_t nrec = 0;  // neighbours recorded for the reverse sweep
-        if (!noNb) {
+        uint32_t nrec = 0;  // neighbours recorded (reverse sweep) or listed (MODE 2 build)
+        if (NL && use) {
+            // walk the kept list; entries within h are buffered and drained as below
+            const uint32_t cnt = a.nl.count[i];
+            const uint32_t* L = a.nl.list;
+            const size_t n = a.n;
+            uint32_t k = 0, nl = 0;
+            for (;;) {
+                while (nl + ARP_BATCH <= (uint32_t)kList && k < cnt) {
+                    uint32_t e[ARP_BATCH], ok[ARP_BATCH];
+#pragma unroll
+                    for (int u = 0; u < ARP_BATCH; u++) {
+                        const uint32_t kk = k + u;
+                        ok[u] = kk < cnt ? 1u : 0u;
+                        e[u] = L[(size_t)(ok[u] ? kk : 0u) * n + i];
+                        e[u] = ok[u] ? e[u] : 0u;
+                    }
+                    float4 xj[ARP_BATCH];
+#pragma unroll
+                    for (int u = 0; u < ARP_BATCH; u++) xj[u] = ((e[u] & kBnd) ? B : X)[e[u] & ~kBnd];
+#pragma unroll
+                    for (int u = 0; u < ARP_BATCH; u++) {
+                        const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                        const uint32_t in = ok[u] & (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
+                        list[nl * 256 + tid] = e[u];
+                        nl += in;
+                    }
+                    k += ARP_BATCH;
+                }
+                drain(nl);
+                nl = 0;
+                if (k >= cnt) break;
+            }
+        } else if (!noNb) {
             uint32_t cx, cy, cz;
             cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
             const uint32_t stampv = *a.stamp;
-            const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
+            // stencil: the 3x3x3 buckets around the particle's (cell size >= h); a MODE 2
+            // build covers every bucket within h + skin (the cells of the box x +- rad)
+            uint32_t xl, xh;
+            int ylo = (int)cy - 1, zlo = (int)cz - 1, nyr = 3, nrw = 9;
+            if (NL) {
+                const GridP& gp = a.grid;
+                const float R = a.nl.rad;
+                xl = cell_coord(xi.x - R, gp.min[0], gp.dim[0], sx);
+                xh = cell_coord(xi.x + R, gp.min[0], gp.dim[0], sx);
+                ylo = (int)cell_coord(xi.y - R, gp.min[1], gp.dim[1], sy);
+                zlo = (int)cell_coord(xi.z - R, gp.min[2], gp.dim[2], sz);
+                nyr = (int)cell_coord(xi.y + R, gp.min[1], gp.dim[1], sy) - ylo + 1;
+                nrw = nyr * ((int)cell_coord(xi.z + R, gp.min[2], gp.dim[2], sz) - zlo + 1);
+            } else {
+                xl = cx > 0 ? cx - 1 : 0;
+                xh = min(cx + 1, sx - 1);
+            }
             // metadata of one neighbour row: [mobile lo, hi), [boundary lo, hi); all four
             // loads are independent, and the next row's are issued before this row is scanned
             auto row_meta = [&](int rw, uint4& m) {
                 m = make_uint4(0, 0, 0, 0);
-                const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
-                if (rw < 9 && nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
+                const int nz = NL ? zlo + rw / nyr : (int)cz + rw / 3 - 1;
+                const int ny = NL ? ylo + rw % nyr : (int)cy + rw % 3 - 1;
+                if (rw < nrw && nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
                     const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
                     const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
                     const size_t cb = (size_t)row * sx;
@@ -642,10 +747,10 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     }
                 }
             };
-            // Candidates within h are buffered in LDS and drained at a single site of the pair
-            // loop whenever the buffer could overflow.  Inside a batch the test and the append
-            // are branch-free: every candidate is written to slot nl, which advances only for
-            // accepted ones.
+            // Candidates within the test radius are buffered in LDS and drained at a single
+            // site of the pair loop whenever the buffer could overflow.  Inside a batch the
+            // test and the append are branch-free: every candidate is written to slot nl,
+            // which advances only for accepted ones.
             uint32_t nl = 0;
             int rw = 0;
             uint4 cur, nxtm;
@@ -665,7 +770,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                             qend = cur.w;
                             continue;
                         }
-                        if (++rw >= 9) break;
+                        if (++rw >= nrw) break;
                         cur = nxtm;
                         row_meta(rw + 1, nxtm);
                         part = 0;
@@ -704,7 +809,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                         const uint32_t v = e[u] | tag;
                         // the self pair contributes exactly zero (grad W(0) = 0)
                         const uint32_t in = ok[u] & (v != i ? 1u : 0u) &
-                                            (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
+                                            (testR2 - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
                         list[nl * 256 + tid] = v;
                         nl += in;
                     }
@@ -715,21 +820,24 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                         if (nrec + k < a.nbCap) a.nbList[(size_t)(nrec + k) * a.n + i] = list[k * 256 + tid];
                     nrec += nl;
                 }
-                // next neighbour's loads are issued before the current pair's arithmetic
-                PairIn nxt;
-                if (nl) nxt = load(list[tid]);
-                for (uint32_t k = 0; k < nl; k++) {
-                    const PairIn cur2 = nxt;
-                    if (k + 1 < nl) nxt = load(list[(k + 1) * 256 + tid]);
-                    pair(cur2);
+                if (NL) {
+                    for (uint32_t k = 0; k < nl; k++)
+                        if (nrec + k < a.nl.cap) a.nl.list[(size_t)(nrec + k) * a.n + i] = list[k * 256 + tid];
+                    nrec += nl;
                 }
+                drain(nl);
                 nl = 0;
-                if (rw >= 9) break;
+                if (rw >= nrw) break;
             }
         }
         if (REC) {
             a.nbCount[i] = min(nrec, a.nbCap);
             if (nrec > a.nbCap) atomicOr(a.nbOver, 1u);
+        }
+        if (NL && !use) {
+            a.nl.count[i] = min(nrec, a.nl.cap);
+            reinterpret_cast<float4*>(a.nl.ref)[i] = xi;
+            if (nrec > a.nl.cap) a.nl.words[2 + (gen & 1u)] = gen;
         }
 
         // derivative (fluid.rs:243-247): den' = sum, vel' = F/den + (0,-g,0), pos' = vel
@@ -745,7 +853,29 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             reinterpret_cast<float4*>(a.KdP)[i] = kP;
             reinterpret_cast<float4*>(a.KdV)[i] = kV;
         } else {
-            rk_update_particle(a.rk, a.grid, Me, i, kP, kV, haveKey, nextKey, nextBin);
+            float4 op;
+            rk_update_particle(a.rk, a.grid, Me, i, kP, kV, haveKey, nextKey, nextBin, op);
+            if (NL) {
+                // the next stage may keep the lists only if this particle stays within skin/2
+                // of its position at the build and its next bucket keeps every entry
+                const float4 r0 = use ? reinterpret_cast<const float4*>(a.nl.ref)[i] : X[i];
+                const float dx = op.x - r0.x, dy = op.y - r0.y, dz = op.z - r0.z;
+                bool stale = !(dx * dx + dy * dy + dz * dz <= a.nl.lim2);
+                uint32_t* cc = a.nl.cellCnt;
+                const size_t nc = a.nl.ncells;
+                if (nextKey != kInvalid) {
+                    const uint32_t c1 = atomicAdd(&cc[((gen + 1u) & 1u) * nc + nextKey], 1u) + 1u;
+                    if (c1 > a.nl.capThresh) {
+                        const uint32_t used = (a.bP[nextKey + 1] - a.bP[nextKey]) + (a.nl.rP[nextKey + 1] - a.nl.rP[nextKey]);
+                        stale = stale || c1 > kBucketCap - used;
+                    }
+                }
+                const float4 xc = X[i];
+                uint32_t ux, uy, uz;
+                if (cell_of(a.grid, xc.x, xc.y, xc.z, ux, uy, uz))
+                    cc[(gen & 1u) * nc + ux + a.grid.sub[0] * (uy + a.grid.sub[1] * uz)] = 0u;
+                if (stale) a.nl.words[2 + (gen & 1u)] = gen;
+            }
         }
     }
     if (a.rk.keys && a.mode == 0) {
@@ -786,13 +916,19 @@ ARP_ENTRIES(k_rowsort, RowSortArgs, rowsort_body)
 // k_fluid: blocks are remapped so that each XCD processes a contiguous share of the
 // cell-sorted particles (of the whole batch)
 // (five waves per SIMD: measured faster than four without the few spilled registers)
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid(FluidArgs a) { fluid_body<false>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid(FluidArgs a) { fluid_body<0>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_batch(BatchArgs b) {
     uint32_t bid, nb;
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
-    fluid_body<false>(a, bid, nb);
+    fluid_body<0>(a, bid, nb);
 }
-extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { fluid_body<true>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { fluid_body<1>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nl(FluidArgs a) { fluid_body<2>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nl_batch(BatchArgs b) {
+    uint32_t bid, nb;
+    const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
+    fluid_body<2>(a, bid, nb);
+}
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA
@@ -1294,7 +1430,8 @@ extern "C" __global__ void __launch_bounds__(256) k_solid(SolidArgs a) {
             reinterpret_cast<float4*>(a.KdP)[i] = kP;
             reinterpret_cast<float4*>(a.KdV)[i] = kV;
         } else {
-            rk_update_particle(a.rk, a.grid, M, i, kP, kV, haveKey, nextKey, nextBin);
+            float4 op;
+            rk_update_particle(a.rk, a.grid, M, i, kP, kV, haveKey, nextKey, nextBin, op);
             rk_update_eps(a.rk, s, a.keps + 16 * (size_t)s);
         }
     } else if (s < a.ns && a.mode == 0) {

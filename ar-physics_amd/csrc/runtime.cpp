@@ -63,6 +63,8 @@ enum Kern {
     K_SOLID_ADJ2,
     K_FLUID_REC,
     K_FLUID_ADJ_REC,
+    K_FLUID_NL,
+    K_FLUID_NL_B,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -72,7 +74,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce",
                                    "k_key_batch", "k_scan_batch", "k_scatter_batch", "k_rowsort_batch",
                                    "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2",
-                                   "k_fluid_rec", "k_fluid_adj_rec"};
+                                   "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -164,6 +166,13 @@ struct arp_ctx {
         gPid, gGrp, stats, scanState, scanTicket, scanEpoch, rowStamp;
     DBuf stampDev;          // build counter written into rowStamp by the row sort
     GridSlot slot;          // where builds go (the buffers above unless recording)
+    // neighbour lists kept across stages (fluid-only scenes, the plain step)
+    float nl_skin = 0.2f;   // list radius h * (1 + skin); 0 disables
+    uint32_t nl_cap = 48;   // entries per particle
+    bool nl_always = false; // rebuild at every stage (testing)
+    bool nl_dirty = true;   // the decision words and bucket counts must be reset
+    int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
+    DBuf nlWords, nlList, nlCount, nlRef, nlCell;
     // adjoint recording
     bool rec_valid = false;
     float rec_dt = 0.0f;
@@ -204,7 +213,7 @@ struct arp_ctx {
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
                 &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp,
-                &stampDev};
+                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
             bufs.push_back(&K[k][1]);
@@ -506,6 +515,7 @@ struct SortIO {
     uint32_t* dstMk;
     uint32_t* dstPid;
     uint32_t* srcOut = nullptr;  // full reorder: sorted slot -> source index
+    const uint32_t* nlw = nullptr;  // neighbour-list words: the sort is skipped when reused
 };
 
 int run_key(arp_ctx* c, const float* posd) {
@@ -544,6 +554,8 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.inv = io.full ? P<uint32_t>(c->inv) : nullptr;
     sc.sflag = g.sflag;
     sc.srcOut = io.full ? io.srcOut : nullptr;
+    sc.nlw = io.nlw;
+    sc.stamp = g.stamp;
     if ((rc = launch(c, K_SCATTER, nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
     r.grid = c->grid;
@@ -575,6 +587,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.stats = P<uint32_t>(c->stats);
     r.sflag = g.sflag;
     r.srcOut = io.full ? io.srcOut : nullptr;
+    r.nlw = io.nlw;
     // one workgroup per occupied row, looping; enough workgroups for about rowsort_div
     // particles each (a tuning knob), at most 4096
     const uint32_t want = std::max<uint32_t>(1, (c->n + c->rowsort_div - 1) / c->rowsort_div);
@@ -601,6 +614,75 @@ FluidArgs fluid_base(arp_ctx* c) {
     f.bvelq = P<float>(c->bvelq);
     f.bmk = P<uint32_t>(c->bmk);
     return f;
+}
+
+// ---- neighbour lists kept across stages --------------------------------------------
+constexpr float kEps32 = 1.1920929e-7f;
+
+// largest coordinate magnitude of the grid box
+float box_extent(const GridP& g) {
+    float M = 0.0f;
+    for (int i = 0; i < 3; i++) M = std::max(M, std::max(std::fabs(g.min[i]), std::fabs(g.min[i] + g.dim[i])));
+    return M;
+}
+
+// Lists are kept for fluid-only scenes whose buckets are wider than h by more than float
+// rounding of the bucket function, so that every pair within h lies in neighbouring buckets
+// (the 27-bucket stencil of neighbor_list.rs) whatever the list says.
+bool nl_enabled(const arp_ctx* c) {
+    if (!(c->nl_skin > 0.0f) || c->solids_active || c->n == 0) return false;
+    const float h = c->phys.h, M = box_extent(c->grid);
+    for (int i = 0; i < 3; i++) {
+        if (!(c->grid.dim[i] > 0.0f)) continue;
+        const float cs = c->grid.dim[i] / (float)c->grid.sub[i];
+        if (!(cs >= h + 16.0f * kEps32 * (M + h))) return false;
+    }
+    return true;
+}
+
+int nl_prepare(arp_ctx* c) {
+    int rc;
+    const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
+    const bool fresh = c->nlWords.p == nullptr;
+    if ((rc = dalloc(c, c->nlWords, 32)) || (rc = dalloc(c, c->nlList, 4 * n * c->nl_cap)) ||
+        (rc = dalloc(c, c->nlCount, 4 * n)) || (rc = dalloc(c, c->nlRef, 16 * n)) ||
+        (rc = dalloc(c, c->nlCell, 8 * (size_t)c->ncells)))
+        return rc;
+    if (fresh && (rc = fill_u32(c, c->nlWords, 8, 0))) return rc;
+    if (c->nl_dirty) {
+        if ((rc = fill_u32(c, c->nlWords, 4, kInvalid))) return rc;
+        if ((rc = fill_u32(c, c->nlCell, (uint32_t)(2 * c->ncells), 0))) return rc;
+        c->nl_dirty = false;
+    }
+    return ARP_OK;
+}
+
+NlArgs nl_args(arp_ctx* c) {
+    NlArgs a{};
+    a.words = P<uint32_t>(c->nlWords);
+    a.list = P<uint32_t>(c->nlList);
+    a.count = P<uint32_t>(c->nlCount);
+    a.ref = P<float>(c->nlRef);
+    a.cellCnt = P<uint32_t>(c->nlCell);
+    a.rP = P<uint32_t>(c->rP);
+    a.ncells = (uint32_t)c->ncells;
+    a.cap = c->nl_cap;
+    if (c->nl_thresh < 0) {
+        uint32_t maxStatic = 0;
+        for (uint64_t q = 0; q < c->ncells; q++)
+            maxStatic = std::max(maxStatic, (c->bP_h[q + 1] - c->bP_h[q]) + (c->rP_h[q + 1] - c->rP_h[q]));
+        c->nl_thresh = maxStatic >= kBucketCap ? 0 : (int64_t)(kBucketCap - maxStatic);
+    }
+    a.capThresh = (uint32_t)c->nl_thresh;
+    const float h = c->phys.h, R = h * (1.0f + c->nl_skin);
+    a.rad2 = R * R;
+    // the stencil covers rad with room for the rounding of x -+ rad and of the list test
+    a.rad = R * (1.0f + 1e-4f) + 8.0f * kEps32 * box_extent(c->grid);
+    // a particle may move (skin/2)(1 - 1e-3) from its build position: pairs within h then
+    // stay within h + skin with a relative margin far above the rounding of the tests
+    const float lim = 0.5f * c->nl_skin * h * (1.0f - 1e-3f);
+    a.lim2 = c->nl_always ? -1.0f : lim * lim;
+    return a;
 }
 
 GridSlot ctx_slot(arp_ctx* c) {
@@ -752,6 +834,18 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     for (int j = 0; j + 1 < S; j++)
         if (T.b[j] != 0.0f) acc = true;
     if ((rc = ensure_k(c, nslots, acc))) return rc;
+    // neighbour lists kept across stages (the plain step of a fluid-only scene)
+    const bool nl = !rec && nl_enabled(c);
+    if (nl) {
+        if (!c->capture) {
+            if ((rc = nl_prepare(c))) return rc;
+        } else if (c->nl_dirty || !c->nlWords.p) {
+            return fail(c, ARP_E_STATE, "neighbour lists not prepared before capture");
+        }
+    } else {
+        c->nl_dirty = true;  // the counters and words no longer follow the state
+    }
+    const uint32_t* nlw = nl ? P<uint32_t>(c->nlWords) : nullptr;
 
     // stage-1 sort: full reorder of y_n into cell-major storage
     if (!c->keys_valid) {
@@ -766,7 +860,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
               P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_mk[c->cur]),
               P<uint32_t>(c->meta_pid[c->cur]),
               Y0p, Y0v, P<uint32_t>(c->meta_mk[nx]),
-              P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr};
+              P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr, nlw};
     if ((rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
@@ -783,7 +877,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     for (int s = 0; s < S; s++) {
         if (s > 0) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
-                       nullptr, nullptr, nullptr, nullptr};
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nlw};
             if (rec) c->slot = rec->slot[s];
             if ((rc = run_sort(c, pio))) return rc;
             if (rec) rec->slot[s] = c->slot;
@@ -848,7 +942,8 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             f.nbOver = P<uint32_t>(rec->nbOver) + s;
             f.nbCap = kRecNb;
         }
-        if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) return rc;
+        if (nl) f.nl = nl_args(c);
+        if ((rc = launch(c, rec ? K_FLUID_REC : (nl ? K_FLUID_NL : K_FLUID), nblk(c->n), 256, f))) return rc;
         if (c->solids_active) {
             SolidArgs sa = solid_base(c);
             sa.Xposd = Xp;
@@ -956,6 +1051,7 @@ int ck_save(arp_ctx* c, Checkpoint& k) {
 // cell, particle id), so replays reproduce the recorded trajectory bit for bit.
 int ck_restore(arp_ctx* c, Checkpoint& k) {
     const size_t n = c->n;
+    c->nl_dirty = true;
     HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, k.P.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, k.V.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->meta_mk[c->cur].p, k.mk.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1095,6 +1191,7 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
 // Per-stage path: device AoS state -> sorted SoA in the ps buffers (full reorder).
 int stage_sort_aos(arp_ctx* c, const void* d_state) {
     int rc;
+    c->nl_dirty = true;  // this build advances the grid's counter outside the step
     if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
     float* lp = P<float>(c->Xb[0][0]);
     float* lv = P<float>(c->Xb[0][1]);
@@ -1427,6 +1524,8 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
             if ((rc = build_solid_lists(c, particles, n, solids, ns, mats, kept))) return rc;
     }
     if ((rc = alloc_state(c))) return rc;
+    c->nl_thresh = -1;
+    c->nl_dirty = true;
     c->have_scene = true;
     c->cur = 0;
     c->time = 0.0;
@@ -1442,6 +1541,7 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     if ((rc = need_scene(c))) return rc;
     if (n != c->n || (n && !particles)) return fail(c, ARP_E_ARG, "state has %u particles, scene %u", n, c->n);
     c->rec_valid = false;
+    c->nl_dirty = true;
     HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     if (c->solids_active) {
         const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
@@ -1780,6 +1880,37 @@ int arp_velocity(arp_ctx* c, float r, const ArpParticle* d_in, const ArpSolidPar
     return launch(c, K_VELOCITY, nblk(std::max(c->n, a.ns)), 256, a);
 }
 
+int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) {
+    if (!c) return ARP_E_ARG;
+    if (!(skin >= 0.0f && skin <= 1.0f)) return fail(c, ARP_E_ARG, "skin must be in [0, 1] (a fraction of h)");
+    if (skin > 0.0f && skin < 0.01f) return fail(c, ARP_E_ARG, "skin below 0.01 h leaves no room for rounding");
+    if (cap == 0) cap = 48;
+    if (cap < 8 || cap > 1024) return fail(c, ARP_E_ARG, "list capacity must be in [8, 1024]");
+    c->nl_skin = skin;
+    c->nl_cap = cap;
+    c->nl_always = (flags & ARP_NL_ALWAYS_BUILD) != 0;
+    c->nl_dirty = true;
+    if (c->nlWords.p) {
+        HIPCHK(c, hipSetDevice(c->device));
+        int rc = fill_u32(c, c->nlWords, 8, 0);  // statistics restart
+        if (rc) return rc;
+    }
+    return ARP_OK;
+}
+
+int arp_neighbor_stats(arp_ctx* c, uint32_t* builds, uint32_t* reused) {
+    if (!c) return ARP_E_ARG;
+    uint32_t w[8] = {};
+    if (c->nlWords.p) {
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipMemcpyAsync(w, c->nlWords.p, sizeof w, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (builds) *builds = w[4];
+    if (reused) *reused = w[5];
+    return ARP_OK;
+}
+
 int arp_profile(arp_ctx* c, int enable) {
     if (!c) return ARP_E_ARG;
     c->prof = enable != 0;
@@ -1986,6 +2117,7 @@ Kern batch_kernel(int k) {
         case K_SCATTER: return K_SCATTER_B;
         case K_ROWSORT: return K_ROWSORT_B;
         case K_FLUID: return K_FLUID_B;
+        case K_FLUID_NL: return K_FLUID_NL_B;
         default: return K_COUNT;
     }
 }
@@ -2163,6 +2295,8 @@ int run_batch(arp_batch* b, float dt, uint32_t nsteps) {
     arp_ctx* L0 = b->lead;
     int rc;
     if (nsteps == 0) return ARP_OK;
+    for (arp_ctx* c : b->ctx)  // outside the captured launches: reset stale list state
+        if (nl_enabled(c) && (rc = nl_prepare(c))) return rc;
     if ((rc = build_plan(b, dt))) return rc;
     if ((rc = join_streams(b, true))) return rc;
     const bool graph = (b->flags & ARP_BATCH_GRAPH) && !L0->prof && nsteps >= 2;

@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256,
                     help="scenes of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--skin", type=float, default=0.2,
+                    help="neighbour lists kept across RK stages with radius h*(1+skin) (0: search at every stage)")
     ap.add_argument("--adjoint-steps", type=int, default=10,
                     help="steps of the recorded rollout whose reverse sweep is timed (0: skip)")
     args = ap.parse_args()
@@ -192,6 +194,8 @@ def main():
     sc = S.fluid_column_3d(shape=shape, rk=args.integrator, dt=dt)
     log("particles %d boundary %d" % (sc.n, sc.nb))
     sim = FluidSim(sc, integrator=args.integrator, timestep=dt)
+    sim.set_neighbor_skin(args.skin)
+    fluid_kernel = "k_fluid_nl" if args.skin > 0 else "k_fluid"
     log("warmup %d steps" % args.warmup)
     sim.update(args.warmup)
     sim.synchronize()
@@ -202,17 +206,19 @@ def main():
             torch.cuda.synchronize()
     except Exception:
         pass
-    sim.profile_select(["k_fluid"] if args.events == "fluid" else None)
+    sim.profile_select([fluid_kernel] if args.events == "fluid" else None)
     sim.profile(args.events != "none")
     sim.profile_reset()
+    nl0 = sim.neighbor_stats()
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
     sim.update(args.steps)
     sim.synchronize()
     t1 = time.perf_counter()
     sec = t1 - t0
+    nl1 = sim.neighbor_stats()
     kern = {}
-    for k in ("k_fluid", "k_rowsort", "k_scatter", "k_scan", "k_key"):
+    for k in ("k_fluid", "k_fluid_nl", "k_rowsort", "k_scatter", "k_scan", "k_key"):
         ms, cnt = sim.profile_query(k)
         if cnt:
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
@@ -227,10 +233,10 @@ def main():
     if args.ensemble > 0:
         ens = time_ensemble(args.ensemble, args.ensemble_steps, dt)
     value = sc.n * args.steps / sec
-    f = kern.get("k_fluid")
+    f = kern.get(fluid_kernel)
     bytes_launch = B_ALG_FLUID * sc.n
     achieved = bytes_launch / (f["avg_us"] * 1e-6) if f else None
-    traffic = read_traffic("k_fluid", "fluid_column_3d")
+    traffic = read_traffic(fluid_kernel, "fluid_column_3d")
     out = {
         "metric": "SPH particle-steps/sec at 1M particles (fwd)",
         "value": value,
@@ -257,7 +263,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_fluid",
+            "kernel": fluid_kernel,
             "achieved": achieved / 1e9 if achieved else None,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -267,6 +273,8 @@ def main():
         },
         "step_roofline_frac": B_ALG_STEP.get(args.integrator, 0.0) * value / HBM_PEAK,
         "kernels": kern,
+        "neighbor_lists": {"skin": args.skin, "stage_builds": nl1[0] - nl0[0],
+                           "stage_reuses": nl1[1] - nl0[1]},
         "state_finite": finite,
         "fwd_adjoint": fwd_adj,
         "ensemble": ens,

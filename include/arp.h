@@ -155,6 +155,23 @@ int arp_download(arp_ctx* ctx, ArpParticle* particles, uint32_t n, ArpSolidParti
 int arp_upload_state(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
                      const ArpSolidParticle* solids, uint32_t ns);
 
+/* ---- neighbour lists kept across RK stages (the fused path's replacement for the
+ * per-stage rebuild of NeighborList::update_contents, neighbor_list.rs:224-294) ----------
+ * For fluid-only scenes arp_update/arp_step/arp_batch_step keep, per particle, the
+ * candidates within h * (1 + skin) found by a grid search, and later stages walk those
+ * lists instead of sorting and searching again, for as long as every particle stays within
+ * skin*h/2 of its position at the search and no bucket would exceed its 128 slots
+ * (checked on the device at every stage; otherwise the stage sorts and searches as the
+ * reference does).  The neighbours within h, and so the forces, are the reference's; only
+ * the order of the per-particle sums follows the list.  skin = 0 disables the lists
+ * (default 0.2); cap = entries per particle (0: 48; a list that overflows is not kept).
+ * ARP_NL_ALWAYS_BUILD rebuilds at every stage (testing).  Resets the statistics. */
+#define ARP_NL_ALWAYS_BUILD 1u
+int arp_set_neighbor_skin(arp_ctx* ctx, float skin, uint32_t cap, uint32_t flags);
+/* Stages that searched and built lists, and stages that reused them, since the last
+ * arp_set_neighbor_skin. */
+int arp_neighbor_stats(arp_ctx* ctx, uint32_t* builds, uint32_t* reused);
+
 /* ---- per-stage drop-ins on device std430 buffers ------------------------------------- */
 /* compute_forces (shaders/mod.rs:11-86): neighbour update at the input positions, fluid
  * forces, and (when the scene has more than one solid) strain + solid forces, summed. */

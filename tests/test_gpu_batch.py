@@ -86,6 +86,10 @@ def test_batch_tune_leaves_states(torch_cuda):
     for k, s in enumerate(sims):
         assert _same(s.particles(), before[k])
         assert s.time == t0[k]
+    # the trials' restores drop the kept neighbour lists (the next stage searches again):
+    # drop the solo scenes' lists too, so both continue from the same structures
+    for s in solo:
+        s.set_neighbor_skin(0.2)
     B.step(dt, 5)
     for s in solo:
         s.step(dt, 5)
