@@ -181,27 +181,32 @@ struct RkArgs {
 // Neighbour lists kept across RK stages (fluid-only scenes).  A stage whose lists are stale
 // searches the grid with radius h + skin and writes, per particle, every candidate within it
 // (entry k of storage index i at list[k * n + i]) and its own position; a later stage uses
-// the lists while every particle stays within skin/2 of that position and no reference
-// bucket would exceed its 128 slots.  The decision is taken on the device from the build
+// the lists while every particle stays within lim (< skin/2) of that position.  Bucket caps:
+// the build counts, for every bucket, the particles within ext (>= lim) of it; no bucket can
+// then hold more than that count while the lists are used, and a count above the bucket's
+// free slots makes the lists unusable.  The decision is taken on the device from the build
 // counter g of the stage (the grid stamp):
 //   words[g & 1]     = g  when the stage at counter g ran with lists (built or reused)
 //   words[2 + (g&1)] = g  when that stage found a stale entry for the next stage
-//   words[4], [5]        number of list builds, number of stages run with lists
+//   words[4 + (g&1)] = g  when that stage built lists (its bucket counts are cleared by the
+//                          next stage, from the buckets each particle recorded in cellRec)
+//   words[6], [7]        number of list builds, number of stages run with lists
 // Stage g reuses iff words[(g-1)&1] == g-1 and words[2+((g-1)&1)] != g-1.
-// cellCnt holds two arrays of per-bucket counts: stage g counts the buckets of its outputs
-// into array (g+1)&1 and clears array g&1 at the buckets of its inputs.
+// cellCnt holds two count arrays; the build at counter g counts into array g&1.
 struct NlArgs {
     uint32_t* words;
     uint32_t* list;
     uint32_t* count;
     float* ref;         // float4 per storage index: position at the build
     uint32_t* cellCnt;  // 2 * ncells
+    uint32_t* cellRec;  // 2 per storage index: first bucket counted at the build, axis mask
     const uint32_t* rP; // reference prefix per cell (ncells+1), with FluidArgs::bP the static entries
     uint32_t ncells;
     uint32_t cap;        // entries per particle
     uint32_t capThresh;  // counts above this are checked against the bucket's static entries
     float rad, rad2;     // h + skin and its square (build radius)
-    float lim2;          // (skin/2)^2, slightly reduced: allowed squared displacement
+    float lim2;          // allowed squared displacement from the build position (< (skin/2)^2)
+    float ext;           // half-width of the box around a particle whose buckets count it
 };
 
 struct FluidArgs {

@@ -506,21 +506,56 @@ struct PairIn {
 #define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
 
+// LDS of the fluid kernels (namespace scope, so that both table variants of a kernel share it)
+__shared__ uint32_t s_list[kList * 256];     // candidates buffered per thread
+__shared__ float4 s_mat[kMatLds];            // mass, visc, bulk_visc
+__shared__ DInter s_inter[kMatLds * kMatLds];
+
+#ifndef ARP_NLB
+#define ARP_NLB 4  // list entries whose neighbour loads are issued together (list walk)
+#endif
+#ifndef ARP_PP_ALL
+#define ARP_PP_ALL 0  // two neighbour register sets in the search kernels' drains too
+#endif
+
 // MODE 0: plain step.  MODE 1: also keep every particle's neighbours within h (the recording
-// of the reverse sweep).  MODE 2: neighbour lists kept across stages (NlArgs): the stage
-// either walks its particles' lists or searches the grid with radius h + skin and writes new
-// lists.  Separate instances, so the plain step carries none of the other modes' registers.
-template <int MODE>
+// of the reverse sweep).  MODES 2, 3: neighbour lists kept across stages (NlArgs), one
+// kernel each, both launched for every stage; the one that does not apply to the stage
+// returns at once.  MODE 2 searches the grid with radius h + skin and writes new lists
+// (the entries found within h first, the others from the end of the particle's list);
+// MODE 3 walks the kept lists.  Separate instances, so each carries only its own registers.
+// LT: the material and interaction tables are read from LDS (at most kMatLds materials);
+// a compile-time choice, so that the pair loop's table reads are LDS reads, never generic
+// (flat) loads, whose completion the compiler could only await together with the
+// neighbour loads already in flight.
+template <int MODE, bool LT>
 __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
     constexpr bool REC = MODE == 1;
-    constexpr bool NL = MODE == 2;
-    __shared__ uint32_t list[kList * 256];
-    __shared__ float4 smat[kMatLds];                // mass, visc, bulk_visc
-    __shared__ DInter sint[kMatLds * kMatLds];
+    constexpr bool NL = MODE >= 2;
+    constexpr bool WALK = MODE == 3;
     const uint32_t tid = threadIdx.x;
+    // MODES 2, 3: the stage's build counter and whether it walks the kept lists
+    uint32_t gen = 0;
+    bool use = false;
+    if (NL) {
+        gen = *a.stamp;
+        use = nl_reuse(a.nl.words, a.stamp);
+        if (use != WALK) return;  // the other kernel of the pair runs this stage
+        if (bid == 0 && tid == 0) {
+            a.nl.words[gen & 1u] = gen;
+            atomicAdd(&a.nl.words[use ? 7 : 6], 1u);
+            if (!use) {
+                a.nl.words[4 + (gen & 1u)] = gen;
+                // a build over a sort that refused particles (bucket cap) is not kept
+                if (a.stats && *a.stats != 0u) a.nl.words[2 + (gen & 1u)] = gen;
+            }
+        }
+    }
+    uint32_t* list = s_list;
+    float4* smat = s_mat;
+    DInter* sint = s_inter;
     const uint32_t nm = a.phys.nm;
-    const bool ldsTab = nm <= (uint32_t)kMatLds;
-    if (ldsTab) {
+    if (LT) {
         for (uint32_t m = tid; m < nm; m += 256)
             smat[m] = make_float4(a.mats[m].mass, a.mats[m].visc, a.mats[m].bulk_visc, 0.f);
         for (uint32_t m = tid; m < nm * nm; m += 256) sint[m] = a.inter[m];
@@ -533,19 +568,6 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
     const float4* B = reinterpret_cast<const float4*>(a.bposd);
     const float4* BV = reinterpret_cast<const float4*>(a.bvelq);
     const float h = a.phys.h, hh = h * h, kn = a.phys.k;
-    // MODE 2: the stage's build counter and whether it walks the kept lists
-    uint32_t gen = 0;
-    bool use = false;
-    if (NL) {
-        gen = *a.stamp;
-        use = nl_reuse(a.nl.words, a.stamp);
-        if (bid == 0 && tid == 0) {
-            a.nl.words[gen & 1u] = gen;
-            atomicAdd(&a.nl.words[use ? 5 : 4], 1u);
-            // a build over a sort that refused particles (bucket cap) is not kept
-            if (!use && a.stats && *a.stats != 0u) a.nl.words[2 + (gen & 1u)] = gen;
-        }
-    }
     // candidates are tested against this radius (MODE 2 builds: h + skin)
     const float testR2 = (NL && !use) ? a.nl.rad2 : hh;
 
@@ -591,7 +613,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const bool elastic2 = (in.mk2 & kMkElastic) != 0;
             float m2, visc2, bulk2;
             DInter it;
-            if (ldsTab) {
+            if constexpr (LT) {
                 const float4 mm = smat[mat2];
                 m2 = mm.x;
                 visc2 = mm.y;
@@ -608,7 +630,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
             const float rr = rx * rx + ry * ry + rz * rz;
             // a build's list also holds candidates between h and h + skin: no terms
-            if (NL && !(hh - rr > 0.0f)) return;
+            if (NL && !WALK && !(hh - rr > 0.0f)) return;
             const float diff = fmaxf(hh - rr, 0.0f);
             const float gs = -6.0f * kn * diff * diff;
             const float gx = gs * rx, gy = gs * ry, gz = gs * rz;
@@ -661,49 +683,65 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             }
         };
         // the buffered candidates' terms; the next neighbour's loads are issued before the
-        // current pair's arithmetic
+        // current pair's arithmetic, into the other of two register sets (no copies, so a
+        // pair waits only for its own loads)
         auto drain = [&](uint32_t nl) {
-            PairIn nxt;
-            if (nl) nxt = load(list[tid]);
-            for (uint32_t k = 0; k < nl; k++) {
-                const PairIn cur2 = nxt;
-                if (k + 1 < nl) nxt = load(list[(k + 1) * 256 + tid]);
-                pair(cur2);
+            if (WALK || ARP_PP_ALL) {
+                PairIn pa, pb;
+                if (nl) pa = load(list[tid]);
+                for (uint32_t k = 0; k < nl; k += 2) {
+                    if (k + 1 < nl) pb = load(list[(k + 1) * 256 + tid]);
+                    pair(pa);
+                    if (k + 1 >= nl) break;
+                    if (k + 2 < nl) pa = load(list[(k + 2) * 256 + tid]);
+                    pair(pb);
+                }
+            } else {  // one register set (the search kernels keep five waves per SIMD)
+                PairIn nxt;
+                if (nl) nxt = load(list[tid]);
+                for (uint32_t k = 0; k < nl; k++) {
+                    const PairIn cur2 = nxt;
+                    if (k + 1 < nl) nxt = load(list[(k + 1) * 256 + tid]);
+                    pair(cur2);
+                }
             }
         };
 
-        uint32_t nrec = 0;  // neighbours recorded (reverse sweep) or listed (MODE 2 build)
-        if (NL && use) {
-            // walk the kept list; entries within h are buffered and drained as below
-            const uint32_t cnt = a.nl.count[i];
+        uint32_t nrec = 0;  // neighbours recorded (reverse sweep)
+        uint32_t nfront = 0, nback = 0;  // list entries within h / within h + skin (MODE 2)
+        if constexpr (WALK) {
+            // walk the kept list: the entries now within h are buffered in LDS (branch-free
+            // test, ARP_NLB entries per batch) and drained as in the search
+            const uint32_t cw = a.nl.count[i];
+            const uint32_t fa = cw & 0xFFFFu, fb = cw >> 16, m = fa + fb, cap = a.nl.cap;
             const uint32_t* L = a.nl.list;
             const size_t n = a.n;
             uint32_t k = 0, nl = 0;
             for (;;) {
-                while (nl + ARP_BATCH <= (uint32_t)kList && k < cnt) {
-                    uint32_t e[ARP_BATCH], ok[ARP_BATCH];
+                while (nl + ARP_NLB <= (uint32_t)kList && k < m) {
+                    uint32_t e[ARP_NLB], ok[ARP_NLB];
 #pragma unroll
-                    for (int u = 0; u < ARP_BATCH; u++) {
+                    for (int u = 0; u < ARP_NLB; u++) {
                         const uint32_t kk = k + u;
-                        ok[u] = kk < cnt ? 1u : 0u;
-                        e[u] = L[(size_t)(ok[u] ? kk : 0u) * n + i];
-                        e[u] = ok[u] ? e[u] : 0u;
+                        const uint32_t pos = kk < fa ? kk : cap - fb + (kk - fa);
+                        ok[u] = kk < m ? 1u : 0u;
+                        e[u] = ok[u] ? L[(size_t)pos * n + i] : 0u;
                     }
-                    float4 xj[ARP_BATCH];
+                    float4 xj[ARP_NLB];
 #pragma unroll
-                    for (int u = 0; u < ARP_BATCH; u++) xj[u] = ((e[u] & kBnd) ? B : X)[e[u] & ~kBnd];
+                    for (int u = 0; u < ARP_NLB; u++) xj[u] = ((e[u] & kBnd) ? B : X)[e[u] & ~kBnd];
 #pragma unroll
-                    for (int u = 0; u < ARP_BATCH; u++) {
+                    for (int u = 0; u < ARP_NLB; u++) {
                         const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
                         const uint32_t in = ok[u] & (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
                         list[nl * 256 + tid] = e[u];
                         nl += in;
                     }
-                    k += ARP_BATCH;
+                    k += ARP_NLB;
                 }
                 drain(nl);
                 nl = 0;
-                if (k >= cnt) break;
+                if (k >= m) break;
             }
         } else if (!noNb) {
             uint32_t cx, cy, cz;
@@ -821,9 +859,22 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     nrec += nl;
                 }
                 if (NL) {
-                    for (uint32_t k = 0; k < nl; k++)
-                        if (nrec + k < a.nl.cap) a.nl.list[(size_t)(nrec + k) * a.n + i] = list[k * 256 + tid];
-                    nrec += nl;
+                    // the candidates within h go to the front of the particle's list, the
+                    // others (h .. h + skin) to its end
+                    const uint32_t cap = a.nl.cap;
+                    for (uint32_t k = 0; k < nl; k++) {
+                        const uint32_t v = list[k * 256 + tid];
+                        const uint32_t j = v & ~kBnd;
+                        const float4 xj = ((v & kBnd) ? B : X)[j];
+                        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                        if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) {
+                            if (nfront < cap) a.nl.list[(size_t)nfront * a.n + i] = v;
+                            nfront++;
+                        } else {
+                            if (nback < cap) a.nl.list[(size_t)(cap - 1u - nback) * a.n + i] = v;
+                            nback++;
+                        }
+                    }
                 }
                 drain(nl);
                 nl = 0;
@@ -834,10 +885,55 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             a.nbCount[i] = min(nrec, a.nbCap);
             if (nrec > a.nbCap) atomicOr(a.nbOver, 1u);
         }
+        if (NL) {
+            // clear the bucket counts of the previous stage's build (the buckets recorded at
+            // this storage index; every index is visited once, so every count is cleared)
+            const uint32_t gp = gen - 1u;
+            if (a.nl.words[4 + (gp & 1u)] == gp) {
+                const uint2 cr = reinterpret_cast<const uint2*>(a.nl.cellRec)[i];
+                if (cr.y != kInvalid) {
+                    uint32_t* cc = a.nl.cellCnt + (size_t)(gp & 1u) * a.nl.ncells;
+                    const uint32_t sx = a.grid.sub[0], sxy = sx * a.grid.sub[1];
+                    for (uint32_t q = 0; q < 8; q++)
+                        if ((q & cr.y) == q) cc[cr.x + (q & 1u) + ((q >> 1) & 1u) * sx + (q >> 2) * sxy] = 0u;
+                }
+            }
+        }
         if (NL && !use) {
-            a.nl.count[i] = min(nrec, a.nl.cap);
+            bool stale = nfront + nback > a.nl.cap;
+            a.nl.count[i] = stale ? 0u : nfront | (nback << 16);
             reinterpret_cast<float4*>(a.nl.ref)[i] = xi;
-            if (nrec > a.nl.cap) a.nl.words[2 + (gen & 1u)] = gen;
+            // count this particle in every bucket within ext of it (it can be in no other
+            // bucket while the lists are used); the buckets must keep their slots
+            uint2 cr = make_uint2(0u, kInvalid);
+            if (isfinite(xi.x) && isfinite(xi.y) && isfinite(xi.z)) {
+                const GridP& gp = a.grid;
+                const float e = a.nl.ext;
+                uint32_t lo[3], m = 0;
+                const float xs[3] = {xi.x, xi.y, xi.z};
+                for (int ax = 0; ax < 3; ax++) {
+                    lo[ax] = cell_coord(xs[ax] - e, gp.min[ax], gp.dim[ax], gp.sub[ax]);
+                    const uint32_t hi = cell_coord(xs[ax] + e, gp.min[ax], gp.dim[ax], gp.sub[ax]);
+                    if (hi > lo[ax]) m |= 1u << ax;  // ext < bucket size: at most two per axis
+                }
+                const uint32_t sx = gp.sub[0], sxy = sx * gp.sub[1];
+                cr.x = lo[0] + sx * (lo[1] + gp.sub[1] * lo[2]);
+                cr.y = m;
+                uint32_t* cc = a.nl.cellCnt + (size_t)(gen & 1u) * a.nl.ncells;
+                for (uint32_t q = 0; q < 8; q++) {
+                    if ((q & m) != q) continue;
+                    const uint32_t key = cr.x + (q & 1u) + ((q >> 1) & 1u) * sx + (q >> 2) * sxy;
+                    const uint32_t c1 = atomicAdd(&cc[key], 1u) + 1u;
+                    if (c1 > a.nl.capThresh) {
+                        const uint32_t used = (a.bP[key + 1] - a.bP[key]) + (a.nl.rP[key + 1] - a.nl.rP[key]);
+                        stale = stale || c1 > kBucketCap - used;
+                    }
+                }
+            } else {
+                stale = true;
+            }
+            reinterpret_cast<uint2*>(a.nl.cellRec)[i] = cr;
+            if (stale) a.nl.words[2 + (gen & 1u)] = gen;
         }
 
         // derivative (fluid.rs:243-247): den' = sum, vel' = F/den + (0,-g,0), pos' = vel
@@ -856,25 +952,11 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             float4 op;
             rk_update_particle(a.rk, a.grid, Me, i, kP, kV, haveKey, nextKey, nextBin, op);
             if (NL) {
-                // the next stage may keep the lists only if this particle stays within skin/2
-                // of its position at the build and its next bucket keeps every entry
+                // the next stage may keep the lists only if this particle stays within lim of
+                // its position at the build
                 const float4 r0 = use ? reinterpret_cast<const float4*>(a.nl.ref)[i] : X[i];
                 const float dx = op.x - r0.x, dy = op.y - r0.y, dz = op.z - r0.z;
-                bool stale = !(dx * dx + dy * dy + dz * dz <= a.nl.lim2);
-                uint32_t* cc = a.nl.cellCnt;
-                const size_t nc = a.nl.ncells;
-                if (nextKey != kInvalid) {
-                    const uint32_t c1 = atomicAdd(&cc[((gen + 1u) & 1u) * nc + nextKey], 1u) + 1u;
-                    if (c1 > a.nl.capThresh) {
-                        const uint32_t used = (a.bP[nextKey + 1] - a.bP[nextKey]) + (a.nl.rP[nextKey + 1] - a.nl.rP[nextKey]);
-                        stale = stale || c1 > kBucketCap - used;
-                    }
-                }
-                const float4 xc = X[i];
-                uint32_t ux, uy, uz;
-                if (cell_of(a.grid, xc.x, xc.y, xc.z, ux, uy, uz))
-                    cc[(gen & 1u) * nc + ux + a.grid.sub[0] * (uy + a.grid.sub[1] * uz)] = 0u;
-                if (stale) a.nl.words[2 + (gen & 1u)] = gen;
+                if (!(dx * dx + dy * dy + dz * dz <= a.nl.lim2)) a.nl.words[2 + (gen & 1u)] = gen;
             }
         }
     }
@@ -916,19 +998,30 @@ ARP_ENTRIES(k_rowsort, RowSortArgs, rowsort_body)
 // k_fluid: blocks are remapped so that each XCD processes a contiguous share of the
 // cell-sorted particles (of the whole batch)
 // (five waves per SIMD: measured faster than four without the few spilled registers)
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid(FluidArgs a) { fluid_body<0>(a, xcd_block(), gridDim.x); }
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_batch(BatchArgs b) {
+// the material tables' location is chosen once per launch (uniform branch)
+template <int MODE>
+__device__ __forceinline__ void fluid_entry(const FluidArgs& a, uint32_t bid, uint32_t nb) {
+    if (a.phys.nm <= (uint32_t)kMatLds)
+        fluid_body<MODE, true>(a, bid, nb);
+    else
+        fluid_body<MODE, false>(a, bid, nb);
+}
+template <int MODE>
+__device__ __forceinline__ void fluid_entry_batch(const BatchArgs& b) {
     uint32_t bid, nb;
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
-    fluid_body<0>(a, bid, nb);
+    fluid_entry<MODE>(a, bid, nb);
 }
-extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { fluid_body<1>(a, xcd_block(), gridDim.x); }
-extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nl(FluidArgs a) { fluid_body<2>(a, xcd_block(), gridDim.x); }
-extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nl_batch(BatchArgs b) {
-    uint32_t bid, nb;
-    const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
-    fluid_body<2>(a, bid, nb);
-}
+
+// (five waves per SIMD: measured faster than four without the few spilled registers)
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid(FluidArgs a) { fluid_entry<0>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_batch(BatchArgs b) { fluid_entry_batch<0>(b); }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { fluid_entry<1>(a, xcd_block(), gridDim.x); }
+// the list pair of kernels: k_fluid_nlb searches and builds, k_fluid_nl walks the lists
+extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb(FluidArgs a) { fluid_entry<2>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb_batch(BatchArgs b) { fluid_entry_batch<2>(b); }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_nl(FluidArgs a) { fluid_entry<3>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_nl_batch(BatchArgs b) { fluid_entry_batch<3>(b); }
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA

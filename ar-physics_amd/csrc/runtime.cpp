@@ -65,6 +65,8 @@ enum Kern {
     K_FLUID_ADJ_REC,
     K_FLUID_NL,
     K_FLUID_NL_B,
+    K_FLUID_NLB,
+    K_FLUID_NLB_B,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -74,7 +76,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_adj_w",   "k_adj_perm", "k_grad_aos", "k_reduce",
                                    "k_key_batch", "k_scan_batch", "k_scatter_batch", "k_rowsort_batch",
                                    "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2",
-                                   "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch"};
+                                   "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch",
+                                   "k_fluid_nlb", "k_fluid_nlb_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -172,7 +175,7 @@ struct arp_ctx {
     bool nl_always = false; // rebuild at every stage (testing)
     bool nl_dirty = true;   // the decision words and bucket counts must be reset
     int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
-    DBuf nlWords, nlList, nlCount, nlRef, nlCell;
+    DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec;
     // adjoint recording
     bool rec_valid = false;
     float rec_dt = 0.0f;
@@ -213,7 +216,7 @@ struct arp_ctx {
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
                 &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp,
-                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell};
+                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell, &nlCellRec};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
             bufs.push_back(&K[k][1]);
@@ -646,11 +649,11 @@ int nl_prepare(arp_ctx* c) {
     const bool fresh = c->nlWords.p == nullptr;
     if ((rc = dalloc(c, c->nlWords, 32)) || (rc = dalloc(c, c->nlList, 4 * n * c->nl_cap)) ||
         (rc = dalloc(c, c->nlCount, 4 * n)) || (rc = dalloc(c, c->nlRef, 16 * n)) ||
-        (rc = dalloc(c, c->nlCell, 8 * (size_t)c->ncells)))
+        (rc = dalloc(c, c->nlCell, 8 * (size_t)c->ncells)) || (rc = dalloc(c, c->nlCellRec, 8 * n)))
         return rc;
     if (fresh && (rc = fill_u32(c, c->nlWords, 8, 0))) return rc;
     if (c->nl_dirty) {
-        if ((rc = fill_u32(c, c->nlWords, 4, kInvalid))) return rc;
+        if ((rc = fill_u32(c, c->nlWords, 6, kInvalid))) return rc;
         if ((rc = fill_u32(c, c->nlCell, (uint32_t)(2 * c->ncells), 0))) return rc;
         c->nl_dirty = false;
     }
@@ -664,6 +667,7 @@ NlArgs nl_args(arp_ctx* c) {
     a.count = P<uint32_t>(c->nlCount);
     a.ref = P<float>(c->nlRef);
     a.cellCnt = P<uint32_t>(c->nlCell);
+    a.cellRec = P<uint32_t>(c->nlCellRec);
     a.rP = P<uint32_t>(c->rP);
     a.ncells = (uint32_t)c->ncells;
     a.cap = c->nl_cap;
@@ -682,6 +686,9 @@ NlArgs nl_args(arp_ctx* c) {
     // stay within h + skin with a relative margin far above the rounding of the tests
     const float lim = 0.5f * c->nl_skin * h * (1.0f - 1e-3f);
     a.lim2 = c->nl_always ? -1.0f : lim * lim;
+    // a particle within lim of its build position lies in a bucket within ext of that
+    // position, with room for the rounding of the bucket function
+    a.ext = lim * (1.0f + 1e-3f) + 8.0f * kEps32 * box_extent(c->grid);
     return a;
 }
 
@@ -942,8 +949,15 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             f.nbOver = P<uint32_t>(rec->nbOver) + s;
             f.nbCap = kRecNb;
         }
-        if (nl) f.nl = nl_args(c);
-        if ((rc = launch(c, rec ? K_FLUID_REC : (nl ? K_FLUID_NL : K_FLUID), nblk(c->n), 256, f))) return rc;
+        if (nl) {
+            // the list pair: one searches and builds, the other walks the kept lists; the
+            // device decides which of the two works (the other returns at once)
+            f.nl = nl_args(c);
+            if ((rc = launch(c, K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
+            if ((rc = launch(c, K_FLUID_NL, nblk(c->n), 256, f))) return rc;
+        } else if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) {
+            return rc;
+        }
         if (c->solids_active) {
             SolidArgs sa = solid_base(c);
             sa.Xposd = Xp;
@@ -1906,8 +1920,8 @@ int arp_neighbor_stats(arp_ctx* c, uint32_t* builds, uint32_t* reused) {
         HIPCHK(c, hipMemcpyAsync(w, c->nlWords.p, sizeof w, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    if (builds) *builds = w[4];
-    if (reused) *reused = w[5];
+    if (builds) *builds = w[6];
+    if (reused) *reused = w[7];
     return ARP_OK;
 }
 
@@ -2118,6 +2132,7 @@ Kern batch_kernel(int k) {
         case K_ROWSORT: return K_ROWSORT_B;
         case K_FLUID: return K_FLUID_B;
         case K_FLUID_NL: return K_FLUID_NL_B;
+        case K_FLUID_NLB: return K_FLUID_NLB_B;
         default: return K_COUNT;
     }
 }

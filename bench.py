@@ -218,7 +218,7 @@ def main():
     sec = t1 - t0
     nl1 = sim.neighbor_stats()
     kern = {}
-    for k in ("k_fluid", "k_fluid_nl", "k_rowsort", "k_scatter", "k_scan", "k_key"):
+    for k in ("k_fluid", "k_fluid_nl", "k_fluid_nlb", "k_rowsort", "k_scatter", "k_scan", "k_key"):
         ms, cnt = sim.profile_query(k)
         if cnt:
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
