@@ -64,6 +64,7 @@ struct KeyArgs {
     uint32_t* keys;
     uint32_t* ranks;
     uint32_t* binCount;
+    const uint32_t* nlw;  // neighbour-list words (may be null): nothing to do when the stage reuses
 };
 
 constexpr int kMatLds = 16;  // material / interaction tables cached in LDS up to this many
@@ -82,6 +83,8 @@ struct ScanArgs {
     uint32_t* epoch;                // 1
     uint32_t* stats;                // zeroed for the following row sort
     uint32_t* stamp;                // build counter of the target grid, advanced once per build
+    const uint32_t* nlw;            // neighbour-list words (may be null): only the counter
+                                    // advances when the stage reuses its lists
 };
 
 struct ScatterArgs {
@@ -106,10 +109,12 @@ struct ScatterArgs {
     uint32_t* inv;  // pid -> storage index (full mode), may be null
     uint32_t* sflag;
     uint32_t* srcOut;  // full mode: previous storage index of every new slot (may be null)
-    // neighbour-list reuse words (may be null): when the stage reuses its lists the scatter
-    // copies the state in place of the full reorder and places nothing
+    // neighbour-list reuse words (may be null): a stage that reuses its lists places nothing;
+    // with copyOnReuse a full build then copies the state to the sorted buffers (when the
+    // previous step did not leave it there)
     const uint32_t* nlw;
     const uint32_t* stamp;
+    int copyOnReuse;
 };
 
 struct RowSortArgs {
@@ -161,6 +166,8 @@ struct RkArgs {
     int last;
     float* Oposd;  // X_{s+1} or y_{n+1}
     float* Ovelq;
+    float* Oposd2;  // second copy of the output (may be null)
+    float* Ovelq2;
     float* AccPOut;  // may be null
     float* AccVOut;
     float* KpOut;  // may be null
@@ -190,7 +197,9 @@ struct RkArgs {
 //   words[2 + (g&1)] = g  when that stage found a stale entry for the next stage
 //   words[4 + (g&1)] = g  when that stage built lists (its bucket counts are cleared by the
 //                          next stage, from the buckets each particle recorded in cellRec)
-//   words[6], [7]        number of list builds, number of stages run with lists
+//   words[6]         = g  of the last stage that ran with lists (read by the next stage's
+//                          kernels that run before its counter advances)
+//   words[8], [9]        number of list builds, number of stages run with lists
 // Stage g reuses iff words[(g-1)&1] == g-1 and words[2+((g-1)&1)] != g-1.
 // cellCnt holds two count arrays; the build at counter g counts into array g&1.
 struct NlArgs {
@@ -207,6 +216,8 @@ struct NlArgs {
     float rad, rad2;     // h + skin and its square (build radius)
     float lim2;          // allowed squared displacement from the build position (< (skin/2)^2)
     float ext;           // half-width of the box around a particle whose buckets count it
+    float grey2;         // pairs farther apart than this (squared) are checked to lie in
+                         // neighbouring buckets, as the reference's 27-bucket stencil requires
 };
 
 struct FluidArgs {
@@ -244,7 +255,12 @@ struct FluidArgs {
     uint32_t* nbCount;
     uint32_t* nbOver;
     uint32_t nbCap;
-    NlArgs nl;  // neighbour-list reuse (k_fluid_nl only)
+    NlArgs nl;  // neighbour-list reuse (k_fluid_nl, k_fluid_nlb)
+    // a list build after a full reorder also writes the sorted mk / pid into the buffers the
+    // reorder read (so that both copies agree and a reusing step needs no reorder)
+    uint32_t* mkMirror;
+    uint32_t* pidMirror;
+    const uint32_t* pidIn;
 };
 
 // Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists

@@ -126,25 +126,31 @@ __device__ __forceinline__ float contact_potential(uint32_t eq, float r, float s
 // ------------------------------------------------------------------------------------
 
 // Cell key of every particle + slot in its (y,z) row, reserved by wave-aggregated atomics.
-__device__ __forceinline__ void key_body(const KeyArgs& a, uint32_t bid, uint32_t) {
-    const uint32_t i = bid * 256u + threadIdx.x;
-    const bool in = i < a.n;
-    uint32_t bin = 0, key = kInvalid;
-    if (in) {
-        float4 p = reinterpret_cast<const float4*>(a.posd)[i];
-        uint32_t cx, cy, cz;
-        if (cell_of(a.grid, p.x, p.y, p.z, cx, cy, cz)) {
-            uint32_t row = cy + a.grid.sub[1] * cz;
-            key = cx + a.grid.sub[0] * row;
-            bin = row;
-        } else {
-            bin = a.grid.nrows;
+__device__ __forceinline__ bool nl_reuse_prev(const uint32_t* w);
+
+// (grid-stride over blocks of 256 particles: a gated launch that has nothing to do is short)
+__device__ __forceinline__ void key_body(const KeyArgs& a, uint32_t bid, uint32_t nb) {
+    if (a.nlw && nl_reuse_prev(a.nlw)) return;  // the stage reuses its neighbour lists
+    for (uint32_t base = bid * 256u; base < a.n; base += nb * 256u) {
+        const uint32_t i = base + threadIdx.x;
+        const bool in = i < a.n;
+        uint32_t bin = 0, key = kInvalid;
+        if (in) {
+            float4 p = reinterpret_cast<const float4*>(a.posd)[i];
+            uint32_t cx, cy, cz;
+            if (cell_of(a.grid, p.x, p.y, p.z, cx, cy, cz)) {
+                uint32_t row = cy + a.grid.sub[1] * cz;
+                key = cx + a.grid.sub[0] * row;
+                bin = row;
+            } else {
+                bin = a.grid.nrows;
+            }
         }
-    }
-    uint32_t rank = agg_rank(a.binCount, bin, in);
-    if (in) {
-        a.keys[i] = key;
-        a.ranks[i] = rank;
+        uint32_t rank = agg_rank(a.binCount, bin, in);
+        if (in) {
+            a.keys[i] = key;
+            a.ranks[i] = rank;
+        }
     }
 }
 
@@ -160,11 +166,18 @@ __device__ __forceinline__ void st_state(unsigned long long* p, unsigned long lo
 }
 constexpr unsigned long long kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStMask = 3ull << 62;
 
-__device__ __forceinline__ void scan_body(const ScanArgs& a, uint32_t, uint32_t) {
+__device__ __forceinline__ void scan_body(const ScanArgs& a, uint32_t bid, uint32_t) {
     __shared__ uint32_t sTile, sEpoch;
     __shared__ uint32_t wsumC[4], wsumF[4];
     __shared__ unsigned long long sPrefix;
     const uint32_t tid = threadIdx.x;
+    if (a.nlw && nl_reuse_prev(a.nlw)) {  // the stage reuses its lists: no counts to scan
+        if (bid == 0 && tid == 0) {
+            if (a.stats) a.stats[0] = 0;
+            if (a.stamp) *a.stamp += 1u;
+        }
+        return;
+    }
     if (tid == 0) {
         const uint32_t ep = __hip_atomic_load(a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sEpoch = ep;
@@ -255,6 +268,14 @@ __device__ __forceinline__ void move_particle(const ScatterArgs& a, uint32_t src
     if (a.inv) a.inv[p] = dst;
 }
 
+// Whether two positions lie in the same or neighbouring buckets (27-bucket stencil,
+// neighbor_list.rs:10-33 / fluid.rs:137-138).
+__device__ __forceinline__ bool buckets_adjacent(const GridP& g, const float4& p, const float4& q) {
+    uint32_t a0, a1, a2, b0, b1, b2;
+    if (!cell_of(g, p.x, p.y, p.z, a0, a1, a2) || !cell_of(g, q.x, q.y, q.z, b0, b1, b2)) return false;
+    return a0 + 1u >= b0 && b0 + 1u >= a0 && a1 + 1u >= b1 && b1 + 1u >= a1 && a2 + 1u >= b2 && b2 + 1u >= a2;
+}
+
 // Whether the stage about to run reuses the neighbour lists (see NlArgs).  Read after the
 // stage's row-count scan advanced the counter; the words it reads are written only by the
 // stage before.
@@ -262,23 +283,34 @@ __device__ __forceinline__ bool nl_reuse(const uint32_t* w, const uint32_t* stam
     const uint32_t g = *stamp, p = (g - 1u) & 1u;
     return w[p] == g - 1u && w[2 + p] != g - 1u;
 }
+// The same decision before the stage's counter advances (key pass, scan): from the counter
+// the previous stage recorded.
+__device__ __forceinline__ bool nl_reuse_prev(const uint32_t* w) {
+    const uint32_t g1 = w[6];
+    return g1 != kInvalid && w[g1 & 1u] == g1 && w[2 + (g1 & 1u)] != g1;
+}
 
 // Scatter (key, source) into the reserved row slots; particles with non-finite positions
 // go straight to their final place after all rows (neighbor_list.rs:86-93 drops them).
-// A stage that reuses its neighbour lists keeps the storage order: a full build then only
-// copies the state to the sorted buffers.
-__device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t) {
-    const uint32_t i = bid * 256u + threadIdx.x;
-    if (i >= a.n) return;
-    if (a.nlw && nl_reuse(a.nlw, a.stamp)) {
-        if (a.full) {
+// A stage that reuses its neighbour lists keeps the storage order and places nothing (with
+// copyOnReuse a full build copies the state to the sorted buffers).  Grid-stride over the
+// particles, so that a launch with nothing to do is short.
+__device__ __forceinline__ void scatter_one(const ScatterArgs& a, uint32_t i);
+__device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t nb) {
+    const bool reuse = a.nlw && nl_reuse(a.nlw, a.stamp);
+    if (reuse && !(a.full && a.copyOnReuse)) return;
+    for (uint32_t i = bid * 256u + threadIdx.x; i < a.n; i += nb * 256u) {
+        if (reuse) {
             reinterpret_cast<float4*>(a.dstPosd)[i] = reinterpret_cast<const float4*>(a.srcPosd)[i];
             reinterpret_cast<float4*>(a.dstVelq)[i] = reinterpret_cast<const float4*>(a.srcVelq)[i];
             a.dstMk[i] = a.srcMk[i];
             a.dstPid[i] = a.srcPid[i];
+        } else {
+            scatter_one(a, i);
         }
-        return;
     }
+}
+__device__ __forceinline__ void scatter_one(const ScatterArgs& a, uint32_t i) {
     const uint32_t key = a.keys[i], rank = a.ranks[i];
     if (key != kInvalid) {
         uint32_t row = key / a.grid.sub[0];
@@ -452,6 +484,10 @@ __device__ __forceinline__ void rk_update_particle(const RkArgs& r, const GridP&
     ov.w = pressure_q(M, op.w);
     opOut = op;
     reinterpret_cast<float4*>(r.Oposd)[i] = op;
+    if (r.Oposd2) {
+        reinterpret_cast<float4*>(r.Oposd2)[i] = op;
+        reinterpret_cast<float4*>(r.Ovelq2)[i] = ov;
+    }
     reinterpret_cast<float4*>(r.Ovelq)[i] = ov;
     if (r.keys) {
         uint32_t cx, cy, cz;
@@ -543,7 +579,8 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         if (use != WALK) return;  // the other kernel of the pair runs this stage
         if (bid == 0 && tid == 0) {
             a.nl.words[gen & 1u] = gen;
-            atomicAdd(&a.nl.words[use ? 7 : 6], 1u);
+            a.nl.words[6] = gen;
+            atomicAdd(&a.nl.words[use ? 9 : 8], 1u);
             if (!use) {
                 a.nl.words[4 + (gen & 1u)] = gen;
                 // a build over a sort that refused particles (bucket cap) is not kept
@@ -631,6 +668,14 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const float rr = rx * rx + ry * ry + rz * rz;
             // a build's list also holds candidates between h and h + skin: no terms
             if (NL && !WALK && !(hh - rr > 0.0f)) return;
+            // the reference pairs only particles in neighbouring buckets; a pair within h can
+            // be two buckets apart only within the rounding of the bucket function of h
+            if (NL && rr > a.nl.grey2) {
+                // (rare: the buckets are recomputed here, not kept across the loop)
+                float4 pi = xi;
+                asm volatile("" : "+v"(pi.x), "+v"(pi.y), "+v"(pi.z));
+                if (!buckets_adjacent(a.grid, pi, xj)) return;
+            }
             const float diff = fmaxf(hh - rr, 0.0f);
             const float gs = -6.0f * kn * diff * diff;
             const float gx = gs * rx, gy = gs * ry, gz = gs * rz;
@@ -899,6 +944,10 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                 }
             }
         }
+        if (NL && !use && a.mkMirror) {
+            a.mkMirror[i] = mk1;
+            a.pidMirror[i] = a.pidIn[i];
+        }
         if (NL && !use) {
             bool stale = nfront + nback > a.nl.cap;
             a.nl.count[i] = stale ? 0u : nfront | (nback << 16);
@@ -1020,7 +1069,7 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { flu
 // the list pair of kernels: k_fluid_nlb searches and builds, k_fluid_nl walks the lists
 extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb(FluidArgs a) { fluid_entry<2>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb_batch(BatchArgs b) { fluid_entry_batch<2>(b); }
-extern "C" __global__ void __launch_bounds__(256) k_fluid_nl(FluidArgs a) { fluid_entry<3>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl(FluidArgs a) { fluid_entry<3>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nl_batch(BatchArgs b) { fluid_entry_batch<3>(b); }
 
 // ------------------------------------------------------------------------------------

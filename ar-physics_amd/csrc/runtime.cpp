@@ -175,6 +175,7 @@ struct arp_ctx {
     bool nl_always = false; // rebuild at every stage (testing)
     bool nl_dirty = true;   // the decision words and bucket counts must be reset
     int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
+    bool nl_suspend = false;  // a batch whose scenes cannot all keep lists runs without
     DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec;
     // adjoint recording
     bool rec_valid = false;
@@ -519,11 +520,12 @@ struct SortIO {
     uint32_t* dstPid;
     uint32_t* srcOut = nullptr;  // full reorder: sorted slot -> source index
     const uint32_t* nlw = nullptr;  // neighbour-list words: the sort is skipped when reused
+    int copyOnReuse = 0;            // full build of a reusing stage: copy src -> dst
 };
 
-int run_key(arp_ctx* c, const float* posd) {
-    KeyArgs k{c->grid, c->n, posd, P<uint32_t>(c->keys), P<uint32_t>(c->ranks), P<uint32_t>(c->binCount)};
-    return launch(c, K_KEY, nblk(c->n), 256, k);
+int run_key(arp_ctx* c, const float* posd, const uint32_t* nlw = nullptr) {
+    KeyArgs k{c->grid, c->n, posd, P<uint32_t>(c->keys), P<uint32_t>(c->ranks), P<uint32_t>(c->binCount), nlw};
+    return launch(c, K_KEY, nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, k);
 }
 
 int run_sort(arp_ctx* c, const SortIO& io) {
@@ -533,7 +535,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     ScanArgs s{nbins, ntiles, P<uint32_t>(c->binCount), P<uint32_t>(c->binStart),
                P<uint32_t>(c->activeRows), P<uint32_t>(c->numActive),
                P<unsigned long long>(c->scanState), P<uint32_t>(c->scanTicket), P<uint32_t>(c->scanEpoch),
-               P<uint32_t>(c->stats), c->slot.stamp};
+               P<uint32_t>(c->stats), c->slot.stamp, io.nlw};
     if ((rc = launch(c, K_SCAN, ntiles, 256, s))) return rc;
     ScatterArgs sc{};
     sc.grid = c->grid;
@@ -559,7 +561,9 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.srcOut = io.full ? io.srcOut : nullptr;
     sc.nlw = io.nlw;
     sc.stamp = g.stamp;
-    if ((rc = launch(c, K_SCATTER, nblk(c->n), 256, sc))) return rc;
+    sc.copyOnReuse = io.copyOnReuse;
+    // (grid-stride: with lists kept the launch is mostly a no-op, so it stays small)
+    if ((rc = launch(c, K_SCATTER, io.nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
     r.grid = c->grid;
     r.n = c->n;
@@ -629,31 +633,26 @@ float box_extent(const GridP& g) {
     return M;
 }
 
-// Lists are kept for fluid-only scenes whose buckets are wider than h by more than float
-// rounding of the bucket function, so that every pair within h lies in neighbouring buckets
-// (the 27-bucket stencil of neighbor_list.rs) whatever the list says.
+// Lists are kept for fluid-only scenes (the solid path searches the grid itself).
 bool nl_enabled(const arp_ctx* c) {
-    if (!(c->nl_skin > 0.0f) || c->solids_active || c->n == 0) return false;
-    const float h = c->phys.h, M = box_extent(c->grid);
-    for (int i = 0; i < 3; i++) {
-        if (!(c->grid.dim[i] > 0.0f)) continue;
-        const float cs = c->grid.dim[i] / (float)c->grid.sub[i];
-        if (!(cs >= h + 16.0f * kEps32 * (M + h))) return false;
-    }
-    return true;
+    return c->nl_skin > 0.0f && !c->solids_active && c->n != 0 && !c->nl_suspend;
 }
 
 int nl_prepare(arp_ctx* c) {
     int rc;
     const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
     const bool fresh = c->nlWords.p == nullptr;
-    if ((rc = dalloc(c, c->nlWords, 32)) || (rc = dalloc(c, c->nlList, 4 * n * c->nl_cap)) ||
+    if ((rc = dalloc(c, c->nlWords, 64)) || (rc = dalloc(c, c->nlList, 4 * n * c->nl_cap)) ||
         (rc = dalloc(c, c->nlCount, 4 * n)) || (rc = dalloc(c, c->nlRef, 16 * n)) ||
         (rc = dalloc(c, c->nlCell, 8 * (size_t)c->ncells)) || (rc = dalloc(c, c->nlCellRec, 8 * n)))
         return rc;
-    if (fresh && (rc = fill_u32(c, c->nlWords, 8, 0))) return rc;
+    if (fresh && (rc = fill_u32(c, c->nlWords, 16, 0))) return rc;
+    if (c->keys_valid) {  // the plain step's epilogue binned the state; the list step bins per stage
+        if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
+        c->keys_valid = false;
+    }
     if (c->nl_dirty) {
-        if ((rc = fill_u32(c, c->nlWords, 6, kInvalid))) return rc;
+        if ((rc = fill_u32(c, c->nlWords, 7, kInvalid))) return rc;
         if ((rc = fill_u32(c, c->nlCell, (uint32_t)(2 * c->ncells), 0))) return rc;
         c->nl_dirty = false;
     }
@@ -689,6 +688,12 @@ NlArgs nl_args(arp_ctx* c) {
     // a particle within lim of its build position lies in a bucket within ext of that
     // position, with room for the rounding of the bucket function
     a.ext = lim * (1.0f + 1e-3f) + 8.0f * kEps32 * box_extent(c->grid);
+    // The bucket of a coordinate is off by at most sub * 1.5 ulp of the bucket index (the
+    // rounding of (p - min) / dim * sub); buckets are at least h wide up to that rounding, so
+    // a pair closer than h (1 - 2 w) always lies in neighbouring buckets.
+    uint32_t smax = std::max(c->grid.sub[0], std::max(c->grid.sub[1], c->grid.sub[2]));
+    const float w = 4.0f * (float)smax * kEps32 + 1e-6f;
+    a.grey2 = h * h * std::max(0.0f, 1.0f - 2.0f * w);
     return a;
 }
 
@@ -854,8 +859,11 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     }
     const uint32_t* nlw = nl ? P<uint32_t>(c->nlWords) : nullptr;
 
-    // stage-1 sort: full reorder of y_n into cell-major storage
-    if (!c->keys_valid) {
+    // stage-1 sort: full reorder of y_n into cell-major storage (with lists kept, the key
+    // pass runs at every stage that sorts, gated on the device)
+    if (nl) {
+        if ((rc = run_key(c, P<float>(c->Yland[0]), nlw))) return rc;
+    } else if (!c->keys_valid) {
         if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
     }
     const int nx = c->cur ^ 1;
@@ -867,7 +875,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
               P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_mk[c->cur]),
               P<uint32_t>(c->meta_pid[c->cur]),
               Y0p, Y0v, P<uint32_t>(c->meta_mk[nx]),
-              P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr, nlw};
+              P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr, nlw,
+              // one-stage tableaus leave y_{n+1} only in the landing buffers
+              S == 1 ? 1 : 0};
     if ((rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
@@ -886,6 +896,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
                        nullptr, nullptr, nullptr, nullptr, nullptr, nlw};
             if (rec) c->slot = rec->slot[s];
+            if (nl && (rc = run_key(c, Xp, nlw))) return rc;
             if ((rc = run_sort(c, pio))) return rc;
             if (rec) rec->slot[s] = c->slot;
         }
@@ -923,6 +934,13 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             r.Oposd = P<float>(c->Yland[0]);
             r.Ovelq = P<float>(c->Yland[1]);
             r.Oeps = P<float>(c->Yeps);  // in place: each solid's thread reads then writes
+            // with lists kept, y_{n+1} also goes to the sorted buffers, where a next step that
+            // reuses its lists (and so keeps the storage order) finds it; a thread overwrites
+            // only its own y_n there, and the stage's neighbours come from its own input
+            if (nl && S > 1) {
+                r.Oposd2 = Y0p;
+                r.Ovelq2 = Y0v;
+            }
         }
         r.Yeps = P<float>(c->Yeps);
         if (kslot[s] >= 0) {
@@ -930,9 +948,11 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             r.KvOut = P<float>(c->K[kslot[s]][1]);
             r.KeOut = c->solids_active ? P<float>(c->Ke[kslot[s]]) : nullptr;
         }
-        r.keys = P<uint32_t>(c->keys);
-        r.ranks = P<uint32_t>(c->ranks);
-        r.binCount = P<uint32_t>(c->binCount);
+        if (!nl) {  // the next stage's keys (with lists kept: the gated key pass)
+            r.keys = P<uint32_t>(c->keys);
+            r.ranks = P<uint32_t>(c->ranks);
+            r.binCount = P<uint32_t>(c->binCount);
+        }
 
         FluidArgs f = fluid_base(c);
         f.Xposd = Xp;
@@ -953,6 +973,11 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             // the list pair: one searches and builds, the other walks the kept lists; the
             // device decides which of the two works (the other returns at once)
             f.nl = nl_args(c);
+            if (s == 0) {  // a build after the full reorder mirrors the sorted meta
+                f.mkMirror = P<uint32_t>(c->meta_mk[c->cur ^ 1]);
+                f.pidMirror = P<uint32_t>(c->meta_pid[c->cur ^ 1]);
+                f.pidIn = P<uint32_t>(c->meta_pid[c->cur]);
+            }
             if ((rc = launch(c, K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
             if ((rc = launch(c, K_FLUID_NL, nblk(c->n), 256, f))) return rc;
         } else if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) {
@@ -977,7 +1002,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         }
     }
     if (rec) c->slot = own;
-    c->keys_valid = true;  // the last epilogue binned y_{n+1}
+    c->keys_valid = !nl;  // the plain step's last epilogue binned y_{n+1}
     c->have_build = false;
     return ARP_OK;
 }
@@ -1906,7 +1931,8 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
     c->nl_dirty = true;
     if (c->nlWords.p) {
         HIPCHK(c, hipSetDevice(c->device));
-        int rc = fill_u32(c, c->nlWords, 8, 0);  // statistics restart
+        FillArgs fa{P<uint32_t>(c->nlWords) + 8, 2, 0};  // statistics restart
+        int rc = launch(c, K_FILL, 1, 256, fa);
         if (rc) return rc;
     }
     return ARP_OK;
@@ -1914,14 +1940,14 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
 
 int arp_neighbor_stats(arp_ctx* c, uint32_t* builds, uint32_t* reused) {
     if (!c) return ARP_E_ARG;
-    uint32_t w[8] = {};
+    uint32_t w[16] = {};
     if (c->nlWords.p) {
         HIPCHK(c, hipSetDevice(c->device));
         HIPCHK(c, hipMemcpyAsync(w, c->nlWords.p, sizeof w, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    if (builds) *builds = w[6];
-    if (reused) *reused = w[7];
+    if (builds) *builds = w[8];
+    if (reused) *reused = w[9];
     return ARP_OK;
 }
 
@@ -2193,7 +2219,7 @@ int build_plan(arp_batch* b, float dt) {
     arp_ctx* L0 = b->lead;
     int rc;
     for (arp_ctx* c : b->ctx)
-        if (!c->keys_valid) {  // bin the current state once so every step has the same launches
+        if (!c->keys_valid && !nl_enabled(c)) {  // bin the current state once so every step has the same launches
             if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
             c->keys_valid = true;
         }
@@ -2310,6 +2336,19 @@ int run_batch(arp_batch* b, float dt, uint32_t nsteps) {
     arp_ctx* L0 = b->lead;
     int rc;
     if (nsteps == 0) return ARP_OK;
+    // one launch sequence for all scenes: lists only when every scene keeps them
+    bool all = true;
+    for (arp_ctx* c : b->ctx) {
+        c->nl_suspend = false;
+        all = all && nl_enabled(c);
+    }
+    for (arp_ctx* c : b->ctx) c->nl_suspend = !all;
+    struct Resume {
+        arp_batch* b;
+        ~Resume() {
+            for (arp_ctx* c : b->ctx) c->nl_suspend = false;
+        }
+    } resume{b};
     for (arp_ctx* c : b->ctx)  // outside the captured launches: reset stale list state
         if (nl_enabled(c) && (rc = nl_prepare(c))) return rc;
     if ((rc = build_plan(b, dt))) return rc;
