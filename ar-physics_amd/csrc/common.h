@@ -83,8 +83,9 @@ struct ScanArgs {
     uint32_t* epoch;                // 1
     uint32_t* stats;                // zeroed for the following row sort
     uint32_t* stamp;                // build counter of the target grid, advanced once per build
-    const uint32_t* nlw;            // neighbour-list words (may be null): only the counter
-                                    // advances when the stage reuses its lists
+    uint32_t* nlw;                  // neighbour-list words (may be null): advances their stage
+                                    // counter; only the counters advance when the stage
+                                    // reuses its lists
 };
 
 struct ScatterArgs {
@@ -191,8 +192,8 @@ struct RkArgs {
 // the lists while every particle stays within lim (< skin/2) of that position.  Bucket caps:
 // the build counts, for every bucket, the particles within ext (>= lim) of it; no bucket can
 // then hold more than that count while the lists are used, and a count above the bucket's
-// free slots makes the lists unusable.  The decision is taken on the device from the build
-// counter g of the stage (the grid stamp):
+// free slots makes the lists unusable.  The decision is taken on the device from the stage
+// counter g = words[10] (advanced by the row-count scan that opens every stage):
 //   words[g & 1]     = g  when the stage at counter g ran with lists (built or reused)
 //   words[2 + (g&1)] = g  when that stage found a stale entry for the next stage
 //   words[4 + (g&1)] = g  when that stage built lists (its bucket counts are cleared by the
@@ -200,6 +201,7 @@ struct RkArgs {
 //   words[6]         = g  of the last stage that ran with lists (read by the next stage's
 //                          kernels that run before its counter advances)
 //   words[8], [9]        number of list builds, number of stages run with lists
+//   words[10]            the stage counter
 // Stage g reuses iff words[(g-1)&1] == g-1 and words[2+((g-1)&1)] != g-1.
 // cellCnt holds two count arrays; the build at counter g counts into array g&1.
 struct NlArgs {
@@ -255,7 +257,8 @@ struct FluidArgs {
     uint32_t* nbCount;
     uint32_t* nbOver;
     uint32_t nbCap;
-    NlArgs nl;  // neighbour-list reuse (k_fluid_nl, k_fluid_nlb)
+    NlArgs nl;  // neighbour-list reuse (k_fluid_nl, k_fluid_nlb and their recording forms)
+    uint32_t* recUse;  // recording with lists: set to whether the stage walked kept lists
     // a list build after a full reorder also writes the sorted mk / pid into the buffers the
     // reorder read (so that both copies agree and a reusing step needs no reorder)
     uint32_t* mkMirror;
@@ -361,6 +364,10 @@ struct AdjArgs {
     const uint32_t* nbList;
     const uint32_t* nbCount;
     const uint32_t* nbOver;
+    uint32_t fastMat;  // the mobile particles' material (kernels without the _g suffix)
+    // recorded with kept lists: nonzero when the forward stage walked them (it then kept
+    // the storage order and sorted nothing: threads map to storage indices one to one)
+    const uint32_t* stageUse;
 };
 
 // W = cl * lambda + sum_t c_t * G_t (stage cotangent of k)

@@ -88,6 +88,11 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsu
     return pre + inc - v;
 }
 
+// Quotients of the pair terms through the hardware reciprocal (v_rcp_f32, 1 ulp) instead of
+// the correctly rounded division sequence
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // EOS (fluid.rs:45-52)
 __device__ __forceinline__ float eos_pressure(uint32_t eq, float density, float c, float d0) {
     switch (eq) {
@@ -171,12 +176,17 @@ __device__ __forceinline__ void scan_body(const ScanArgs& a, uint32_t bid, uint3
     __shared__ uint32_t wsumC[4], wsumF[4];
     __shared__ unsigned long long sPrefix;
     const uint32_t tid = threadIdx.x;
-    if (a.nlw && nl_reuse_prev(a.nlw)) {  // the stage reuses its lists: no counts to scan
-        if (bid == 0 && tid == 0) {
-            if (a.stats) a.stats[0] = 0;
-            if (a.stamp) *a.stamp += 1u;
+    if (a.nlw) {
+        const bool reuse = nl_reuse_prev(a.nlw);  // (read before the counter below advances)
+        __syncthreads();
+        if (bid == 0 && tid == 0) a.nlw[10] += 1u;  // the stage counter of the lists
+        if (reuse) {  // the stage reuses its lists: no counts to scan
+            if (bid == 0 && tid == 0) {
+                if (a.stats) a.stats[0] = 0;
+                if (a.stamp) *a.stamp += 1u;
+            }
+            return;
         }
-        return;
     }
     if (tid == 0) {
         const uint32_t ep = __hip_atomic_load(a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -279,8 +289,8 @@ __device__ __forceinline__ bool buckets_adjacent(const GridP& g, const float4& p
 // Whether the stage about to run reuses the neighbour lists (see NlArgs).  Read after the
 // stage's row-count scan advanced the counter; the words it reads are written only by the
 // stage before.
-__device__ __forceinline__ bool nl_reuse(const uint32_t* w, const uint32_t* stamp) {
-    const uint32_t g = *stamp, p = (g - 1u) & 1u;
+__device__ __forceinline__ bool nl_reuse(const uint32_t* w) {
+    const uint32_t g = w[10], p = (g - 1u) & 1u;
     return w[p] == g - 1u && w[2 + p] != g - 1u;
 }
 // The same decision before the stage's counter advances (key pass, scan): from the counter
@@ -297,7 +307,7 @@ __device__ __forceinline__ bool nl_reuse_prev(const uint32_t* w) {
 // particles, so that a launch with nothing to do is short.
 __device__ __forceinline__ void scatter_one(const ScatterArgs& a, uint32_t i);
 __device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t nb) {
-    const bool reuse = a.nlw && nl_reuse(a.nlw, a.stamp);
+    const bool reuse = a.nlw && nl_reuse(a.nlw);
     if (reuse && !(a.full && a.copyOnReuse)) return;
     for (uint32_t i = bid * 256u + threadIdx.x; i < a.n; i += nb * 256u) {
         if (reuse) {
@@ -305,6 +315,7 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid,
             reinterpret_cast<float4*>(a.dstVelq)[i] = reinterpret_cast<const float4*>(a.srcVelq)[i];
             a.dstMk[i] = a.srcMk[i];
             a.dstPid[i] = a.srcPid[i];
+            if (a.srcOut) a.srcOut[i] = i;
         } else {
             scatter_one(a, i);
         }
@@ -341,7 +352,7 @@ __device__ __forceinline__ void rowsort_body(const RowSortArgs& a, uint32_t bid,
     __shared__ uint16_t eX[kRowCap];
     __shared__ uint32_t wsum[4];
     const uint32_t tid = threadIdx.x;
-    if (a.nlw && nl_reuse(a.nlw, a.stamp)) return;  // the stage reuses its neighbour lists
+    if (a.nlw && nl_reuse(a.nlw)) return;  // the stage reuses its neighbour lists
     const uint32_t sx = a.grid.sub[0];
     const uint32_t nact = *a.numActive;
     for (uint32_t ri = bid; ri < nact; ri += nb) {
@@ -559,25 +570,27 @@ __shared__ DInter s_inter[kMatLds * kMatLds];
 // kernel each, both launched for every stage; the one that does not apply to the stage
 // returns at once.  MODE 2 searches the grid with radius h + skin and writes new lists
 // (the entries found within h first, the others from the end of the particle's list);
-// MODE 3 walks the kept lists.  Separate instances, so each carries only its own registers.
+// MODE 3 walks the kept lists.  MODES 4, 5: 2, 3 keeping, as MODE 1 does, each particle's
+// neighbours within h.  Separate instances, so each carries only its own registers.
 // LT: the material and interaction tables are read from LDS (at most kMatLds materials);
 // a compile-time choice, so that the pair loop's table reads are LDS reads, never generic
 // (flat) loads, whose completion the compiler could only await together with the
 // neighbour loads already in flight.
 template <int MODE, bool LT>
 __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
-    constexpr bool REC = MODE == 1;
+    constexpr bool REC = MODE == 1 || MODE >= 4;
     constexpr bool NL = MODE >= 2;
-    constexpr bool WALK = MODE == 3;
+    constexpr bool WALK = MODE == 3 || MODE == 5;
     const uint32_t tid = threadIdx.x;
     // MODES 2, 3: the stage's build counter and whether it walks the kept lists
     uint32_t gen = 0;
     bool use = false;
     if (NL) {
-        gen = *a.stamp;
-        use = nl_reuse(a.nl.words, a.stamp);
+        gen = a.nl.words[10];
+        use = nl_reuse(a.nl.words);
         if (use != WALK) return;  // the other kernel of the pair runs this stage
         if (bid == 0 && tid == 0) {
+            if (REC) *a.recUse = use ? 1u : 0u;
             a.nl.words[gen & 1u] = gen;
             a.nl.words[6] = gen;
             atomicAdd(&a.nl.words[use ? 9 : 8], 1u);
@@ -686,23 +699,23 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             acc.fy += pf * gy;
             acc.fz += pf * gz;
             if (cp != 0.0f) {
-                const float cs = __fdividef(m2 * cp, d2);
+                const float cs = fdiv(m2 * cp, d2);
                 const float cx = cs * gx, cy = cs * gy, cz = cs * gz;
                 if ((elastic1 || elastic2 || bnd) && mat1 != mat2 && it.friction != 0.0f) {
                     // friction along the tangential relative velocity (fluid.rs:191-199)
-                    const float irr = __fdividef(1.0f, rr);
+                    const float irr = fdiv(1.0f, rr);
                     const float ix = rx * irr, iy = ry * irr, iz = rz * irr;
                     const float cfr = cx * rx + cy * ry + cz * rz;
                     const float vr = vx * rx + vy * ry + vz * rz;
                     const float nx = ix * cfr, ny = iy * cfr, nz = iz * cfr;
                     float tx = vx - ix * vr, ty = vy - iy * vr, tz = vz - iz * vr;
-                    const float itl = __fdividef(1.0f, sqrtf(tx * tx + ty * ty + tz * tz));
+                    const float itl = fdiv(1.0f, sqrtf(tx * tx + ty * ty + tz * tz));
                     tx *= itl;
                     ty *= itl;
                     tz *= itl;
                     if (isfinite(tx) && isfinite(ty) && isfinite(tz)) {
                         const float fl = it.friction * sqrtf(nx * nx + ny * ny + nz * nz);
-                        const float itl2 = __fdividef(fl, sqrtf(tx * tx + ty * ty + tz * tz));
+                        const float itl2 = fdiv(fl, sqrtf(tx * tx + ty * ty + tz * tz));
                         acc.fx += itl2 * tx;
                         acc.fy += itl2 * ty;
                         acc.fz += itl2 * tz;
@@ -715,13 +728,13 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             if (!bnd) {
                 const float den_ = rr + a.phys.eps_hh;
                 if (!elastic1 && f1 + visc2 != 0.0f) {
-                    const float s = __fdividef(m1 * m2 * (f1 + visc2) * (rx * gx + ry * gy + rz * gz), d1 * d2 * den_);
+                    const float s = fdiv(m1 * m2 * (f1 + visc2) * (rx * gx + ry * gy + rz * gz), d1 * d2 * den_);
                     acc.fx -= s * vx;
                     acc.fy -= s * vy;
                     acc.fz -= s * vz;
                 }
                 const float ff = mat1 == mat2 ? bf1 + bulk2 : it.dampening;
-                const float s = m1 * m2 * __fdividef(ff * (vx * rx + vy * ry + vz * rz), (d1 + d2) * den_);
+                const float s = m1 * m2 * fdiv(ff * (vx * rx + vy * ry + vz * rz), (d1 + d2) * den_);
                 acc.fx -= s * gx;
                 acc.fy -= s * gy;
                 acc.fz -= s * gz;
@@ -783,6 +796,11 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                         nl += in;
                     }
                     k += ARP_NLB;
+                }
+                if (REC) {  // the buffered entries are the neighbours within h
+                    for (uint32_t q = 0; q < nl; q++)
+                        if (nrec + q < a.nbCap) a.nbList[(size_t)(nrec + q) * a.n + i] = list[q * 256 + tid];
+                    nrec += nl;
                 }
                 drain(nl);
                 nl = 0;
@@ -898,7 +916,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     }
                     q += ARP_BATCH;
                 }
-                if (REC) {
+                if (REC && !NL) {
                     for (uint32_t k = 0; k < nl; k++)
                         if (nrec + k < a.nbCap) a.nbList[(size_t)(nrec + k) * a.n + i] = list[k * 256 + tid];
                     nrec += nl;
@@ -915,6 +933,10 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                         if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) {
                             if (nfront < cap) a.nl.list[(size_t)nfront * a.n + i] = v;
                             nfront++;
+                            if (REC) {
+                                if (nrec < a.nbCap) a.nbList[(size_t)nrec * a.n + i] = v;
+                                nrec++;
+                            }
                         } else {
                             if (nback < cap) a.nl.list[(size_t)(cap - 1u - nback) * a.n + i] = v;
                             nback++;
@@ -991,7 +1013,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         uint32_t mte = mat1;
         asm volatile("" : "+v"(i), "+v"(mte));
         const DMat& Me = a.mats[mte];
-        const float id1 = __fdividef(1.0f, d1);
+        const float id1 = fdiv(1.0f, d1);
         const float4 kP = make_float4(vi.x, vi.y, vi.z, acc.den);
         const float4 kV = make_float4(acc.fx * id1, acc.fy * id1 - a.phys.g, acc.fz * id1, 0.0f);
         if (a.mode == 2 || (a.skip_elastic && elastic1)) {
@@ -1071,6 +1093,9 @@ extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb(FluidArgs a) { 
 extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb_batch(BatchArgs b) { fluid_entry_batch<2>(b); }
 extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl(FluidArgs a) { fluid_entry<3>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nl_batch(BatchArgs b) { fluid_entry_batch<3>(b); }
+// the same pair recording each stage's neighbours within h (the reverse sweep reads them)
+extern "C" __global__ void __launch_bounds__(256) k_fluid_nlbr(FluidArgs a) { fluid_entry<4>(a, xcd_block(), gridDim.x); }
+extern "C" __global__ void __launch_bounds__(256) k_fluid_nlr(FluidArgs a) { fluid_entry<5>(a, xcd_block(), gridDim.x); }
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA
@@ -1689,83 +1714,103 @@ __device__ void pair_vjp(const PairSide& P, const DInter& it, float rx, float ry
         const float cp = contact_potential(it.potential, L, it.radius, it.strength);
         if (cp != 0.0f) {
             const float dcp = contact_potential_d(it.potential, L, it.radius, it.strength);
-            const float cs = P.mj * cp / P.rhoj;
-            const float iL = 1.0f / L;
+            const float irj = frcp(P.rhoj);
+            const float cs = P.mj * cp * irj;
+            const float iL = frcp(L);
             for (int c = 0; c < 3; c++) {
                 o.F[c] += cs * gs * r[c];
-                o.gr[c] += (P.mj / P.rhoj) * (dcp * r[c] * iL * gs * ar + cp * (gs * A[c] + 2.0f * dgs * ar * r[c]));
+                o.gr[c] += (P.mj * irj) * (dcp * r[c] * iL * gs * ar + cp * (gs * A[c] + 2.0f * dgs * ar * r[c]));
             }
-            o.grj += -cs * gs * ar / P.rhoj;
+            o.grj += -cs * gs * ar * irj;
             if ((P.eli || P.elj || P.bnd) && P.mati != P.matj && it.friction != 0.0f) {
                 float t[3];
-                for (int c = 0; c < 3; c++) t[c] = v[c] - r[c] * vr / rr;
+                const float irr = frcp(rr);
+                for (int c = 0; c < 3; c++) t[c] = v[c] - r[c] * vr * irr;
                 const float tl = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
                 if (tl > 0.0f && isfinite(tl)) {
                     const float mu = it.friction;
                     const float c0 = cs * gs;
                     const float N = fabsf(c0) * L;
                     float th[3];
-                    for (int c = 0; c < 3; c++) th[c] = t[c] / tl;
+                    const float itl = frcp(tl);
+                    for (int c = 0; c < 3; c++) th[c] = t[c] * itl;
                     const float at = A[0] * th[0] + A[1] * th[1] + A[2] * th[2];
                     float ut[3];
                     for (int c = 0; c < 3; c++) {
                         o.F[c] += mu * N * th[c];
-                        ut[c] = mu * N * (A[c] - th[c] * at) / tl;
+                        ut[c] = mu * N * (A[c] - th[c] * at) * itl;
                     }
                     const float rut = r[0] * ut[0] + r[1] * ut[1] + r[2] * ut[2];
                     const float sg = c0 >= 0.0f ? 1.0f : -1.0f;
                     for (int c = 0; c < 3; c++) {
-                        o.gv[c] += ut[c] - r[c] * rut / rr;
-                        o.gr[c] += -(vr / rr * ut[c] + v[c] * rut / rr - 2.0f * vr * rut * r[c] / (rr * rr));
-                        const float dc0 = (P.mj / P.rhoj) * (dcp * r[c] * iL * gs + cp * 2.0f * dgs * r[c]);
+                        o.gv[c] += ut[c] - r[c] * rut * irr;
+                        o.gr[c] += -(vr * irr * ut[c] + v[c] * rut * irr - 2.0f * vr * rut * r[c] * (irr * irr));
+                        const float dc0 = (P.mj * irj) * (dcp * r[c] * iL * gs + cp * 2.0f * dgs * r[c]);
                         o.gr[c] += mu * at * sg * (L * dc0 + c0 * r[c] * iL);
                     }
-                    o.grj += mu * at * (-N / P.rhoj);
+                    o.grj += mu * at * (-N * irj);
                 }
             }
         }
     }
     if (!P.bnd) {
-        const float D = rr + eps_hh;
+        const float D = rr + eps_hh, iD = frcp(D), iRR = frcp(P.rhoi * P.rhoj);
         if (!P.eli && P.fi + P.fj != 0.0f) {  // viscosity: -C (fi+fj) gs rr / (rhoi rhoj D) v
-            const float u = gs * rr / D;
-            const float du = (dgs * rr + gs) / D - gs * rr / (D * D);
-            const float K4 = C * (P.fi + P.fj) / (P.rhoi * P.rhoj);
+            const float u = gs * rr * iD;
+            const float du = (dgs * rr + gs) * iD - gs * rr * (iD * iD);
+            const float K4 = C * (P.fi + P.fj) * iRR;
             const float s4 = K4 * u;
             for (int c = 0; c < 3; c++) {
                 o.F[c] -= s4 * v[c];
                 o.gv[c] -= s4 * A[c];
                 o.gr[c] -= av * K4 * du * 2.0f * r[c];
             }
-            o.gri += av * s4 / P.rhoi;
-            o.grj += av * s4 / P.rhoj;
+            o.gri += av * s4 * frcp(P.rhoi);
+            o.grj += av * s4 * frcp(P.rhoj);
         }
-        if (!P.eli) o.dvisc += -av * C * (gs * rr / D) / (P.rhoi * P.rhoj);
+        if (!P.eli) o.dvisc += -av * C * (gs * rr * iD) * iRR;
         // artificial viscosity: -C ff (v.r)/((rhoi+rhoj) D) grad W
         const float ff = P.same ? P.bfi + P.bfj : it.dampening;
-        const float E0 = -C / (P.rhoi + P.rhoj);
+        const float iS = frcp(P.rhoi + P.rhoj);
+        const float E0 = -C * iS;
         const float E = E0 * ff;
-        const float h5 = gs / D, dh5 = dgs / D - gs / (D * D);
+        const float h5 = gs * iD, dh5 = dgs * iD - gs * (iD * iD);
         const float phi = E * vr * ar * h5;
         for (int c = 0; c < 3; c++) {
             o.F[c] += E * vr * h5 * r[c];
             o.gv[c] += E * ar * h5 * r[c];
             o.gr[c] += E * (ar * h5 * v[c] + vr * h5 * A[c] + vr * ar * dh5 * 2.0f * r[c]);
         }
-        o.gri += -phi / (P.rhoi + P.rhoj);
-        o.grj += -phi / (P.rhoi + P.rhoj);
+        o.gri += -phi * iS;
+        o.grj += -phi * iS;
         if (P.same) o.dbulk += E0 * vr * ar * h5;
     }
 }
 
-// REC: the neighbours come from the forward stage's record instead of a new sweep
-template <bool REC>
+// One neighbour's data as the reverse stage reads it
+struct AdjIn {
+    float4 xj, vj, wpj, wvj;
+    uint32_t mk2;
+};
+
+// REC: the neighbours come from the forward stage's record instead of a new sweep.
+// FAST: every mobile particle has material a.fastMat and there are at most kMatLds
+// materials: the tables are read from LDS and the parameter partials are two scalars per
+// thread (the general form keeps a partial per material, indexed per pair).
+template <bool REC, bool FAST>
 __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
-    __shared__ uint32_t list[kList * 256];
+    uint32_t* list = s_list;
+    float4* smat = s_mat;
+    DInter* sint = s_inter;
     __shared__ float spg[4][2 * kMatLds];  // per wave, summed in a fixed order
     const uint32_t tid = threadIdx.x;
     const uint32_t nm = a.phys.nm;
     for (uint32_t m = tid; m < 4 * 2 * kMatLds; m += 256) spg[m / (2 * kMatLds)][m % (2 * kMatLds)] = 0.0f;
+    if (FAST) {
+        for (uint32_t m = tid; m < nm; m += 256)
+            smat[m] = make_float4(a.mats[m].mass, a.mats[m].visc, a.mats[m].bulk_visc, 0.f);
+        for (uint32_t m = tid; m < nm * nm; m += 256) sint[m] = a.inter[m];
+    }
     __syncthreads();
     const uint32_t t = xcd_block() * 256u + tid;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
@@ -1775,12 +1820,16 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
     const float4* WP = reinterpret_cast<const float4*>(a.Wp);
     const float4* WV = reinterpret_cast<const float4*>(a.Wv);
     const float h = a.phys.h, hh = h * h, kn = a.phys.k;
-    float pb[kMatLds], pv[kMatLds];  // per-thread parameter partials (nm <= kMatLds)
-    for (int m = 0; m < kMatLds; m++) pb[m] = pv[m] = 0.0f;
+    constexpr int NP = FAST ? 1 : kMatLds;
+    float pb[NP], pv[NP];  // per-thread parameter partials (FAST: of a.fastMat only)
+    for (int m = 0; m < NP; m++) pb[m] = pv[m] = 0.0f;
     if (t < a.n) {
         uint32_t i;
         bool noNb;
-        if (a.identity) {
+        if (a.stageUse && *a.stageUse) {  // the forward stage walked kept lists
+            i = t;
+            noNb = false;
+        } else if (a.identity) {
             i = t;
             noNb = a.sflag[t] != 0u;
         } else {
@@ -1797,7 +1846,7 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
         float4 wp = WP[i], wv = WV[i];
         wp = make_float4(ws * wp.x, ws * wp.y, ws * wp.z, ws * wp.w);
         wv = make_float4(ws * wv.x, ws * wv.y, ws * wv.z, 0.0f);
-        const float ir1 = 1.0f / rho1;
+        const float ir1 = frcp(rho1);
         const float a1x = wv.x * ir1, a1y = wv.y * ir1, a1z = wv.z * ir1;
         const float dq1 = pressure_q_d(M1, rho1);
         float G[3] = {0, 0, 0}, Gv[3] = {0, 0, 0}, Gr = 0.0f, F[3] = {0, 0, 0};
@@ -1900,18 +1949,44 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                     }
                     q += ARP_BATCH;
                 }
-                for (uint32_t k = 0; k < nl; k++) {
-                    const uint32_t e = list[k * 256 + tid];
+                // the recorded / found neighbours; the next one's loads are issued before the
+                // current pair's arithmetic, into the other of two register sets
+                auto ld = [&](uint32_t e) {
+                    AdjIn in;
                     const bool bnd = (e & kBnd) != 0;
                     const uint32_t j = e & ~kBnd;
-                    const float4 xj = bnd ? B[j] : X[j];
-                    const float4 vj = bnd ? BV[j] : XV[j];
-                    const uint32_t mk2 = bnd ? a.bmk[j] : a.mk[j];
+                    in.xj = bnd ? B[j] : X[j];
+                    in.vj = bnd ? BV[j] : XV[j];
+                    in.mk2 = bnd ? a.bmk[j] : a.mk[j];
+                    in.wpj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WP[j];
+                    in.wvj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WV[j];
+                    return in;
+                };
+                auto proc = [&](uint32_t e, const AdjIn& in) {
+                    const bool bnd = (e & kBnd) != 0;
+                    const float4 xj = in.xj, vj = in.vj;
+                    const uint32_t mk2 = in.mk2;
                     const uint32_t mat2 = mk2 & kMatMask;
-                    const DMat& M2 = a.mats[mat2];
+                    float m2, f2, bf2;
+                    DInter itA, itB;
+                    if constexpr (FAST) {
+                        const float4 mm = smat[mat2];
+                        m2 = mm.x;
+                        f2 = mm.y;
+                        bf2 = mm.z;
+                        itA = sint[mat1 * nm + mat2];
+                        itB = sint[mat2 * nm + mat1];
+                    } else {
+                        const DMat& M2 = a.mats[mat2];
+                        m2 = M2.mass;
+                        f2 = M2.visc;
+                        bf2 = M2.bulk_visc;
+                        itA = a.inter[mat1 * nm + mat2];
+                        itB = a.inter[mat2 * nm + mat1];
+                    }
                     PairSide P;
                     P.mi = M1.mass;
-                    P.mj = M2.mass;
+                    P.mj = m2;
                     P.rhoi = rho1;
                     P.rhoj = xj.w;
                     P.qi = vi.w;
@@ -1919,9 +1994,9 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                     P.dqi = dq1;
                     P.dqj = 0.0f;
                     P.fi = M1.visc;
-                    P.fj = M2.visc;
+                    P.fj = f2;
                     P.bfi = M1.bulk_visc;
-                    P.bfj = M2.bulk_visc;
+                    P.bfj = bf2;
                     P.eli = (mk1 & kMkElastic) != 0;
                     P.elj = (mk2 & kMkElastic) != 0;
                     P.bnd = bnd;
@@ -1930,7 +2005,7 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                     P.matj = mat2;
                     PairVjp o;
                     // side a: this particle's own terms
-                    pair_vjp(P, a.inter[mat1 * nm + mat2], xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x,
+                    pair_vjp(P, itA, xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x,
                              vj.y - vi.y, vj.z - vi.z, a1x, a1y, a1z, wp.w, hh, kn, a.phys.eps_hh, o);
                     for (int c = 0; c < 3; c++) {
                         G[c] -= o.gr[c];
@@ -1938,7 +2013,16 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         F[c] += o.F[c];
                     }
                     Gr += o.gri;
-                    if (nm <= (uint32_t)kMatLds) {
+                    if constexpr (FAST) {
+                        // mobile pairs are of one material; boundary pairs carry no
+                        // viscosity terms
+                        if (!bnd) {
+                            pb[0] += o.dbulk;
+                            pb[0] += o.dbulk;
+                            pv[0] += o.dvisc;
+                            pv[0] += o.dvisc;
+                        }
+                    } else if (nm <= (uint32_t)kMatLds) {
                         pb[mat1] += o.dbulk;
                         pb[mat2] += o.dbulk;
                         pv[mat1] += o.dvisc;
@@ -1946,12 +2030,12 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                     }
                     if (!bnd) {
                         // side b: the neighbour's terms as seen from this particle
-                        float4 wpj = WP[j], wvj = WV[j];
+                        float4 wpj = in.wpj, wvj = in.wvj;
                         wpj.w *= ws;
                         wvj = make_float4(ws * wvj.x, ws * wvj.y, ws * wvj.z, 0.0f);
-                        const float irj = 1.0f / xj.w;
+                        const float irj = frcp(xj.w);
                         PairSide Q;
-                        Q.mi = M2.mass;
+                        Q.mi = m2;
                         Q.mj = M1.mass;
                         Q.rhoi = xj.w;
                         Q.rhoj = rho1;
@@ -1959,9 +2043,9 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         Q.qj = vi.w;
                         Q.dqi = 0.0f;
                         Q.dqj = dq1;
-                        Q.fi = M2.visc;
+                        Q.fi = f2;
                         Q.fj = M1.visc;
-                        Q.bfi = M2.bulk_visc;
+                        Q.bfi = bf2;
                         Q.bfj = M1.bulk_visc;
                         Q.eli = P.elj;
                         Q.elj = P.eli;
@@ -1969,7 +2053,7 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         Q.same = P.same;
                         Q.mati = mat2;
                         Q.matj = mat1;
-                        pair_vjp(Q, a.inter[mat2 * nm + mat1], xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, vi.x - vj.x,
+                        pair_vjp(Q, itB, xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, vi.x - vj.x,
                                  vi.y - vj.y, vi.z - vj.z, wvj.x * irj, wvj.y * irj, wvj.z * irj, wpj.w, hh, kn,
                                  a.phys.eps_hh, o);
                         for (int c = 0; c < 3; c++) {
@@ -1977,6 +2061,27 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                             Gv[c] += o.gv[c];
                         }
                         Gr += o.grj;
+                    }
+                };
+                {
+                    AdjIn pa, pbn;
+                    uint32_t ea = 0, eb = 0;
+                    if (nl) {
+                        ea = list[tid];
+                        pa = ld(ea);
+                    }
+                    for (uint32_t k = 0; k < nl; k += 2) {
+                        if (k + 1 < nl) {
+                            eb = list[(k + 1) * 256 + tid];
+                            pbn = ld(eb);
+                        }
+                        proc(ea, pa);
+                        if (k + 1 >= nl) break;
+                        if (k + 2 < nl) {
+                            ea = list[(k + 2) * 256 + tid];
+                            pa = ld(ea);
+                        }
+                        proc(eb, pbn);
                     }
                 }
                 nl = 0;
@@ -2002,7 +2107,8 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
     // parameter partials: block sum in LDS, one row of partials per block
     if (nm <= (uint32_t)kMatLds) {
         for (uint32_t m = 0; m < nm; m++) {
-            float b = pb[m], v = pv[m];
+            if (FAST && m != a.fastMat) continue;  // (uniform) the other materials' partials are 0
+            float b = pb[FAST ? 0 : m], v = pv[FAST ? 0 : m];
             for (int o = 32; o >= 1; o >>= 1) {
                 b += __shfl_xor(b, o);
                 v += __shfl_xor(v, o);
@@ -2018,8 +2124,12 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
     }
 }
 
-extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) { fluid_adj_body<false>(a); }
-extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec(AdjArgs a) { fluid_adj_body<true>(a); }
+// reverse stage: searching the grid / on the recorded neighbours; _g: general material
+// tables (more than kMatLds materials or more than one mobile material)
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) { fluid_adj_body<false, true>(a); }
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec(AdjArgs a) { fluid_adj_body<true, true>(a); }
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_g(AdjArgs a) { fluid_adj_body<false, false>(a); }
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec_g(AdjArgs a) { fluid_adj_body<true, false>(a); }
 
 extern "C" __global__ void __launch_bounds__(256) k_adj_w(AdjWArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;

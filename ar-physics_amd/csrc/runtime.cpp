@@ -67,6 +67,10 @@ enum Kern {
     K_FLUID_NL_B,
     K_FLUID_NLB,
     K_FLUID_NLB_B,
+    K_FLUID_ADJ_G,
+    K_FLUID_ADJ_REC_G,
+    K_FLUID_NLBR,
+    K_FLUID_NLR,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -77,7 +81,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_key_batch", "k_scan_batch", "k_scatter_batch", "k_rowsort_batch",
                                    "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2",
                                    "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch",
-                                   "k_fluid_nlb", "k_fluid_nlb_batch"};
+                                   "k_fluid_nlb", "k_fluid_nlb_batch",
+                                   "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -105,7 +110,7 @@ struct GridSlot {
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
 struct RecStep {
-    DBuf mk, pid, src, stamps, nbOver;
+    DBuf mk, pid, src, stamps, nbOver, use;
     uint32_t over[ARP_MAX_STAGES] = {};  // host copy of nbOver after the recording
     DBuf nbList[ARP_MAX_STAGES], nbCount[ARP_MAX_STAGES];
     DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
@@ -144,6 +149,7 @@ struct arp_ctx {
     PhysP phys{};
     std::vector<ArpMaterial> mats_h;
     std::vector<uint32_t> mat_by_pid, sid_by_pid, mk_by_pid;
+    int64_t mobile_mat = -1;  // the material of every mobile particle, or -1 (none or several)
     std::vector<ArpSolidParticle> solids_h;
     std::vector<float> kscale_h;  // per-solid stiffness scale (empty: 1)
     std::vector<uint32_t> b_orig;  // sorted boundary slot -> original boundary id
@@ -519,7 +525,7 @@ struct SortIO {
     uint32_t* dstMk;
     uint32_t* dstPid;
     uint32_t* srcOut = nullptr;  // full reorder: sorted slot -> source index
-    const uint32_t* nlw = nullptr;  // neighbour-list words: the sort is skipped when reused
+    uint32_t* nlw = nullptr;  // neighbour-list words: the sort is skipped when reused
     int copyOnReuse = 0;            // full build of a reusing stage: copy src -> dst
 };
 
@@ -697,6 +703,11 @@ NlArgs nl_args(arp_ctx* c) {
     return a;
 }
 
+// Entries per particle and stage of a recording: with kept lists at least the lists'
+// capacity, so that a stage walking its list never overflows the record (the neighbours
+// within h are list entries); a stage that searched can overflow, and its grid is recorded.
+uint32_t rec_cap(const arp_ctx* c) { return nl_enabled(c) ? std::max<uint32_t>(kRecNb, c->nl_cap) : kRecNb; }
+
 GridSlot ctx_slot(arp_ctx* c) {
     GridSlot g;
     g.perm = P<uint32_t>(c->perm);
@@ -847,7 +858,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         if (T.b[j] != 0.0f) acc = true;
     if ((rc = ensure_k(c, nslots, acc))) return rc;
     // neighbour lists kept across stages (the plain step of a fluid-only scene)
-    const bool nl = !rec && nl_enabled(c);
+    const bool nl = nl_enabled(c);
     if (nl) {
         if (!c->capture) {
             if ((rc = nl_prepare(c))) return rc;
@@ -857,7 +868,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     } else {
         c->nl_dirty = true;  // the counters and words no longer follow the state
     }
-    const uint32_t* nlw = nl ? P<uint32_t>(c->nlWords) : nullptr;
+    uint32_t* nlw = nl ? P<uint32_t>(c->nlWords) : nullptr;
 
     // stage-1 sort: full reorder of y_n into cell-major storage (with lists kept, the key
     // pass runs at every stage that sorts, gated on the device)
@@ -876,8 +887,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
               P<uint32_t>(c->meta_pid[c->cur]),
               Y0p, Y0v, P<uint32_t>(c->meta_mk[nx]),
               P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr, nlw,
-              // one-stage tableaus leave y_{n+1} only in the landing buffers
-              S == 1 ? 1 : 0};
+              // one-stage tableaus, and recordings (whose stage input is kept apart), find
+              // y_n only in the landing buffers
+              (S == 1 || rec) ? 1 : 0};
     if ((rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
@@ -937,7 +949,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             // with lists kept, y_{n+1} also goes to the sorted buffers, where a next step that
             // reuses its lists (and so keeps the storage order) finds it; a thread overwrites
             // only its own y_n there, and the stage's neighbours come from its own input
-            if (nl && S > 1) {
+            if (nl && S > 1 && !rec) {
                 r.Oposd2 = Y0p;
                 r.Ovelq2 = Y0v;
             }
@@ -967,7 +979,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             f.nbList = P<uint32_t>(rec->nbList[s]);
             f.nbCount = P<uint32_t>(rec->nbCount[s]);
             f.nbOver = P<uint32_t>(rec->nbOver) + s;
-            f.nbCap = kRecNb;
+            f.nbCap = rec_cap(c);
         }
         if (nl) {
             // the list pair: one searches and builds, the other walks the kept lists; the
@@ -978,8 +990,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
                 f.pidMirror = P<uint32_t>(c->meta_pid[c->cur ^ 1]);
                 f.pidIn = P<uint32_t>(c->meta_pid[c->cur]);
             }
-            if ((rc = launch(c, K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
-            if ((rc = launch(c, K_FLUID_NL, nblk(c->n), 256, f))) return rc;
+            if (rec) f.recUse = P<uint32_t>(rec->use) + s;
+            if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
+            if ((rc = launch(c, rec ? K_FLUID_NLR : K_FLUID_NL, nblk(c->n), 256, f))) return rc;
         } else if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) {
             return rc;
         }
@@ -1029,9 +1042,10 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
         if ((rc = dalloc(c, *b, 4 * n))) return rc;
     if ((rc = dalloc(c, R.stamps, 4 * ARP_MAX_STAGES))) return rc;
     HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
-    if ((rc = dalloc(c, R.nbOver, 4 * ARP_MAX_STAGES))) return rc;
+    if ((rc = dalloc(c, R.nbOver, 4 * ARP_MAX_STAGES)) || (rc = dalloc(c, R.use, 4 * ARP_MAX_STAGES))) return rc;
+    HIPCHK(c, hipMemsetAsync(R.use.p, 0, 4 * ARP_MAX_STAGES, c->stream));
     for (int s = 0; s < S; s++)
-        if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)kRecNb * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
+        if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)rec_cap(c) * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
     for (int s = 0; s < S; s++) {
         if ((rc = dalloc(c, R.Xp[s], 16 * n)) || (rc = dalloc(c, R.Xv[s], 16 * n)) ||
             (rc = dalloc(c, R.perm[s], 4 * n)) || (rc = dalloc(c, R.sflag[s], 4 * n)) ||
@@ -1061,7 +1075,7 @@ void free_buf(DBuf& b) {
 void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
     for (size_t i = nrec; i < c->rec.size(); i++) {
         RecStep& R = c->rec[i];
-        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps, &R.nbOver}) free_buf(*b);
+        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps, &R.nbOver, &R.use}) free_buf(*b);
         for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
@@ -1187,8 +1201,13 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         a.nbList = P<uint32_t>(R.nbList[s]);
         a.nbCount = P<uint32_t>(R.nbCount[s]);
         a.nbOver = P<uint32_t>(R.nbOver) + s;
+        a.stageUse = P<uint32_t>(R.use) + s;
         // the recorded neighbours, unless a particle had more than the record holds
-        if ((rc = launch(c, R.over[s] ? K_FLUID_ADJ : K_FLUID_ADJ_REC, nblk(c->n), 256, a))) return rc;
+        // the fast kernels: LDS tables, one mobile material
+        const bool fast = c->mobile_mat >= 0 && c->nm <= (uint32_t)kMatLds;
+        a.fastMat = fast ? (uint32_t)c->mobile_mat : 0u;
+        const Kern ka = R.over[s] ? (fast ? K_FLUID_ADJ : K_FLUID_ADJ_G) : (fast ? K_FLUID_ADJ_REC : K_FLUID_ADJ_REC_G);
+        if ((rc = launch(c, ka, nblk(c->n), 256, a))) return rc;
         if (nm <= (uint32_t)kMatLds) {
             ReduceArgs r{nblk(c->n), 2 * nm, P<float>(c->pgradPart), P<float>(c->pgradSum)};
             if ((rc = launch(c, K_REDUCE, 2 * nm, 256, r))) return rc;
@@ -1501,6 +1520,9 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     }
     if ((rc = upload(c, c->mkStatic, c->mk_by_pid.data(), 4 * (size_t)n))) return rc;
     if ((rc = upload(c, c->matByPid, c->mat_by_pid.data(), 4 * (size_t)n))) return rc;
+    c->mobile_mat = n ? (int64_t)c->mat_by_pid[0] : -1;
+    for (uint32_t i = 1; i < n && c->mobile_mat >= 0; i++)
+        if (c->mat_by_pid[i] != (uint32_t)c->mobile_mat) c->mobile_mat = -1;
     if ((rc = upload(c, c->sidByPid, c->sid_by_pid.data(), 4 * (size_t)n))) return rc;
 
     // static boundary grid (mode 0 fill, once: neighbor_list.rs:247-264)
@@ -1657,7 +1679,12 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     for (RecStep& R : c->rec)
         if ((rc = alloc_rec(c, R, S))) return rc;
     for (uint32_t j = 0; j < nsteps; j++) {
-        if (j % K == 0 && (rc = ck_save(c, c->ck[j / K]))) return rc;
+        if (j % K == 0) {
+            if ((rc = ck_save(c, c->ck[j / K]))) return rc;
+            // kept neighbour lists restart with every segment, as in the replays from the
+            // segment's checkpoint, so that a replay repeats the recorded steps exactly
+            c->nl_dirty = true;
+        }
         if ((rc = step_once(c, dt, &c->rec[j % K]))) return rc;
         c->time += dt;
     }

@@ -164,6 +164,7 @@ def test_adjoint_leaves_state_and_replays_exactly(torch_cuda):
     dt = 5e-5
     g = np.random.default_rng(5).standard_normal((sc.n, 7))
     sim = FluidSim(sc, integrator="rk2", timestep=dt)
+    sim.set_neighbor_skin(0.0)  # the search at every stage, as a plain step (compared below)
     sim.upload_state(sc.particles)
     sim.adjoint_record(dt, 5, 2)
     end = sim.particles().copy()
@@ -177,17 +178,35 @@ def test_adjoint_leaves_state_and_replays_exactly(torch_cuda):
         assert _same_state(sim.particles(), end)
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
-    # one segment (no replay) gives the same gradient as checkpoints every 2 steps
-    sim2 = FluidSim(sc, integrator="rk2", timestep=dt)
-    sim2.upload_state(sc.particles)
-    sim2.adjoint_record(dt, 5, 5)
-    d_in2 = torch_cuda.zeros((sc.n, 12), dtype=torch_cuda.float32, device="cuda")
-    sim2.adjoint_backward(d_out, d_in2)
-    sim2.synchronize()
-    assert np.array_equal(d_in2.cpu().numpy(), outs[0][0])
-    # continuing after the backward equals continuing a plain run (the search at every
-    # stage, as the recording does; kept neighbour lists would sum in another order)
-    sim.set_neighbor_skin(0.0)
+    # one segment (no replay) gives the same gradient as checkpoints every 2 steps: bitwise
+    # with the search at every stage; with kept neighbour lists (which restart with every
+    # segment) up to the order of the per-particle sums
+    for skin in (0.0, 0.2):
+        s1 = FluidSim(sc, integrator="rk2", timestep=dt)
+        s1.set_neighbor_skin(skin)
+        s1.upload_state(sc.particles)
+        s1.adjoint_record(dt, 5, 2)
+        d_in1 = torch_cuda.zeros((sc.n, 12), dtype=torch_cuda.float32, device="cuda")
+        s1.adjoint_backward(d_out, d_in1)
+        d_in1b = torch_cuda.zeros((sc.n, 12), dtype=torch_cuda.float32, device="cuda")
+        s1.adjoint_backward(d_out, d_in1b)  # replays repeat the recording exactly
+        assert torch_cuda.equal(d_in1, d_in1b)
+        sim2 = FluidSim(sc, integrator="rk2", timestep=dt)
+        sim2.set_neighbor_skin(skin)
+        sim2.upload_state(sc.particles)
+        sim2.adjoint_record(dt, 5, 5)
+        d_in2 = torch_cuda.zeros((sc.n, 12), dtype=torch_cuda.float32, device="cuda")
+        sim2.adjoint_backward(d_out, d_in2)
+        sim2.synchronize()
+        g1, g2 = d_in1.cpu().numpy(), d_in2.cpu().numpy()
+        if skin == 0.0:
+            assert np.array_equal(g2, g1)
+        else:
+            assert np.max(np.abs(g2 - g1)) <= 1e-4 * np.max(np.abs(g1))
+        s1.close()
+        if skin == 0.0:
+            sim2.close()
+    # continuing after the backward equals continuing a plain run
     sim.step(dt, 3)
     sim3 = FluidSim(sc, integrator="rk2", timestep=dt)
     sim3.set_neighbor_skin(0.0)
