@@ -86,6 +86,8 @@ struct ScanArgs {
     uint32_t* nlw;                  // neighbour-list words (may be null): advances their stage
                                     // counter; only the counters advance when the stage
                                     // reuses its lists
+    uint32_t* clear;                // words zeroed before the stage's kernels (may be null)
+    uint32_t nclear;
 };
 
 struct ScatterArgs {
@@ -116,6 +118,8 @@ struct ScatterArgs {
     const uint32_t* nlw;
     const uint32_t* stamp;
     int copyOnReuse;
+    uint32_t* recMk;   // full build of a recording: second copy of the sorted mk / pid
+    uint32_t* recPid;
 };
 
 struct RowSortArgs {
@@ -148,6 +152,8 @@ struct RowSortArgs {
     uint32_t* sflag;  // per storage index: cap/non-finite flag (may be null)
     uint32_t* srcOut;
     const uint32_t* nlw;  // neighbour-list reuse words (may be null): no row sort when reused
+    uint32_t* recMk;      // full build of a recording: second copy of the sorted mk / pid
+    uint32_t* recPid;
 };
 
 // RK stage epilogue: k -> next stage input / y_{n+1}, the b-weighted accumulator and the
@@ -359,7 +365,7 @@ struct AdjArgs {
     float* Gv;         // (v.xyz, 0)
     float* LamP;       // step cotangent accumulator (+= G)
     float* LamV;
-    float* pgrad;      // per block: nm * 2 (d bulk_visc, d visc)
+    float* pgrad;      // per block: nm * 2 (d bulk_visc, d visc), accumulated over the sweep
     // neighbours recorded by the forward stage (null, or *nbOver set: search again)
     const uint32_t* nbList;
     const uint32_t* nbCount;
@@ -368,6 +374,18 @@ struct AdjArgs {
     // recorded with kept lists: nonzero when the forward stage walked them (it then kept
     // the storage order and sorted nothing: threads map to storage indices one to one)
     const uint32_t* stageUse;
+    // first reverse stage of a step: the accumulator starts from these (else LamP/LamV)
+    const float* LamInP;
+    const float* LamInV;
+    // with LamInP: also form the next reverse stage's W = wl * Lam_in + wg * G here
+    float* WnP;
+    float* WnV;
+    float wl, wg;
+    // last reverse stage of a step (no solid passes after it): the accumulator goes to
+    // LamOut[permOut[i]] (the storage order before the step) instead of LamP
+    const uint32_t* permOut;
+    float* LamOutP;
+    float* LamOutV;
 };
 
 // W = cl * lambda + sum_t c_t * G_t (stage cotangent of k)

@@ -176,6 +176,7 @@ __device__ __forceinline__ void scan_body(const ScanArgs& a, uint32_t bid, uint3
     __shared__ uint32_t wsumC[4], wsumF[4];
     __shared__ unsigned long long sPrefix;
     const uint32_t tid = threadIdx.x;
+    if (a.clear && bid == 0 && tid < a.nclear) a.clear[tid] = 0u;
     if (a.nlw) {
         const bool reuse = nl_reuse_prev(a.nlw);  // (read before the counter below advances)
         __syncthreads();
@@ -276,6 +277,10 @@ __device__ __forceinline__ void move_particle(const ScatterArgs& a, uint32_t src
     uint32_t p = a.srcPid[src];
     a.dstPid[dst] = p;
     if (a.inv) a.inv[p] = dst;
+    if (a.recMk) {
+        a.recMk[dst] = a.srcMk[src];
+        a.recPid[dst] = p;
+    }
 }
 
 // Whether two positions lie in the same or neighbouring buckets (27-bucket stencil,
@@ -308,14 +313,20 @@ __device__ __forceinline__ bool nl_reuse_prev(const uint32_t* w) {
 __device__ __forceinline__ void scatter_one(const ScatterArgs& a, uint32_t i);
 __device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t nb) {
     const bool reuse = a.nlw && nl_reuse(a.nlw);
-    if (reuse && !(a.full && a.copyOnReuse)) return;
+    if (reuse && !(a.full && (a.copyOnReuse || a.recMk))) return;
     for (uint32_t i = bid * 256u + threadIdx.x; i < a.n; i += nb * 256u) {
         if (reuse) {
-            reinterpret_cast<float4*>(a.dstPosd)[i] = reinterpret_cast<const float4*>(a.srcPosd)[i];
-            reinterpret_cast<float4*>(a.dstVelq)[i] = reinterpret_cast<const float4*>(a.srcVelq)[i];
-            a.dstMk[i] = a.srcMk[i];
-            a.dstPid[i] = a.srcPid[i];
+            if (a.copyOnReuse) {
+                reinterpret_cast<float4*>(a.dstPosd)[i] = reinterpret_cast<const float4*>(a.srcPosd)[i];
+                reinterpret_cast<float4*>(a.dstVelq)[i] = reinterpret_cast<const float4*>(a.srcVelq)[i];
+                a.dstMk[i] = a.srcMk[i];
+                a.dstPid[i] = a.srcPid[i];
+            }
             if (a.srcOut) a.srcOut[i] = i;
+            if (a.recMk) {
+                a.recMk[i] = a.srcMk[i];
+                a.recPid[i] = a.srcPid[i];
+            }
         } else {
             scatter_one(a, i);
         }
@@ -433,6 +444,10 @@ __device__ __forceinline__ void rowsort_body(const RowSortArgs& a, uint32_t bid,
                 a.dstMk[dst] = a.srcMk[src];
                 a.dstPid[dst] = p;
                 if (a.inv) a.inv[p] = dst;
+                if (a.recMk) {
+                    a.recMk[dst] = a.srcMk[src];
+                    a.recPid[dst] = p;
+                }
                 if (a.srcOut) a.srcOut[dst] = src;
             } else {
                 a.perm[dst] = src | flag;
@@ -1793,6 +1808,97 @@ struct AdjIn {
     uint32_t mk2;
 };
 
+// Both sides of a pair of non-elastic mobile particles of one material at once: the
+// cotangent of x_i, v_i, rho_i from a_i.(force on i) + wd_i (density rate of i) and from
+// a_j.(force on j) + wd_j (density rate of j).  The pressure, viscosity and artificial
+// viscosity forces are antisymmetric under i <-> j, so their two contractions are one
+// contraction with a_i - a_j; the density rates differ only by the mass factor; the
+// contact forces differ by (m/rho) of the other side.  r = x_j - x_i, v = v_j - v_i.
+// Returns the amounts to subtract from G (x_i) and Gv (v_i), to add to Gr (rho_i), the
+// force on i (F) and the side-i parameter partials (as pair_vjp's side a).
+struct PairSym {
+    float gr[3], gv[3], gri, F[3], dbulk, dvisc;
+};
+__device__ __forceinline__ void pair_vjp_sym(float mi, float mj, float rhoi, float rhoj, float qi, float qj, float dqi,
+                                             float f, float bf, const DInter& it, float rx, float ry, float rz,
+                                             float vx, float vy, float vz, float aix, float aiy, float aiz,
+                                             float ajx, float ajy, float ajz, float wdi, float wdj, float hh,
+                                             float kn, float eps_hh, PairSym& o) {
+    for (int c = 0; c < 3; c++) o.gr[c] = o.gv[c] = o.F[c] = 0.0f;
+    o.gri = o.dbulk = o.dvisc = 0.0f;
+    const float rr = rx * rx + ry * ry + rz * rz;
+    const float d = hh - rr;
+    if (!(d > 0.0f)) return;
+    const float gs = -6.0f * kn * d * d, dgs = 12.0f * kn * d;
+    const float r[3] = {rx, ry, rz}, v[3] = {vx, vy, vz};
+    const float Ae[3] = {aix - ajx, aiy - ajy, aiz - ajz};
+    const float vr = vx * rx + vy * ry + vz * rz;
+    const float are = Ae[0] * rx + Ae[1] * ry + Ae[2] * rz;
+    const float ari = aix * rx + aiy * ry + aiz * rz;
+    const float ave = Ae[0] * vx + Ae[1] * vy + Ae[2] * vz;
+    const float avi = aix * vx + aiy * vy + aiz * vz;
+    const float C = mi * mj;
+    {  // density rates: (wd_i m_j + wd_j m_i) gs (v.r)
+        const float c1 = wdi * mj + wdj * mi;
+        for (int c = 0; c < 3; c++) {
+            o.gv[c] += c1 * gs * r[c];
+            o.gr[c] += c1 * (gs * v[c] + 2.0f * dgs * vr * r[c]);
+        }
+    }
+    {  // pressure: C (qi + qj) grad W
+        const float Q = qi + qj;
+        for (int c = 0; c < 3; c++) {
+            o.F[c] += C * Q * gs * r[c];
+            o.gr[c] += C * Q * (gs * Ae[c] + 2.0f * dgs * are * r[c]);
+        }
+        o.gri += C * dqi * gs * are;
+    }
+    if (it.potential) {  // contact: (m_j / rho_j) on side i, (m_i / rho_i) on side j
+        const float L = sqrtf(rr);
+        const float cp = contact_potential(it.potential, L, it.radius, it.strength);
+        if (cp != 0.0f) {
+            const float dcp = contact_potential_d(it.potential, L, it.radius, it.strength);
+            const float irj = frcp(rhoj), iri = frcp(rhoi), iL = frcp(L);
+            const float sa = mj * irj, sb = mi * iri;
+            const float Ac[3] = {sa * aix - sb * ajx, sa * aiy - sb * ajy, sa * aiz - sb * ajz};
+            const float arc = Ac[0] * rx + Ac[1] * ry + Ac[2] * rz;
+            const float arj = ajx * rx + ajy * ry + ajz * rz;
+            for (int c = 0; c < 3; c++) {
+                o.F[c] += sa * cp * gs * r[c];
+                o.gr[c] += dcp * r[c] * iL * gs * arc + cp * (gs * Ac[c] + 2.0f * dgs * arc * r[c]);
+            }
+            o.gri += sb * cp * gs * arj * iri;
+        }
+    }
+    const float D = rr + eps_hh, iD = frcp(D), iRR = frcp(rhoi * rhoj);
+    if (f != 0.0f) {  // viscosity: -C f gs rr / (rhoi rhoj D) v
+        const float u = gs * rr * iD;
+        const float du = (dgs * rr + gs) * iD - gs * rr * (iD * iD);
+        const float K4 = C * f * iRR;
+        const float s4 = K4 * u;
+        for (int c = 0; c < 3; c++) {
+            o.F[c] -= s4 * v[c];
+            o.gv[c] -= s4 * Ae[c];
+            o.gr[c] -= ave * K4 * du * 2.0f * r[c];
+        }
+        o.gri += ave * s4 * frcp(rhoi);
+    }
+    o.dvisc += -avi * C * (gs * rr * iD) * iRR;
+    {  // artificial viscosity: -C bf (v.r)/((rhoi+rhoj) D) grad W
+        const float iS = frcp(rhoi + rhoj);
+        const float E0 = -C * iS;
+        const float E = E0 * bf;
+        const float h5 = gs * iD, dh5 = dgs * iD - gs * (iD * iD);
+        for (int c = 0; c < 3; c++) {
+            o.F[c] += E * vr * h5 * r[c];
+            o.gv[c] += E * are * h5 * r[c];
+            o.gr[c] += E * (are * h5 * v[c] + vr * h5 * Ae[c] + vr * are * dh5 * 2.0f * r[c]);
+        }
+        o.gri += -E * vr * are * h5 * iS;
+        o.dbulk += E0 * vr * ari * h5;
+    }
+}
+
 // REC: the neighbours come from the forward stage's record instead of a new sweep.
 // FAST: every mobile particle has material a.fastMat and there are at most kMatLds
 // materials: the tables are read from LDS and the parameter partials are two scalars per
@@ -1984,6 +2090,27 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         itA = a.inter[mat1 * nm + mat2];
                         itB = a.inter[mat2 * nm + mat1];
                     }
+                    if (FAST && !bnd && !((mk1 | mk2) & kMkElastic)) {
+                        // both sides at once (one material, no elastic particle)
+                        const float4 wpj = in.wpj, wvj = in.wvj;
+                        const float irj = frcp(xj.w);
+                        PairSym o;
+                        pair_vjp_sym(M1.mass, m2, rho1, xj.w, vi.w, vj.w, dq1, M1.visc + f2, M1.bulk_visc + bf2, itA,
+                                     xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z,
+                                     a1x, a1y, a1z, ws * wvj.x * irj, ws * wvj.y * irj, ws * wvj.z * irj, wp.w,
+                                     ws * wpj.w, hh, kn, a.phys.eps_hh, o);
+                        for (int c = 0; c < 3; c++) {
+                            G[c] -= o.gr[c];
+                            Gv[c] -= o.gv[c];
+                            F[c] += o.F[c];
+                        }
+                        Gr += o.gri;
+                        pb[0] += o.dbulk;
+                        pb[0] += o.dbulk;
+                        pv[0] += o.dvisc;
+                        pv[0] += o.dvisc;
+                        return;
+                    }
                     PairSide P;
                     P.mi = M1.mass;
                     P.mj = m2;
@@ -2094,15 +2221,37 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
         Gv[2] += wp.z;
         Gr += -(wv.x * F[0] + wv.y * F[1] + wv.z * F[2]) * ir1 * ir1;
         const float4 gp = make_float4(G[0], G[1], G[2], Gr), gv = make_float4(Gv[0], Gv[1], Gv[2], 0.0f);
-        reinterpret_cast<float4*>(a.Gp)[i] = gp;
-        reinterpret_cast<float4*>(a.Gv)[i] = gv;
+        if (a.Gp) {  // (kept when a later reverse stage reads it)
+            reinterpret_cast<float4*>(a.Gp)[i] = gp;
+            reinterpret_cast<float4*>(a.Gv)[i] = gv;
+        }
         float4* LP = reinterpret_cast<float4*>(a.LamP) + i;
         float4* LV = reinterpret_cast<float4*>(a.LamV) + i;
-        float4 lp = *LP, lv = *LV;
+        float4 lp, lv;
+        if (a.LamInP) {
+            lp = reinterpret_cast<const float4*>(a.LamInP)[i];
+            lv = reinterpret_cast<const float4*>(a.LamInV)[i];
+            if (a.WnP) {
+                const float wl = a.wl, wg = a.wg;
+                reinterpret_cast<float4*>(a.WnP)[i] =
+                    make_float4(wl * lp.x + wg * gp.x, wl * lp.y + wg * gp.y, wl * lp.z + wg * gp.z, wl * lp.w + wg * gp.w);
+                reinterpret_cast<float4*>(a.WnV)[i] =
+                    make_float4(wl * lv.x + wg * gv.x, wl * lv.y + wg * gv.y, wl * lv.z + wg * gv.z, 0.0f);
+            }
+        } else {
+            lp = *LP;
+            lv = *LV;
+        }
         lp.x += gp.x; lp.y += gp.y; lp.z += gp.z; lp.w += gp.w;
         lv.x += gv.x; lv.y += gv.y; lv.z += gv.z;
-        *LP = lp;
-        *LV = lv;
+        if (a.permOut) {
+            const uint32_t d = a.permOut[i];
+            reinterpret_cast<float4*>(a.LamOutP)[d] = lp;
+            reinterpret_cast<float4*>(a.LamOutV)[d] = lv;
+        } else {
+            *LP = lp;
+            *LV = lv;
+        }
     }
     // parameter partials: block sum in LDS, one row of partials per block
     if (nm <= (uint32_t)kMatLds) {
@@ -2120,7 +2269,7 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
         }
         __syncthreads();
         for (uint32_t m = tid; m < 2 * nm; m += 256)
-            a.pgrad[(size_t)blockIdx.x * 2 * nm + m] = ((spg[0][m] + spg[1][m]) + spg[2][m]) + spg[3][m];
+            a.pgrad[(size_t)blockIdx.x * 2 * nm + m] += ((spg[0][m] + spg[1][m]) + spg[2][m]) + spg[3][m];
     }
 }
 

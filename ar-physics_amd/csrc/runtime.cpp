@@ -110,7 +110,7 @@ struct GridSlot {
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
 struct RecStep {
-    DBuf mk, pid, src, stamps, nbOver, use;
+    DBuf mk, pid, src, stamps, flags;  // flags: nbOver[ARP_MAX_STAGES], use[ARP_MAX_STAGES]
     uint32_t over[ARP_MAX_STAGES] = {};  // host copy of nbOver after the recording
     DBuf nbList[ARP_MAX_STAGES], nbCount[ARP_MAX_STAGES];
     DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
@@ -182,6 +182,10 @@ struct arp_ctx {
     bool nl_dirty = true;   // the decision words and bucket counts must be reset
     int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
     bool nl_suspend = false;  // a batch whose scenes cannot all keep lists runs without
+    // recording: the step recorded after the current one (its stage input can be written by
+    // the current step), and the stage-input buffer that already holds the live state
+    RecStep* rec_next = nullptr;
+    const void* rec_prefilled = nullptr;
     DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec;
     // adjoint recording
     bool rec_valid = false;
@@ -190,7 +194,7 @@ struct arp_ctx {
     ArpTableau rec_tab{};
     std::vector<Checkpoint> ck;
     std::vector<RecStep> rec;
-    DBuf LamIn[2], LamAcc[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
+    DBuf LamIn[2], LamAcc[2], LamNext[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
     DBuf Fbar, spar;  // solid reverse mode: per-solid F cotangents, per-solid stiffness partials
     DBuf kscale;      // per-solid stiffness scale (empty: 1)
     bool keys_valid = false;
@@ -527,6 +531,10 @@ struct SortIO {
     uint32_t* srcOut = nullptr;  // full reorder: sorted slot -> source index
     uint32_t* nlw = nullptr;  // neighbour-list words: the sort is skipped when reused
     int copyOnReuse = 0;            // full build of a reusing stage: copy src -> dst
+    uint32_t* recMk = nullptr;      // full build of a recording: the recorded meta
+    uint32_t* recPid = nullptr;
+    uint32_t* clear = nullptr;      // words the scan zeroes first (a recording's stage flags)
+    uint32_t nclear = 0;
 };
 
 int run_key(arp_ctx* c, const float* posd, const uint32_t* nlw = nullptr) {
@@ -541,7 +549,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     ScanArgs s{nbins, ntiles, P<uint32_t>(c->binCount), P<uint32_t>(c->binStart),
                P<uint32_t>(c->activeRows), P<uint32_t>(c->numActive),
                P<unsigned long long>(c->scanState), P<uint32_t>(c->scanTicket), P<uint32_t>(c->scanEpoch),
-               P<uint32_t>(c->stats), c->slot.stamp, io.nlw};
+               P<uint32_t>(c->stats), c->slot.stamp, io.nlw, io.clear, io.nclear};
     if ((rc = launch(c, K_SCAN, ntiles, 256, s))) return rc;
     ScatterArgs sc{};
     sc.grid = c->grid;
@@ -568,6 +576,8 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.nlw = io.nlw;
     sc.stamp = g.stamp;
     sc.copyOnReuse = io.copyOnReuse;
+    sc.recMk = io.full ? io.recMk : nullptr;
+    sc.recPid = io.full ? io.recPid : nullptr;
     // (grid-stride: with lists kept the launch is mostly a no-op, so it stays small)
     if ((rc = launch(c, K_SCATTER, io.nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
@@ -601,6 +611,8 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.sflag = g.sflag;
     r.srcOut = io.full ? io.srcOut : nullptr;
     r.nlw = io.nlw;
+    r.recMk = io.full ? io.recMk : nullptr;
+    r.recPid = io.full ? io.recPid : nullptr;
     // one workgroup per occupied row, looping; enough workgroups for about rowsort_div
     // particles each (a tuning knob), at most 4096
     const uint32_t want = std::max<uint32_t>(1, (c->n + c->rowsort_div - 1) / c->rowsort_div);
@@ -887,17 +899,17 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
               P<uint32_t>(c->meta_pid[c->cur]),
               Y0p, Y0v, P<uint32_t>(c->meta_mk[nx]),
               P<uint32_t>(c->meta_pid[nx]), rec ? P<uint32_t>(rec->src) : nullptr, nlw,
-              // one-stage tableaus, and recordings (whose stage input is kept apart), find
-              // y_n only in the landing buffers
-              (S == 1 || rec) ? 1 : 0};
+              // one-stage tableaus find y_n only in the landing buffers; so does a recording
+              // unless the previous recorded step also left it in this step's stage input
+              (S == 1 || (rec && c->rec_prefilled != rec->Xp[0].p)) ? 1 : 0,
+              rec ? P<uint32_t>(rec->mk) : nullptr, rec ? P<uint32_t>(rec->pid) : nullptr,
+              // the recording's per-stage flags start at zero
+              rec ? P<uint32_t>(rec->flags) : nullptr, rec ? 2u * ARP_MAX_STAGES : 0u};
     if ((rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
     if (rec) {
-        if ((rc = fill_u32(c, rec->nbOver, ARP_MAX_STAGES, 0))) return rc;
-        rec->slot[0] = c->slot;
-        HIPCHK(c, hipMemcpyAsync(rec->mk.p, c->meta_mk[nx].p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(rec->pid.p, c->meta_pid[nx].p, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+        rec->slot[0] = c->slot;  // (the sort also wrote the recorded meta)
     }
 
     const float* Xp = Y0p;
@@ -952,6 +964,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             if (nl && S > 1 && !rec) {
                 r.Oposd2 = Y0p;
                 r.Ovelq2 = Y0v;
+            } else if (nl && S > 1 && rec && c->rec_next && c->rec_next != rec) {  // the next recorded step's input
+                r.Oposd2 = P<float>(c->rec_next->Xp[0]);
+                r.Ovelq2 = P<float>(c->rec_next->Xv[0]);
             }
         }
         r.Yeps = P<float>(c->Yeps);
@@ -978,7 +993,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         if (rec) {  // keep the stage's neighbour lists for the reverse sweep
             f.nbList = P<uint32_t>(rec->nbList[s]);
             f.nbCount = P<uint32_t>(rec->nbCount[s]);
-            f.nbOver = P<uint32_t>(rec->nbOver) + s;
+            f.nbOver = P<uint32_t>(rec->flags) + s;
             f.nbCap = rec_cap(c);
         }
         if (nl) {
@@ -990,7 +1005,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
                 f.pidMirror = P<uint32_t>(c->meta_pid[c->cur ^ 1]);
                 f.pidIn = P<uint32_t>(c->meta_pid[c->cur]);
             }
-            if (rec) f.recUse = P<uint32_t>(rec->use) + s;
+            if (rec) f.recUse = P<uint32_t>(rec->flags) + ARP_MAX_STAGES + s;
             if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
             if ((rc = launch(c, rec ? K_FLUID_NLR : K_FLUID_NL, nblk(c->n), 256, f))) return rc;
         } else if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) {
@@ -1016,6 +1031,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     }
     if (rec) c->slot = own;
     c->keys_valid = !nl;  // the plain step's last epilogue binned y_{n+1}
+    c->rec_prefilled = (rec && nl && S > 1 && c->rec_next && c->rec_next != rec) ? c->rec_next->Xp[0].p : nullptr;
     c->have_build = false;
     return ARP_OK;
 }
@@ -1040,19 +1056,22 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
     int rc;
     for (DBuf* b : {&R.mk, &R.pid, &R.src})
         if ((rc = dalloc(c, *b, 4 * n))) return rc;
+    // the slots' build counters and row stamps start at zero when (re)allocated; kept
+    // buffers keep growing counters, so rows left from an older build never match
+    const void* st0 = R.stamps.p;
     if ((rc = dalloc(c, R.stamps, 4 * ARP_MAX_STAGES))) return rc;
-    HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
-    if ((rc = dalloc(c, R.nbOver, 4 * ARP_MAX_STAGES)) || (rc = dalloc(c, R.use, 4 * ARP_MAX_STAGES))) return rc;
-    HIPCHK(c, hipMemsetAsync(R.use.p, 0, 4 * ARP_MAX_STAGES, c->stream));
+    if (R.stamps.p != st0) HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
+    if ((rc = dalloc(c, R.flags, 8 * ARP_MAX_STAGES))) return rc;
     for (int s = 0; s < S; s++)
         if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)rec_cap(c) * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
     for (int s = 0; s < S; s++) {
+        const void* rs0 = R.rowStamp[s].p;
         if ((rc = dalloc(c, R.Xp[s], 16 * n)) || (rc = dalloc(c, R.Xv[s], 16 * n)) ||
             (rc = dalloc(c, R.perm[s], 4 * n)) || (rc = dalloc(c, R.sflag[s], 4 * n)) ||
             (rc = dalloc(c, R.rowOff[s], 4 * nr * c->grid.rowstride)) || (rc = dalloc(c, R.rowStamp[s], 4 * nr)))
             return rc;
-        // the slot's counter only grows, so rows left from an older build never match
-        HIPCHK(c, hipMemsetAsync(R.rowStamp[s].p, 0, 4 * nr, c->stream));
+        if (R.rowStamp[s].p != rs0 || R.stamps.p != st0)
+            HIPCHK(c, hipMemsetAsync(R.rowStamp[s].p, 0, 4 * nr, c->stream));
         GridSlot& g = R.slot[s];
         g.perm = P<uint32_t>(R.perm[s]);
         g.sflag = P<uint32_t>(R.sflag[s]);
@@ -1075,7 +1094,7 @@ void free_buf(DBuf& b) {
 void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
     for (size_t i = nrec; i < c->rec.size(); i++) {
         RecStep& R = c->rec[i];
-        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps, &R.nbOver, &R.use}) free_buf(*b);
+        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps, &R.flags}) free_buf(*b);
         for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
@@ -1105,6 +1124,7 @@ int ck_save(arp_ctx* c, Checkpoint& k) {
 int ck_restore(arp_ctx* c, Checkpoint& k) {
     const size_t n = c->n;
     c->nl_dirty = true;
+    c->rec_prefilled = nullptr;
     HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, k.P.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, k.V.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->meta_mk[c->cur].p, k.mk.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1119,7 +1139,7 @@ int ck_restore(arp_ctx* c, Checkpoint& k) {
 // host copies of the neighbour-record overflow flags of the first nrec recorded steps
 int fetch_over(arp_ctx* c, uint32_t nrec) {
     for (uint32_t j = 0; j < nrec; j++)
-        HIPCHK(c, hipMemcpyAsync(c->rec[j].over, c->rec[j].nbOver.p, sizeof(c->rec[j].over), hipMemcpyDeviceToHost,
+        HIPCHK(c, hipMemcpyAsync(c->rec[j].over, c->rec[j].flags.p, sizeof(c->rec[j].over), hipMemcpyDeviceToHost,
                                  c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ARP_OK;
@@ -1134,12 +1154,13 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
     const size_t n = c->n;
     const uint32_t nm = c->nm;
     int rc;
-    HIPCHK(c, hipMemcpyAsync(c->LamAcc[0].p, c->LamIn[0].p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->LamAcc[1].p, c->LamIn[1].p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    (void)n;
     if (c->solids_active) {  // particle id -> storage index of this step
         InvArgs ia{c->n, P<uint32_t>(R.pid), P<uint32_t>(c->inv)};
         if ((rc = launch(c, K_INV, nblk(c->n), 256, ia))) return rc;
     }
+    bool wReady = false;   // Wb already holds this stage's W (formed by the previous kernel)
+    bool folded = false;   // the last kernel wrote dL/dy_n in the previous order (LamNext)
     for (int s = S - 1; s >= 0; s--) {
         AdjWArgs w{};
         w.n = c->n;
@@ -1159,17 +1180,22 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         const float* wP = P<float>(c->Wb[0]);
         const float* wV = P<float>(c->Wb[1]);
         float wscale = 1.0f;
-        if (w.ng == 0) {
+        bool wIsWb = true;
+        if (wReady) {
+        } else if (w.ng == 0) {
             wP = w.LP;
             wV = w.LV;
             wscale = w.cl;
+            wIsWb = false;
         } else if (w.ng == 1 && w.cl == 0.0f) {
             wP = w.GP[0];
             wV = w.GV[0];
             wscale = w.cg[0];
+            wIsWb = false;
         } else if ((rc = launch(c, K_ADJ_W, nblk(c->n), 256, w))) {
             return rc;
         }
+        wReady = false;
         const GridSlot& g = R.slot[s];
         AdjArgs a{};
         a.grid = c->grid;
@@ -1193,25 +1219,50 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         a.Wp = wP;
         a.Wv = wV;
         a.wscale = wscale;
-        a.Gp = P<float>(c->Gb[s][0]);
-        a.Gv = P<float>(c->Gb[s][1]);
+        // the next reverse stage's W formed here when it is wl * Lam_in + wg * G_s alone
+        bool fuseW = false;
+        if (s == S - 1 && s >= 1 && !wIsWb && !c->solids_active) {
+            fuseW = true;
+            for (int t = s; t < S; t++)  // terms of W_{s-1}: t > s-1 with a[t][s-1] != 0
+                if (T.a[t][s - 1] != 0.0f && t != s) fuseW = false;
+            if (T.a[s][s - 1] == 0.0f) fuseW = false;
+        }
+        // G_s is kept when a later reverse stage reads it (beyond a fused W)
+        bool needG = c->solids_active;
+        for (int t = 0; t < s; t++)
+            if (T.a[s][t] != 0.0f && !(fuseW && t == s - 1)) needG = true;
+        a.Gp = needG ? P<float>(c->Gb[s][0]) : nullptr;
+        a.Gv = needG ? P<float>(c->Gb[s][1]) : nullptr;
+        if (fuseW) {
+            a.WnP = P<float>(c->Wb[0]);
+            a.WnV = P<float>(c->Wb[1]);
+            a.wl = dt * T.b[s - 1];
+            a.wg = dt * T.a[s][s - 1];
+            wReady = true;
+        }
+        if (s == 0 && !c->solids_active) {  // the step's permutation, folded in
+            a.permOut = P<uint32_t>(R.src);
+            a.LamOutP = P<float>(c->LamNext[0]);
+            a.LamOutV = P<float>(c->LamNext[1]);
+            folded = true;
+        }
         a.LamP = P<float>(c->LamAcc[0]);
         a.LamV = P<float>(c->LamAcc[1]);
         a.pgrad = P<float>(c->pgradPart);
         a.nbList = P<uint32_t>(R.nbList[s]);
         a.nbCount = P<uint32_t>(R.nbCount[s]);
-        a.nbOver = P<uint32_t>(R.nbOver) + s;
-        a.stageUse = P<uint32_t>(R.use) + s;
+        a.nbOver = P<uint32_t>(R.flags) + s;
+        a.stageUse = P<uint32_t>(R.flags) + ARP_MAX_STAGES + s;
+        if (s == S - 1) {  // the step's accumulator starts from dL/dy_{n+1}
+            a.LamInP = P<float>(c->LamIn[0]);
+            a.LamInV = P<float>(c->LamIn[1]);
+        }
         // the recorded neighbours, unless a particle had more than the record holds
         // the fast kernels: LDS tables, one mobile material
         const bool fast = c->mobile_mat >= 0 && c->nm <= (uint32_t)kMatLds;
         a.fastMat = fast ? (uint32_t)c->mobile_mat : 0u;
         const Kern ka = R.over[s] ? (fast ? K_FLUID_ADJ : K_FLUID_ADJ_G) : (fast ? K_FLUID_ADJ_REC : K_FLUID_ADJ_REC_G);
         if ((rc = launch(c, ka, nblk(c->n), 256, a))) return rc;
-        if (nm <= (uint32_t)kMatLds) {
-            ReduceArgs r{nblk(c->n), 2 * nm, P<float>(c->pgradPart), P<float>(c->pgradSum)};
-            if ((rc = launch(c, K_REDUCE, 2 * nm, 256, r))) return rc;
-        }
         if (c->solids_active) {
             // the stage's stress and F again, then the two solid passes
             SolidArgs sa = solid_base(c);
@@ -1240,6 +1291,11 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             if ((rc = launch(c, K_SOLID_ADJ1, nblk(c->ns), 256, sj))) return rc;
             if ((rc = launch(c, K_SOLID_ADJ2, nblk(c->ns), 256, sj))) return rc;
         }
+    }
+    if (folded) {  // dL/dy_n is in LamNext
+        std::swap(c->LamIn[0], c->LamNext[0]);
+        std::swap(c->LamIn[1], c->LamNext[1]);
+        return ARP_OK;
     }
     AdjPermArgs pa{c->n, P<uint32_t>(R.src), P<float>(c->LamAcc[0]), P<float>(c->LamAcc[1]),
                    P<float>(c->LamIn[0]), P<float>(c->LamIn[1])};
@@ -1322,8 +1378,8 @@ void arp_destroy(arp_ctx* c) {
     for (DBuf* b : c->bufs)
         if (b->p) hipFree(b->p);
     free_rec(c);
-    for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->Wb[0], &c->Wb[1], &c->pgradPart,
-                    &c->pgradSum})
+    for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->LamNext[0], &c->LamNext[1],
+                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->kscale, &c->gaos})
         free_buf(*b);
     for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(c->Gb[s][0]), free_buf(c->Gb[s][1]);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1603,6 +1659,7 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     if (n != c->n || (n && !particles)) return fail(c, ARP_E_ARG, "state has %u particles, scene %u", n, c->n);
     c->rec_valid = false;
     c->nl_dirty = true;
+    c->rec_prefilled = nullptr;
     HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     if (c->solids_active) {
         const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
@@ -1685,9 +1742,13 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
             // segment's checkpoint, so that a replay repeats the recorded steps exactly
             c->nl_dirty = true;
         }
-        if ((rc = step_once(c, dt, &c->rec[j % K]))) return rc;
+        c->rec_next = j + 1 < nsteps ? &c->rec[(j + 1) % K] : nullptr;
+        rc = step_once(c, dt, &c->rec[j % K]);
+        c->rec_next = nullptr;
+        if (rc) return rc;
         c->time += dt;
     }
+    c->rec_prefilled = nullptr;
     if ((rc = ck_save(c, c->ck[nseg]))) return rc;
     if ((rc = fetch_over(c, (nsteps - 1) % K + 1))) return rc;
     c->rec_dt = dt;
@@ -1748,12 +1809,13 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
     const int S = c->tab.stages;
     for (int j = 0; j < 2; j++) {
         if ((rc = dalloc(c, c->LamIn[j], 16 * n)) || (rc = dalloc(c, c->LamAcc[j], 16 * n)) ||
-            (rc = dalloc(c, c->Wb[j], 16 * n)))
+            (rc = dalloc(c, c->LamNext[j], 16 * n)) || (rc = dalloc(c, c->Wb[j], 16 * n)))
             return rc;
         for (int s = 0; s < S; s++)
             if ((rc = dalloc(c, c->Gb[s][j], 16 * n))) return rc;
     }
     if ((rc = dalloc(c, c->pgradPart, 4 * (size_t)nblk(c->n) * 2 * kMatLds))) return rc;
+    if ((rc = fill_u32(c, c->pgradPart, nblk(c->n) * 2 * kMatLds, 0))) return rc;  // accumulated per block
     if (c->solids_active) {
         const size_t ns = std::max<uint32_t>(c->ns, 1);
         if ((rc = dalloc(c, c->Fbar, 4 * 18 * ns)) || (rc = dalloc(c, c->spar, 4 * 4 * ns))) return rc;
@@ -1772,13 +1834,22 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
         if ((uint32_t)seg != c->rec_seg) {
             // replay the segment from its checkpoint, recording
             if ((rc = ck_restore(c, c->ck[seg]))) return rc;
-            for (uint32_t j = j0; j < j1; j++)
-                if ((rc = step_once(c, c->rec_dt, &c->rec[j - j0]))) return rc;
+            for (uint32_t j = j0; j < j1; j++) {
+                c->rec_next = j + 1 < j1 ? &c->rec[j + 1 - j0] : nullptr;
+                rc = step_once(c, c->rec_dt, &c->rec[j - j0]);
+                c->rec_next = nullptr;
+                if (rc) return rc;
+            }
+            c->rec_prefilled = nullptr;
             if ((rc = fetch_over(c, j1 - j0))) return rc;
             c->rec_seg = (uint32_t)seg;
         }
         for (uint32_t j = j1; j-- > j0;)
             if ((rc = reverse_step(c, c->rec[j - j0], c->rec_dt))) return rc;
+    }
+    if (c->nm <= (uint32_t)kMatLds) {  // the per-block parameter partials of the whole sweep
+        ReduceArgs r{nblk(c->n), 2 * c->nm, P<float>(c->pgradPart), P<float>(c->pgradSum)};
+        if ((rc = launch(c, K_REDUCE, 2 * c->nm, 256, r))) return rc;
     }
     // dL/dy_0 is now in the storage order of the recorded start state
     GradAosArgs out{c->n, 1, P<uint32_t>(c->ck[0].pid), d_grad_in, P<float>(c->LamIn[0]), P<float>(c->LamIn[1])};
