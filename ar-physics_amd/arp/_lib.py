@@ -17,7 +17,7 @@ SYMBOLS = [
     "arp_profile_reset", "arp_adjoint_record", "arp_adjoint_backward",
     "arp_profile_select", "arp_batch_create", "arp_batch_destroy", "arp_batch_last_error", "arp_batch_step",
     "arp_batch_tune", "arp_set_stiffness_field", "arp_adjoint_stiffness_grad",
-    "arp_set_neighbor_skin", "arp_neighbor_stats",
+    "arp_set_neighbor_skin", "arp_neighbor_stats", "arp_set_neighbor_max_age",
 ]
 
 
@@ -87,6 +87,7 @@ def lib():
     L.arp_set_stiffness_field.argtypes = [vp, vp, C.c_uint32]
     L.arp_adjoint_stiffness_grad.argtypes = [vp, vp, C.c_uint32]
     L.arp_set_neighbor_skin.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
+    L.arp_set_neighbor_max_age.argtypes = [vp, C.c_uint32]
     L.arp_neighbor_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_adjoint_backward.argtypes = [vp, vp, vp, vp]

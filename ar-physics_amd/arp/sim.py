@@ -151,6 +151,10 @@ class FluidSim:
         48 entries per particle).  always_build rebuilds at every stage (testing)."""
         check(self.ctx, lib().arp_set_neighbor_skin(self.ctx, float(skin), int(cap), 1 if always_build else 0))
 
+    def set_neighbor_max_age(self, stages):
+        """Rebuild kept lists older than this many RK stages (0: only when invalid)."""
+        check(self.ctx, lib().arp_set_neighbor_max_age(self.ctx, int(stages)))
+
     def neighbor_stats(self):
         """(stages that searched and built lists, stages that reused them)."""
         b, r = C.c_uint32(0), C.c_uint32(0)

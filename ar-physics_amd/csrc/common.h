@@ -208,6 +208,8 @@ struct RkArgs {
 //                          kernels that run before its counter advances)
 //   words[8], [9]        number of list builds, number of stages run with lists
 //   words[10]            the stage counter
+//   words[12 + (g&1)]    stages since the lists in use at stage g were built
+//   words[14]            rebuild lists older than this many stages (0: never for age)
 // Stage g reuses iff words[(g-1)&1] == g-1 and words[2+((g-1)&1)] != g-1.
 // cellCnt holds two count arrays; the build at counter g counts into array g&1.
 struct NlArgs {

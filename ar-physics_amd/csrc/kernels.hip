@@ -294,15 +294,17 @@ __device__ __forceinline__ bool buckets_adjacent(const GridP& g, const float4& p
 // Whether the stage about to run reuses the neighbour lists (see NlArgs).  Read after the
 // stage's row-count scan advanced the counter; the words it reads are written only by the
 // stage before.
+// (Lists older than w[14] stages, if nonzero, are rebuilt: the build also re-sorts the
+// storage, which the neighbour gathers' locality depends on.)
 __device__ __forceinline__ bool nl_reuse(const uint32_t* w) {
     const uint32_t g = w[10], p = (g - 1u) & 1u;
-    return w[p] == g - 1u && w[2 + p] != g - 1u;
+    return w[p] == g - 1u && w[2 + p] != g - 1u && (w[14] == 0u || w[12 + p] + 1u < w[14]);
 }
 // The same decision before the stage's counter advances (key pass, scan): from the counter
 // the previous stage recorded.
 __device__ __forceinline__ bool nl_reuse_prev(const uint32_t* w) {
-    const uint32_t g1 = w[6];
-    return g1 != kInvalid && w[g1 & 1u] == g1 && w[2 + (g1 & 1u)] != g1;
+    const uint32_t g1 = w[6], p = g1 & 1u;
+    return g1 != kInvalid && w[p] == g1 && w[2 + p] != g1 && (w[14] == 0u || w[12 + p] + 1u < w[14]);
 }
 
 // Scatter (key, source) into the reserved row slots; particles with non-finite positions
@@ -608,6 +610,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             if (REC) *a.recUse = use ? 1u : 0u;
             a.nl.words[gen & 1u] = gen;
             a.nl.words[6] = gen;
+            a.nl.words[12 + (gen & 1u)] = use ? a.nl.words[12 + ((gen - 1u) & 1u)] + 1u : 0u;  // age
             atomicAdd(&a.nl.words[use ? 9 : 8], 1u);
             if (!use) {
                 a.nl.words[4 + (gen & 1u)] = gen;
@@ -1804,7 +1807,8 @@ __device__ void pair_vjp(const PairSide& P, const DInter& it, float rx, float ry
 
 // One neighbour's data as the reverse stage reads it
 struct AdjIn {
-    float4 xj, vj, wpj, wvj;
+    float4 xj, vj;
+    float wvx, wvy, wvz, wd;  // the neighbour's W: velocity part and density part
     uint32_t mk2;
 };
 
@@ -1903,7 +1907,9 @@ __device__ __forceinline__ void pair_vjp_sym(float mi, float mj, float rhoi, flo
 // FAST: every mobile particle has material a.fastMat and there are at most kMatLds
 // materials: the tables are read from LDS and the parameter partials are two scalars per
 // thread (the general form keeps a partial per material, indexed per pair).
-template <bool REC, bool FAST>
+// FLUID (with FAST): no elastic particle in the scene, so that every mobile pair takes the
+// two-sided form and the one-sided form serves boundary pairs only.
+template <bool REC, bool FAST, bool FLUID = false>
 __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
     uint32_t* list = s_list;
     float4* smat = s_mat;
@@ -2064,8 +2070,11 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                     in.xj = bnd ? B[j] : X[j];
                     in.vj = bnd ? BV[j] : XV[j];
                     in.mk2 = bnd ? a.bmk[j] : a.mk[j];
-                    in.wpj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WP[j];
-                    in.wvj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WV[j];
+                    const float4 wvj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WV[j];
+                    in.wvx = wvj.x;
+                    in.wvy = wvj.y;
+                    in.wvz = wvj.z;
+                    in.wd = bnd ? 0.0f : reinterpret_cast<const float*>(WP)[4 * (size_t)j + 3];
                     return in;
                 };
                 auto proc = [&](uint32_t e, const AdjIn& in) {
@@ -2090,15 +2099,14 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         itA = a.inter[mat1 * nm + mat2];
                         itB = a.inter[mat2 * nm + mat1];
                     }
-                    if (FAST && !bnd && !((mk1 | mk2) & kMkElastic)) {
+                    if (FAST && !bnd && (FLUID || !((mk1 | mk2) & kMkElastic))) {
                         // both sides at once (one material, no elastic particle)
-                        const float4 wpj = in.wpj, wvj = in.wvj;
                         const float irj = frcp(xj.w);
                         PairSym o;
                         pair_vjp_sym(M1.mass, m2, rho1, xj.w, vi.w, vj.w, dq1, M1.visc + f2, M1.bulk_visc + bf2, itA,
                                      xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z,
-                                     a1x, a1y, a1z, ws * wvj.x * irj, ws * wvj.y * irj, ws * wvj.z * irj, wp.w,
-                                     ws * wpj.w, hh, kn, a.phys.eps_hh, o);
+                                     a1x, a1y, a1z, ws * in.wvx * irj, ws * in.wvy * irj, ws * in.wvz * irj, wp.w,
+                                     ws * in.wd, hh, kn, a.phys.eps_hh, o);
                         for (int c = 0; c < 3; c++) {
                             G[c] -= o.gr[c];
                             Gv[c] -= o.gv[c];
@@ -2155,11 +2163,10 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         pv[mat1] += o.dvisc;
                         pv[mat2] += o.dvisc;
                     }
-                    if (!bnd) {
+                    if (!FLUID && !bnd) {
                         // side b: the neighbour's terms as seen from this particle
-                        float4 wpj = in.wpj, wvj = in.wvj;
-                        wpj.w *= ws;
-                        wvj = make_float4(ws * wvj.x, ws * wvj.y, ws * wvj.z, 0.0f);
+                        float4 wpj = make_float4(0.f, 0.f, 0.f, ws * in.wd);
+                        const float4 wvj = make_float4(ws * in.wvx, ws * in.wvy, ws * in.wvz, 0.0f);
                         const float irj = frcp(xj.w);
                         PairSide Q;
                         Q.mi = m2;
@@ -2279,6 +2286,9 @@ extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj(AdjArgs a) { fl
 extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec(AdjArgs a) { fluid_adj_body<true, true>(a); }
 extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_g(AdjArgs a) { fluid_adj_body<false, false>(a); }
 extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec_g(AdjArgs a) { fluid_adj_body<true, false>(a); }
+// _f: scenes without elastic particles
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_f(AdjArgs a) { fluid_adj_body<false, true, true>(a); }
+extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec_f(AdjArgs a) { fluid_adj_body<true, true, true>(a); }
 
 extern "C" __global__ void __launch_bounds__(256) k_adj_w(AdjWArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;

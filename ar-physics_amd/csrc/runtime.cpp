@@ -71,6 +71,8 @@ enum Kern {
     K_FLUID_ADJ_REC_G,
     K_FLUID_NLBR,
     K_FLUID_NLR,
+    K_FLUID_ADJ_F,
+    K_FLUID_ADJ_REC_F,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -82,7 +84,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_batch", "k_inv", "k_solid_adj1", "k_solid_adj2",
                                    "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch",
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
-                                   "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr"};
+                                   "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
+                                   "k_fluid_adj_f", "k_fluid_adj_rec_f"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -150,6 +153,7 @@ struct arp_ctx {
     std::vector<ArpMaterial> mats_h;
     std::vector<uint32_t> mat_by_pid, sid_by_pid, mk_by_pid;
     int64_t mobile_mat = -1;  // the material of every mobile particle, or -1 (none or several)
+    bool any_elastic = false; // some particle is elastic (a solid of a stiff material)
     std::vector<ArpSolidParticle> solids_h;
     std::vector<float> kscale_h;  // per-solid stiffness scale (empty: 1)
     std::vector<uint32_t> b_orig;  // sorted boundary slot -> original boundary id
@@ -179,6 +183,7 @@ struct arp_ctx {
     float nl_skin = 0.2f;   // list radius h * (1 + skin); 0 disables
     uint32_t nl_cap = 48;   // entries per particle
     bool nl_always = false; // rebuild at every stage (testing)
+    uint32_t nl_max_age = 0;  // rebuild lists older than this many stages (0: never for age)
     bool nl_dirty = true;   // the decision words and bucket counts must be reset
     int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
     bool nl_suspend = false;  // a batch whose scenes cannot all keep lists runs without
@@ -671,6 +676,8 @@ int nl_prepare(arp_ctx* c) {
     }
     if (c->nl_dirty) {
         if ((rc = fill_u32(c, c->nlWords, 7, kInvalid))) return rc;
+        FillArgs fa{P<uint32_t>(c->nlWords) + 14, 1, c->nl_max_age};
+        if ((rc = launch(c, K_FILL, 1, 256, fa))) return rc;
         if ((rc = fill_u32(c, c->nlCell, (uint32_t)(2 * c->ncells), 0))) return rc;
         c->nl_dirty = false;
     }
@@ -1261,7 +1268,9 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         // the fast kernels: LDS tables, one mobile material
         const bool fast = c->mobile_mat >= 0 && c->nm <= (uint32_t)kMatLds;
         a.fastMat = fast ? (uint32_t)c->mobile_mat : 0u;
-        const Kern ka = R.over[s] ? (fast ? K_FLUID_ADJ : K_FLUID_ADJ_G) : (fast ? K_FLUID_ADJ_REC : K_FLUID_ADJ_REC_G);
+        const bool fl = fast && !c->any_elastic;
+        const Kern ka = R.over[s] ? (fl ? K_FLUID_ADJ_F : fast ? K_FLUID_ADJ : K_FLUID_ADJ_G)
+                                  : (fl ? K_FLUID_ADJ_REC_F : fast ? K_FLUID_ADJ_REC : K_FLUID_ADJ_REC_G);
         if ((rc = launch(c, ka, nblk(c->n), 256, a))) return rc;
         if (c->solids_active) {
             // the stage's stress and F again, then the two solid passes
@@ -1577,6 +1586,9 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     if ((rc = upload(c, c->mkStatic, c->mk_by_pid.data(), 4 * (size_t)n))) return rc;
     if ((rc = upload(c, c->matByPid, c->mat_by_pid.data(), 4 * (size_t)n))) return rc;
     c->mobile_mat = n ? (int64_t)c->mat_by_pid[0] : -1;
+    c->any_elastic = false;
+    for (uint32_t i = 0; i < n; i++)
+        if (c->mk_by_pid[i] & kMkElastic) c->any_elastic = true;
     for (uint32_t i = 1; i < n && c->mobile_mat >= 0; i++)
         if (c->mat_by_pid[i] != (uint32_t)c->mobile_mat) c->mobile_mat = -1;
     if ((rc = upload(c, c->sidByPid, c->sid_by_pid.data(), 4 * (size_t)n))) return rc;
@@ -1738,9 +1750,10 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     for (uint32_t j = 0; j < nsteps; j++) {
         if (j % K == 0) {
             if ((rc = ck_save(c, c->ck[j / K]))) return rc;
-            // kept neighbour lists restart with every segment, as in the replays from the
-            // segment's checkpoint, so that a replay repeats the recorded steps exactly
-            c->nl_dirty = true;
+            // kept neighbour lists restart with every segment that the reverse sweep
+            // replays (all but the last), as its replay from the checkpoint does, so that
+            // the replay repeats the recorded steps exactly
+            if (j / K + 1 < nseg) c->nl_dirty = true;
         }
         c->rec_next = j + 1 < nsteps ? &c->rec[(j + 1) % K] : nullptr;
         rc = step_once(c, dt, &c->rec[j % K]);
@@ -2033,6 +2046,13 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
         int rc = launch(c, K_FILL, 1, 256, fa);
         if (rc) return rc;
     }
+    return ARP_OK;
+}
+
+int arp_set_neighbor_max_age(arp_ctx* c, uint32_t stages) {
+    if (!c) return ARP_E_ARG;
+    c->nl_max_age = stages;
+    c->nl_dirty = true;
     return ARP_OK;
 }
 

@@ -78,7 +78,8 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
     sim.adjoint_record(dt, nsteps, nsteps)
     sim.adjoint_backward(g, gin, want_params=False)
     sim.synchronize()
-    sim.profile_select(["k_fluid_adj", "k_fluid_adj_rec"])
+    adj_kernels = ["k_fluid_adj" + sfx for sfx in ("", "_rec", "_f", "_rec_f", "_g", "_rec_g")]
+    sim.profile_select(adj_kernels)
     sim.profile(True)
     sim.profile_reset()
     t0 = time.perf_counter()
@@ -88,9 +89,12 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
     sim.adjoint_backward(g, gin, want_params=True)
     sim.synchronize()
     t2 = time.perf_counter()
-    ms, cnt = sim.profile_query("k_fluid_adj_rec")  # reverse stage on the recorded neighbours
-    ms2, cnt2 = sim.profile_query("k_fluid_adj")   # the same after a neighbour-record overflow
-    ms, cnt = ms + ms2, cnt + cnt2
+    # the reverse stage (on the recorded neighbours, or searching after a record overflow;
+    # fluid-only / one-material / general table forms)
+    ms, cnt = 0.0, 0
+    for k in adj_kernels:
+        m1, c1 = sim.profile_query(k)
+        ms, cnt = ms + m1, cnt + c1
     sim.profile(False)
     finite = bool(torch.isfinite(gin).all().item())
     rec_ms, bwd_ms = 1e3 * (t1 - t0), 1e3 * (t2 - t1)
@@ -164,6 +168,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--skin", type=float, default=0.2,
                     help="neighbour lists kept across RK stages with radius h*(1+skin) (0: search at every stage)")
+    ap.add_argument("--max-age", type=int, default=None,
+                    help="rebuild kept neighbour lists older than this many stages (default: the library's)")
     ap.add_argument("--adjoint-steps", type=int, default=10,
                     help="steps of the recorded rollout whose reverse sweep is timed (0: skip)")
     args = ap.parse_args()
@@ -195,6 +201,8 @@ def main():
     log("particles %d boundary %d" % (sc.n, sc.nb))
     sim = FluidSim(sc, integrator=args.integrator, timestep=dt)
     sim.set_neighbor_skin(args.skin)
+    if args.max_age is not None:
+        sim.set_neighbor_max_age(args.max_age)
     fluid_kernel = "k_fluid_nl" if args.skin > 0 else "k_fluid"
     log("warmup %d steps" % args.warmup)
     sim.update(args.warmup)

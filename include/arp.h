@@ -168,6 +168,10 @@ int arp_upload_state(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
  * ARP_NL_ALWAYS_BUILD rebuilds at every stage (testing).  Resets the statistics. */
 #define ARP_NL_ALWAYS_BUILD 1u
 int arp_set_neighbor_skin(arp_ctx* ctx, float skin, uint32_t cap, uint32_t flags);
+/* Rebuild lists older than this many RK stages even when they are still valid (0: only when
+ * invalid).  A build re-sorts the particles into bucket order, which keeps the neighbour
+ * gathers local as the particles move. */
+int arp_set_neighbor_max_age(arp_ctx* ctx, uint32_t stages);
 /* Stages that searched and built lists, and stages that reused them, since the last
  * arp_set_neighbor_skin. */
 int arp_neighbor_stats(arp_ctx* ctx, uint32_t* builds, uint32_t* reused);
