@@ -1,4 +1,5 @@
-"""Explicit Runge-Kutta tableaus for the integrator names of examples/test.rs:165-180.
+"""Explicit Runge-Kutta tableaus for the integrator names of examples/test.rs:165-180, and
+the marker of velocity Verlet ("verlet", the TOML default, examples/test_mods/config.rs:451).
 
 The reference takes them from the un-vendored ``numerical-integration 0.0.1`` crate; the
 coefficients below are the textbook tableaus for those names.  "rk2" is taken to be the
@@ -31,10 +32,15 @@ _T = {
     ),
 }
 
-NAMES = sorted(_T)
+NAMES = sorted(_T) + ["verlet"]
+VERLET = -1  # ArpTableau.stages of velocity Verlet (ARP_VERLET)
 
 
 def tableau(name):
+    if name.lower() == "verlet":
+        t = np.zeros((), wire.TABLEAU)
+        t["stages"] = VERLET
+        return t
     a, b = _T[name.lower()]
     t = np.zeros((), wire.TABLEAU)
     s = len(b)

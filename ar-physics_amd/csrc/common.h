@@ -120,6 +120,11 @@ struct ScatterArgs {
     int copyOnReuse;
     uint32_t* recMk;   // full build of a recording: second copy of the sorted mk / pid
     uint32_t* recPid;
+    // full build: a second state pair reordered along (velocity Verlet's force point)
+    const float* srcAuxP;
+    const float* srcAuxV;
+    float* dstAuxP;
+    float* dstAuxV;
 };
 
 struct RowSortArgs {
@@ -154,6 +159,10 @@ struct RowSortArgs {
     const uint32_t* nlw;  // neighbour-list reuse words (may be null): no row sort when reused
     uint32_t* recMk;      // full build of a recording: second copy of the sorted mk / pid
     uint32_t* recPid;
+    const float* srcAuxP;  // full build: a second state pair reordered along
+    const float* srcAuxV;
+    float* dstAuxP;
+    float* dstAuxV;
 };
 
 // RK stage epilogue: k -> next stage input / y_{n+1}, the b-weighted accumulator and the
@@ -170,6 +179,8 @@ struct RkArgs {
     int nk;
     float a_next;  // dt * a(s+1, s)
     float b_cur;   // dt * b(s)
+    float b_pos;   // the same for the position part of k (velocity Verlet: 0, else b_cur)
+    float vdt;     // velocity Verlet: the step (the next force point is x + dt (v + dt/2 a))
     int last;
     float* Oposd;  // X_{s+1} or y_{n+1}
     float* Ovelq;
@@ -469,6 +480,26 @@ struct LincombArgs {
     const void* sterms[8];  // ArpSolidParticle*
     void* out;
     void* sout;
+};
+
+// Velocity Verlet before the force evaluation of a step, from y and the last evaluation
+// D = (a, rho', eps'): in place Z = (x + dt (v + dt/2 a), v + dt/2 a, rho + dt/2 rho',
+// eps + dt/2 eps') (the base of the step's kick), and the force point
+// X = (Z.x, Z.v + dt/2 a, Z.rho + dt/2 rho', Z.eps + dt/2 eps'); q = p/rho^2 of each.
+struct VerletArgs {
+    uint32_t n, ns;
+    float dt;
+    float* posd;        // y -> Z
+    float* velq;
+    float* Xposd;       // X
+    float* Xvelq;
+    const float* Dp;    // the last force evaluation: (v, rho')
+    const float* Dv;    // (a, 0)
+    const uint32_t* mk;
+    const DMat* mats;
+    float* eps;         // plastic strain registers (16 per solid), may be null: -> Z
+    float* Xeps;        // -> X
+    const float* De;    // their last rates
 };
 
 struct FillArgs {

@@ -281,6 +281,10 @@ __device__ __forceinline__ void move_particle(const ScatterArgs& a, uint32_t src
         a.recMk[dst] = a.srcMk[src];
         a.recPid[dst] = p;
     }
+    if (a.dstAuxP) {
+        reinterpret_cast<float4*>(a.dstAuxP)[dst] = reinterpret_cast<const float4*>(a.srcAuxP)[src];
+        reinterpret_cast<float4*>(a.dstAuxV)[dst] = reinterpret_cast<const float4*>(a.srcAuxV)[src];
+    }
 }
 
 // Whether two positions lie in the same or neighbouring buckets (27-bucket stencil,
@@ -323,6 +327,10 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs& a, uint32_t bid,
                 reinterpret_cast<float4*>(a.dstVelq)[i] = reinterpret_cast<const float4*>(a.srcVelq)[i];
                 a.dstMk[i] = a.srcMk[i];
                 a.dstPid[i] = a.srcPid[i];
+                if (a.dstAuxP) {
+                    reinterpret_cast<float4*>(a.dstAuxP)[i] = reinterpret_cast<const float4*>(a.srcAuxP)[i];
+                    reinterpret_cast<float4*>(a.dstAuxV)[i] = reinterpret_cast<const float4*>(a.srcAuxV)[i];
+                }
             }
             if (a.srcOut) a.srcOut[i] = i;
             if (a.recMk) {
@@ -450,6 +458,10 @@ __device__ __forceinline__ void rowsort_body(const RowSortArgs& a, uint32_t bid,
                     a.recMk[dst] = a.srcMk[src];
                     a.recPid[dst] = p;
                 }
+                if (a.dstAuxP) {
+                    reinterpret_cast<float4*>(a.dstAuxP)[dst] = reinterpret_cast<const float4*>(a.srcAuxP)[src];
+                    reinterpret_cast<float4*>(a.dstAuxV)[dst] = reinterpret_cast<const float4*>(a.srcAuxV)[src];
+                }
                 if (a.srcOut) a.srcOut[dst] = src;
             } else {
                 a.perm[dst] = src | flag;
@@ -482,8 +494,8 @@ __device__ __forceinline__ void rk_update_particle(const RkArgs& r, const GridP&
         ap = reinterpret_cast<const float4*>(r.AccP)[i];
         av = reinterpret_cast<const float4*>(r.AccV)[i];
     }
-    const float b = r.b_cur;
-    ap.x += b * kP.x; ap.y += b * kP.y; ap.z += b * kP.z; ap.w += b * kP.w;
+    const float b = r.b_cur, bp = r.b_pos;
+    ap.x += bp * kP.x; ap.y += bp * kP.y; ap.z += bp * kP.z; ap.w += b * kP.w;
     av.x += b * kV.x; av.y += b * kV.y; av.z += b * kV.z;
     float4 op, ov;
     if (r.last) {
@@ -511,6 +523,12 @@ __device__ __forceinline__ void rk_update_particle(const RkArgs& r, const GridP&
     }
     ov.w = pressure_q(M, op.w);
     opOut = op;
+    if (r.vdt != 0.0f) {  // velocity Verlet: the next force point (as k_verlet_pre forms it)
+        const float dt = r.vdt, h = 0.5f * dt;
+        opOut.x = op.x + dt * (ov.x + h * kV.x);
+        opOut.y = op.y + dt * (ov.y + h * kV.y);
+        opOut.z = op.z + dt * (ov.z + h * kV.z);
+    }
     reinterpret_cast<float4*>(r.Oposd)[i] = op;
     if (r.Oposd2) {
         reinterpret_cast<float4*>(r.Oposd2)[i] = op;
@@ -2307,6 +2325,44 @@ extern "C" __global__ void __launch_bounds__(256) k_adj_w(AdjWArgs a) {
     }
     reinterpret_cast<float4*>(a.WP)[i] = p;
     reinterpret_cast<float4*>(a.WV)[i] = v;
+}
+
+// Velocity Verlet before the step's force evaluation: the kick base Z (in place) and the
+// force point X from y and the last evaluation (VerletArgs).
+extern "C" __global__ void __launch_bounds__(256) k_verlet_pre(VerletArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const float dt = a.dt, h = 0.5f * dt;
+    if (i < a.n) {
+        float4 p = reinterpret_cast<const float4*>(a.posd)[i];
+        float4 v = reinterpret_cast<const float4*>(a.velq)[i];
+        const float4 dp = reinterpret_cast<const float4*>(a.Dp)[i];
+        const float4 dv = reinterpret_cast<const float4*>(a.Dv)[i];
+        const DMat& M = a.mats[a.mk[i] & kMatMask];
+        v.x = v.x + h * dv.x;
+        v.y = v.y + h * dv.y;
+        v.z = v.z + h * dv.z;
+        p.x = p.x + dt * v.x;
+        p.y = p.y + dt * v.y;
+        p.z = p.z + dt * v.z;
+        p.w = p.w + h * dp.w;
+        v.w = pressure_q(M, p.w);
+        reinterpret_cast<float4*>(a.posd)[i] = p;
+        reinterpret_cast<float4*>(a.velq)[i] = v;
+        v.x = v.x + h * dv.x;
+        v.y = v.y + h * dv.y;
+        v.z = v.z + h * dv.z;
+        p.w = p.w + h * dp.w;
+        v.w = pressure_q(M, p.w);
+        reinterpret_cast<float4*>(a.Xposd)[i] = p;
+        reinterpret_cast<float4*>(a.Xvelq)[i] = v;
+    }
+    if (a.eps && i < a.ns)
+        for (int q = 0; q < 16; q++) {
+            const size_t k = 16 * (size_t)i + q;
+            const float z = a.eps[k] + h * a.De[k];
+            a.eps[k] = z;
+            a.Xeps[k] = z + h * a.De[k];
+        }
 }
 
 extern "C" __global__ void __launch_bounds__(256) k_adj_perm(AdjPermArgs a) {

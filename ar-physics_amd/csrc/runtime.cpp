@@ -73,6 +73,7 @@ enum Kern {
     K_FLUID_NLR,
     K_FLUID_ADJ_F,
     K_FLUID_ADJ_REC_F,
+    K_VERLET_PRE,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -85,7 +86,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch",
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
-                                   "k_fluid_adj_f", "k_fluid_adj_rec_f"};
+                                   "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -167,6 +168,9 @@ struct arp_ctx {
     DBuf Xb[2][2];  // [buffer][posd/velq]
     DBuf Acc[2];
     DBuf K[kMaxK][2];
+    DBuf Dv[2], De;            // velocity Verlet: the last force evaluation (storage order)
+    DBuf XF[2][2], XFe;        // velocity Verlet: the force point, [landing / sorted][posd / velq]
+    bool verlet_valid = false; // Dv/De hold the derivative at the live state's last force point
     DBuf inv, sflag, KF[2];
     // solids: static reference-neighbour data and plastic-strain registers (solid-id order)
     bool solids_active = false;
@@ -232,7 +236,8 @@ struct arp_ctx {
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
                 &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp,
-                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell, &nlCellRec};
+                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell, &nlCellRec,
+                &Dv[0],    &Dv[1],     &De,       &XF[0][0], &XF[0][1], &XF[1][0], &XF[1][1], &XFe};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
             bufs.push_back(&K[k][1]);
@@ -540,6 +545,10 @@ struct SortIO {
     uint32_t* recPid = nullptr;
     uint32_t* clear = nullptr;      // words the scan zeroes first (a recording's stage flags)
     uint32_t nclear = 0;
+    const float* srcAuxP = nullptr; // full build: a second state pair reordered along
+    const float* srcAuxV = nullptr;
+    float* dstAuxP = nullptr;
+    float* dstAuxV = nullptr;
 };
 
 int run_key(arp_ctx* c, const float* posd, const uint32_t* nlw = nullptr) {
@@ -583,6 +592,12 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     sc.copyOnReuse = io.copyOnReuse;
     sc.recMk = io.full ? io.recMk : nullptr;
     sc.recPid = io.full ? io.recPid : nullptr;
+    if (io.full && io.dstAuxP) {
+        sc.srcAuxP = io.srcAuxP;
+        sc.srcAuxV = io.srcAuxV;
+        sc.dstAuxP = io.dstAuxP;
+        sc.dstAuxV = io.dstAuxV;
+    }
     // (grid-stride: with lists kept the launch is mostly a no-op, so it stays small)
     if ((rc = launch(c, K_SCATTER, io.nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
@@ -618,6 +633,12 @@ int run_sort(arp_ctx* c, const SortIO& io) {
     r.nlw = io.nlw;
     r.recMk = io.full ? io.recMk : nullptr;
     r.recPid = io.full ? io.recPid : nullptr;
+    if (io.full && io.dstAuxP) {
+        r.srcAuxP = io.srcAuxP;
+        r.srcAuxV = io.srcAuxV;
+        r.dstAuxP = io.dstAuxP;
+        r.dstAuxV = io.dstAuxV;
+    }
     // one workgroup per occupied row, looping; enough workgroups for about rowsort_div
     // particles each (a tuning knob), at most 4096
     const uint32_t want = std::max<uint32_t>(1, (c->n + c->rowsort_div - 1) / c->rowsort_div);
@@ -857,10 +878,79 @@ SolidArgs solid_base(arp_ctx* c) {
 // One RK step of length dt on the internal state (Yland/meta[cur] on entry and exit).
 // With rec, the step also keeps what its adjoint reads: the sorted y_n and every stage
 // input, the sorted meta, the reorder map and each stage's neighbour structure.
+// Velocity Verlet: D = F(y) at the live state (the first step after the state was set, as
+// init_with_vel), then the kick and drift of the step in place.  The evaluation sorts the
+// state into the storage order it leaves in the landing buffers, so D and y share it.
+int verlet_prepare(arp_ctx* c, float dt) {
+    int rc;
+    const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
+    if ((rc = dalloc(c, c->Dv[0], 16 * n)) || (rc = dalloc(c, c->Dv[1], 16 * n))) return rc;
+    for (int k = 0; k < 2; k++)
+        if ((rc = dalloc(c, c->XF[k][0], 16 * n)) || (rc = dalloc(c, c->XF[k][1], 16 * n))) return rc;
+    if (c->solids_active && ((rc = dalloc(c, c->De, 64 * (size_t)std::max<uint32_t>(c->ns, 1))) ||
+                             (rc = dalloc(c, c->XFe, 64 * (size_t)std::max<uint32_t>(c->ns, 1)))))
+        return rc;
+    if (!c->verlet_valid) {
+        if (!c->keys_valid && (rc = run_key(c, P<float>(c->Yland[0])))) return rc;
+        const int nx = c->cur ^ 1;
+        const GridSlot own = c->slot;
+        SortIO io{1,
+                  P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_mk[c->cur]),
+                  P<uint32_t>(c->meta_pid[c->cur]), P<float>(c->Ysort[0]), P<float>(c->Ysort[1]),
+                  P<uint32_t>(c->meta_mk[nx]), P<uint32_t>(c->meta_pid[nx])};
+        if ((rc = run_sort(c, io))) return rc;
+        c->cur = nx;
+        c->keys_valid = false;
+        c->nl_dirty = true;  // the storage order changed outside the kept lists
+        FluidArgs f = fluid_base(c);
+        f.Xposd = P<float>(c->Ysort[0]);
+        f.Xvelq = P<float>(c->Ysort[1]);
+        f.mk = P<uint32_t>(c->meta_mk[c->cur]);
+        f.mode = 2;
+        f.KdP = P<float>(c->Dv[0]);
+        f.KdV = P<float>(c->Dv[1]);
+        if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
+        if (c->solids_active) {
+            SolidArgs sa = solid_base(c);
+            sa.Xposd = P<float>(c->Ysort[0]);
+            sa.Xvelq = P<float>(c->Ysort[1]);
+            sa.mk = P<uint32_t>(c->meta_mk[c->cur]);
+            sa.pid = P<uint32_t>(c->meta_pid[c->cur]);
+            sa.Xeps = P<float>(c->Yeps);
+            sa.mode = 2;
+            sa.KFp = P<float>(c->Dv[0]);
+            sa.KFv = P<float>(c->Dv[1]);
+            sa.KdP = P<float>(c->Dv[0]);
+            sa.KdV = P<float>(c->Dv[1]);
+            sa.keps = P<float>(c->De);
+            if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
+            if ((rc = launch(c, K_SOLID, nblk(c->ns), 256, sa))) return rc;
+        }
+        // the state in the order of D
+        HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, c->Ysort[0].p, 16 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, c->Ysort[1].p, 16 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+        c->slot = own;
+        c->verlet_valid = true;
+    }
+    if (c->keys_valid) {  // the bins counted for the old positions are void
+        if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
+        c->keys_valid = false;
+    }
+    VerletArgs v{c->n, c->solids_active ? c->ns : 0u, dt, P<float>(c->Yland[0]), P<float>(c->Yland[1]),
+                 P<float>(c->XF[0][0]), P<float>(c->XF[0][1]),
+                 P<float>(c->Dv[0]), P<float>(c->Dv[1]), P<uint32_t>(c->meta_mk[c->cur]), P<DMat>(c->mats),
+                 c->solids_active ? P<float>(c->Yeps) : nullptr, c->solids_active ? P<float>(c->XFe) : nullptr,
+                 c->solids_active ? P<float>(c->De) : nullptr};
+    return launch(c, K_VERLET_PRE, nblk(std::max(c->n, v.ns)), 256, v);
+}
+
 int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     const ArpTableau& T = c->tab;
-    const int S = T.stages;
+    const bool verlet = T.stages == ARP_VERLET;
+    const int S = verlet ? 1 : T.stages;
     int rc;
+    if (verlet && rec) return fail(c, ARP_E_STATE, "reverse mode covers the Runge-Kutta tableaus");
+    if (!verlet) c->verlet_valid = false;
     // which stage derivatives must be kept for later (non-subdiagonal) stage inputs
     int kslot[ARP_MAX_STAGES];
     int nslots = 0;
@@ -870,12 +960,15 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             if (T.a[t][j] != 0.0f) kslot[j] = 0;
         if (kslot[j] == 0) kslot[j] = nslots++;
     }
+    if (verlet) kslot[0] = -1;
     if (nslots > kMaxK) return fail(c, ARP_E_LIMIT, "tableau needs %d stored stages (max %d)", nslots, kMaxK);
     if (!rec) c->rec_valid = false;  // the live state leaves the recorded trajectory
     bool acc = false;
     for (int j = 0; j + 1 < S; j++)
         if (T.b[j] != 0.0f) acc = true;
     if ((rc = ensure_k(c, nslots, acc))) return rc;
+    // velocity Verlet: the kick and drift before the force evaluation (moves the positions)
+    if (verlet && (rc = verlet_prepare(c, dt))) return rc;
     // neighbour lists kept across stages (the plain step of a fluid-only scene)
     const bool nl = nl_enabled(c);
     if (nl) {
@@ -912,6 +1005,12 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
               rec ? P<uint32_t>(rec->mk) : nullptr, rec ? P<uint32_t>(rec->pid) : nullptr,
               // the recording's per-stage flags start at zero
               rec ? P<uint32_t>(rec->flags) : nullptr, rec ? 2u * ARP_MAX_STAGES : 0u};
+    if (verlet) {  // the force point is reordered along with the kick base
+        io.srcAuxP = P<float>(c->XF[0][0]);
+        io.srcAuxV = P<float>(c->XF[0][1]);
+        io.dstAuxP = P<float>(c->XF[1][0]);
+        io.dstAuxV = P<float>(c->XF[1][1]);
+    }
     if ((rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
@@ -919,9 +1018,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         rec->slot[0] = c->slot;  // (the sort also wrote the recorded meta)
     }
 
-    const float* Xp = Y0p;
-    const float* Xv = Y0v;
-    const float* Xe = P<float>(c->Yeps);
+    const float* Xp = verlet ? P<float>(c->XF[1][0]) : Y0p;  // the first stage's input
+    const float* Xv = verlet ? P<float>(c->XF[1][1]) : Y0v;
+    const float* Xe = verlet ? P<float>(c->XFe) : P<float>(c->Yeps);
     for (int s = 0; s < S; s++) {
         if (s > 0) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
@@ -935,7 +1034,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         RkArgs r{};
         r.Yposd = Y0p;
         r.Yvelq = Y0v;
-        r.b_cur = dt * T.b[s];
+        r.b_cur = verlet ? 0.5f * dt : dt * T.b[s];
+        r.b_pos = verlet ? 0.0f : r.b_cur;
+        r.vdt = verlet ? dt : 0.0f;
         r.last = last ? 1 : 0;
         if (acc && s > 0) {
             r.AccP = P<float>(c->Acc[0]);
@@ -982,7 +1083,12 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             r.KvOut = P<float>(c->K[kslot[s]][1]);
             r.KeOut = c->solids_active ? P<float>(c->Ke[kslot[s]]) : nullptr;
         }
-        if (!nl) {  // the next stage's keys (with lists kept: the gated key pass)
+        if (verlet) {  // the force evaluation is kept for the next step's first kick
+            r.KpOut = P<float>(c->Dv[0]);
+            r.KvOut = P<float>(c->Dv[1]);
+            r.KeOut = c->solids_active ? P<float>(c->De) : nullptr;
+        }
+        if (!nl && !verlet) {  // the next stage's keys (with lists kept: the gated key pass)
             r.keys = P<uint32_t>(c->keys);
             r.ranks = P<uint32_t>(c->ranks);
             r.binCount = P<uint32_t>(c->binCount);
@@ -1037,7 +1143,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         }
     }
     if (rec) c->slot = own;
-    c->keys_valid = !nl;  // the plain step's last epilogue binned y_{n+1}
+    c->keys_valid = !nl && !verlet;  // the plain step's last epilogue binned y_{n+1}
     c->rec_prefilled = (rec && nl && S > 1 && c->rec_next && c->rec_next != rec) ? c->rec_next->Xp[0].p : nullptr;
     c->have_build = false;
     return ARP_OK;
@@ -1131,6 +1237,7 @@ int ck_save(arp_ctx* c, Checkpoint& k) {
 int ck_restore(arp_ctx* c, Checkpoint& k) {
     const size_t n = c->n;
     c->nl_dirty = true;
+    c->verlet_valid = false;
     c->rec_prefilled = nullptr;
     HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, k.P.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, k.V.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1462,7 +1569,9 @@ int arp_tableau_by_name(const char* name, ArpTableau* out) {
             t.c[i] = cs;
         }
     };
-    if (s == "euler" || s == "rk1")
+    if (s == "verlet")
+        t.stages = ARP_VERLET;
+    else if (s == "euler" || s == "rk1")
         set(1, {0.0f}, {1.0f});
     else if (s == "midpoint" || s == "rk2")
         set(2, {0, 0, 0.5f, 0}, {0.0f, 1.0f});
@@ -1672,6 +1781,7 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     c->rec_valid = false;
     c->nl_dirty = true;
     c->rec_prefilled = nullptr;
+    c->verlet_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     if (c->solids_active) {
         const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
@@ -1691,6 +1801,13 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
 
 int arp_set_integrator(arp_ctx* c, const ArpTableau* t, float timestep, uint32_t subticks) {
     if (!c || !t) return ARP_E_ARG;
+    c->verlet_valid = false;
+    if (t->stages == ARP_VERLET) {
+        c->tab = *t;
+        c->timestep = timestep;
+        c->subticks = subticks ? subticks : 1;
+        return ARP_OK;
+    }
     if (t->stages < 1 || t->stages > ARP_MAX_STAGES) return fail(c, ARP_E_ARG, "tableau has %d stages", t->stages);
     for (int i = 0; i < t->stages; i++)
         for (int j = i; j < t->stages; j++)
@@ -1729,6 +1846,8 @@ double arp_time(const arp_ctx* c) { return c ? c->time : 0.0; }
 int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoint_every) {
     int rc;
     if ((rc = need_scene(c))) return rc;
+    if (c->tab.stages == ARP_VERLET)
+        return fail(c, ARP_E_STATE, "reverse mode covers the Runge-Kutta tableaus (not velocity Verlet)");
     if (c->solids_active)
         for (uint32_t m = 0; m < c->nm; m++) {
             const ArpMaterial& M = c->mats_h[m];
@@ -2514,6 +2633,8 @@ int arp_batch_create(arp_ctx* const* ctxs, uint32_t nscenes, uint32_t flags, arp
         if (!c->have_scene) return fail(L0, ARP_E_STATE, "scene %u has no scene uploaded", s);
         if (c->device != L0->device) return fail(L0, ARP_E_ARG, "scene %u is on another device", s);
         if (c->solids_active) return fail(L0, ARP_E_ARG, "scene %u runs the solid path (batches hold fluid scenes)", s);
+        if (c->tab.stages == ARP_VERLET)
+            return fail(L0, ARP_E_ARG, "scene %u uses velocity Verlet (batches run Runge-Kutta tableaus)", s);
         if (memcmp(&c->tab, &L0->tab, sizeof(ArpTableau)) != 0)
             return fail(L0, ARP_E_ARG, "scene %u uses another integrator", s);
         for (uint32_t t = 0; t < s; t++)

@@ -85,6 +85,14 @@ typedef struct ArpAABB {
 /* Explicit Runge-Kutta tableau (the role of the numerical-integration crate's
  * VelIntegrates implementors, examples/test.rs:165-180).  a is strictly lower triangular. */
 #define ARP_MAX_STAGES 6
+/* stages == ARP_VERLET selects velocity Verlet (examples/test.rs:167, the TOML default
+ * config.rs:451): one force evaluation per step.  With D = (a, rho', eps') the last
+ * evaluation, Z = (x + dt (v + dt/2 a), v + dt/2 a, rho + dt/2 rho', eps + dt/2 eps'), the
+ * forces D' are evaluated at the predicted X = (Z.x, Z.v + dt/2 a, Z.rho + dt/2 rho',
+ * Z.eps + dt/2 eps') and y' = Z + dt/2 D' in v, rho and eps (second order; D' is kept for
+ * the next step; the first step of a state evaluates D first, as init_with_vel does,
+ * soft_body/mod.rs:220-232).  a, b, c are unused. */
+#define ARP_VERLET (-1)
 typedef struct ArpTableau {
     int32_t stages;
     float a[ARP_MAX_STAGES][ARP_MAX_STAGES];
@@ -126,7 +134,7 @@ void arp_default_interaction(const ArpMaterial* a, const ArpMaterial* b, float h
 void arp_resolve_interactions(uint32_t nm, const ArpMaterial* mats,
                               const ArpMatInteraction* cfg, const uint8_t* present, float h,
                               ArpMatInteraction* out);
-/* Named integrators of examples/test.rs:165-180 ("euler", "rk1", "midpoint", "rk2",
+/* Named integrators of examples/test.rs:165-180 ("verlet", "euler", "rk1", "midpoint", "rk2",
  * "heun2", "heun", "ralston", "rk3", "heun3", "rk4", "rk_3_8"); case-insensitive. */
 int arp_tableau_by_name(const char* name, ArpTableau* out);
 

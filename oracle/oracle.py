@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+VERLET = -1  # the stages field of a tableau that selects velocity Verlet
 
 u32p = C.POINTER(C.c_uint32)
 f32p = C.POINTER(C.c_float)
@@ -170,6 +171,8 @@ class Oracle:
         return out
 
     def step(self, st, tab, dt, nsteps=1):
+        if int(tab["stages"]) == VERLET:
+            return self.verlet(st, dt, nsteps)
         st = {k: v.copy() for k, v in st.items()}
         if self._prec(st) == "f":
             rc = lib().orc_step_f(C.byref(self.sc), _p(tab), C.c_float(dt), nsteps, C.byref(self._cs(st)))
@@ -177,6 +180,43 @@ class Oracle:
             rc = lib().orc_step_d(C.byref(self.sc), _p(tab), C.c_double(dt), nsteps, C.byref(self._cs(st)))
         assert rc == 0, rc
         return st
+
+    def verlet(self, st, dt, nsteps=1, D=None):
+        """Velocity Verlet over this oracle's force function, in the state's precision: the
+        integrator the reference names "verlet" (examples/test.rs:167) and steps with
+        step_with_vel(t, state, dt, vel_fn, force_fn) (soft_body/mod.rs:252-265) after
+        init_with_vel evaluated the forces once (soft_body/mod.rs:220-232).  The integrator
+        crate (numerical-integration 0.0.1) is not vendored, so this is the textbook scheme
+        x' = x + dt v + dt^2/2 a, v' = v + dt/2 (a + a'), with the same trapezoid for the
+        density and the plastic strain, one force evaluation per step (parity unpinned
+        beyond the properties tests/test_oracle.py checks).  The forces depend on velocity
+        and density too, so a' is evaluated at the predicted v + dt a, rho + dt rho'
+        (second order; the evaluation is kept for the next step):
+            Z = (x + dt (v + dt/2 a), v + dt/2 a, rho + dt/2 rho', eps + dt/2 eps')
+            X = (Z.x, Z.v + dt/2 a, Z.rho + dt/2 rho', Z.eps + dt/2 eps')
+            a', rho'', eps'' = force_fn(X)
+            y' = (Z.x, Z.v + dt/2 a', Z.rho + dt/2 rho'', Z.eps + dt/2 eps'')
+        """
+        s = {k: v.copy() for k, v in st.items()}
+        f = s["den"].dtype.type
+        dt = f(dt)
+        h = f(0.5) * dt
+        if D is None:
+            D = self.compute_forces(s)
+        for _ in range(nsteps):
+            s["vel"][:, :3] = s["vel"][:, :3] + h * D["vel"][:, :3]
+            s["pos"][:, :3] = s["pos"][:, :3] + dt * s["vel"][:, :3]
+            s["den"] = s["den"] + h * D["den"]
+            s["stress"] = s["stress"] + h * D["stress"]
+            X = {k: v.copy() for k, v in s.items()}
+            X["vel"][:, :3] = s["vel"][:, :3] + h * D["vel"][:, :3]
+            X["den"] = s["den"] + h * D["den"]
+            X["stress"] = s["stress"] + h * D["stress"]
+            D = self.compute_forces(X)
+            s["vel"][:, :3] = s["vel"][:, :3] + h * D["vel"][:, :3]
+            s["den"] = s["den"] + h * D["den"]
+            s["stress"] = s["stress"] + h * D["stress"]
+        return s
 
     def lincomb(self, coefs, terms):
         out = self.zeros_like(terms[0])

@@ -105,7 +105,7 @@ def test_compute_forces_parity(torch_cuda, oracle_mod, name):
     sim.close()
 
 
-@pytest.mark.parametrize("integ", ["rk4", "rk2", "heun2", "rk_3_8", "euler"])
+@pytest.mark.parametrize("integ", ["rk4", "rk2", "heun2", "rk_3_8", "euler", "verlet"])
 def test_fused_step_parity(torch_cuda, oracle_mod, integ):
     sc = S.dam_break_2d()
     rng = np.random.default_rng(1)
@@ -141,6 +141,45 @@ def test_multi_step_3d_close_to_oracle(torch_cuda, oracle_mod):
     errs = _report("20 steps rk2", _state_of(got), y32, y64)
     for k, (e32, _) in errs.items():
         assert e32 <= 1e-4, (k, e32)
+
+
+def test_verlet_multi_step_and_resume(torch_cuda, oracle_mod):
+    """Velocity Verlet keeps the last force evaluation between steps: 20 steps in one call,
+    and 7 + 13 in two, match the oracle's scheme; a state set from the host evaluates the
+    forces afresh."""
+    sc = S.small_fluid_3d(n_side=12, box=(0.12, 0.1, 0.12))
+    print("case verlet multistep", sc.n, sc.nb, flush=True)
+    o = oracle_mod.Oracle(sc)
+    y32 = o.step(o.state("f"), T.tableau("verlet"), 5e-5, 20)
+    y64 = o.step(o.state("d"), T.tableau("verlet"), 5e-5, 20)
+    a = _sim(sc, integrator="verlet", timestep=5e-5)
+    a.update(20)
+    b = _sim(sc, integrator="verlet", timestep=5e-5)
+    b.update(7)
+    b.update(13)
+    ga, gb = a.particles(), b.particles()
+    for k in ("den", "pos", "vel"):
+        assert np.array_equal(ga[k], gb[k]), k
+    errs = _report("20 steps verlet", _state_of(ga), y32, y64)
+    for k, (e32, _) in errs.items():
+        assert e32 <= 1e-4, (k, e32)
+    # resetting the state drops the kept evaluation: the oracle restarted at step 7
+    c = _sim(sc, integrator="verlet", timestep=5e-5)
+    c.update(7)
+    mid = c.particles()
+    c.upload_state(mid)
+    c.update(13)
+    o2 = oracle_mod.Oracle(sc)
+    st = o2.state("f")
+    st7 = o2.step(st, T.tableau("verlet"), 5e-5, 7)
+    ref = o2.step(st7, T.tableau("verlet"), 5e-5, 13)
+    gc = c.particles()
+    for k in ("den", "pos", "vel"):
+        e = oracle_mod.rel_err(gc[k], ref[k])
+        print("  restart %s %.2e" % (k, e), flush=True)
+        assert e <= 1e-4, (k, e)
+    for s in (a, b, c):
+        s.close()
 
 
 def test_lincomb_and_velocity(torch_cuda, oracle_mod):
@@ -277,7 +316,7 @@ def test_solid_compute_forces_parity(torch_cuda, oracle_mod, case):
     assert np.all(gs["ref_pos"] == 0)
 
 
-@pytest.mark.parametrize("integ", ["rk4", "rk2"])
+@pytest.mark.parametrize("integ", ["rk4", "rk2", "verlet"])
 def test_solid_step_parity(torch_cuda, oracle_mod, integ):
     sc = _solid_scene(dim=3, plastic=True)
     print("case solid step", integ, flush=True)

@@ -257,7 +257,8 @@ def test_tableaus_match_c_library():
         assert np.array_equal(t["a"], ref["a"]) and np.array_equal(t["b"], ref["b"])
 
 
-@pytest.mark.parametrize("name,exact", [("rk2", True), ("rk4", True), ("heun2", True), ("euler", False)])
+@pytest.mark.parametrize("name,exact", [("rk2", True), ("rk4", True), ("heun2", True), ("euler", False),
+                                        ("verlet", True)])
 def test_free_fall_polynomial_exactness(oracle_mod, name, exact):
     sc = box_scene([[0.5, 0.9, 0.5]], vel=[[0.3, 0.5, -0.2]], h=0.05)
     o = oracle_mod.Oracle(sc)
@@ -391,3 +392,22 @@ def test_lattice_mass_and_count():
     assert m == np.float32(1000.0 / 150)
     xs = np.sort(np.unique(p["pos"][:, 0]))
     assert len(xs) == 20 and xs[1] == np.float32(0.05)
+
+
+def test_verlet_second_order(oracle_mod):
+    """Velocity Verlet on a small SPH column (FP64): halving dt divides the error against a
+    fine RK4 solution by about 4 (a missing half kick or a drift with the old velocity would
+    make it first order), and it takes one force evaluation per step."""
+    sc = S.small_fluid_3d(n_side=6, box=(0.06, 0.06, 0.06))
+    rng = np.random.default_rng(4)
+    sc.particles["vel"][:, :3] = rng.normal(0, 0.05, (sc.n, 3))
+    o = oracle_mod.Oracle(sc)
+    T_end = 4e-4
+    ref = o.step(o.state("d"), T.tableau("rk4"), T_end / 64, 64)
+    errs = []
+    for k in (4, 8, 16):
+        y = o.step(o.state("d"), T.tableau("verlet"), T_end / k, k)
+        errs.append(max(np.abs(y[f] - ref[f]).max() / np.abs(ref[f]).max() for f in ("pos", "vel", "den")))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    print("verlet errors", errs, "ratios", r1, r2)
+    assert 3.0 < r1 < 5.5 and 3.0 < r2 < 5.5
