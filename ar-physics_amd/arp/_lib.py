@@ -18,6 +18,8 @@ SYMBOLS = [
     "arp_profile_select", "arp_batch_create", "arp_batch_destroy", "arp_batch_last_error", "arp_batch_step",
     "arp_batch_tune", "arp_set_stiffness_field", "arp_adjoint_stiffness_grad",
     "arp_set_neighbor_skin", "arp_neighbor_stats", "arp_set_neighbor_max_age",
+    "arp_scene_parse", "arp_scene_free", "arp_scene_error", "arp_scene_settings", "arp_scene_object",
+    "arp_scene_generate", "arp_scene_arrays", "arp_scene_contains", "arp_scene_apply", "arp_toml_to_json",
 ]
 
 
@@ -91,6 +93,18 @@ def lib():
     L.arp_neighbor_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_adjoint_backward.argtypes = [vp, vp, vp, vp]
+    L.arp_scene_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_size_t]
+    L.arp_scene_free.argtypes = [vp]
+    L.arp_scene_free.restype = None
+    L.arp_scene_error.argtypes = [vp]
+    L.arp_scene_error.restype = C.c_char_p
+    L.arp_scene_settings.argtypes = [vp, vp]
+    L.arp_scene_object.argtypes = [vp, C.c_uint32, vp]
+    L.arp_scene_generate.argtypes = [vp, C.c_uint32]
+    L.arp_scene_arrays.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp]
+    L.arp_scene_contains.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
+    L.arp_scene_apply.argtypes = [vp, vp]
+    L.arp_toml_to_json.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     _lib = L
     return L
 

@@ -2,16 +2,14 @@
 
 Restated from the reference (paths relative to the reference repository):
 
-* regions (src/soft_body/material_region/region.rs:11-272): AABB, AABE, their ``border``
-  differences, Union / Intersection / Difference;
-* the staggered-lattice particle generator and material mass
-  (src/soft_body/material_region/mod.rs:56-155), evaluated in f32 in the same order;
 * material constructors (material.rs:61-121, examples/test_mods/config.rs:330-391);
 * the interaction matrix (soft_body/mod.rs:93-111, interaction.rs:42-50);
 * the grid dimensionality (soft_body/mod.rs:113-117).
 
-The benchmark workloads (2-D dam break, 3-D fluid column, elastic cantilever, fluid-solid
-contact, ensembles) are built here from their stated sizes and parameters.
+Regions and the staggered lattice are generated by the native scene-file reader
+(``arp.config``); the 2-D dam break is a TOML scene read by it.  The 3-D benchmark workloads
+(fluid column, elastic cantilever, fluid-solid contact, ensembles) are built here from their
+stated sizes and parameters.
 """
 from __future__ import annotations
 
@@ -23,175 +21,6 @@ import numpy as np
 from . import wire
 
 f32 = np.float32
-
-
-# ---------------------------------------------------------------------------------------
-# regions
-# ---------------------------------------------------------------------------------------
-class AABB:
-    def __init__(self, mn, dim):
-        self.min = np.zeros(4, f32)
-        self.dim = np.zeros(4, f32)
-        self.min[: len(mn)] = mn
-        self.dim[: len(dim)] = dim
-
-    def bound(self):
-        return self
-
-    def contains(self, p):
-        # p[i] >= min[i] && p[i]-min[i] <= dim[i] on every axis (region.rs:64-71)
-        ok = np.ones(len(p), bool)
-        for i in range(4):
-            ok &= ~(p[:, i] < self.min[i])
-            ok &= ~((p[:, i] - self.min[i]) > self.dim[i])
-        return ok
-
-    def center(self):
-        return np.array([self.min[i] + f32(0.5) * self.dim[i] for i in range(4)], f32)
-
-    def border(self, b):
-        # AABB::border (region.rs:33-43)
-        b = f32(b)
-        c = self.center()
-        inner_min = np.array([min(self.min[i] + b, c[i]) for i in range(4)], f32)
-        inner_dim = np.array([max(self.dim[i] - f32(2.0) * b, f32(0.0)) for i in range(4)], f32)
-        return Difference(self, AABB(inner_min, inner_dim))
-
-
-class AABE:
-    def __init__(self, center, radii):
-        self.c = np.zeros(4, f32)
-        self.r = np.zeros(4, f32)
-        self.c[: len(center)] = center
-        self.r[: len(radii)] = radii
-
-    def bound(self):
-        return AABB(self.c - self.r, f32(2.0) * self.r)
-
-    def contains(self, p):
-        # region.rs:88-106, evaluated in f64 like the reference
-        lhs = np.zeros(len(p))
-        for i in range(4):
-            d = (p[:, i] - self.c[i]).astype(np.float64)
-            r = float(self.r[i])
-            if r > 0.0:
-                coeff = 1.0
-                for j in range(4):
-                    if self.r[j] > 0.0 and j != i:
-                        coeff *= float(self.r[j])
-                lhs = lhs + d * d * coeff * coeff
-        rhs = 1.0
-        for i in range(4):
-            if self.r[i] > 0.0:
-                rhs *= float(self.r[i])
-        return lhs <= rhs * rhs
-
-    def border(self, b):
-        inner = AABE(self.c, np.maximum(self.r - f32(b), f32(0)))
-        return Difference(self, inner)
-
-
-class Difference:
-    def __init__(self, a, b):
-        self.a, self.b = a, b
-
-    def bound(self):
-        return self.a.bound()
-
-    def contains(self, p):
-        return self.a.contains(p) & ~self.b.contains(p)
-
-
-class Union:
-    def __init__(self, a, b):
-        self.a, self.b = a, b
-
-    def bound(self):
-        l, r = self.a.bound(), self.b.bound()
-        mn = np.minimum(l.min, r.min)
-        mx = np.maximum(l.min + l.dim, r.min + r.dim)
-        return AABB(mn, mx - mn)
-
-    def contains(self, p):
-        return self.a.contains(p) | self.b.contains(p)
-
-
-class Intersection:
-    def __init__(self, a, b):
-        self.a, self.b = a, b
-
-    def bound(self):
-        l, r = self.a.bound(), self.b.bound()
-        mn = np.maximum(l.min, r.min)
-        mx = np.minimum(l.min + l.dim, r.min + r.dim)
-        return AABB(mn, mx - mn)
-
-    def contains(self, p):
-        return self.a.contains(p) & self.b.contains(p)
-
-
-def _axis_run(start, step, origin, extent):
-    """f32 run start, start+step, ... while (pos - origin) <= extent, the first always."""
-    out = [start]
-    pos = start
-    while True:
-        pos = f32(pos + step)
-        if (pos - origin) > extent or extent == f32(0.0):
-            break
-        out.append(pos)
-    return out, pos
-
-
-def gen_particles(region, packing, kernel_radius, mat_id, start_den, vel_fn=None):
-    """MaterialRegion::gen_particles (material_region/mod.rs:56-155).
-
-    Returns (particle array, mass).  The lattice is staggered: x step s, y step s/sqrt(2)
-    with alternating x offset s/2, z step s/sqrt(3) with alternating offset s/sqrt(8),
-    accumulated in f32; mass = start_den * bbox volume / lattice points tested (f64)."""
-    h = f32(f32(packing) * f32(kernel_radius))
-    b = region.bound()
-    mn, dm = b.min, b.dim
-    sq2, sq3, sq8 = f32(math.sqrt(2.0)), f32(math.sqrt(3.0)), f32(math.sqrt(8.0))
-    pts = []
-    pos = mn.copy()
-    offset2 = f32(0.0)
-    while True:
-        offset = f32(0.0)
-        pos[1] = f32(mn[1] + offset2)
-        while True:
-            x0 = f32(mn[0] + f32(offset + offset2))
-            xs, _ = _axis_run(x0, h, mn[0], dm[0])
-            row = np.empty((len(xs), 4), f32)
-            row[:, 0] = xs
-            row[:, 1] = pos[1]
-            row[:, 2] = pos[2]
-            row[:, 3] = pos[3]
-            pts.append(row)
-            pos[1] = f32(pos[1] + f32(h / sq2))
-            offset = f32(h / f32(2.0)) if offset == f32(0.0) else f32(0.0)
-            if (pos[1] - mn[1]) > dm[1] or dm[1] == f32(0.0):
-                break
-        pos[2] = f32(pos[2] + f32(h / sq3))
-        offset2 = f32(h / sq8) if offset2 == f32(0.0) else f32(0.0)
-        if (pos[2] - mn[2]) > dm[2] or dm[2] == f32(0.0):
-            break
-    P = np.concatenate(pts) if pts else np.zeros((0, 4), f32)
-    num_in_box = len(P)
-    keep = region.contains(P)
-    P = P[keep]
-    out = wire.particles(len(P))
-    out["pos"] = P
-    if vel_fn is not None:
-        out["vel"] = vel_fn(P)
-    out["den"] = f32(start_den)
-    out["mat"] = mat_id
-    box = float(start_den)
-    vol = 1.0
-    for i in range(4):
-        if dm[i] > 0.0:
-            vol *= float(dm[i])
-    mass = f32((box * vol) / float(max(num_in_box, 1)))
-    return out, mass
 
 
 # ---------------------------------------------------------------------------------------
@@ -310,6 +139,7 @@ class Scene:
     subticks: int = 1
     integrator: str = "rk4"
     name: str = ""
+    objects: list = None  # per-object records of a scene file (wire.SCENE_OBJECT)
 
     def __post_init__(self):
         if self.solids is None:
@@ -385,33 +215,43 @@ def uniform_pm1(seed, n):
 # ---------------------------------------------------------------------------------------
 # workloads
 # ---------------------------------------------------------------------------------------
+DAM_BREAK_2D = """\
+name = "Dam break 2D"
+kernel_radius = {h!r}
+time_step = 0.0001
+gravity = 9.81
+artificial_viscocity = {alpha!r}
+integrator = "RK4"
+min = [-0.03, -0.03]
+dim = [1.06, 1.06]
+
+[boundaries.wall]
+spacing = {s!r}
+region = {{min = [-0.03, -0.03], dim = [1.06, 1.06], border = {border!r}}}
+
+[objects.water]
+density = 1000.0
+speed_of_sound = 20.0
+viscocity = 0.0
+spacing = {s!r}
+region = {{min = [0.0, 0.0], dim = [{wx!r}, {wy!r}]}}
+"""
+
+
 def dam_break_2d(spacing=0.01, kernel_radius=0.013, rows=64, cols=64, alpha=0.02):
-    """2-D weakly compressible dam break, built like a repo TOML scene would be:
-
-        kernel_radius = 0.013, time_step = 1e-4, gravity = 9.81, integrator = "RK4",
-        artificial_viscocity = 0.02, min = [-0.03,-0.03], dim = [1.06,1.06]
-        [boundaries.wall] spacing = 0.01, region = {min=[-0.03,-0.03], dim=[1.06,1.06],
-                                                     border = 0.03}
-        [objects.water]  density = 1000, speed_of_sound = 20, viscocity = 0,
-                         spacing = 0.01, region = {min = [0,0], dim = [63.75 s, 63.5 s/sqrt2]}
-
+    """2-D weakly compressible dam break, read from a TOML scene (DAM_BREAK_2D) by the
+    native scene-file reader: a wall shell of three lattice rows around [-0.03, 1.03]^2 and
+    a water block of cols x rows lattice points (spacing = lattice step) at the origin.
     Materials follow the loader's order: boundaries, then objects, each sorted by name."""
-    h = f32(kernel_radius)
-    s = spacing
-    packing = f32(float(s) / float(h))
-    wall_region = AABB([-0.03, -0.03], [1.06, 1.06]).border(3 * s)
-    fluid_region = AABB([0.0, 0.0], [(cols - 0.25) * s, (rows - 0.5) * s / math.sqrt(2.0)])
-    wall_mat = immobile_material(0.0)
-    water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)
-    bnd, m_b = gen_particles(wall_region, packing, h, 0, wall_mat["start_den"])
-    flu, m_f = gen_particles(fluid_region, packing, h, 1, water["start_den"])
-    wall_mat["mass"] = m_b
-    water["mass"] = m_f
-    mats = np.array([wall_mat, water], wire.MATERIAL)
-    inter = resolve_interactions(mats, h)
-    bounds = wire.aabb([-0.03, -0.03], [1.06, 1.06])
-    return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, alpha, dt=1e-4, subticks=1,
-                 integrator="rk4", name="dam_break_2d")
+    from . import config
+
+    s = float(spacing)
+    text = DAM_BREAK_2D.format(h=float(kernel_radius), alpha=float(alpha), s=s, border=3 * s,
+                               wx=(cols - 0.25) * s, wy=(rows - 0.5) * s / math.sqrt(2.0))
+    sc = config.loads(text)
+    sc.integrator = "rk4"
+    sc.name = "dam_break_2d"
+    return sc
 
 
 def _cubic_block(n, spacing, origin, jitter, seed):

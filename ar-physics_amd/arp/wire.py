@@ -83,6 +83,44 @@ GRID_INFO = np.dtype(
     align=True,
 )
 
+# scene-file records (ArpSceneSettings / ArpSceneObject of include/arp.h)
+SCENE_SETTINGS = np.dtype(
+    [
+        ("title", "S128"),
+        ("integrator", "S32"),
+        ("kernel_radius", "<f4"),
+        ("time_step", "<f4"),
+        ("artificial_viscosity", "<f4"),
+        ("gravity", "<f4"),
+        ("render_radius", "<f4"),
+        ("subticks", "<u4"),
+        ("lighting", "<u4"),
+        ("bounds", AABB),
+        ("n_objects", "<u4"),
+        ("n_on_click", "<u4"),
+        ("n_particles", "<u4"),
+        ("n_boundary", "<u4"),
+        ("n_solids", "<u4"),
+    ]
+)
+assert SCENE_SETTINGS.itemsize == 240
+
+SCENE_OBJECT = np.dtype(
+    [
+        ("name", "S64"),
+        ("colors", "<f4", (3, 4)),
+        ("packing", "<f4"),
+        ("immobile", "<u4"),
+        ("on_click", "<u4"),
+        ("bound", AABB),
+        ("material", MATERIAL),
+        ("first", "<u4"),
+        ("count", "<u4"),
+        ("lattice_points", "<u4"),
+    ]
+)
+assert SCENE_OBJECT.itemsize == 248
+
 # state equations (material.rs:53-59) and contact potentials (interaction.rs:18-25)
 EQ_ZERO, EQ_CONSTANT, EQ_IDEAL_GAS, EQ_TAIT = 0, 1, 2, 3
 POT_ZERO, POT_CONSTANT, POT_LINEAR, POT_TAIT, POT_LENNARD_JONES, POT_HERTZ = range(6)

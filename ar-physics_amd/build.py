@@ -4,7 +4,8 @@
 
 1. hipcc --genco --offload-arch=gfx950 csrc/kernels.hip  -> csrc/_build/kernels.hsaco
 2. the code object is embedded (csrc/embed.S, .incbin) into the host runtime
-   csrc/runtime.cpp, compiled with g++ against the HIP runtime API      -> arp/libarp.so
+   csrc/runtime.cpp, compiled with g++ against the HIP runtime API, and linked with the
+   host-only scene-file reader (csrc/toml.cpp, csrc/scene_file.cpp)    -> arp/libarp.so
 """
 import os
 import shutil
@@ -46,8 +47,15 @@ def build(verbose=False):
     rt = os.path.join(BUILD, "runtime.o")
     run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-D__HIP_PLATFORM_AMD__",
          "-I" + os.path.join(ROCM, "include"), "-c", "-o", rt, os.path.join(CSRC, "runtime.cpp")])
+    # host-only scene files: no contraction, so the f32 arithmetic is the reference's
+    host_objs = []
+    for name in ("toml", "scene_file"):
+        o = os.path.join(BUILD, name + ".o")
+        run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math",
+             "-I" + os.path.join(ROCM, "include"), "-c", "-o", o, os.path.join(CSRC, name + ".cpp")])
+        host_objs.append(o)
     tmp = OUT + ".tmp"
-    run(["g++", "-shared", "-o", tmp, rt, emb, "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+    run(["g++", "-shared", "-o", tmp, rt] + host_objs + [emb, "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
          "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-Wl,--no-undefined"])
     shutil.move(tmp, OUT)
     if verbose:

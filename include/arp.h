@@ -270,6 +270,73 @@ int arp_batch_step(arp_batch* batch, float dt, uint32_t nsteps);
  * restored after every trial. */
 int arp_batch_tune(arp_batch* batch, float dt, uint32_t trial_steps, uint32_t* chosen_div);
 
+/* ---- scene files: the example runner's TOML scenes (host only, no device needed) ------- */
+/* Config::parse (examples/test_mods/config.rs:298-474) and the particle generation of
+ * FluidSim::new (soft_body/mod.rs:58-111: MaterialRegion::gen_particles per object, the
+ * boundary split, init_solid_particles, the interaction matrix).  Objects are numbered as
+ * the reference numbers materials: [boundaries.*] first, then [objects.*] without on_click,
+ * each group in byte order of the names.  Keys, defaults and type rules are the reference's
+ * (a float key given as an integer is an error, as its as_float().unwrap() is); STL models
+ * ("model") are read relative to base_dir, or the working directory when it is NULL. */
+typedef struct arp_scene_file arp_scene_file;
+
+typedef struct ArpSceneSettings {
+    char title[128];            /* name (default "Fluid Test") */
+    char integrator[32];        /* integrator as written (default "verlet") */
+    float kernel_radius;        /* kernel_radius (1/64) */
+    float time_step;            /* time_step (0.01) */
+    float artificial_viscosity; /* artificial_viscocity (50) */
+    float gravity;              /* gravity (1) */
+    float render_radius;        /* kernel_radius * particle_render_factor */
+    uint32_t subticks;          /* subticks (1) */
+    uint32_t lighting;          /* lighting (default: light_pos given) */
+    ArpAABB bounds;             /* min ([-1,-1,0,0]), dim ([2,2,0,0]) */
+    uint32_t n_objects;         /* simulation objects = materials */
+    uint32_t n_on_click;        /* interactive objects (indices n_objects..) */
+    uint32_t n_particles, n_boundary, n_solids; /* after arp_scene_generate */
+} ArpSceneSettings;
+
+typedef struct ArpSceneObject {
+    char name[64];
+    float colors[3][4];       /* color, color_low_density, color_high_density */
+    float packing;            /* lattice spacing / kernel radius */
+    uint32_t immobile;        /* a [boundaries.*] object */
+    uint32_t on_click;        /* 0 simulated, 1 on_click, 2 on_click = "Relative" */
+    ArpAABB bound;            /* the region's bounding box */
+    ArpMaterial material;     /* with the lattice mass once generated */
+    uint32_t first, count;    /* its range in the particle or boundary array */
+    uint32_t lattice_points;  /* lattice points tested (the mass divisor) */
+} ArpSceneObject;
+
+/* Parses a scene; on error returns ARP_E_ARG with the message in err (may be NULL). */
+int arp_scene_parse(const char* toml_text, size_t len, const char* base_dir, arp_scene_file** out,
+                    char* err, size_t err_len);
+void arp_scene_free(arp_scene_file* f);
+const char* arp_scene_error(const arp_scene_file* f);
+int arp_scene_settings(const arp_scene_file* f, ArpSceneSettings* out);
+/* i < n_objects: simulation objects (material i); n_objects <= i: on_click objects. */
+int arp_scene_object(const arp_scene_file* f, uint32_t i, ArpSceneObject* out);
+/* Generates every object's particles (regions tested on `threads` host threads, 0 = all
+ * cores); the result does not depend on the thread count. */
+int arp_scene_generate(arp_scene_file* f, uint32_t threads);
+/* Copies the generated arrays (each may be NULL; counts must match the settings):
+ * mobile particles, solid particles, boundary particles, materials and the resolved
+ * n_objects x n_objects interaction matrix. */
+int arp_scene_arrays(const arp_scene_file* f, ArpParticle* particles, uint32_t n,
+                     ArpSolidParticle* solids, uint32_t ns, ArpParticle* boundary, uint32_t nb,
+                     ArpMaterial* mats, uint32_t nm, ArpMatInteraction* inter);
+/* Region::contains of object i for n points (4 floats each): out[k] = 0 or 1. */
+int arp_scene_contains(const arp_scene_file* f, uint32_t object, const float* points,
+                       uint32_t n, uint8_t* out);
+/* FluidSim::new + its integrator (examples/test.rs:160-205): arp_scene_upload of the
+ * generated scene with the file's kernel radius, gravity, artificial viscosity and bounds,
+ * then arp_set_integrator with the named integrator, time_step and subticks. */
+int arp_scene_apply(arp_ctx* ctx, const arp_scene_file* f);
+/* The parsed TOML document as JSON (no counterpart in the reference: it checks the TOML
+ * reader).  *needed = bytes including the terminator; returns ARP_E_ARG with the parse
+ * message in out when the text is not valid TOML. */
+int arp_toml_to_json(const char* text, size_t len, char* out, size_t cap, size_t* needed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -382,16 +382,6 @@ def test_dam_break_scene_layout():
     assert it["potential"] == wire.POT_LINEAR and it["strength"] == 10000.0
 
 
-def test_lattice_mass_and_count():
-    # staggered lattice: rows of s, s/sqrt(2); mass = density * area / points tested
-    reg = S.AABB([0, 0], [1.0, 1.0])
-    p, m = S.gen_particles(reg, 1.0, 0.1, 0, 1000.0)
-    # f32 accumulation of 0.1 passes 1.0 after 10 points (offsets 0 and 0.05 alike);
-    # 15 rows 0.0707 apart fit in the unit height
-    assert len(p) == 10 * 15
-    assert m == np.float32(1000.0 / 150)
-    xs = np.sort(np.unique(p["pos"][:, 0]))
-    assert len(xs) == 20 and xs[1] == np.float32(0.05)
 
 
 def test_verlet_second_order(oracle_mod):
