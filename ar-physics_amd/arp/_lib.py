@@ -20,6 +20,8 @@ SYMBOLS = [
     "arp_set_neighbor_skin", "arp_neighbor_stats", "arp_set_neighbor_max_age",
     "arp_scene_parse", "arp_scene_free", "arp_scene_error", "arp_scene_settings", "arp_scene_object",
     "arp_scene_generate", "arp_scene_arrays", "arp_scene_contains", "arp_scene_apply", "arp_toml_to_json",
+    "arp_kernel_radius", "arp_num_materials", "arp_add_particles", "arp_scene_spawn", "arp_num_particles",
+    "arp_num_solids",
 ]
 
 
@@ -105,6 +107,15 @@ def lib():
     L.arp_scene_contains.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
     L.arp_scene_apply.argtypes = [vp, vp]
     L.arp_toml_to_json.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.arp_kernel_radius.argtypes = [vp]
+    L.arp_kernel_radius.restype = C.c_float
+    L.arp_num_materials.argtypes = [vp]
+    L.arp_num_materials.restype = C.c_uint32
+    for nm in ("arp_num_particles", "arp_num_solids"):
+        getattr(L, nm).argtypes = [vp]
+        getattr(L, nm).restype = C.c_uint32
+    L.arp_add_particles.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.arp_scene_spawn.argtypes = [vp, vp, C.c_uint32, vp, C.POINTER(C.c_uint32)]
     _lib = L
     return L
 

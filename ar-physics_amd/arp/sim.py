@@ -72,14 +72,40 @@ class FluidSim:
 
     # ---- state ----------------------------------------------------------------------
     def particles(self):
-        out = wire.particles(self.scene.n)
+        out = wire.particles(lib().arp_num_particles(self.ctx))
         check(self.ctx, lib().arp_download(self.ctx, _p(out), len(out), None, 0))
         return out
 
     def solids(self):
-        out = wire.solids(self.scene.ns)
+        out = wire.solids(lib().arp_num_solids(self.ctx))
         check(self.ctx, lib().arp_download(self.ctx, None, 0, _p(out), len(out)))
         return out
+
+    @property
+    def kernel_radius(self):
+        return lib().arp_kernel_radius(self.ctx)
+
+    def num_materials(self):
+        return lib().arp_num_materials(self.ctx)
+
+    def add_particles(self, material, particles):
+        """FluidSim::add_particles with generated particles (Particles::add_particles):
+        returns True when a material was added."""
+        m = np.ascontiguousarray(material, wire.MATERIAL)
+        p = np.ascontiguousarray(particles, wire.PARTICLE)
+        added = C.c_uint32(0)
+        check(self.ctx, lib().arp_add_particles(self.ctx, _p(m), _p(p), len(p), C.byref(added)))
+        return bool(added.value)
+
+    def add_object(self, scene_file, index, offset=None):
+        """FluidSim::add_particles of object ``index`` of an arp.config.SceneFile (the way
+        the example runner spawns its on_click objects), optionally offset."""
+        off = None if offset is None else np.zeros(4, np.float32)
+        if off is not None:
+            off[: len(offset)] = offset
+        added = C.c_uint32(0)
+        check(self.ctx, lib().arp_scene_spawn(self.ctx, scene_file._h, index, _p(off), C.byref(added)))
+        return bool(added.value)
 
     def upload_state(self, particles, solids=None):
         particles = np.ascontiguousarray(particles)
@@ -124,7 +150,7 @@ class FluidSim:
         after / before the recorded steps).  Returns an (nmat, 6) array of dL/d
         (bulk_viscosity, viscosity, normal_stiffness, shear_stiffness, normal_damp,
         shear_damp) per material, or None."""
-        nm = len(self.scene.materials)
+        nm = lib().arp_num_materials(self.ctx)
         gp = np.zeros(6 * nm, np.float32) if want_params else None
         check(self.ctx, lib().arp_adjoint_backward(self.ctx, self._dp(d_grad_out), self._dp(d_grad_in), _p(gp)))
         return gp.reshape(nm, 6) if want_params else None
@@ -140,7 +166,7 @@ class FluidSim:
 
     def stiffness_grad(self):
         """dL/dk_s per solid after adjoint_backward."""
-        out = np.zeros(self.scene.ns, np.float32)
+        out = np.zeros(lib().arp_num_solids(self.ctx), np.float32)
         check(self.ctx, lib().arp_adjoint_stiffness_grad(self.ctx, _p(out), len(out)))
         return out
 
@@ -168,7 +194,7 @@ class FluidSim:
         return g
 
     def neighbor_dump(self):
-        n = self.scene.n
+        n = lib().arp_num_particles(self.ctx)
         cell = np.zeros(n, np.uint32)
         cc = np.zeros(n, np.uint32)
         check(self.ctx, lib().arp_neighbor_dump(self.ctx, _p(cell), _p(cc), None, 0))

@@ -156,6 +156,10 @@ struct arp_ctx {
     int64_t mobile_mat = -1;  // the material of every mobile particle, or -1 (none or several)
     bool any_elastic = false; // some particle is elastic (a solid of a stiff material)
     std::vector<ArpSolidParticle> solids_h;
+    std::vector<ArpParticle> boundary_h;    // the uploaded boundary, interactions and bounds
+    std::vector<ArpMatInteraction> inter_h; // (kept for arp_add_particles)
+    ArpAABB bounds_h{};
+    float alpha_h = 0.0f;
     std::vector<float> kscale_h;  // per-solid stiffness scale (empty: 1)
     std::vector<uint32_t> b_orig;  // sorted boundary slot -> original boundary id
     std::vector<uint32_t> bP_h, rP_h;
@@ -1642,6 +1646,12 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     c->nm = nm;
     c->mats_h.assign(mats, mats + nm);
     c->solids_h.assign(solids, solids + ns);
+    c->boundary_h.assign(boundary, boundary + nb);
+    c->inter_h.assign(inter, inter + (size_t)nm * nm);
+    c->bounds_h = *bounds;
+    c->alpha_h = artificial_viscosity;
+    c->kscale_h.clear();
+    free_buf(c->kscale);
 
     std::vector<DMat> dm(nm);
     for (uint32_t i = 0; i < nm; i++) {
@@ -2040,6 +2050,109 @@ int arp_download(arp_ctx* c, ArpParticle* particles, uint32_t n, ArpSolidParticl
         }
     }
     return arp_synchronize(c);
+}
+
+namespace {
+
+// Material equality of the reference's derived PartialEq: field by field, floats by value.
+bool same_material(const ArpMaterial& a, const ArpMaterial& b) {
+    return a.immobile == b.immobile && a.mass == b.mass && a.start_den == b.start_den && a.state_eq == b.state_eq &&
+           a.sound_speed == b.sound_speed && a.target_den == b.target_den && a.visc == b.visc &&
+           a.bulk_visc == b.bulk_visc && a.strain_order == b.strain_order &&
+           a.normal_stiffness == b.normal_stiffness && a.shear_stiffness == b.shear_stiffness &&
+           a.normal_damp == b.normal_damp && a.shear_damp == b.shear_damp && a.plastic == b.plastic &&
+           a.yield_strength == b.yield_strength && a.work_hardening == b.work_hardening &&
+           a.work_hardening_exp == b.work_hardening_exp && a.kinematic_hardening == b.kinematic_hardening &&
+           a.thermal_softening == b.thermal_softening && a.relaxation_time == b.relaxation_time;
+}
+
+}  // namespace
+
+float arp_kernel_radius(const arp_ctx* c) { return c && c->have_scene ? c->phys.h : 0.0f; }
+
+uint32_t arp_num_materials(const arp_ctx* c) { return c && c->have_scene ? c->nm : 0u; }
+
+uint32_t arp_num_particles(const arp_ctx* c) { return c && c->have_scene ? c->n : 0u; }
+
+uint32_t arp_num_solids(const arp_ctx* c) { return c && c->have_scene ? c->ns : 0u; }
+
+int arp_add_particles(arp_ctx* c, const ArpMaterial* material, const ArpParticle* particles, uint32_t count,
+                      uint32_t* material_added) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (!material || (count && !particles)) return fail(c, ARP_E_ARG, "arp_add_particles: missing argument");
+    if (material_added) *material_added = 0;
+    if (count == 0) return ARP_OK;  // Particles::add_particles returns early
+    if ((uint64_t)c->n + count > 0xFFFFFFF0ull) return fail(c, ARP_E_LIMIT, "more than 2^32 particles");
+    const uint32_t n = c->n, nm = c->nm;
+    // the live state in the caller's order (plastic strain included)
+    std::vector<ArpParticle> P(n + (size_t)count);
+    std::vector<ArpSolidParticle> S(c->ns);
+    if ((rc = arp_download(c, P.data(), n, S.data(), c->ns))) return rc;
+    // the material: a stiff one is always new, else the first equal one (particles.rs:549-565)
+    const bool solid = is_stiff(*material);
+    uint32_t mid = nm;
+    if (!solid)
+        for (uint32_t i = 0; i < nm; i++)
+            if (same_material(c->mats_h[i], *material)) {
+                mid = i;
+                break;
+            }
+    std::vector<ArpMaterial> M = c->mats_h;
+    std::vector<ArpMatInteraction> I = c->inter_h;
+    const float h = c->phys.h;
+    if (mid == nm) {
+        // the grown matrix (particles.rs:573-598): old pairs kept, default_between for the
+        // new row and column, and for the new material with itself unless it is stiff
+        if (nm + 1 > 0xFFFF) return fail(c, ARP_E_LIMIT, "at most 65535 materials");
+        M.push_back(*material);
+        const uint32_t m2 = nm + 1;
+        std::vector<ArpMatInteraction> J((size_t)m2 * m2);
+        for (uint32_t i = 0; i < nm; i++) {
+            for (uint32_t j = 0; j < nm; j++) J[(size_t)i * m2 + j] = I[(size_t)i * nm + j];
+            arp_default_interaction(&M[i], &M[nm], h, &J[(size_t)i * m2 + nm]);
+        }
+        for (uint32_t i = 0; i < nm; i++) arp_default_interaction(&M[nm], &M[i], h, &J[(size_t)nm * m2 + i]);
+        if (solid)
+            J[(size_t)nm * m2 + nm] = ArpMatInteraction{0, 0.0f, 0.0f, 0.0f, 0.0f};
+        else
+            arp_default_interaction(&M[nm], &M[nm], h, &J[(size_t)nm * m2 + nm]);
+        I.swap(J);
+        if (material_added) *material_added = 1;
+    }
+    // the new particles (particles.rs:601-619); a scene without solids holds one placeholder
+    // solid, which new solids replace
+    if (solid && S.size() == 1 && S[0].part_id == ARP_INVALID_INDEX) S.clear();
+    for (uint32_t k = 0; k < count; k++) {
+        ArpParticle q = particles[k];
+        q.mat = mid;
+        q.solid_id = ARP_INVALID_INDEX;
+        if (solid) {
+            q.solid_id = (uint32_t)S.size();
+            ArpSolidParticle sp{};
+            sp.part_id = n + k;
+            std::memcpy(sp.ref_pos, q.pos, sizeof sp.ref_pos);
+            S.push_back(sp);
+        }
+        P[n + k] = q;
+    }
+    if (S.empty()) {
+        ArpSolidParticle sp{};
+        sp.part_id = ARP_INVALID_INDEX;
+        S.push_back(sp);
+    }
+    std::vector<float> ks = c->kscale_h;
+    if (!ks.empty()) ks.resize(S.size(), 1.0f);
+    const std::vector<ArpParticle> B = c->boundary_h;
+    const ArpAABB bounds = c->bounds_h;
+    const double t = c->time;
+    if ((rc = arp_scene_upload(c, P.data(), (uint32_t)P.size(), S.data(), (uint32_t)S.size(), B.data(),
+                               (uint32_t)B.size(), M.data(), (uint32_t)M.size(), I.data(), &bounds, h, c->phys.g,
+                               c->alpha_h)))
+        return rc;
+    c->time = t;
+    if (!ks.empty() && (rc = arp_set_stiffness_field(c, ks.data(), (uint32_t)ks.size()))) return rc;
+    return ARP_OK;
 }
 
 int arp_neighbors_update(arp_ctx* c, const ArpParticle* d_state) {

@@ -1209,6 +1209,32 @@ int arp_scene_apply(arp_ctx* ctx, const arp_scene_file* f) {
     return arp_set_integrator(ctx, &tab, f->dt, f->subticks);
 }
 
+int arp_scene_spawn(arp_ctx* ctx, const arp_scene_file* f, uint32_t object, const float* offset,
+                    uint32_t* material_added) {
+    if (!ctx || !f) return ARP_E_ARG;
+    const size_t no = f->objects.size();
+    if (object >= no + f->on_click.size()) return ARP_E_ARG;
+    const Object& o = object < no ? f->objects[object] : f->on_click[object - no];
+    const float h = arp_kernel_radius(ctx);
+    if (!(h > 0.0f)) return ARP_E_STATE;
+    std::vector<ArpParticle> p;
+    ArpMaterial mat{};
+    uint32_t tested = 0;
+    try {
+        gen_object(o, h, 0, std::max(1u, std::min(16u, std::thread::hardware_concurrency())), p, mat, tested);
+    } catch (SceneError&) {
+        return ARP_E_LIMIT;
+    } catch (std::bad_alloc&) {
+        return ARP_E_NOMEM;
+    } catch (std::system_error&) {
+        return ARP_E_ARG;
+    }
+    if (offset)
+        for (ArpParticle& q : p)
+            for (int i = 0; i < 4; i++) q.pos[i] += offset[i];
+    return arp_add_particles(ctx, &mat, p.data(), (uint32_t)p.size(), material_added);
+}
+
 int arp_toml_to_json(const char* text, size_t len, char* out, size_t cap, size_t* needed) {
     std::string err;
     arp_toml::VPtr root = arp_toml::parse(text ? text : "", text ? len : 0, err);

@@ -270,6 +270,26 @@ int arp_batch_step(arp_batch* batch, float dt, uint32_t nsteps);
  * restored after every trial. */
 int arp_batch_tune(arp_batch* batch, float dt, uint32_t trial_steps, uint32_t* chosen_div);
 
+/* ---- particle insertion (FluidSim::add_particles, soft_body/mod.rs:180-206) ----------- */
+/* FluidSim::kernel_radius (soft_body/mod.rs:166) and num_materials (mod.rs:176-178). */
+float arp_kernel_radius(const arp_ctx* ctx);
+uint32_t arp_num_materials(const arp_ctx* ctx);
+/* Lengths of the particle and solid-particle buffers (Particles::particles / solids). */
+uint32_t arp_num_particles(const arp_ctx* ctx);
+uint32_t arp_num_solids(const arp_ctx* ctx);
+/* Particles::add_particles (particle_state/particles.rs:547-631): appends count particles of
+ * `material` (their den, pos, vel are used; mat and solid_id are assigned).  A stiff material
+ * is always added as a new material; otherwise the first equal material is reused.  A new
+ * material's interactions are default_between with every material, itself included unless
+ * it is stiff (particles.rs:573-598).  New particles of a stiff material become solids with
+ * their current position as reference position.  *material_added (may be NULL) is 1 when a
+ * material was added.  The live state, time, integrator, neighbour-list settings and
+ * stiffness field (new solids: 1) carry over; recordings are dropped.  Deviation: a scene
+ * without solids holds one placeholder solid (part_id ARP_INVALID_INDEX) which new solids
+ * replace; the reference appends after it. */
+int arp_add_particles(arp_ctx* ctx, const ArpMaterial* material, const ArpParticle* particles,
+                      uint32_t count, uint32_t* material_added);
+
 /* ---- scene files: the example runner's TOML scenes (host only, no device needed) ------- */
 /* Config::parse (examples/test_mods/config.rs:298-474) and the particle generation of
  * FluidSim::new (soft_body/mod.rs:58-111: MaterialRegion::gen_particles per object, the
@@ -332,6 +352,12 @@ int arp_scene_contains(const arp_scene_file* f, uint32_t object, const float* po
  * generated scene with the file's kernel radius, gravity, artificial viscosity and bounds,
  * then arp_set_integrator with the named integrator, time_step and subticks. */
 int arp_scene_apply(arp_ctx* ctx, const arp_scene_file* f);
+/* FluidSim::add_particles of object i of a scene file (examples/test_mods/renderer.rs:
+ * 386-404 spawns the on_click objects this way): its lattice is generated with the context's
+ * kernel radius, offset by offset[0..3] when offset is not NULL (positions only), and added
+ * with arp_add_particles. */
+int arp_scene_spawn(arp_ctx* ctx, const arp_scene_file* f, uint32_t object, const float* offset,
+                    uint32_t* material_added);
 /* The parsed TOML document as JSON (no counterpart in the reference: it checks the TOML
  * reader).  *needed = bytes including the terminator; returns ARP_E_ARG with the parse
  * message in out when the text is not valid TOML. */

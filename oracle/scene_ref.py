@@ -737,3 +737,54 @@ def generate(cfg):
                 it = dict(potential=0, strength=f32(0), radius=f32(0), dampening=f32(0), friction=f32(0))
             inter[i][j] = inter[j][i] = it
     return dict(particles=P, boundary=B, solids=S, materials=mats, inter=inter)
+
+
+def add_particles(particles, solids, mats, inter, material, new, h):
+    """Particles::add_particles (particle_state/particles.rs:547-631) on structured arrays in
+    the wire layout: (particles, solids, materials, interactions, material id, added).
+
+    A stiff material is always new; otherwise the first material equal field by field is
+    reused.  A new material gets default_between with every material (particles.rs:573-598),
+    with itself too unless it is stiff.  New particles of a stiff material become solids with
+    their position as reference; a placeholder solid (part_id INVALID, the only one of a scene
+    without solids) is replaced by them (the reference appends after it)."""
+    nm = len(mats)
+    solid = bool(material["normal_stiffness"] != 0 or material["shear_stiffness"] != 0)
+    mid = nm
+    if not solid:
+        for i in range(nm):
+            if all(mats[i][f] == material[f] for f in MAT_FIELDS):
+                mid = i
+                break
+    added = mid == nm
+    if added:
+        mats = np.concatenate([mats, np.asarray(material, mats.dtype).reshape(1)])
+        m2 = nm + 1
+        J = np.zeros(m2 * m2, inter.dtype)
+
+        def put(k, d):
+            for f, v in d.items():
+                J[k][f] = v
+
+        for i in range(nm):
+            for j in range(nm):
+                J[i * m2 + j] = inter[i * nm + j]
+            put(i * m2 + nm, default_between(mats[i], mats[nm], h))
+        for i in range(nm):
+            put(nm * m2 + i, default_between(mats[nm], mats[i], h))
+        if not solid:
+            put(nm * m2 + nm, default_between(mats[nm], mats[nm], h))
+        inter = J
+    new = np.array(new, copy=True)
+    new["mat"] = mid
+    new["solid_id"] = INVALID
+    n = len(particles)
+    if solid:
+        if len(solids) == 1 and solids["part_id"][0] == INVALID:
+            solids = solids[:0]
+        extra = np.zeros(len(new), solids.dtype)
+        extra["part_id"] = n + np.arange(len(new), dtype=np.uint32)
+        extra["ref_pos"] = new["pos"]
+        new["solid_id"] = len(solids) + np.arange(len(new), dtype=np.uint32)
+        solids = np.concatenate([solids, extra])
+    return np.concatenate([particles, new]), solids, mats, inter, mid, added

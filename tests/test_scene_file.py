@@ -455,3 +455,52 @@ def test_reference_examples(name):
         assert np.array_equal(f.contains(i, L[sample]), want), (name, o["name"])
         assert [bytes(L[k]) in have for k in sample] == want.tolist(), (name, o["name"])
     f.close()
+
+
+def test_add_particles_oracle_rules():
+    """The oracle's Particles::add_particles: equal fluid materials are reused, a new one
+    gets default_between everywhere (itself included), a stiff one is always new with a
+    zero self-interaction, and its particles become solids replacing the placeholder."""
+    from arp import wire
+
+    h = np.float32(0.02)
+    wall = np.zeros((), wire.MATERIAL)
+    wall["immobile"], wall["visc"], wall["start_den"], wall["target_den"] = 1, 0.5, 1.0, 1.0
+    water = np.zeros((), wire.MATERIAL)
+    water["state_eq"], water["start_den"], water["sound_speed"], water["visc"] = 3, 1000.0, 20.0, 0.1
+    mats = np.array([wall, water], wire.MATERIAL)
+    inter = np.zeros(4, wire.INTERACTION)
+    inter[1]["potential"] = inter[2]["potential"] = 2
+    parts = wire.particles(3)
+    parts["mat"] = 1
+    solids = wire.solids(1)
+    solids["part_id"] = wire.INVALID
+    new = wire.particles(2)
+    new["pos"][:, 0] = [1.0, 2.0]
+    # an equal fluid: reused, nothing grows
+    P, S, M, I, mid, added = R.add_particles(parts, solids, mats, inter, water.copy(), new, h)
+    assert (mid, added, len(M), len(I), len(P)) == (1, False, 2, 4, 5) and list(P["mat"][3:]) == [1, 1]
+    # a different fluid: a new material, Linear contact (radius 2h, friction = sum of
+    # viscosities) with the wall, none with water or itself
+    oil = water.copy()
+    oil["start_den"] = 800.0
+    P, S, M, I, mid, added = R.add_particles(P, S, M, I, oil, new, h)
+    assert (mid, added, len(M), len(I)) == (2, True, 3, 9)
+    it = I.reshape(3, 3)
+    assert it[0, 2]["potential"] == 2 and it[2, 0]["potential"] == 2
+    assert it[0, 2]["radius"] == np.float32(0.04) and it[0, 2]["friction"] == np.float32(0.6)
+    assert it[1, 2]["potential"] == 0 and it[2, 2]["potential"] == 0 and it[0, 1]["potential"] == 2
+    # a zero-EOS material touches itself with Linear contact
+    sand = water.copy()
+    sand["state_eq"] = 0
+    _, _, _, I2, _, _ = R.add_particles(P, S, M, I, sand, new, h)
+    assert I2.reshape(4, 4)[3, 3]["potential"] == 2
+    # a stiff material: new every time, zero self-interaction, solids replace the placeholder
+    rub = water.copy()
+    rub["normal_stiffness"] = -1e4
+    P, S, M, I, mid, added = R.add_particles(P, S, M, I, rub, new, h)
+    assert (mid, added) == (3, True) and I.reshape(4, 4)[3, 3]["potential"] == 0
+    assert list(S["part_id"]) == [7, 8] and list(P["solid_id"][7:]) == [0, 1]
+    assert np.array_equal(S["ref_pos"], new["pos"])
+    P, S, M, I, mid, added = R.add_particles(P, S, M, I, rub.copy(), new, h)
+    assert (mid, added) == (4, True) and list(S["part_id"]) == [7, 8, 9, 10]
