@@ -283,6 +283,10 @@ struct FluidArgs {
     uint32_t* mkMirror;
     uint32_t* pidMirror;
     const uint32_t* pidIn;
+    // one mobile material and one boundary material, no elastic particles: the list walk
+    // takes the materials of a pair from these (k_fluid_nl_f, k_fluid_nlr_f)
+    uint32_t fastMat;
+    uint32_t fastBnd;
 };
 
 // Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists

@@ -611,8 +611,12 @@ __shared__ DInter s_inter[kMatLds * kMatLds];
 // a compile-time choice, so that the pair loop's table reads are LDS reads, never generic
 // (flat) loads, whose completion the compiler could only await together with the
 // neighbour loads already in flight.
-template <int MODE, bool LT>
+// FM (walks only): every mobile particle has material fastMat, every boundary particle
+// fastBnd, and none is elastic; a pair's material parameters are then selected by whether
+// the neighbour is a boundary particle, without per-neighbour material loads.
+template <int MODE, bool LT, bool FM = false>
 __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
+    static_assert(!FM || MODE == 3 || MODE == 5, "the fast-material form walks kept lists");
     constexpr bool REC = MODE == 1 || MODE >= 4;
     constexpr bool NL = MODE >= 2;
     constexpr bool WALK = MODE == 3 || MODE == 5;
@@ -674,13 +678,26 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         }
         const float4 xi = X[i];
         const float4 vi = XV[i];
-        const uint32_t mk1 = a.mk[i];
+        const uint32_t mk1 = FM ? a.fastMat : a.mk[i];
         const uint32_t mat1 = mk1 & kMatMask;
-        const bool elastic1 = (mk1 & kMkElastic) != 0;
+        const bool elastic1 = FM ? false : (mk1 & kMkElastic) != 0;
         const DMat& M1 = a.mats[mat1];
         const float m1 = M1.mass, d1 = xi.w, q1 = vi.w;
         const float f1 = M1.visc, bf1 = M1.bulk_visc;
         PairAcc acc = {0.f, 0.f, 0.f, 0.f};
+        // FM: the two neighbour kinds' parameters (uniform, loaded once)
+        float fmM[2] = {0.f, 0.f}, fmV[2] = {0.f, 0.f}, fmB[2] = {0.f, 0.f};
+        DInter fmI[2] = {};
+        if constexpr (FM) {
+            const uint32_t mm[2] = {a.fastMat, a.fastBnd};
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                fmM[k] = a.mats[mm[k]].mass;
+                fmV[k] = a.mats[mm[k]].visc;
+                fmB[k] = a.mats[mm[k]].bulk_visc;
+                fmI[k] = a.inter[mat1 * nm + mm[k]];
+            }
+        }
 
         auto load = [&](uint32_t e) {
             PairIn in;
@@ -688,7 +705,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const uint32_t j = e & ~kBnd;
             in.xj = (in.bnd ? B : X)[j];
             in.vj = (in.bnd ? BV : XV)[j];
-            in.mk2 = (in.bnd ? a.bmk : a.mk)[j];
+            in.mk2 = FM ? (in.bnd ? a.fastBnd : a.fastMat) : (in.bnd ? a.bmk : a.mk)[j];
             return in;
         };
         // one neighbour's terms (fluid.rs:153-214)
@@ -696,10 +713,21 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const bool bnd = in.bnd;
             const float4 xj = in.xj, vj = in.vj;
             const uint32_t mat2 = in.mk2 & kMatMask;
-            const bool elastic2 = (in.mk2 & kMkElastic) != 0;
+            const bool elastic2 = FM ? false : (in.mk2 & kMkElastic) != 0;
             float m2, visc2, bulk2;
             DInter it;
-            if constexpr (LT) {
+            if constexpr (FM) {
+                const int k = bnd ? 1 : 0;
+                m2 = bnd ? fmM[1] : fmM[0];
+                visc2 = bnd ? fmV[1] : fmV[0];
+                bulk2 = bnd ? fmB[1] : fmB[0];
+                it.potential = bnd ? fmI[1].potential : fmI[0].potential;
+                it.strength = bnd ? fmI[1].strength : fmI[0].strength;
+                it.radius = bnd ? fmI[1].radius : fmI[0].radius;
+                it.dampening = bnd ? fmI[1].dampening : fmI[0].dampening;
+                it.friction = bnd ? fmI[1].friction : fmI[0].friction;
+                (void)k;
+            } else if constexpr (LT) {
                 const float4 mm = smat[mat2];
                 m2 = mm.x;
                 visc2 = mm.y;
@@ -1132,6 +1160,13 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_nl_batch(BatchArgs b) 
 // the same pair recording each stage's neighbours within h (the reverse sweep reads them)
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nlbr(FluidArgs a) { fluid_entry<4>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nlr(FluidArgs a) { fluid_entry<5>(a, xcd_block(), gridDim.x); }
+// the walks of scenes with one mobile and one boundary material and no elastic particles
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f(FluidArgs a) {
+    fluid_body<3, false, true>(a, xcd_block(), gridDim.x);
+}
+extern "C" __global__ void __launch_bounds__(256) k_fluid_nlr_f(FluidArgs a) {
+    fluid_body<5, false, true>(a, xcd_block(), gridDim.x);
+}
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA

@@ -74,6 +74,8 @@ enum Kern {
     K_FLUID_ADJ_F,
     K_FLUID_ADJ_REC_F,
     K_VERLET_PRE,
+    K_FLUID_NL_F,
+    K_FLUID_NLR_F,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -86,7 +88,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch",
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
-                                   "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre"};
+                                   "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -114,7 +116,8 @@ struct GridSlot {
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
 struct RecStep {
-    DBuf mk, pid, src, stamps, flags;  // flags: nbOver[ARP_MAX_STAGES], use[ARP_MAX_STAGES]
+    DBuf mk, pid, src, stamps;
+    uint32_t* fl = nullptr;  // nbOver[ARP_MAX_STAGES], use[ARP_MAX_STAGES] (in the context's recFlags)
     uint32_t over[ARP_MAX_STAGES] = {};  // host copy of nbOver after the recording
     DBuf nbList[ARP_MAX_STAGES], nbCount[ARP_MAX_STAGES];
     DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
@@ -154,6 +157,8 @@ struct arp_ctx {
     std::vector<ArpMaterial> mats_h;
     std::vector<uint32_t> mat_by_pid, sid_by_pid, mk_by_pid;
     int64_t mobile_mat = -1;  // the material of every mobile particle, or -1 (none or several)
+    int64_t bnd_mat = -1;     // the material of every boundary particle (none: -1, several: -2)
+    bool bnd_elastic = false; // some boundary word carries the elastic flag
     bool any_elastic = false; // some particle is elastic (a solid of a stiff material)
     std::vector<ArpSolidParticle> solids_h;
     std::vector<ArpParticle> boundary_h;    // the uploaded boundary, interactions and bounds
@@ -210,6 +215,7 @@ struct arp_ctx {
     DBuf LamIn[2], LamAcc[2], LamNext[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
     DBuf Fbar, spar;  // solid reverse mode: per-solid F cotangents, per-solid stiffness partials
     DBuf kscale;      // per-solid stiffness scale (empty: 1)
+    DBuf recFlags;    // the recorded steps' stage flags, 2 * ARP_MAX_STAGES words per step
     bool keys_valid = false;
     // what the last build described (for the dump)
     const DBuf* last_pid = nullptr;
@@ -1008,7 +1014,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
               (S == 1 || (rec && c->rec_prefilled != rec->Xp[0].p)) ? 1 : 0,
               rec ? P<uint32_t>(rec->mk) : nullptr, rec ? P<uint32_t>(rec->pid) : nullptr,
               // the recording's per-stage flags start at zero
-              rec ? P<uint32_t>(rec->flags) : nullptr, rec ? 2u * ARP_MAX_STAGES : 0u};
+              rec ? rec->fl : nullptr, rec ? 2u * ARP_MAX_STAGES : 0u};
     if (verlet) {  // the force point is reordered along with the kick base
         io.srcAuxP = P<float>(c->XF[0][0]);
         io.srcAuxV = P<float>(c->XF[0][1]);
@@ -1110,7 +1116,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         if (rec) {  // keep the stage's neighbour lists for the reverse sweep
             f.nbList = P<uint32_t>(rec->nbList[s]);
             f.nbCount = P<uint32_t>(rec->nbCount[s]);
-            f.nbOver = P<uint32_t>(rec->flags) + s;
+            f.nbOver = rec->fl + s;
             f.nbCap = rec_cap(c);
         }
         if (nl) {
@@ -1122,9 +1128,16 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
                 f.pidMirror = P<uint32_t>(c->meta_pid[c->cur ^ 1]);
                 f.pidIn = P<uint32_t>(c->meta_pid[c->cur]);
             }
-            if (rec) f.recUse = P<uint32_t>(rec->flags) + ARP_MAX_STAGES + s;
+            if (rec) f.recUse = rec->fl + ARP_MAX_STAGES + s;
+            // one mobile and one boundary material, nothing elastic: the walk selects a
+            // pair's materials by kind (batches keep the general form, shared by all scenes)
+            const bool fm = !c->capture && c->mobile_mat >= 0 && !c->any_elastic && !c->bnd_elastic &&
+                            c->bnd_mat != -2;
+            f.fastMat = fm ? (uint32_t)c->mobile_mat : 0u;
+            f.fastBnd = fm ? (uint32_t)(c->bnd_mat >= 0 ? c->bnd_mat : c->mobile_mat) : 0u;
             if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
-            if ((rc = launch(c, rec ? K_FLUID_NLR : K_FLUID_NL, nblk(c->n), 256, f))) return rc;
+            const Kern walk = rec ? (fm ? K_FLUID_NLR_F : K_FLUID_NLR) : (fm ? K_FLUID_NL_F : K_FLUID_NL);
+            if ((rc = launch(c, walk, nblk(c->n), 256, f))) return rc;
         } else if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) {
             return rc;
         }
@@ -1178,7 +1191,6 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
     const void* st0 = R.stamps.p;
     if ((rc = dalloc(c, R.stamps, 4 * ARP_MAX_STAGES))) return rc;
     if (R.stamps.p != st0) HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
-    if ((rc = dalloc(c, R.flags, 8 * ARP_MAX_STAGES))) return rc;
     for (int s = 0; s < S; s++)
         if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)rec_cap(c) * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
     for (int s = 0; s < S; s++) {
@@ -1211,7 +1223,7 @@ void free_buf(DBuf& b) {
 void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
     for (size_t i = nrec; i < c->rec.size(); i++) {
         RecStep& R = c->rec[i];
-        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps, &R.flags}) free_buf(*b);
+        for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps}) free_buf(*b);
         for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
@@ -1225,12 +1237,14 @@ void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
 
 void free_rec(arp_ctx* c) { trim_rec(c, 0, 0); }
 
-// the live state (y in its storage order and the meta of that order)
-int ck_save(arp_ctx* c, Checkpoint& k) {
+// the live state (y in its storage order and the meta of that order); without `full` only the
+// order (a recording of one segment is never replayed: its checkpoints give only orders)
+int ck_save(arp_ctx* c, Checkpoint& k, bool full = true) {
     const size_t n = c->n;
     int rc;
-    if ((rc = d2d(c, k.P, c->Yland[0], 16 * n)) || (rc = d2d(c, k.V, c->Yland[1], 16 * n)) ||
-        (rc = d2d(c, k.mk, c->meta_mk[c->cur], 4 * n)) || (rc = d2d(c, k.pid, c->meta_pid[c->cur], 4 * n)))
+    if ((rc = d2d(c, k.pid, c->meta_pid[c->cur], 4 * n))) return rc;
+    if (full && ((rc = d2d(c, k.P, c->Yland[0], 16 * n)) || (rc = d2d(c, k.V, c->Yland[1], 16 * n)) ||
+                 (rc = d2d(c, k.mk, c->meta_mk[c->cur], 4 * n))))
         return rc;
     return ARP_OK;
 }
@@ -1256,10 +1270,11 @@ int ck_restore(arp_ctx* c, Checkpoint& k) {
 
 // host copies of the neighbour-record overflow flags of the first nrec recorded steps
 int fetch_over(arp_ctx* c, uint32_t nrec) {
-    for (uint32_t j = 0; j < nrec; j++)
-        HIPCHK(c, hipMemcpyAsync(c->rec[j].over, c->rec[j].flags.p, sizeof(c->rec[j].over), hipMemcpyDeviceToHost,
-                                 c->stream));
+    constexpr size_t W = 2 * ARP_MAX_STAGES;
+    std::vector<uint32_t> h(W * (size_t)std::max<uint32_t>(nrec, 1));
+    if (nrec) HIPCHK(c, hipMemcpyAsync(h.data(), c->recFlags.p, 4 * W * nrec, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t j = 0; j < nrec; j++) std::memcpy(c->rec[j].over, &h[W * j], sizeof(c->rec[j].over));
     return ARP_OK;
 }
 
@@ -1369,8 +1384,8 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
         a.pgrad = P<float>(c->pgradPart);
         a.nbList = P<uint32_t>(R.nbList[s]);
         a.nbCount = P<uint32_t>(R.nbCount[s]);
-        a.nbOver = P<uint32_t>(R.flags) + s;
-        a.stageUse = P<uint32_t>(R.flags) + ARP_MAX_STAGES + s;
+        a.nbOver = R.fl + s;
+        a.stageUse = R.fl + ARP_MAX_STAGES + s;
         if (s == S - 1) {  // the step's accumulator starts from dL/dy_{n+1}
             a.LamInP = P<float>(c->LamIn[0]);
             a.LamInV = P<float>(c->LamIn[1]);
@@ -1499,7 +1514,7 @@ void arp_destroy(arp_ctx* c) {
         if (b->p) hipFree(b->p);
     free_rec(c);
     for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->LamNext[0], &c->LamNext[1],
-                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->kscale, &c->gaos})
+                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->kscale, &c->gaos, &c->recFlags})
         free_buf(*b);
     for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(c->Gb[s][0]), free_buf(c->Gb[s][1]);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1745,6 +1760,14 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
             if (id < n && particles[id].solid_id != ARP_INVALID_INDEX && is_stiff(m)) mk |= kMkElastic;
             bm[s] = mk;
         }
+        c->bnd_mat = -1;
+        c->bnd_elastic = false;
+        for (size_t s = 0; s < kept.size(); s++) {
+            const int64_t m = (int64_t)(bm[s] & kMatMask);
+            if (bm[s] & kMkElastic) c->bnd_elastic = true;
+            if (c->bnd_mat == -1) c->bnd_mat = m;
+            else if (c->bnd_mat != m) c->bnd_mat = -2;
+        }
         if ((rc = upload(c, c->bposd, bp.data(), bp.size() * 4))) return rc;
         if ((rc = upload(c, c->bvelq, bv.data(), bv.size() * 4))) return rc;
         if ((rc = upload(c, c->bmk, bm.data(), bm.size() * 4))) return rc;
@@ -1876,9 +1899,12 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     c->ck.resize(nseg + 1);  // segment starts, then the end state
     for (RecStep& R : c->rec)
         if ((rc = alloc_rec(c, R, S))) return rc;
+    // every step's stage flags in one buffer (read back with one copy)
+    if ((rc = dalloc(c, c->recFlags, 4 * 2 * ARP_MAX_STAGES * (size_t)K))) return rc;
+    for (uint32_t j = 0; j < K; j++) c->rec[j].fl = P<uint32_t>(c->recFlags) + 2 * ARP_MAX_STAGES * (size_t)j;
     for (uint32_t j = 0; j < nsteps; j++) {
         if (j % K == 0) {
-            if ((rc = ck_save(c, c->ck[j / K]))) return rc;
+            if ((rc = ck_save(c, c->ck[j / K], nseg > 1))) return rc;
             // kept neighbour lists restart with every segment that the reverse sweep
             // replays (all but the last), as its replay from the checkpoint does, so that
             // the replay repeats the recorded steps exactly
@@ -1891,7 +1917,7 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
         c->time += dt;
     }
     c->rec_prefilled = nullptr;
-    if ((rc = ck_save(c, c->ck[nseg]))) return rc;
+    if ((rc = ck_save(c, c->ck[nseg], nseg > 1))) return rc;
     if ((rc = fetch_over(c, (nsteps - 1) % K + 1))) return rc;
     c->rec_dt = dt;
     c->rec_nsteps = nsteps;
@@ -1971,10 +1997,12 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
     GradAosArgs in{c->n, 0, P<uint32_t>(fin.pid), const_cast<void*>(d_grad_out), P<float>(c->LamIn[0]),
                    P<float>(c->LamIn[1])};
     if ((rc = launch(c, K_GRAD_AOS, nblk(c->n), 256, in))) return rc;
+    bool replayed = false;
     for (int seg = (int)nseg - 1; seg >= 0; seg--) {
         const uint32_t j0 = (uint32_t)seg * K, j1 = std::min(nsteps, j0 + K);
         if ((uint32_t)seg != c->rec_seg) {
             // replay the segment from its checkpoint, recording
+            replayed = true;
             if ((rc = ck_restore(c, c->ck[seg]))) return rc;
             for (uint32_t j = j0; j < j1; j++) {
                 c->rec_next = j + 1 < j1 ? &c->rec[j + 1 - j0] : nullptr;
@@ -1996,8 +2024,9 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
     // dL/dy_0 is now in the storage order of the recorded start state
     GradAosArgs out{c->n, 1, P<uint32_t>(c->ck[0].pid), d_grad_in, P<float>(c->LamIn[0]), P<float>(c->LamIn[1])};
     if ((rc = launch(c, K_GRAD_AOS, nblk(c->n), 256, out))) return rc;
-    // the live state goes back to the end of the recorded trajectory
-    if ((rc = ck_restore(c, fin))) return rc;
+    // the live state goes back to the end of the recorded trajectory (a sweep without replays
+    // left it there, with its kept neighbour lists)
+    if (replayed && (rc = ck_restore(c, fin))) return rc;
     if (grad_params) {
         std::vector<float> pg(2 * (size_t)c->nm);
         HIPCHK(c, hipMemcpyAsync(pg.data(), c->pgradSum.p, 4 * pg.size(), hipMemcpyDeviceToHost, c->stream));

@@ -78,10 +78,6 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
     sim.adjoint_record(dt, nsteps, nsteps)
     sim.adjoint_backward(g, gin, want_params=False)
     sim.synchronize()
-    adj_kernels = ["k_fluid_adj" + sfx for sfx in ("", "_rec", "_f", "_rec_f", "_g", "_rec_g")]
-    sim.profile_select(adj_kernels)
-    sim.profile(True)
-    sim.profile_reset()
     t0 = time.perf_counter()
     sim.adjoint_record(dt, nsteps, nsteps)
     sim.synchronize()
@@ -90,7 +86,14 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
     sim.synchronize()
     t2 = time.perf_counter()
     # the reverse stage (on the recorded neighbours, or searching after a record overflow;
-    # fluid-only / one-material / general table forms)
+    # fluid-only / one-material / general table forms), timed with events in another round
+    adj_kernels = ["k_fluid_adj" + sfx for sfx in ("", "_rec", "_f", "_rec_f", "_g", "_rec_g")]
+    sim.adjoint_record(dt, nsteps, nsteps)
+    sim.profile_select(adj_kernels)
+    sim.profile(True)
+    sim.profile_reset()
+    sim.adjoint_backward(g, gin, want_params=True)
+    sim.synchronize()
     ms, cnt = 0.0, 0
     for k in adj_kernels:
         m1, c1 = sim.profile_query(k)
@@ -158,6 +161,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--event-steps", type=int, default=50,
+                    help="steps of the separate pass that times the kernels with HIP events")
     ap.add_argument("--events", default="fluid", choices=["all", "fluid", "none"],
                     help="kernels timed with HIP events inside the timed region (fluid: the dominant "
                          "kernel only, which keeps the event overhead off the other launches)")
@@ -203,7 +208,9 @@ def main():
     sim.set_neighbor_skin(args.skin)
     if args.max_age is not None:
         sim.set_neighbor_max_age(args.max_age)
-    fluid_kernel = "k_fluid_nl" if args.skin > 0 else "k_fluid"
+    # the list walk (its fast-material form when the scene has one mobile and one boundary
+    # material), or the plain step without lists
+    fluid_kernels = ["k_fluid_nl", "k_fluid_nl_f"] if args.skin > 0 else ["k_fluid"]
     log("warmup %d steps" % args.warmup)
     sim.update(args.warmup)
     sim.synchronize()
@@ -214,9 +221,8 @@ def main():
             torch.cuda.synchronize()
     except Exception:
         pass
-    sim.profile_select([fluid_kernel] if args.events == "fluid" else None)
-    sim.profile(args.events != "none")
-    sim.profile_reset()
+    # the timed steps run without per-kernel events; a separate pass with events on the
+    # stream times the kernels
     nl0 = sim.neighbor_stats()
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
@@ -225,11 +231,20 @@ def main():
     t1 = time.perf_counter()
     sec = t1 - t0
     nl1 = sim.neighbor_stats()
+    sim.profile_select(fluid_kernels if args.events == "fluid" else None)
+    sim.profile(args.events != "none")
+    sim.profile_reset()
+    if args.events != "none":
+        log("kernel timing: %d steps with events" % args.event_steps)
+        sim.update(args.event_steps)
+        sim.synchronize()
     kern = {}
-    for k in ("k_fluid", "k_fluid_nl", "k_fluid_nlb", "k_rowsort", "k_scatter", "k_scan", "k_key"):
+    for k in ("k_fluid", "k_fluid_nl", "k_fluid_nl_f", "k_fluid_nlb", "k_rowsort", "k_scatter", "k_scan", "k_key"):
         ms, cnt = sim.profile_query(k)
         if cnt:
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
+    # the walk kernel that ran (most launches)
+    fluid_kernel = max(fluid_kernels, key=lambda k: kern.get(k, {}).get("launches", 0))
     sim.profile(False)
     fwd_adj = None
     ens = None
