@@ -65,6 +65,10 @@ struct KeyArgs {
     uint32_t* ranks;
     uint32_t* binCount;
     const uint32_t* nlw;  // neighbour-list words (may be null): nothing to do when the stage reuses
+    // when it works: copy the state (posd read above, velq) to these (may be null)
+    const float* cpyVelq;
+    float* cpyDstP;
+    float* cpyDstV;
 };
 
 constexpr int kMatLds = 16;  // material / interaction tables cached in LDS up to this many

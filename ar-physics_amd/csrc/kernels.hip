@@ -142,6 +142,10 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, uint32_t bid, uint32_
         uint32_t bin = 0, key = kInvalid;
         if (in) {
             float4 p = reinterpret_cast<const float4*>(a.posd)[i];
+            if (a.cpyDstP) {  // the state kept only in the sorted buffers goes back to the landing ones
+                reinterpret_cast<float4*>(a.cpyDstP)[i] = p;
+                reinterpret_cast<float4*>(a.cpyDstV)[i] = reinterpret_cast<const float4*>(a.cpyVelq)[i];
+            }
             uint32_t cx, cy, cz;
             if (cell_of(a.grid, p.x, p.y, p.z, cx, cy, cz)) {
                 uint32_t row = cy + a.grid.sub[1] * cz;

@@ -172,6 +172,9 @@ struct arp_ctx {
     DBuf mats, inter, bposd, bvelq, bmk, bP, rP, mkStatic, matByPid, sidByPid;
     // state
     DBuf Ysort[2], Yland[2];  // [0] posd, [1] velq
+    // the live state is only in Ysort (same storage order): a list-reusing RK step of several
+    // stages leaves y_{n+1} there alone; sync_yland copies it back before Yland is read
+    bool yland_stale = false;
     DBuf meta_mk[2], meta_pid[2];
     int cur = 0;
     DBuf Xb[2][2];  // [buffer][posd/velq]
@@ -561,9 +564,24 @@ struct SortIO {
     float* dstAuxV = nullptr;
 };
 
-int run_key(arp_ctx* c, const float* posd, const uint32_t* nlw = nullptr) {
+int run_key(arp_ctx* c, const float* posd, const uint32_t* nlw = nullptr, bool copy_back = false) {
     KeyArgs k{c->grid, c->n, posd, P<uint32_t>(c->keys), P<uint32_t>(c->ranks), P<uint32_t>(c->binCount), nlw};
+    if (copy_back) {  // the key pass of a rebuild from the sorted buffers restores the landing ones
+        k.cpyVelq = P<float>(c->Ysort[1]);
+        k.cpyDstP = P<float>(c->Yland[0]);
+        k.cpyDstV = P<float>(c->Yland[1]);
+    }
     return launch(c, K_KEY, nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, k);
+}
+
+// Yland <- Ysort when the live state is only in the sorted buffers
+int sync_yland(arp_ctx* c) {
+    if (!c->yland_stale) return ARP_OK;
+    if (c->capture) return fail(c, ARP_E_STATE, "state copy inside a captured step");
+    HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, c->Ysort[0].p, 16 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, c->Ysort[1].p, 16 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+    c->yland_stale = false;
+    return ARP_OK;
 }
 
 int run_sort(arp_ctx* c, const SortIO& io) {
@@ -961,6 +979,10 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     int rc;
     if (verlet && rec) return fail(c, ARP_E_STATE, "reverse mode covers the Runge-Kutta tableaus");
     if (!verlet) c->verlet_valid = false;
+    // a plain list-keeping step of several stages keeps y only in the sorted buffers; every
+    // other step reads the landing buffers
+    const bool lazy = !verlet && S > 1 && !rec && !c->capture && nl_enabled(c);
+    if (!lazy && (rc = sync_yland(c))) return rc;
     // which stage derivatives must be kept for later (non-subdiagonal) stage inputs
     int kslot[ARP_MAX_STAGES];
     int nslots = 0;
@@ -994,8 +1016,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
 
     // stage-1 sort: full reorder of y_n into cell-major storage (with lists kept, the key
     // pass runs at every stage that sorts, gated on the device)
+    const bool stale = c->yland_stale;  // (only on the lazy path)
     if (nl) {
-        if ((rc = run_key(c, P<float>(c->Yland[0]), nlw))) return rc;
+        if ((rc = run_key(c, stale ? P<float>(c->Ysort[0]) : P<float>(c->Yland[0]), nlw, stale))) return rc;
     } else if (!c->keys_valid) {
         if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
     }
@@ -1076,10 +1099,14 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
             r.Oposd = P<float>(c->Yland[0]);
             r.Ovelq = P<float>(c->Yland[1]);
             r.Oeps = P<float>(c->Yeps);  // in place: each solid's thread reads then writes
-            // with lists kept, y_{n+1} also goes to the sorted buffers, where a next step that
+            // with lists kept, y_{n+1} goes to the sorted buffers, where a next step that
             // reuses its lists (and so keeps the storage order) finds it; a thread overwrites
-            // only its own y_n there, and the stage's neighbours come from its own input
-            if (nl && S > 1 && !rec) {
+            // only its own y_n there, and the stage's neighbours come from its own input.
+            // The landing buffers are brought up to date when something reads them.
+            if (lazy) {
+                r.Oposd = Y0p;
+                r.Ovelq = Y0v;
+            } else if (nl && S > 1 && !rec) {
                 r.Oposd2 = Y0p;
                 r.Ovelq2 = Y0v;
             } else if (nl && S > 1 && rec && c->rec_next && c->rec_next != rec) {  // the next recorded step's input
@@ -1160,6 +1187,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         }
     }
     if (rec) c->slot = own;
+    c->yland_stale = lazy;
     c->keys_valid = !nl && !verlet;  // the plain step's last epilogue binned y_{n+1}
     c->rec_prefilled = (rec && nl && S > 1 && c->rec_next && c->rec_next != rec) ? c->rec_next->Xp[0].p : nullptr;
     c->have_build = false;
@@ -1242,6 +1270,7 @@ void free_rec(arp_ctx* c) { trim_rec(c, 0, 0); }
 int ck_save(arp_ctx* c, Checkpoint& k, bool full = true) {
     const size_t n = c->n;
     int rc;
+    if (full && (rc = sync_yland(c))) return rc;
     if ((rc = d2d(c, k.pid, c->meta_pid[c->cur], 4 * n))) return rc;
     if (full && ((rc = d2d(c, k.P, c->Yland[0], 16 * n)) || (rc = d2d(c, k.V, c->Yland[1], 16 * n)) ||
                  (rc = d2d(c, k.mk, c->meta_mk[c->cur], 4 * n))))
@@ -1257,6 +1286,7 @@ int ck_restore(arp_ctx* c, Checkpoint& k) {
     c->nl_dirty = true;
     c->verlet_valid = false;
     c->rec_prefilled = nullptr;
+    c->yland_stale = false;
     HIPCHK(c, hipMemcpyAsync(c->Yland[0].p, k.P.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, k.V.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->meta_mk[c->cur].p, k.mk.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1815,6 +1845,7 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
     c->nl_dirty = true;
     c->rec_prefilled = nullptr;
     c->verlet_valid = false;
+    c->yland_stale = false;
     HIPCHK(c, hipMemcpyAsync(c->aos_tmp.p, particles, sizeof(ArpParticle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     if (c->solids_active) {
         const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
@@ -2062,6 +2093,7 @@ int arp_download(arp_ctx* c, ArpParticle* particles, uint32_t n, ArpSolidParticl
     if ((rc = need_scene(c))) return rc;
     if (particles) {
         if (n != c->n) return fail(c, ARP_E_ARG, "download of %u particles, scene has %u", n, c->n);
+        if ((rc = sync_yland(c))) return rc;
         if ((rc = soa_to_aos(c, P<float>(c->Yland[0]), P<float>(c->Yland[1]), P<uint32_t>(c->meta_pid[c->cur]), c->aos_tmp.p)))
             return rc;
         HIPCHK(c, hipMemcpyAsync(particles, c->aos_tmp.p, sizeof(ArpParticle) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
@@ -2728,8 +2760,10 @@ int run_batch(arp_batch* b, float dt, uint32_t nsteps) {
             for (arp_ctx* c : b->ctx) c->nl_suspend = false;
         }
     } resume{b};
-    for (arp_ctx* c : b->ctx)  // outside the captured launches: reset stale list state
+    for (arp_ctx* c : b->ctx) {  // outside the captured launches: reset stale list state
         if (nl_enabled(c) && (rc = nl_prepare(c))) return rc;
+        if ((rc = sync_yland(c))) return rc;  // batched steps read and write the landing buffers
+    }
     if ((rc = build_plan(b, dt))) return rc;
     if ((rc = join_streams(b, true))) return rc;
     const bool graph = (b->flags & ARP_BATCH_GRAPH) && !L0->prof && nsteps >= 2;

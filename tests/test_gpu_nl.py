@@ -208,3 +208,35 @@ def test_batch_keeps_lists_per_scene(torch_cuda):
     B.close()
     for s in sims + solo:
         s.close()
+
+
+def test_reading_the_state_between_steps_changes_nothing(torch_cuda):
+    """A list-reusing step of several stages leaves y only in the sorted buffers; reading
+    the state (which brings the landing buffers up to date), recording a checkpoint of it or
+    switching to a one-stage method in between must not change the trajectory."""
+    from arp.sim import FluidSim
+
+    sc = _moving(S.dam_break_2d(), 1.0, 12)  # fast enough to rebuild between the reads
+    a = FluidSim(sc, integrator="rk4", timestep=1e-4)
+    b = FluidSim(sc, integrator="rk4", timestep=1e-4)
+    for s in (a, b):
+        s.set_neighbor_skin(0.1)
+    a.update(12)
+    for _ in range(4):
+        b.update(3)
+        b.particles()
+    pa, pb = a.particles(), b.particles()
+    assert _same(pa, pb)
+    assert a.neighbor_stats() == b.neighbor_stats() and a.neighbor_stats()[0] > 1
+    # a one-stage step after a lazy step reads the landing buffers
+    a.set_integrator("euler", 1e-4)
+    a.update(2)
+    c = FluidSim(sc, integrator="rk4", timestep=1e-4)
+    c.set_neighbor_skin(0.1)
+    c.update(12)
+    c.particles()
+    c.set_integrator("euler", 1e-4)
+    c.update(2)
+    assert _same(a.particles(), c.particles())
+    for s in (a, b, c):
+        s.close()
