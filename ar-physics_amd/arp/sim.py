@@ -171,11 +171,13 @@ class FluidSim:
         return out
 
     # ---- neighbour lists kept across stages --------------------------------------------
-    def set_neighbor_skin(self, skin=0.2, cap=0, always_build=False):
+    def set_neighbor_skin(self, skin=0.2, cap=0, always_build=False, speculate=True):
         """Lists of the candidates within h*(1+skin) kept across RK stages while every
         particle stays within skin*h/2 of its position at the search (0 disables; cap 0:
-        48 entries per particle).  always_build rebuilds at every stage (testing)."""
-        check(self.ctx, lib().arp_set_neighbor_skin(self.ctx, float(skin), int(cap), 1 if always_build else 0))
+        48 entries per particle).  always_build rebuilds at every stage (testing);
+        speculate=False issues the rebuild chain at every stage (same results)."""
+        flags = (1 if always_build else 0) | (0 if speculate else 2)
+        check(self.ctx, lib().arp_set_neighbor_skin(self.ctx, float(skin), int(cap), flags))
 
     def set_neighbor_max_age(self, stages):
         """Rebuild kept lists older than this many RK stages (0: only when invalid)."""

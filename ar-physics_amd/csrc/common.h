@@ -291,6 +291,30 @@ struct FluidArgs {
     // takes the materials of a pair from these (k_fluid_nl_f, k_fluid_nlr_f)
     uint32_t fastMat;
     uint32_t fastBnd;
+    // speculative walk (k_fluid_nl(_f) of a list-reusing step issued without the rebuild
+    // chain): the stage keeps the chain's bookkeeping itself (the stage counter, the build
+    // stamp, the refused count) and returns at once, recording specStep in the list words and
+    // in *hostAbort, when the lists are not valid for it; later stages then return at once
+    uint32_t spec;
+    uint32_t specGen;     // the stage's list counter value
+    uint32_t specStep;    // (step << 4) | stage
+    uint32_t* specSnap;   // first stage of a step: words[0..14], *stampW, statsW[0] saved here
+    uint32_t* hostAbort;  // host-mapped
+    uint32_t* stampW;
+    uint32_t* statsW;
+};
+
+// words of the neighbour-list state beyond the decision words
+constexpr int kNlAbort = 15;  // the aborting speculative step (0xFFFFFFFF: none)
+constexpr int kNlWords = 32;
+constexpr int kSpecSnap = 20;  // words per speculative step snapshot
+
+// Restore of the list words after an aborted speculative step (one thread).
+struct SpecRestoreArgs {
+    uint32_t* words;
+    const uint32_t* snap;  // words[0..14], stamp, stats (null: only the abort word and the counter)
+    uint32_t* stamp;
+    uint32_t* stats;
 };
 
 // Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists

@@ -628,10 +628,43 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
     // MODES 2, 3: the stage's build counter and whether it walks the kept lists
     uint32_t gen = 0;
     bool use = false;
+    const bool spec = WALK && !REC && a.spec != 0;
     if (NL) {
-        gen = a.nl.words[10];
-        use = nl_reuse(a.nl.words);
-        if (use != WALK) return;  // the other kernel of the pair runs this stage
+        if (spec) {
+            // a speculative stage: its counter value comes from the host, and the decision
+            // reads only the previous stage's words (the other parity from those this kernel
+            // writes), so every block takes the same one
+            if (a.nl.words[kNlAbort] != kInvalid) return;  // an earlier stage was not valid
+            gen = a.specGen;
+            {
+                const uint32_t* w = a.nl.words;
+                const uint32_t p = (gen - 1u) & 1u;
+                use = w[p] == gen - 1u && w[2 + p] != gen - 1u && (w[14] == 0u || w[12 + p] + 1u < w[14]);
+            }
+            if (!use) {
+                if (bid == 0 && tid == 0) {
+                    a.nl.words[kNlAbort] = a.specStep;
+                    __hip_atomic_store(a.hostAbort, a.specStep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __threadfence_system();
+                }
+                return;
+            }
+            if (bid == 0 && tid == 0) {
+                if (a.specSnap) {  // the words as the step found them
+                    for (int q = 0; q < 15; q++) a.specSnap[q] = a.nl.words[q];
+                    a.specSnap[15] = *a.stampW;
+                    a.specSnap[16] = a.statsW[0];
+                }
+                // the bookkeeping of the rebuild chain's reusing stage (no block reads these)
+                a.nl.words[10] = gen;
+                a.statsW[0] = 0u;
+                *a.stampW += 1u;
+            }
+        } else {
+            gen = a.nl.words[10];
+            use = nl_reuse(a.nl.words);
+            if (use != WALK) return;  // the other kernel of the pair runs this stage
+        }
         if (bid == 0 && tid == 0) {
             if (REC) *a.recUse = use ? 1u : 0u;
             a.nl.words[gen & 1u] = gen;
@@ -1106,6 +1139,17 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             a.rk.ranks[i] = rank;
         }
     }
+}
+
+// the list words as an aborted speculative step found them (one thread)
+extern "C" __global__ void __launch_bounds__(64) k_spec_restore(SpecRestoreArgs a) {
+    if (threadIdx.x != 0) return;
+    if (a.snap) {
+        for (int q = 0; q < 15; q++) a.words[q] = a.snap[q];
+        *a.stamp = a.snap[15];
+        a.stats[0] = a.snap[16];
+    }
+    a.words[kNlAbort] = kInvalid;
 }
 
 // ------------------------------------------------------------------------------------

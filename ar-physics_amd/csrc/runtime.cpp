@@ -76,6 +76,7 @@ enum Kern {
     K_VERLET_PRE,
     K_FLUID_NL_F,
     K_FLUID_NLR_F,
+    K_SPEC_RESTORE,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -88,7 +89,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_rec", "k_fluid_adj_rec", "k_fluid_nl", "k_fluid_nl_batch",
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
-                                   "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f"};
+                                   "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f",
+                                   "k_spec_restore"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -199,6 +201,14 @@ struct arp_ctx {
     float nl_skin = 0.2f;   // list radius h * (1 + skin); 0 disables
     uint32_t nl_cap = 48;   // entries per particle
     bool nl_always = false; // rebuild at every stage (testing)
+    bool nl_nospec = false; // issue the rebuild chain at every stage (no speculative steps)
+    // speculative steps: host-mapped abort word, per-step snapshots of the list words, and
+    // the events of the steps in flight
+    uint32_t* hAbort = nullptr;
+    uint32_t* hAbortDev = nullptr;
+    DBuf nlSnap;
+    hipEvent_t specEv[4] = {};
+    uint32_t spec_aborts = 0;
     uint32_t nl_max_age = 0;  // rebuild lists older than this many stages (0: never for age)
     bool nl_dirty = true;   // the decision words and bucket counts must be reset
     int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
@@ -714,11 +724,15 @@ int nl_prepare(arp_ctx* c) {
     int rc;
     const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
     const bool fresh = c->nlWords.p == nullptr;
-    if ((rc = dalloc(c, c->nlWords, 64)) || (rc = dalloc(c, c->nlList, 4 * n * c->nl_cap)) ||
+    if ((rc = dalloc(c, c->nlWords, 4 * kNlWords)) || (rc = dalloc(c, c->nlList, 4 * n * c->nl_cap)) ||
         (rc = dalloc(c, c->nlCount, 4 * n)) || (rc = dalloc(c, c->nlRef, 16 * n)) ||
         (rc = dalloc(c, c->nlCell, 8 * (size_t)c->ncells)) || (rc = dalloc(c, c->nlCellRec, 8 * n)))
         return rc;
-    if (fresh && (rc = fill_u32(c, c->nlWords, 16, 0))) return rc;
+    if (fresh) {
+        if ((rc = fill_u32(c, c->nlWords, kNlWords, 0))) return rc;
+        SpecRestoreArgs sr{P<uint32_t>(c->nlWords), nullptr, nullptr, nullptr};  // no abort pending
+        if ((rc = launch(c, K_SPEC_RESTORE, 1, 64, sr))) return rc;
+    }
     if (c->keys_valid) {  // the plain step's epilogue binned the state; the list step bins per stage
         if ((rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0))) return rc;
         c->keys_valid = false;
@@ -727,6 +741,8 @@ int nl_prepare(arp_ctx* c) {
         if ((rc = fill_u32(c, c->nlWords, 7, kInvalid))) return rc;
         FillArgs fa{P<uint32_t>(c->nlWords) + 14, 1, c->nl_max_age};
         if ((rc = launch(c, K_FILL, 1, 256, fa))) return rc;
+        SpecRestoreArgs sr{P<uint32_t>(c->nlWords), nullptr, nullptr, nullptr};  // no abort pending
+        if ((rc = launch(c, K_SPEC_RESTORE, 1, 64, sr))) return rc;
         if ((rc = fill_u32(c, c->nlCell, (uint32_t)(2 * c->ncells), 0))) return rc;
         c->nl_dirty = false;
     }
@@ -972,7 +988,14 @@ int verlet_prepare(arp_ctx* c, float dt) {
     return launch(c, K_VERLET_PRE, nblk(std::max(c->n, v.ns)), 256, v);
 }
 
-int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
+// A speculative step (SpecArg): a list-reusing step issued as its walk kernels alone.
+struct SpecArg {
+    uint32_t step;  // index among the steps in flight (the abort word records it)
+    uint32_t slot;  // its snapshot slot
+    uint32_t gen;   // the list counter value before its first stage
+};
+
+int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = nullptr) {
     const ArpTableau& T = c->tab;
     const bool verlet = T.stages == ARP_VERLET;
     const int S = verlet ? 1 : T.stages;
@@ -983,6 +1006,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     // other step reads the landing buffers
     const bool lazy = !verlet && S > 1 && !rec && !c->capture && nl_enabled(c);
     if (!lazy && (rc = sync_yland(c))) return rc;
+    if (sp && !lazy) return fail(c, ARP_E_STATE, "speculative step outside the list-reusing path");
     // which stage derivatives must be kept for later (non-subdiagonal) stage inputs
     int kslot[ARP_MAX_STAGES];
     int nslots = 0;
@@ -1017,7 +1041,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     // stage-1 sort: full reorder of y_n into cell-major storage (with lists kept, the key
     // pass runs at every stage that sorts, gated on the device)
     const bool stale = c->yland_stale;  // (only on the lazy path)
-    if (nl) {
+    if (sp) {
+        // (the stage-1 sort of a reusing step does nothing: no key pass)
+    } else if (nl) {
         if ((rc = run_key(c, stale ? P<float>(c->Ysort[0]) : P<float>(c->Yland[0]), nlw, stale))) return rc;
     } else if (!c->keys_valid) {
         if ((rc = run_key(c, P<float>(c->Yland[0])))) return rc;
@@ -1044,7 +1070,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
         io.dstAuxP = P<float>(c->XF[1][0]);
         io.dstAuxV = P<float>(c->XF[1][1]);
     }
-    if ((rc = run_sort(c, io))) return rc;
+    if (!sp && (rc = run_sort(c, io))) return rc;
     c->cur = nx;
     c->keys_valid = false;
     if (rec) {
@@ -1055,7 +1081,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
     const float* Xv = verlet ? P<float>(c->XF[1][1]) : Y0v;
     const float* Xe = verlet ? P<float>(c->XFe) : P<float>(c->Yeps);
     for (int s = 0; s < S; s++) {
-        if (s > 0) {
+        if (s > 0 && !sp) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
                        nullptr, nullptr, nullptr, nullptr, nullptr, nlw};
             if (rec) c->slot = rec->slot[s];
@@ -1162,7 +1188,17 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr) {
                             c->bnd_mat != -2;
             f.fastMat = fm ? (uint32_t)c->mobile_mat : 0u;
             f.fastBnd = fm ? (uint32_t)(c->bnd_mat >= 0 ? c->bnd_mat : c->mobile_mat) : 0u;
-            if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) return rc;
+            if (sp) {  // the walk alone, keeping the chain's bookkeeping
+                f.spec = 1;
+                f.specGen = sp->gen + (uint32_t)s + 1u;
+                f.specStep = (sp->step << 4) | (uint32_t)s;
+                f.specSnap = s == 0 ? P<uint32_t>(c->nlSnap) + (size_t)kSpecSnap * sp->slot : nullptr;
+                f.hostAbort = c->hAbortDev;
+                f.stampW = c->slot.stamp;
+                f.statsW = P<uint32_t>(c->stats);
+            } else if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) {
+                return rc;
+            }
             const Kern walk = rec ? (fm ? K_FLUID_NLR_F : K_FLUID_NLR) : (fm ? K_FLUID_NL_F : K_FLUID_NL);
             if ((rc = launch(c, walk, nblk(c->n), 256, f))) return rc;
         } else if ((rc = launch(c, rec ? K_FLUID_REC : K_FLUID, nblk(c->n), 256, f))) {
@@ -1544,8 +1580,11 @@ void arp_destroy(arp_ctx* c) {
         if (b->p) hipFree(b->p);
     free_rec(c);
     for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->LamNext[0], &c->LamNext[1],
-                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->kscale, &c->gaos, &c->recFlags})
+                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->kscale, &c->gaos, &c->recFlags, &c->nlSnap})
         free_buf(*b);
+    if (c->hAbort) hipHostFree(c->hAbort);
+    for (hipEvent_t e : c->specEv)
+        if (e) hipEventDestroy(e);
     for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(c->Gb[s][0]), free_buf(c->Gb[s][1]);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->mod) hipModuleUnload(c->mod);
@@ -1882,27 +1921,115 @@ int arp_set_integrator(arp_ctx* c, const ArpTableau* t, float timestep, uint32_t
     return ARP_OK;
 }
 
+// Steps of a plain list-keeping multi-stage method are issued speculatively: each stage as
+// its walk kernel alone (no rebuild chain), kSpecW steps in flight.  A stage whose lists are
+// not valid records the step in a host-mapped word and every later stage returns at once;
+// the host then restores the list words the step began with and runs that step with the
+// rebuild chain.  The result is the one of issuing the chain at every stage.
+constexpr uint32_t kSpecW = 4;
+
+static bool spec_ok(arp_ctx* c) {
+    const ArpTableau& T = c->tab;
+    return !c->nl_nospec && !c->capture && !c->nl_always && c->nl_max_age == 0 && T.stages > 1 &&
+           nl_enabled(c) && !c->nl_dirty && c->nlWords.p;
+}
+
+static int spec_setup(arp_ctx* c) {
+    if (!c->hAbort) {
+        HIPCHK(c, hipHostMalloc((void**)&c->hAbort, 64, hipHostMallocMapped));
+        HIPCHK(c, hipHostGetDevicePointer((void**)&c->hAbortDev, c->hAbort, 0));
+        for (hipEvent_t& e : c->specEv) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    return dalloc(c, c->nlSnap, 4 * (size_t)kSpecSnap * kSpecW);
+}
+
+static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
+    int rc;
+    uint32_t k = 0;
+    while (k < nsteps) {
+        if (!spec_ok(c)) {
+            if ((rc = step_once(c, dt))) return rc;
+            c->time += dt;
+            k++;
+            continue;
+        }
+        if ((rc = spec_setup(c))) return rc;
+        __atomic_store_n(c->hAbort, kInvalid, __ATOMIC_RELEASE);
+        // the list counter the speculative stages continue from
+        uint32_t gen0 = 0;
+        HIPCHK(c, hipMemcpyAsync(&gen0, P<uint32_t>(c->nlWords) + 10, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const uint32_t S = (uint32_t)c->tab.stages;
+        struct Host {
+            int cur;
+            bool keys_valid, have_build, yland_stale;
+        } hs[kSpecW];
+        const uint32_t m = nsteps - k;
+        uint32_t issued = 0, done = 0;
+        bool aborted = false;
+        while (done < m) {
+            while (issued < m && issued - done < kSpecW) {
+                hs[issued % kSpecW] = Host{c->cur, c->keys_valid, c->have_build, c->yland_stale};
+                const SpecArg sa{issued, issued % kSpecW, gen0 + issued * S};
+                if ((rc = step_once(c, dt, nullptr, &sa))) {
+                    hipStreamSynchronize(c->stream);
+                    return rc;
+                }
+                HIPCHK(c, hipEventRecord(c->specEv[issued % kSpecW], c->stream));
+                issued++;
+            }
+            HIPCHK(c, hipEventSynchronize(c->specEv[done % kSpecW]));
+            if (__atomic_load_n(c->hAbort, __ATOMIC_ACQUIRE) != kInvalid) {
+                aborted = true;
+                break;
+            }
+            c->time += dt;
+            done++;
+        }
+        if (!aborted) {
+            k += m;
+            continue;
+        }
+        // every issued step has run (those after the abort did nothing)
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const uint32_t code = __atomic_load_n(c->hAbort, __ATOMIC_ACQUIRE);
+        const uint32_t a = code >> 4, stage = code & 15u;
+        if (a < done || a >= issued) return fail(c, ARP_E_STATE, "speculative step %u out of range", a);
+        for (; done < a; done++) c->time += dt;
+        const Host& h = hs[a % kSpecW];
+        c->cur = h.cur;
+        c->keys_valid = h.keys_valid;
+        c->have_build = h.have_build;
+        c->yland_stale = h.yland_stale;
+        // the list words as the step found them (a first stage that aborted changed nothing)
+        SpecRestoreArgs sr{P<uint32_t>(c->nlWords),
+                           stage > 0 ? P<uint32_t>(c->nlSnap) + (size_t)kSpecSnap * (a % kSpecW) : nullptr,
+                           c->slot.stamp, P<uint32_t>(c->stats)};
+        if ((rc = launch(c, K_SPEC_RESTORE, 1, 64, sr))) return rc;
+        __atomic_store_n(c->hAbort, kInvalid, __ATOMIC_RELEASE);
+        c->spec_aborts++;
+        k += a;
+        // that step with the rebuild chain
+        if ((rc = step_once(c, dt))) return rc;
+        c->time += dt;
+        k++;
+    }
+    return ARP_OK;
+}
+
 int arp_step(arp_ctx* c, float dt, uint32_t nsteps) {
     int rc;
     if ((rc = need_scene(c))) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    for (uint32_t s = 0; s < nsteps; s++) {
-        if ((rc = step_once(c, dt))) return rc;
-        c->time += dt;
-    }
-    return ARP_OK;
+    return run_steps(c, dt, nsteps);
 }
 
 int arp_update(arp_ctx* c, uint32_t nframes) {
     int rc;
     if ((rc = need_scene(c))) return rc;
     const float dt = c->timestep / (float)c->subticks;
-    for (uint32_t f = 0; f < nframes; f++)
-        for (uint32_t s = 0; s < c->subticks; s++) {
-            if ((rc = step_once(c, dt))) return rc;
-            c->time += dt;
-        }
-    return ARP_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    return run_steps(c, dt, nframes * c->subticks);
 }
 
 double arp_time(const arp_ctx* c) { return c ? c->time : 0.0; }
@@ -2332,6 +2459,7 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
     c->nl_skin = skin;
     c->nl_cap = cap;
     c->nl_always = (flags & ARP_NL_ALWAYS_BUILD) != 0;
+    c->nl_nospec = (flags & ARP_NL_NO_SPECULATION) != 0;
     c->nl_dirty = true;
     if (c->nlWords.p) {
         HIPCHK(c, hipSetDevice(c->device));

@@ -151,7 +151,10 @@ int arp_scene_upload(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
 int arp_set_integrator(arp_ctx* ctx, const ArpTableau* tableau, float timestep,
                        uint32_t subticks);
 /* FluidSim::update (soft_body/mod.rs:235-274), called nframes times: each frame advances
- * subticks steps of timestep/subticks.  The fused fast path (sort, gather, RK stages). */
+ * subticks steps of timestep/subticks.  The fused fast path (sort, gather, RK stages).
+ * With neighbour lists kept and a method of several stages, steps are issued speculatively
+ * (each stage as its list walk alone, a few steps in flight; a step whose lists turn out
+ * invalid is run again with the rebuild chain), and the call returns once they have run. */
 int arp_update(arp_ctx* ctx, uint32_t nframes);
 /* Advance nsteps single RK steps of length dt (no subtick division). */
 int arp_step(arp_ctx* ctx, float dt, uint32_t nsteps);
@@ -175,6 +178,9 @@ int arp_upload_state(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
  * (default 0.2); cap = entries per particle (0: 48; a list that overflows is not kept).
  * ARP_NL_ALWAYS_BUILD rebuilds at every stage (testing).  Resets the statistics. */
 #define ARP_NL_ALWAYS_BUILD 1u
+/* Issue the rebuild chain at every stage instead of speculative list-reusing steps (the
+ * results are the same; for comparison and debugging). */
+#define ARP_NL_NO_SPECULATION 2u
 int arp_set_neighbor_skin(arp_ctx* ctx, float skin, uint32_t cap, uint32_t flags);
 /* Rebuild lists older than this many RK stages even when they are still valid (0: only when
  * invalid).  A build re-sorts the particles into bucket order, which keeps the neighbour

@@ -240,3 +240,29 @@ def test_reading_the_state_between_steps_changes_nothing(torch_cuda):
     assert _same(a.particles(), c.particles())
     for s in (a, b, c):
         s.close()
+
+
+@pytest.mark.parametrize("integ,sigma", [("rk4", 1.0), ("rk2", 2.0), ("heun3", 0.1)])
+def test_speculative_steps_match_the_rebuild_chain(torch_cuda, integ, sigma):
+    """Steps issued speculatively (each stage as its walk alone) give bitwise the state and
+    statistics of issuing the rebuild chain at every stage, also when lists turn out invalid
+    in the middle of a step and the step is run again."""
+    from arp.sim import FluidSim
+
+    sc = _moving(S.dam_break_2d(), sigma, 21)
+    print("case nl speculation", integ, sigma, flush=True)
+    a = FluidSim(sc, integrator=integ, timestep=1e-4)
+    b = FluidSim(sc, integrator=integ, timestep=1e-4)
+    a.set_neighbor_skin(0.1)
+    b.set_neighbor_skin(0.1, speculate=False)
+    a.update(9)
+    a.update(5)
+    b.update(14)
+    assert _same(a.particles(), b.particles())
+    assert a.neighbor_stats() == b.neighbor_stats()
+    builds, reused = a.neighbor_stats()
+    print("  builds %d reused %d" % (builds, reused), flush=True)
+    assert reused > 0
+    assert a.time == pytest.approx(b.time)
+    for s in (a, b):
+        s.close()
