@@ -1925,8 +1925,11 @@ int arp_set_integrator(arp_ctx* c, const ArpTableau* t, float timestep, uint32_t
 // its walk kernel alone (no rebuild chain), kSpecW steps in flight.  A stage whose lists are
 // not valid records the step in a host-mapped word and every later stage returns at once;
 // the host then restores the list words the step began with and runs that step with the
-// rebuild chain.  The result is the one of issuing the chain at every stage.
+// rebuild chain.  The result is the one of issuing the chain at every stage.  The call
+// returns once its steps have run, so calls of fewer than kSpecMin steps (one frame at a
+// time, say) stay asynchronous and issue the chain.
 constexpr uint32_t kSpecW = 4;
+constexpr uint32_t kSpecMin = 8;
 
 static bool spec_ok(arp_ctx* c) {
     const ArpTableau& T = c->tab;
@@ -1947,7 +1950,7 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
     int rc;
     uint32_t k = 0;
     while (k < nsteps) {
-        if (!spec_ok(c)) {
+        if (nsteps - k < kSpecMin || !spec_ok(c)) {
             if ((rc = step_once(c, dt))) return rc;
             c->time += dt;
             k++;

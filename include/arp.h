@@ -152,9 +152,10 @@ int arp_set_integrator(arp_ctx* ctx, const ArpTableau* tableau, float timestep,
                        uint32_t subticks);
 /* FluidSim::update (soft_body/mod.rs:235-274), called nframes times: each frame advances
  * subticks steps of timestep/subticks.  The fused fast path (sort, gather, RK stages).
- * With neighbour lists kept and a method of several stages, steps are issued speculatively
- * (each stage as its list walk alone, a few steps in flight; a step whose lists turn out
- * invalid is run again with the rebuild chain), and the call returns once they have run. */
+ * With neighbour lists kept, a method of several stages and at least 8 steps in the call,
+ * steps are issued speculatively (each stage as its list walk alone, a few steps in flight;
+ * a step whose lists turn out invalid is run again with the rebuild chain), and the call
+ * returns once they have run; shorter calls return without waiting. */
 int arp_update(arp_ctx* ctx, uint32_t nframes);
 /* Advance nsteps single RK steps of length dt (no subtick division). */
 int arp_step(arp_ctx* ctx, float dt, uint32_t nsteps);
