@@ -1937,13 +1937,24 @@ static bool spec_ok(arp_ctx* c) {
            nl_enabled(c) && !c->nl_dirty && c->nlWords.p;
 }
 
-static int spec_setup(arp_ctx* c) {
+// The host-mapped abort word and the events; false (and speculation off for the context)
+// when the runtime cannot provide them.
+static bool spec_setup(arp_ctx* c) {
     if (!c->hAbort) {
-        HIPCHK(c, hipHostMalloc((void**)&c->hAbort, 64, hipHostMallocMapped));
-        HIPCHK(c, hipHostGetDevicePointer((void**)&c->hAbortDev, c->hAbort, 0));
-        for (hipEvent_t& e : c->specEv) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        bool ok = hipHostMalloc((void**)&c->hAbort, 64, hipHostMallocMapped) == hipSuccess;
+        ok = ok && hipHostGetDevicePointer((void**)&c->hAbortDev, c->hAbort, 0) == hipSuccess;
+        for (hipEvent_t& e : c->specEv) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            c->nl_nospec = true;
+            return false;
+        }
     }
-    return dalloc(c, c->nlSnap, 4 * (size_t)kSpecSnap * kSpecW);
+    if (dalloc(c, c->nlSnap, 4 * (size_t)kSpecSnap * kSpecW) != ARP_OK) {
+        c->nl_nospec = true;
+        return false;
+    }
+    return true;
 }
 
 static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
@@ -1956,7 +1967,7 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
             k++;
             continue;
         }
-        if ((rc = spec_setup(c))) return rc;
+        if (!spec_setup(c)) continue;  // (the steps are then issued with the rebuild chain)
         __atomic_store_n(c->hAbort, kInvalid, __ATOMIC_RELEASE);
         // the list counter the speculative stages continue from
         uint32_t gen0 = 0;
