@@ -874,7 +874,10 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const uint32_t cw = a.nl.count[i];
             const uint32_t fa = cw & 0xFFFFu, fb = cw >> 16, m = fa + fb, cap = a.nl.cap;
             const uint32_t* L = a.nl.list;
-            const size_t n = a.n;
+            // 32-bit element offsets (the host keeps cap x n below 2^32): entry k at k n + i,
+            // the back entries (k >= fa) cap - fb - fa further on
+            const uint32_t n32 = a.n, jump = (cap - fb - fa) * n32;
+            uint32_t offk = i;
             uint32_t k = 0, nl = 0;
             for (;;) {
                 while (nl + ARP_NLB <= (uint32_t)kList && k < m) {
@@ -882,10 +885,15 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) {
                         const uint32_t kk = k + u;
-                        const uint32_t pos = kk < fa ? kk : cap - fb + (kk - fa);
                         ok[u] = kk < m ? 1u : 0u;
-                        e[u] = ok[u] ? L[(size_t)pos * n + i] : 0u;
+                        // unconditional loads: a lane past its count reads its entry 0
+                        const uint32_t off = ok[u] ? offk + (uint32_t)u * n32 + (kk >= fa ? jump : 0u) : i;
+                        // (a 32-bit byte offset from the uniform base: an SGPR-based load)
+                        const uint32_t v = *reinterpret_cast<const uint32_t*>(
+                            reinterpret_cast<const char*>(L) + off * 4u);
+                        e[u] = ok[u] ? v : 0u;
                     }
+                    offk += (uint32_t)ARP_NLB * n32;
                     float4 xj[ARP_NLB];
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) xj[u] = ((e[u] & kBnd) ? B : X)[e[u] & ~kBnd];

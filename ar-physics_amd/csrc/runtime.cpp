@@ -717,7 +717,9 @@ float box_extent(const GridP& g) {
 
 // Lists are kept for fluid-only scenes (the solid path searches the grid itself).
 bool nl_enabled(const arp_ctx* c) {
-    return c->nl_skin > 0.0f && !c->solids_active && c->n != 0 && !c->nl_suspend;
+    // (the walks address list entries with 32-bit offsets)
+    return c->nl_skin > 0.0f && !c->solids_active && c->n != 0 && !c->nl_suspend &&
+           (uint64_t)c->nl_cap * ((uint64_t)c->n + kPad) < (1ull << 32);
 }
 
 int nl_prepare(arp_ctx* c) {
