@@ -846,13 +846,15 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         // pair waits only for its own loads)
         auto drain = [&](uint32_t nl) {
             if (WALK || ARP_PP_ALL) {
+                // (the next loads are unconditional: a lane at its last pair reloads its
+                // first entry rather than branching around the load)
                 PairIn pa, pb;
                 if (nl) pa = load(list[tid]);
                 for (uint32_t k = 0; k < nl; k += 2) {
-                    if (k + 1 < nl) pb = load(list[(k + 1) * 256 + tid]);
+                    pb = load(list[(k + 1 < nl ? k + 1 : 0u) * 256 + tid]);
                     pair(pa);
                     if (k + 1 >= nl) break;
-                    if (k + 2 < nl) pa = load(list[(k + 2) * 256 + tid]);
+                    pa = load(list[(k + 2 < nl ? k + 2 : 0u) * 256 + tid]);
                     pair(pb);
                 }
             } else {  // one register set (the search kernels keep five waves per SIMD)

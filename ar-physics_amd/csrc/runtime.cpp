@@ -2079,10 +2079,11 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     for (uint32_t j = 0; j < nsteps; j++) {
         if (j % K == 0) {
             if ((rc = ck_save(c, c->ck[j / K], nseg > 1))) return rc;
-            // kept neighbour lists restart with every segment that the reverse sweep
-            // replays (all but the last), as its replay from the checkpoint does, so that
-            // the replay repeats the recorded steps exactly
-            if (j / K + 1 < nseg) c->nl_dirty = true;
+            // kept neighbour lists restart with every segment of a recording that the
+            // reverse sweep replays (one of several segments, the last included: a second
+            // sweep replays it too), as its replay from the checkpoint does, so that every
+            // replay repeats the recorded steps exactly
+            if (nseg > 1) c->nl_dirty = true;
         }
         c->rec_next = j + 1 < nsteps ? &c->rec[(j + 1) % K] : nullptr;
         rc = step_once(c, dt, &c->rec[j % K]);
