@@ -2117,8 +2117,12 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                             break;
                         }
                         const uint32_t take = min(rcnt - rpos, (uint32_t)kList - nl);
+                        // 32-bit byte offsets from the uniform base (the host keeps the
+                        // record below 4 GB)
+                        const char* nb = reinterpret_cast<const char*>(a.nbList);
                         for (uint32_t k = 0; k < take; k++)
-                            list[(nl + k) * 256 + tid] = a.nbList[(size_t)(rpos + k) * a.n + i];
+                            list[(nl + k) * 256 + tid] =
+                                *reinterpret_cast<const uint32_t*>(nb + ((rpos + k) * a.n + i) * 4u);
                         nl += take;
                         rpos += take;
                         continue;
@@ -2309,6 +2313,8 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                     }
                 };
                 {
+                    // (the next loads are unconditional: a lane at its last pair reloads its
+                    // first entry rather than branching around the load)
                     AdjIn pa, pbn;
                     uint32_t ea = 0, eb = 0;
                     if (nl) {
@@ -2316,16 +2322,12 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                         pa = ld(ea);
                     }
                     for (uint32_t k = 0; k < nl; k += 2) {
-                        if (k + 1 < nl) {
-                            eb = list[(k + 1) * 256 + tid];
-                            pbn = ld(eb);
-                        }
+                        eb = list[(k + 1 < nl ? k + 1 : 0u) * 256 + tid];
+                        pbn = ld(eb);
                         proc(ea, pa);
                         if (k + 1 >= nl) break;
-                        if (k + 2 < nl) {
-                            ea = list[(k + 2) * 256 + tid];
-                            pa = ld(ea);
-                        }
+                        ea = list[(k + 2 < nl ? k + 2 : 0u) * 256 + tid];
+                        pa = ld(ea);
                         proc(eb, pbn);
                     }
                 }

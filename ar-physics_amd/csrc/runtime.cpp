@@ -2064,6 +2064,8 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
         }
     if (nsteps == 0 || checkpoint_every == 0 || !(dt > 0.0f) || !std::isfinite(dt))
         return fail(c, ARP_E_ARG, "adjoint_record needs nsteps >= 1, checkpoint_every >= 1 and dt > 0");
+    if ((uint64_t)rec_cap(c) * ((uint64_t)c->n + kPad) * 4u >= (1ull << 32))
+        return fail(c, ARP_E_LIMIT, "the neighbour record of %u particles exceeds 4 GB per stage", c->n);
     HIPCHK(c, hipSetDevice(c->device));
     const uint32_t K = std::min(checkpoint_every, nsteps);
     const uint32_t nseg = (nsteps + K - 1) / K;
