@@ -390,7 +390,17 @@ class Parser {
         v.i = (int64_t)x;
     }
 
+    int depth = 0;  // nesting of arrays and inline tables (bounded: the parser recurses)
+    struct Nest {
+        Parser& p;
+        explicit Nest(Parser& q) : p(q) {
+            if (++p.depth > 128) p.fail("arrays or inline tables nested more than 128 deep");
+        }
+        ~Nest() { --p.depth; }
+    };
+
     VPtr value() {
+        Nest nest(*this);
         auto v = std::make_shared<Value>();
         v->defined = true;
         char c = peek();

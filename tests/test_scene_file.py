@@ -263,6 +263,10 @@ def test_toml_reader_refuses_what_scenes_never_hold():
     for doc in ('a = 1979-05-27\n', 't = 07:32:00\n', 'a = {b = 1,\nc = 2}\n'):
         with pytest.raises(ArpError, match="TOML line"):
             K.toml_to_json(doc)
+    # nesting is bounded (the reader recurses); 100 levels still read
+    with pytest.raises(ArpError, match="nested"):
+        K.toml_to_json("a = " + "[" * 200 + "]" * 200 + "\n")
+    assert K.toml_to_json("a = " + "[" * 100 + "]" * 100 + "\n")["a"] is not None
 
 
 # ---------------------------------------------------------------------------------------
