@@ -17,7 +17,7 @@ SYMBOLS = [
     "arp_profile_reset", "arp_adjoint_record", "arp_adjoint_backward",
     "arp_profile_select", "arp_batch_create", "arp_batch_destroy", "arp_batch_last_error", "arp_batch_step",
     "arp_batch_tune", "arp_set_stiffness_field", "arp_adjoint_stiffness_grad",
-    "arp_set_neighbor_skin", "arp_neighbor_stats", "arp_set_neighbor_max_age",
+    "arp_set_neighbor_skin", "arp_neighbor_stats", "arp_speculation_stats", "arp_set_neighbor_max_age",
     "arp_scene_parse", "arp_scene_free", "arp_scene_error", "arp_scene_settings", "arp_scene_object",
     "arp_scene_generate", "arp_scene_arrays", "arp_scene_contains", "arp_scene_apply", "arp_toml_to_json",
     "arp_kernel_radius", "arp_num_materials", "arp_add_particles", "arp_scene_spawn", "arp_num_particles",
@@ -93,6 +93,7 @@ def lib():
     L.arp_set_neighbor_skin.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_set_neighbor_max_age.argtypes = [vp, C.c_uint32]
     L.arp_neighbor_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.arp_speculation_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_adjoint_backward.argtypes = [vp, vp, vp, vp]
     L.arp_scene_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_size_t]

@@ -189,6 +189,12 @@ class FluidSim:
         check(self.ctx, lib().arp_neighbor_stats(self.ctx, C.byref(b), C.byref(r)))
         return int(b.value), int(r.value)
 
+    def speculation_stats(self):
+        """(speculative steps completed, speculative steps run again with the rebuild chain)."""
+        s, a = C.c_uint64(0), C.c_uint64(0)
+        check(self.ctx, lib().arp_speculation_stats(self.ctx, C.byref(s), C.byref(a)))
+        return int(s.value), int(a.value)
+
     # ---- diagnostics ------------------------------------------------------------------
     def grid_info(self):
         g = np.zeros((), wire.GRID_INFO)

@@ -291,10 +291,11 @@ struct FluidArgs {
     // takes the materials of a pair from these (k_fluid_nl_f, k_fluid_nlr_f)
     uint32_t fastMat;
     uint32_t fastBnd;
-    // speculative walk (k_fluid_nl(_f) of a list-reusing step issued without the rebuild
-    // chain): the stage keeps the chain's bookkeeping itself (the stage counter, the build
-    // stamp, the refused count) and returns at once, recording specStep in the list words and
-    // in *hostAbort, when the lists are not valid for it; later stages then return at once
+    // speculative walk (k_fluid_nl(_f), k_fluid_nlr(_f) of a list-reusing step issued without
+    // the rebuild chain): the stage keeps the chain's bookkeeping itself (the stage counter,
+    // the build stamp, the refused count) and returns at once, recording specStep in the list
+    // words and in *hostAbort, when the lists are not valid for it; later stages then return
+    // at once
     uint32_t spec;
     uint32_t specGen;     // the stage's list counter value
     uint32_t specStep;    // (step << 4) | stage
@@ -302,12 +303,20 @@ struct FluidArgs {
     uint32_t* hostAbort;  // host-mapped
     uint32_t* stampW;
     uint32_t* statsW;
+    // first stage of a speculative recorded step: the step's record flags to clear
+    // (kRecFlags words) and its storage-order records (identity source index, mk,
+    // pid), which the rebuild chain's reusing sort writes otherwise
+    uint32_t* specClear;
+    uint32_t* recSrc;
+    uint32_t* recMk;
+    uint32_t* recPid;
 };
 
 // words of the neighbour-list state beyond the decision words
 constexpr int kNlAbort = 15;  // the aborting speculative step (0xFFFFFFFF: none)
 constexpr int kNlWords = 32;
 constexpr int kSpecSnap = 20;  // words per speculative step snapshot
+constexpr int kRecFlags = 12;  // a recorded step's flags: overflow and list use per stage (2 * 6 stages)
 
 // Restore of the list words after an aborted speculative step (one thread).
 struct SpecRestoreArgs {
@@ -315,6 +324,10 @@ struct SpecRestoreArgs {
     const uint32_t* snap;  // words[0..14], stamp, stats (null: only the abort word and the counter)
     uint32_t* stamp;
     uint32_t* stats;
+    // a recorded step keeps one build stamp per stage: those of its later stages that ran
+    // (each advanced by one) go back by one
+    uint32_t* stampDec;
+    uint32_t ndec;
 };
 
 // Solid path (strain.rs, solid.rs): static per-solid reference-neighbour lists

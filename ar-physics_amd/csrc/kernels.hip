@@ -628,7 +628,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
     // MODES 2, 3: the stage's build counter and whether it walks the kept lists
     uint32_t gen = 0;
     bool use = false;
-    const bool spec = WALK && !REC && a.spec != 0;
+    const bool spec = WALK && a.spec != 0;
     if (NL) {
         if (spec) {
             // a speculative stage: its counter value comes from the host, and the decision
@@ -659,6 +659,18 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                 a.nl.words[10] = gen;
                 a.statsW[0] = 0u;
                 *a.stampW += 1u;
+                // a recorded step's flags start at zero (a walk never overflows the record,
+                // whose capacity is at least the lists', so no other block writes them)
+                if (REC && a.specClear)
+                    for (int q = 0; q < kRecFlags; q++) a.specClear[q] = 0u;
+            }
+            if (REC && a.recSrc) {  // the reusing sort's records: storage order unchanged
+                const uint32_t t0 = bid * 256u + tid;
+                if (t0 < a.n) {
+                    a.recSrc[t0] = t0;
+                    a.recMk[t0] = a.mk[t0];
+                    a.recPid[t0] = a.pidIn[t0];
+                }
             }
         } else {
             gen = a.nl.words[10];
@@ -909,8 +921,13 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     k += ARP_NLB;
                 }
                 if (REC) {  // the buffered entries are the neighbours within h
-                    for (uint32_t q = 0; q < nl; q++)
-                        if (nrec + q < a.nbCap) a.nbList[(size_t)(nrec + q) * a.n + i] = list[q * 256 + tid];
+                    // (32-bit byte offsets from the uniform base: the host keeps the record
+                    // below 4 GB)
+                    const uint32_t nw = nrec < a.nbCap ? min(nl, a.nbCap - nrec) : 0u;
+                    char* nbw = reinterpret_cast<char*>(a.nbList);
+                    uint32_t off = (nrec * n32 + i) * 4u;
+                    for (uint32_t q = 0; q < nw; q++, off += n32 * 4u)
+                        *reinterpret_cast<uint32_t*>(nbw + off) = list[q * 256 + tid];
                     nrec += nl;
                 }
                 drain(nl);
@@ -1158,6 +1175,7 @@ extern "C" __global__ void __launch_bounds__(64) k_spec_restore(SpecRestoreArgs 
         for (int q = 0; q < 15; q++) a.words[q] = a.snap[q];
         *a.stamp = a.snap[15];
         a.stats[0] = a.snap[16];
+        for (uint32_t q = 0; q < a.ndec; q++) a.stampDec[q] -= 1u;
     }
     a.words[kNlAbort] = kInvalid;
 }
@@ -2074,265 +2092,270 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
         const float dq1 = pressure_q_d(M1, rho1);
         float G[3] = {0, 0, 0}, Gv[3] = {0, 0, 0}, Gr = 0.0f, F[3] = {0, 0, 0};
         if (!noNb) {
-            uint32_t cx, cy, cz;
-            cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
-            const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
-            const uint32_t stampv = *a.stamp;
-            const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
-            // the candidate sweep of the forward kernel: branch-free batches, one drain site
-            auto row_meta = [&](int rw, uint4& m) {
-                m = make_uint4(0, 0, 0, 0);
-                const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
-                if (rw < 9 && nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
-                    const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
-                    const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
-                    const size_t cb = (size_t)row * sx;
-                    const uint32_t st = a.rowStamp[row];
-                    const uint32_t lo = ro[xl], hi = ro[xh + 1];
-                    m.z = a.bP[cb + xl];
-                    m.w = a.bP[cb + xh + 1];
-                    if (st == stampv) {
-                        m.x = lo;
-                        m.y = hi;
-                    }
-                }
+            // the recorded / found neighbours; the next one's loads are issued before the
+            // current pair's arithmetic, into the other of two register sets
+            auto ld = [&](uint32_t e) {
+                AdjIn in;
+                const bool bnd = (e & kBnd) != 0;
+                const uint32_t j = e & ~kBnd;
+                in.xj = bnd ? B[j] : X[j];
+                in.vj = bnd ? BV[j] : XV[j];
+                in.mk2 = bnd ? a.bmk[j] : a.mk[j];
+                const float4 wvj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WV[j];
+                in.wvx = wvj.x;
+                in.wvy = wvj.y;
+                in.wvz = wvj.z;
+                in.wd = bnd ? 0.0f : reinterpret_cast<const float*>(WP)[4 * (size_t)j + 3];
+                return in;
             };
-            uint32_t nl = 0;
-            int rw = 0, part = 0;
-            // the forward stage's record of this particle's neighbours, when it is complete
-            const bool useRec = REC;
-            uint32_t rpos = 0;
-            const uint32_t rcnt = useRec ? a.nbCount[i] : 0u;
-            uint4 cur = make_uint4(0, 0, 0, 0), nxtm = cur;
-            if (!useRec) {
-                row_meta(0, cur);
-                row_meta(1, nxtm);
-            }
-            uint32_t q = cur.x, qend = cur.y;
-            for (;;) {
-                while (nl + ARP_BATCH <= (uint32_t)kList) {
-                    if (useRec) {
-                        if (rpos >= rcnt) {
-                            rw = 9;
-                            break;
-                        }
-                        const uint32_t take = min(rcnt - rpos, (uint32_t)kList - nl);
-                        // 32-bit byte offsets from the uniform base (the host keeps the
-                        // record below 4 GB)
-                        const char* nb = reinterpret_cast<const char*>(a.nbList);
-                        for (uint32_t k = 0; k < take; k++)
-                            list[(nl + k) * 256 + tid] =
-                                *reinterpret_cast<const uint32_t*>(nb + ((rpos + k) * a.n + i) * 4u);
-                        nl += take;
-                        rpos += take;
-                        continue;
-                    }
-                    if (q >= qend) {
-                        if (part == 0) {
-                            part = 1;
-                            q = cur.z;
-                            qend = cur.w;
-                            continue;
-                        }
-                        if (++rw >= 9) break;
-                        cur = nxtm;
-                        row_meta(rw + 1, nxtm);
-                        part = 0;
-                        q = cur.x;
-                        qend = cur.y;
-                        continue;
-                    }
-                    const bool bseg = part != 0;
-                    const float4* base = bseg ? B : X;
-                    const uint32_t tag = bseg ? kBnd : 0u;
-                    uint32_t e[ARP_BATCH], ok[ARP_BATCH];
-#pragma unroll
-                    for (int u = 0; u < ARP_BATCH; u++) {
-                        const uint32_t qq = q + u;  // < qend + ARP_BATCH: inside the padding
-                        const uint32_t inr = qq < qend ? 1u : 0u;
-                        uint32_t v = qq, okv = inr;
-                        if (!a.identity) {
-                            const uint32_t pe = a.perm[bseg ? 0u : qq];
-                            okv = inr & ((bseg || !(pe & kFlag)) ? 1u : 0u);
-                            v = bseg ? qq : (inr ? (pe & ~kFlag) : 0u);
-                        } else {
-                            const uint32_t fl = a.sflag[bseg ? 0u : qq];
-                            okv = inr & ((bseg || fl == 0u) ? 1u : 0u);
-                        }
-                        e[u] = v;
-                        ok[u] = okv;
-                    }
-                    float4 xj[ARP_BATCH];
-#pragma unroll
-                    for (int u = 0; u < ARP_BATCH; u++) xj[u] = base[e[u]];
-#pragma unroll
-                    for (int u = 0; u < ARP_BATCH; u++) {
-                        const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                        const uint32_t v = e[u] | tag;
-                        const uint32_t in = ok[u] & (v != i ? 1u : 0u) &
-                                            (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
-                        list[nl * 256 + tid] = v;
-                        nl += in;
-                    }
-                    q += ARP_BATCH;
+            auto proc = [&](uint32_t e, const AdjIn& in) {
+                const bool bnd = (e & kBnd) != 0;
+                const float4 xj = in.xj, vj = in.vj;
+                const uint32_t mk2 = in.mk2;
+                const uint32_t mat2 = mk2 & kMatMask;
+                float m2, f2, bf2;
+                DInter itA, itB;
+                if constexpr (FAST) {
+                    const float4 mm = smat[mat2];
+                    m2 = mm.x;
+                    f2 = mm.y;
+                    bf2 = mm.z;
+                    itA = sint[mat1 * nm + mat2];
+                    itB = sint[mat2 * nm + mat1];
+                } else {
+                    const DMat& M2 = a.mats[mat2];
+                    m2 = M2.mass;
+                    f2 = M2.visc;
+                    bf2 = M2.bulk_visc;
+                    itA = a.inter[mat1 * nm + mat2];
+                    itB = a.inter[mat2 * nm + mat1];
                 }
-                // the recorded / found neighbours; the next one's loads are issued before the
-                // current pair's arithmetic, into the other of two register sets
-                auto ld = [&](uint32_t e) {
-                    AdjIn in;
-                    const bool bnd = (e & kBnd) != 0;
-                    const uint32_t j = e & ~kBnd;
-                    in.xj = bnd ? B[j] : X[j];
-                    in.vj = bnd ? BV[j] : XV[j];
-                    in.mk2 = bnd ? a.bmk[j] : a.mk[j];
-                    const float4 wvj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WV[j];
-                    in.wvx = wvj.x;
-                    in.wvy = wvj.y;
-                    in.wvz = wvj.z;
-                    in.wd = bnd ? 0.0f : reinterpret_cast<const float*>(WP)[4 * (size_t)j + 3];
-                    return in;
-                };
-                auto proc = [&](uint32_t e, const AdjIn& in) {
-                    const bool bnd = (e & kBnd) != 0;
-                    const float4 xj = in.xj, vj = in.vj;
-                    const uint32_t mk2 = in.mk2;
-                    const uint32_t mat2 = mk2 & kMatMask;
-                    float m2, f2, bf2;
-                    DInter itA, itB;
-                    if constexpr (FAST) {
-                        const float4 mm = smat[mat2];
-                        m2 = mm.x;
-                        f2 = mm.y;
-                        bf2 = mm.z;
-                        itA = sint[mat1 * nm + mat2];
-                        itB = sint[mat2 * nm + mat1];
-                    } else {
-                        const DMat& M2 = a.mats[mat2];
-                        m2 = M2.mass;
-                        f2 = M2.visc;
-                        bf2 = M2.bulk_visc;
-                        itA = a.inter[mat1 * nm + mat2];
-                        itB = a.inter[mat2 * nm + mat1];
-                    }
-                    if (FAST && !bnd && (FLUID || !((mk1 | mk2) & kMkElastic))) {
-                        // both sides at once (one material, no elastic particle)
-                        const float irj = frcp(xj.w);
-                        PairSym o;
-                        pair_vjp_sym(M1.mass, m2, rho1, xj.w, vi.w, vj.w, dq1, M1.visc + f2, M1.bulk_visc + bf2, itA,
-                                     xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z,
-                                     a1x, a1y, a1z, ws * in.wvx * irj, ws * in.wvy * irj, ws * in.wvz * irj, wp.w,
-                                     ws * in.wd, hh, kn, a.phys.eps_hh, o);
-                        for (int c = 0; c < 3; c++) {
-                            G[c] -= o.gr[c];
-                            Gv[c] -= o.gv[c];
-                            F[c] += o.F[c];
-                        }
-                        Gr += o.gri;
-                        pb[0] += o.dbulk;
-                        pb[0] += o.dbulk;
-                        pv[0] += o.dvisc;
-                        pv[0] += o.dvisc;
-                        return;
-                    }
-                    PairSide P;
-                    P.mi = M1.mass;
-                    P.mj = m2;
-                    P.rhoi = rho1;
-                    P.rhoj = xj.w;
-                    P.qi = vi.w;
-                    P.qj = vj.w;
-                    P.dqi = dq1;
-                    P.dqj = 0.0f;
-                    P.fi = M1.visc;
-                    P.fj = f2;
-                    P.bfi = M1.bulk_visc;
-                    P.bfj = bf2;
-                    P.eli = (mk1 & kMkElastic) != 0;
-                    P.elj = (mk2 & kMkElastic) != 0;
-                    P.bnd = bnd;
-                    P.same = mat1 == mat2;
-                    P.mati = mat1;
-                    P.matj = mat2;
-                    PairVjp o;
-                    // side a: this particle's own terms
-                    pair_vjp(P, itA, xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x,
-                             vj.y - vi.y, vj.z - vi.z, a1x, a1y, a1z, wp.w, hh, kn, a.phys.eps_hh, o);
+                if (FAST && !bnd && (FLUID || !((mk1 | mk2) & kMkElastic))) {
+                    // both sides at once (one material, no elastic particle)
+                    const float irj = frcp(xj.w);
+                    PairSym o;
+                    pair_vjp_sym(M1.mass, m2, rho1, xj.w, vi.w, vj.w, dq1, M1.visc + f2, M1.bulk_visc + bf2, itA,
+                                 xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z,
+                                 a1x, a1y, a1z, ws * in.wvx * irj, ws * in.wvy * irj, ws * in.wvz * irj, wp.w,
+                                 ws * in.wd, hh, kn, a.phys.eps_hh, o);
                     for (int c = 0; c < 3; c++) {
                         G[c] -= o.gr[c];
                         Gv[c] -= o.gv[c];
                         F[c] += o.F[c];
                     }
                     Gr += o.gri;
-                    if constexpr (FAST) {
-                        // mobile pairs are of one material; boundary pairs carry no
-                        // viscosity terms
-                        if (!bnd) {
-                            pb[0] += o.dbulk;
-                            pb[0] += o.dbulk;
-                            pv[0] += o.dvisc;
-                            pv[0] += o.dvisc;
-                        }
-                    } else if (nm <= (uint32_t)kMatLds) {
-                        pb[mat1] += o.dbulk;
-                        pb[mat2] += o.dbulk;
-                        pv[mat1] += o.dvisc;
-                        pv[mat2] += o.dvisc;
+                    pb[0] += o.dbulk;
+                    pb[0] += o.dbulk;
+                    pv[0] += o.dvisc;
+                    pv[0] += o.dvisc;
+                    return;
+                }
+                PairSide P;
+                P.mi = M1.mass;
+                P.mj = m2;
+                P.rhoi = rho1;
+                P.rhoj = xj.w;
+                P.qi = vi.w;
+                P.qj = vj.w;
+                P.dqi = dq1;
+                P.dqj = 0.0f;
+                P.fi = M1.visc;
+                P.fj = f2;
+                P.bfi = M1.bulk_visc;
+                P.bfj = bf2;
+                P.eli = (mk1 & kMkElastic) != 0;
+                P.elj = (mk2 & kMkElastic) != 0;
+                P.bnd = bnd;
+                P.same = mat1 == mat2;
+                P.mati = mat1;
+                P.matj = mat2;
+                PairVjp o;
+                // side a: this particle's own terms
+                pair_vjp(P, itA, xj.x - xi.x, xj.y - xi.y, xj.z - xi.z, vj.x - vi.x,
+                         vj.y - vi.y, vj.z - vi.z, a1x, a1y, a1z, wp.w, hh, kn, a.phys.eps_hh, o);
+                for (int c = 0; c < 3; c++) {
+                    G[c] -= o.gr[c];
+                    Gv[c] -= o.gv[c];
+                    F[c] += o.F[c];
+                }
+                Gr += o.gri;
+                if constexpr (FAST) {
+                    // mobile pairs are of one material; boundary pairs carry no
+                    // viscosity terms
+                    if (!bnd) {
+                        pb[0] += o.dbulk;
+                        pb[0] += o.dbulk;
+                        pv[0] += o.dvisc;
+                        pv[0] += o.dvisc;
                     }
-                    if (!FLUID && !bnd) {
-                        // side b: the neighbour's terms as seen from this particle
-                        float4 wpj = make_float4(0.f, 0.f, 0.f, ws * in.wd);
-                        const float4 wvj = make_float4(ws * in.wvx, ws * in.wvy, ws * in.wvz, 0.0f);
-                        const float irj = frcp(xj.w);
-                        PairSide Q;
-                        Q.mi = m2;
-                        Q.mj = M1.mass;
-                        Q.rhoi = xj.w;
-                        Q.rhoj = rho1;
-                        Q.qi = vj.w;
-                        Q.qj = vi.w;
-                        Q.dqi = 0.0f;
-                        Q.dqj = dq1;
-                        Q.fi = f2;
-                        Q.fj = M1.visc;
-                        Q.bfi = bf2;
-                        Q.bfj = M1.bulk_visc;
-                        Q.eli = P.elj;
-                        Q.elj = P.eli;
-                        Q.bnd = false;
-                        Q.same = P.same;
-                        Q.mati = mat2;
-                        Q.matj = mat1;
-                        pair_vjp(Q, itB, xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, vi.x - vj.x,
-                                 vi.y - vj.y, vi.z - vj.z, wvj.x * irj, wvj.y * irj, wvj.z * irj, wpj.w, hh, kn,
-                                 a.phys.eps_hh, o);
-                        for (int c = 0; c < 3; c++) {
-                            G[c] += o.gr[c];
-                            Gv[c] += o.gv[c];
-                        }
-                        Gr += o.grj;
+                } else if (nm <= (uint32_t)kMatLds) {
+                    pb[mat1] += o.dbulk;
+                    pb[mat2] += o.dbulk;
+                    pv[mat1] += o.dvisc;
+                    pv[mat2] += o.dvisc;
+                }
+                if (!FLUID && !bnd) {
+                    // side b: the neighbour's terms as seen from this particle
+                    float4 wpj = make_float4(0.f, 0.f, 0.f, ws * in.wd);
+                    const float4 wvj = make_float4(ws * in.wvx, ws * in.wvy, ws * in.wvz, 0.0f);
+                    const float irj = frcp(xj.w);
+                    PairSide Q;
+                    Q.mi = m2;
+                    Q.mj = M1.mass;
+                    Q.rhoi = xj.w;
+                    Q.rhoj = rho1;
+                    Q.qi = vj.w;
+                    Q.qj = vi.w;
+                    Q.dqi = 0.0f;
+                    Q.dqj = dq1;
+                    Q.fi = f2;
+                    Q.fj = M1.visc;
+                    Q.bfi = bf2;
+                    Q.bfj = M1.bulk_visc;
+                    Q.eli = P.elj;
+                    Q.elj = P.eli;
+                    Q.bnd = false;
+                    Q.same = P.same;
+                    Q.mati = mat2;
+                    Q.matj = mat1;
+                    pair_vjp(Q, itB, xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, vi.x - vj.x,
+                             vi.y - vj.y, vi.z - vj.z, wvj.x * irj, wvj.y * irj, wvj.z * irj, wpj.w, hh, kn,
+                             a.phys.eps_hh, o);
+                    for (int c = 0; c < 3; c++) {
+                        G[c] += o.gr[c];
+                        Gv[c] += o.gv[c];
                     }
+                    Gr += o.grj;
+                }
+            };
+            if constexpr (REC) {
+                // the forward stage's record of this particle's neighbours, streamed from
+                // the record (rows of n entries: coalesced) in the recorded order; an entry
+                // is loaded a pair ahead of its data, and an index past the count reads
+                // the first row (in bounds; the host keeps the record below 4 GB)
+                const char* nb = reinterpret_cast<const char*>(a.nbList);
+                const uint32_t rcnt = a.nbCount[i], n32 = a.n;
+                auto ent = [&](uint32_t k) {
+                    return *reinterpret_cast<const uint32_t*>(nb + ((k < rcnt ? k : 0u) * n32 + i) * 4u);
                 };
-                {
-                    // (the next loads are unconditional: a lane at its last pair reloads its
-                    // first entry rather than branching around the load)
-                    AdjIn pa, pbn;
-                    uint32_t ea = 0, eb = 0;
-                    if (nl) {
-                        ea = list[tid];
-                        pa = ld(ea);
-                    }
-                    for (uint32_t k = 0; k < nl; k += 2) {
-                        eb = list[(k + 1 < nl ? k + 1 : 0u) * 256 + tid];
+                if (rcnt) {
+                    uint32_t ea = ent(0), eb = ent(1);
+                    AdjIn pa = ld(ea), pbn;
+                    for (uint32_t k = 0; k < rcnt; k += 2) {
+                        const uint32_t ec = ent(k + 2);
                         pbn = ld(eb);
                         proc(ea, pa);
-                        if (k + 1 >= nl) break;
-                        ea = list[(k + 2 < nl ? k + 2 : 0u) * 256 + tid];
-                        pa = ld(ea);
+                        if (k + 1 >= rcnt) break;
+                        const uint32_t ed = ent(k + 3);
+                        pa = ld(ec);
                         proc(eb, pbn);
+                        ea = ec;
+                        eb = ed;
                     }
                 }
-                nl = 0;
-                if (rw >= 9) break;
+            } else {
+                uint32_t cx, cy, cz;
+                cell_of(a.grid, xi.x, xi.y, xi.z, cx, cy, cz);
+                const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
+                const uint32_t stampv = *a.stamp;
+                const uint32_t xl = cx > 0 ? cx - 1 : 0, xh = min(cx + 1, sx - 1);
+                // the candidate sweep of the forward kernel: branch-free batches, one drain site
+                auto row_meta = [&](int rw, uint4& m) {
+                    m = make_uint4(0, 0, 0, 0);
+                    const int nz = (int)cz + rw / 3 - 1, ny = (int)cy + rw % 3 - 1;
+                    if (rw < 9 && nz >= 0 && nz < (int)sz && ny >= 0 && ny < (int)sy) {
+                        const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
+                        const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+                        const size_t cb = (size_t)row * sx;
+                        const uint32_t st = a.rowStamp[row];
+                        const uint32_t lo = ro[xl], hi = ro[xh + 1];
+                        m.z = a.bP[cb + xl];
+                        m.w = a.bP[cb + xh + 1];
+                        if (st == stampv) {
+                            m.x = lo;
+                            m.y = hi;
+                        }
+                    }
+                };
+                uint32_t nl = 0;
+                int rw = 0, part = 0;
+                uint4 cur, nxtm;
+                row_meta(0, cur);
+                row_meta(1, nxtm);
+                uint32_t q = cur.x, qend = cur.y;
+                for (;;) {
+                    while (nl + ARP_BATCH <= (uint32_t)kList) {
+                        if (q >= qend) {
+                            if (part == 0) {
+                                part = 1;
+                                q = cur.z;
+                                qend = cur.w;
+                                continue;
+                            }
+                            if (++rw >= 9) break;
+                            cur = nxtm;
+                            row_meta(rw + 1, nxtm);
+                            part = 0;
+                            q = cur.x;
+                            qend = cur.y;
+                            continue;
+                        }
+                        const bool bseg = part != 0;
+                        const float4* base = bseg ? B : X;
+                        const uint32_t tag = bseg ? kBnd : 0u;
+                        uint32_t e[ARP_BATCH], ok[ARP_BATCH];
+    #pragma unroll
+                        for (int u = 0; u < ARP_BATCH; u++) {
+                            const uint32_t qq = q + u;  // < qend + ARP_BATCH: inside the padding
+                            const uint32_t inr = qq < qend ? 1u : 0u;
+                            uint32_t v = qq, okv = inr;
+                            if (!a.identity) {
+                                const uint32_t pe = a.perm[bseg ? 0u : qq];
+                                okv = inr & ((bseg || !(pe & kFlag)) ? 1u : 0u);
+                                v = bseg ? qq : (inr ? (pe & ~kFlag) : 0u);
+                            } else {
+                                const uint32_t fl = a.sflag[bseg ? 0u : qq];
+                                okv = inr & ((bseg || fl == 0u) ? 1u : 0u);
+                            }
+                            e[u] = v;
+                            ok[u] = okv;
+                        }
+                        float4 xj[ARP_BATCH];
+    #pragma unroll
+                        for (int u = 0; u < ARP_BATCH; u++) xj[u] = base[e[u]];
+    #pragma unroll
+                        for (int u = 0; u < ARP_BATCH; u++) {
+                            const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                            const uint32_t v = e[u] | tag;
+                            const uint32_t in = ok[u] & (v != i ? 1u : 0u) &
+                                                (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
+                            list[nl * 256 + tid] = v;
+                            nl += in;
+                        }
+                        q += ARP_BATCH;
+                    }
+                    {
+                        // (the next loads are unconditional: a lane at its last pair reloads its
+                        // first entry rather than branching around the load)
+                        AdjIn pa, pbn;
+                        uint32_t ea = 0, eb = 0;
+                        if (nl) {
+                            ea = list[tid];
+                            pa = ld(ea);
+                        }
+                        for (uint32_t k = 0; k < nl; k += 2) {
+                            eb = list[(k + 1 < nl ? k + 1 : 0u) * 256 + tid];
+                            pbn = ld(eb);
+                            proc(ea, pa);
+                            if (k + 1 >= nl) break;
+                            ea = list[(k + 2 < nl ? k + 2 : 0u) * 256 + tid];
+                            pa = ld(ea);
+                            proc(eb, pbn);
+                        }
+                    }
+                    nl = 0;
+                    if (rw >= 9) break;
+                }
             }
         }
         // pos' = v; vel' = F/rho + g
@@ -2401,7 +2424,7 @@ extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_g(AdjArgs a) { 
 extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec_g(AdjArgs a) { fluid_adj_body<true, false>(a); }
 // _f: scenes without elastic particles
 extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_f(AdjArgs a) { fluid_adj_body<false, true, true>(a); }
-extern "C" __global__ void __launch_bounds__(256, 3) k_fluid_adj_rec_f(AdjArgs a) { fluid_adj_body<true, true, true>(a); }
+extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_adj_rec_f(AdjArgs a) { fluid_adj_body<true, true, true>(a); }
 
 extern "C" __global__ void __launch_bounds__(256) k_adj_w(AdjWArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;

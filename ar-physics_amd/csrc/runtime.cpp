@@ -117,6 +117,8 @@ struct GridSlot {
 };
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
+static_assert(kRecFlags == 2 * ARP_MAX_STAGES, "recorded step flags: two per stage");
+
 struct RecStep {
     DBuf mk, pid, src, stamps;
     uint32_t* fl = nullptr;  // nbOver[ARP_MAX_STAGES], use[ARP_MAX_STAGES] (in the context's recFlags)
@@ -208,7 +210,8 @@ struct arp_ctx {
     uint32_t* hAbortDev = nullptr;
     DBuf nlSnap;
     hipEvent_t specEv[4] = {};
-    uint32_t spec_aborts = 0;
+    uint64_t spec_aborts = 0;
+    uint64_t spec_steps = 0;  // speculative steps that completed
     uint32_t nl_max_age = 0;  // rebuild lists older than this many stages (0: never for age)
     bool nl_dirty = true;   // the decision words and bucket counts must be reset
     int64_t nl_thresh = -1; // bucket counts above this are checked exactly (-1: not derived)
@@ -1008,7 +1011,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
     // other step reads the landing buffers
     const bool lazy = !verlet && S > 1 && !rec && !c->capture && nl_enabled(c);
     if (!lazy && (rc = sync_yland(c))) return rc;
-    if (sp && !lazy) return fail(c, ARP_E_STATE, "speculative step outside the list-reusing path");
+    // (a recorded step speculates when the previous one left its input in place)
+    if (sp && !lazy && !(rec && S > 1 && !c->capture && nl_enabled(c) && c->rec_prefilled == rec->Xp[0].p))
+        return fail(c, ARP_E_STATE, "speculative step outside the list-reusing path");
     // which stage derivatives must be kept for later (non-subdiagonal) stage inputs
     int kslot[ARP_MAX_STAGES];
     int nslots = 0;
@@ -1073,20 +1078,21 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
         io.dstAuxV = P<float>(c->XF[1][1]);
     }
     if (!sp && (rc = run_sort(c, io))) return rc;
+    if (sp && rec) c->slot.full = true;  // (as the full reorder marks its slot)
     c->cur = nx;
     c->keys_valid = false;
     if (rec) {
-        rec->slot[0] = c->slot;  // (the sort also wrote the recorded meta)
+        rec->slot[0] = c->slot;  // (the sort, or the speculative walk, writes the recorded meta)
     }
 
     const float* Xp = verlet ? P<float>(c->XF[1][0]) : Y0p;  // the first stage's input
     const float* Xv = verlet ? P<float>(c->XF[1][1]) : Y0v;
     const float* Xe = verlet ? P<float>(c->XFe) : P<float>(c->Yeps);
     for (int s = 0; s < S; s++) {
+        if (s > 0 && rec) c->slot = rec->slot[s];
         if (s > 0 && !sp) {
             SortIO pio{0, nullptr, nullptr, nullptr, P<uint32_t>(c->meta_pid[c->cur]),
                        nullptr, nullptr, nullptr, nullptr, nullptr, nlw};
-            if (rec) c->slot = rec->slot[s];
             if (nl && (rc = run_key(c, Xp, nlw))) return rc;
             if ((rc = run_sort(c, pio))) return rc;
             if (rec) rec->slot[s] = c->slot;
@@ -1198,6 +1204,12 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
                 f.hostAbort = c->hAbortDev;
                 f.stampW = c->slot.stamp;
                 f.statsW = P<uint32_t>(c->stats);
+                if (rec && s == 0) {
+                    f.specClear = rec->fl;
+                    f.recSrc = P<uint32_t>(rec->src);
+                    f.recMk = P<uint32_t>(rec->mk);
+                    f.recPid = P<uint32_t>(rec->pid);
+                }
             } else if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) {
                 return rc;
             }
@@ -1959,13 +1971,35 @@ static bool spec_setup(arp_ctx* c) {
     return true;
 }
 
-static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
+// Steps [j0, j0 + count) of a recording of nsteps steps, into the recorded steps j % K;
+// advance: whether the steps advance the clock (a replay of a segment does not).
+struct RecRange {
+    uint32_t j0, nsteps, K;
+    bool advance;
+};
+
+// Step k of a run (of a recording, when rr is given).
+static int issue_step(arp_ctx* c, float dt, const RecRange* rr, uint32_t k, const SpecArg* sp) {
+    if (!rr) return step_once(c, dt, nullptr, sp);
+    const uint32_t j = rr->j0 + k;
+    c->rec_next = j + 1 < rr->nsteps ? &c->rec[(j + 1) % rr->K] : nullptr;
+    const int rc = step_once(c, dt, &c->rec[j % rr->K], sp);
+    c->rec_next = nullptr;
+    return rc;
+}
+
+// Runs nsteps steps; a run of kSpecMin steps or more on kept lists is issued speculatively,
+// kSpecW steps in flight, and a step whose lists turn out invalid runs again with the chain.
+// A recorded step speculates when the previous one left its input in its stage buffers.
+static int run_steps(arp_ctx* c, float dt, uint32_t nsteps, const RecRange* rr = nullptr) {
     int rc;
     uint32_t k = 0;
+    const bool adv = !rr || rr->advance;
     while (k < nsteps) {
-        if (nsteps - k < kSpecMin || !spec_ok(c)) {
-            if ((rc = step_once(c, dt))) return rc;
-            c->time += dt;
+        if (nsteps - k < kSpecMin || !spec_ok(c) ||
+            (rr && (rr->K < 2 || c->rec_prefilled != c->rec[(rr->j0 + k) % rr->K].Xp[0].p))) {
+            if ((rc = issue_step(c, dt, rr, k, nullptr))) return rc;
+            if (adv) c->time += dt;
             k++;
             continue;
         }
@@ -1979,15 +2013,16 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
         struct Host {
             int cur;
             bool keys_valid, have_build, yland_stale;
+            const void* prefilled;
         } hs[kSpecW];
         const uint32_t m = nsteps - k;
         uint32_t issued = 0, done = 0;
         bool aborted = false;
         while (done < m) {
             while (issued < m && issued - done < kSpecW) {
-                hs[issued % kSpecW] = Host{c->cur, c->keys_valid, c->have_build, c->yland_stale};
+                hs[issued % kSpecW] = Host{c->cur, c->keys_valid, c->have_build, c->yland_stale, c->rec_prefilled};
                 const SpecArg sa{issued, issued % kSpecW, gen0 + issued * S};
-                if ((rc = step_once(c, dt, nullptr, &sa))) {
+                if ((rc = issue_step(c, dt, rr, k + issued, &sa))) {
                     hipStreamSynchronize(c->stream);
                     return rc;
                 }
@@ -1999,7 +2034,8 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
                 aborted = true;
                 break;
             }
-            c->time += dt;
+            if (adv) c->time += dt;
+            c->spec_steps++;
             done++;
         }
         if (!aborted) {
@@ -2011,23 +2047,36 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps) {
         const uint32_t code = __atomic_load_n(c->hAbort, __ATOMIC_ACQUIRE);
         const uint32_t a = code >> 4, stage = code & 15u;
         if (a < done || a >= issued) return fail(c, ARP_E_STATE, "speculative step %u out of range", a);
-        for (; done < a; done++) c->time += dt;
+        for (; done < a; done++) {
+            if (adv) c->time += dt;
+            c->spec_steps++;
+        }
         const Host& h = hs[a % kSpecW];
         c->cur = h.cur;
         c->keys_valid = h.keys_valid;
         c->have_build = h.have_build;
         c->yland_stale = h.yland_stale;
-        // the list words as the step found them (a first stage that aborted changed nothing)
+        c->rec_prefilled = h.prefilled;
+        // the list words as the step found them (a first stage that aborted changed nothing);
+        // a recorded step's stages advanced the build stamps of its own slots
         SpecRestoreArgs sr{P<uint32_t>(c->nlWords),
                            stage > 0 ? P<uint32_t>(c->nlSnap) + (size_t)kSpecSnap * (a % kSpecW) : nullptr,
-                           c->slot.stamp, P<uint32_t>(c->stats)};
+                           c->slot.stamp, P<uint32_t>(c->stats), nullptr, 0u};
+        if (rr) {
+            const RecStep& R = c->rec[(rr->j0 + k + a) % rr->K];
+            sr.stamp = R.slot[0].stamp;
+            if (stage > 1) {
+                sr.stampDec = R.slot[1].stamp;  // (the slots' stamps are consecutive words)
+                sr.ndec = stage - 1;
+            }
+        }
         if ((rc = launch(c, K_SPEC_RESTORE, 1, 64, sr))) return rc;
         __atomic_store_n(c->hAbort, kInvalid, __ATOMIC_RELEASE);
         c->spec_aborts++;
         k += a;
         // that step with the rebuild chain
-        if ((rc = step_once(c, dt))) return rc;
-        c->time += dt;
+        if ((rc = issue_step(c, dt, rr, k, nullptr))) return rc;
+        if (adv) c->time += dt;
         k++;
     }
     return ARP_OK;
@@ -2078,20 +2127,16 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     // every step's stage flags in one buffer (read back with one copy)
     if ((rc = dalloc(c, c->recFlags, 4 * 2 * ARP_MAX_STAGES * (size_t)K))) return rc;
     for (uint32_t j = 0; j < K; j++) c->rec[j].fl = P<uint32_t>(c->recFlags) + 2 * ARP_MAX_STAGES * (size_t)j;
-    for (uint32_t j = 0; j < nsteps; j++) {
-        if (j % K == 0) {
-            if ((rc = ck_save(c, c->ck[j / K], nseg > 1))) return rc;
-            // kept neighbour lists restart with every segment of a recording that the
-            // reverse sweep replays (one of several segments, the last included: a second
-            // sweep replays it too), as its replay from the checkpoint does, so that every
-            // replay repeats the recorded steps exactly
-            if (nseg > 1) c->nl_dirty = true;
-        }
-        c->rec_next = j + 1 < nsteps ? &c->rec[(j + 1) % K] : nullptr;
-        rc = step_once(c, dt, &c->rec[j % K]);
-        c->rec_next = nullptr;
-        if (rc) return rc;
-        c->time += dt;
+    for (uint32_t seg = 0; seg < nseg; seg++) {
+        const uint32_t j0 = seg * K, j1 = std::min(nsteps, j0 + K);
+        if ((rc = ck_save(c, c->ck[seg], nseg > 1))) return rc;
+        // kept neighbour lists restart with every segment of a recording that the reverse
+        // sweep replays (one of several segments, the last included: a second sweep replays
+        // it too), as its replay from the checkpoint does, so that every replay repeats the
+        // recorded steps exactly
+        if (nseg > 1) c->nl_dirty = true;
+        const RecRange rr{j0, nsteps, K, true};
+        if ((rc = run_steps(c, dt, j1 - j0, &rr))) return rc;
     }
     c->rec_prefilled = nullptr;
     if ((rc = ck_save(c, c->ck[nseg], nseg > 1))) return rc;
@@ -2181,12 +2226,8 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
             // replay the segment from its checkpoint, recording
             replayed = true;
             if ((rc = ck_restore(c, c->ck[seg]))) return rc;
-            for (uint32_t j = j0; j < j1; j++) {
-                c->rec_next = j + 1 < j1 ? &c->rec[j + 1 - j0] : nullptr;
-                rc = step_once(c, c->rec_dt, &c->rec[j - j0]);
-                c->rec_next = nullptr;
-                if (rc) return rc;
-            }
+            const RecRange rr{j0, j1, K, false};
+            if ((rc = run_steps(c, c->rec_dt, j1 - j0, &rr))) return rc;
             c->rec_prefilled = nullptr;
             if ((rc = fetch_over(c, j1 - j0))) return rc;
             c->rec_seg = (uint32_t)seg;
@@ -2506,6 +2547,13 @@ int arp_neighbor_stats(arp_ctx* c, uint32_t* builds, uint32_t* reused) {
     }
     if (builds) *builds = w[8];
     if (reused) *reused = w[9];
+    return ARP_OK;
+}
+
+int arp_speculation_stats(const arp_ctx* c, uint64_t* steps, uint64_t* aborted) {
+    if (!c) return ARP_E_ARG;
+    if (steps) *steps = c->spec_steps;
+    if (aborted) *aborted = c->spec_aborts;
     return ARP_OK;
 }
 

@@ -74,17 +74,23 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
     g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(rng.standard_normal((sc.n, 7)).astype(np.float32)).cuda()
     gin = torch.zeros_like(g)
     torch.cuda.synchronize()
-    # one untimed round (allocates the recording and the reverse-mode buffers)
+    # one untimed round of the timed calls (allocates the recording and the reverse-mode
+    # buffers), then the median of three timed rounds
     sim.adjoint_record(dt, nsteps, nsteps)
-    sim.adjoint_backward(g, gin, want_params=False)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.adjoint_record(dt, nsteps, nsteps)
-    sim.synchronize()
-    t1 = time.perf_counter()
     sim.adjoint_backward(g, gin, want_params=True)
     sim.synchronize()
-    t2 = time.perf_counter()
+    rounds = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        sim.adjoint_record(dt, nsteps, nsteps)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        sim.adjoint_backward(g, gin, want_params=True)
+        sim.synchronize()
+        t2 = time.perf_counter()
+        rounds.append((t1 - t0, t2 - t1))
+    rec_s = sorted(r[0] for r in rounds)[1]
+    bwd_s = sorted(r[1] for r in rounds)[1]
     # the reverse stage (on the recorded neighbours, or searching after a record overflow;
     # fluid-only / one-material / general table forms), timed with events in another round
     adj_kernels = ["k_fluid_adj" + sfx for sfx in ("", "_rec", "_f", "_rec_f", "_g", "_rec_g")]
@@ -100,8 +106,8 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
         ms, cnt = ms + m1, cnt + c1
     sim.profile(False)
     finite = bool(torch.isfinite(gin).all().item())
-    rec_ms, bwd_ms = 1e3 * (t1 - t0), 1e3 * (t2 - t1)
-    return dict(steps=nsteps, checkpoint_every=nsteps, record_ms=rec_ms, backward_ms=bwd_ms,
+    rec_ms, bwd_ms = 1e3 * rec_s, 1e3 * bwd_s
+    return dict(steps=nsteps, checkpoint_every=nsteps, record_ms=rec_ms, backward_ms=bwd_ms, timed_rounds=3,
                 forward_ms=fwd_ms_per_step * nsteps,
                 ratio_fwd_plus_adjoint_to_fwd=(rec_ms + bwd_ms) / (fwd_ms_per_step * nsteps),
                 k_fluid_adj_avg_us=1e3 * ms / cnt if cnt else None, grad_finite=finite)
@@ -224,6 +230,7 @@ def main():
     # the timed steps run without per-kernel events; a separate pass with events on the
     # stream times the kernels
     nl0 = sim.neighbor_stats()
+    sp0 = sim.speculation_stats()
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
     sim.update(args.steps)
@@ -231,6 +238,7 @@ def main():
     t1 = time.perf_counter()
     sec = t1 - t0
     nl1 = sim.neighbor_stats()
+    sp1 = sim.speculation_stats()
     sim.profile_select(fluid_kernels if args.events == "fluid" else None)
     sim.profile(args.events != "none")
     sim.profile_reset()
@@ -297,7 +305,9 @@ def main():
         "step_roofline_frac": B_ALG_STEP.get(args.integrator, 0.0) * value / HBM_PEAK,
         "kernels": kern,
         "neighbor_lists": {"skin": args.skin, "stage_builds": nl1[0] - nl0[0],
-                           "stage_reuses": nl1[1] - nl0[1]},
+                           "stage_reuses": nl1[1] - nl0[1],
+                           # timed steps issued as list walks alone, and those run again
+                           "speculative_steps": sp1[0] - sp0[0], "speculation_aborts": sp1[1] - sp0[1]},
         "state_finite": finite,
         "fwd_adjoint": fwd_adj,
         "ensemble": ens,

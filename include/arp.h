@@ -190,6 +190,10 @@ int arp_set_neighbor_max_age(arp_ctx* ctx, uint32_t stages);
 /* Stages that searched and built lists, and stages that reused them, since the last
  * arp_set_neighbor_skin. */
 int arp_neighbor_stats(arp_ctx* ctx, uint32_t* builds, uint32_t* reused);
+/* Steps issued speculatively (each stage as its list walk alone) that completed, and those
+ * whose lists turned out invalid and ran again with the rebuild chain, since the context was
+ * created.  No counterpart in the reference (whose step always searches); diagnostics. */
+int arp_speculation_stats(const arp_ctx* ctx, uint64_t* steps, uint64_t* aborted);
 
 /* ---- per-stage drop-ins on device std430 buffers ------------------------------------- */
 /* compute_forces (shaders/mod.rs:11-86): neighbour update at the input positions, fluid

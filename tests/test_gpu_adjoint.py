@@ -217,6 +217,51 @@ def test_adjoint_leaves_state_and_replays_exactly(torch_cuda):
         s.close()
 
 
+@pytest.mark.parametrize("integ,sigma,nsteps,K", [("rk2", 1.0, 20, 20), ("rk4", 1.0, 24, 12), ("heun3", 0.05, 18, 9)])
+def test_speculative_recording_matches_the_chain(torch_cuda, integ, sigma, nsteps, K):
+    """Recorded steps issued speculatively (each stage as its list walk alone, also in the
+    replays of the reverse sweep) leave bitwise the state, list statistics and gradients of
+    issuing the rebuild chain at every stage, also when the lists of a speculated step turn
+    out invalid and the step is recorded again with the chain (fast particles: lists rebuilt
+    every few steps)."""
+    from arp.sim import FluidSim
+
+    sc = _scene(False)
+    rng = np.random.default_rng(11)
+    sc.particles["vel"][:, :3] = (sigma * rng.standard_normal((sc.n, 3))).astype(np.float32)
+    dt = 5e-5
+    g = np.random.default_rng(5).standard_normal((sc.n, 7))
+    d_out = _records(torch_cuda, g.astype(np.float32))
+    print("case speculative recording", integ, sigma, nsteps, K, flush=True)
+    res = []
+    for spec in (True, False):
+        sim = FluidSim(sc, integrator=integ, timestep=dt)
+        sim.set_neighbor_skin(0.2, speculate=spec)
+        sim.upload_state(sc.particles)
+        sim.adjoint_record(dt, nsteps, K)
+        end = sim.particles().copy()
+        stats = sim.neighbor_stats()
+        grads = []
+        for _ in range(2):  # (a second sweep replays the segments again)
+            d_in = torch_cuda.zeros((sc.n, 12), dtype=torch_cuda.float32, device="cuda")
+            gp = sim.adjoint_backward(d_out, d_in)
+            sim.synchronize()
+            grads.append((d_in.cpu().numpy().copy(), gp.copy()))
+        res.append((end, stats, grads, sim.time, sim.speculation_stats()))
+        sim.close()
+    (ea, sa, ga, ta, spa), (eb, sb, gb, tb, spb) = res
+    print("  lists %s, speculative steps %d, run again %d" % (sa, spa[0], spa[1]), flush=True)
+    assert spb == (0, 0)
+    assert spa[0] > 0
+    if sigma >= 1.0:
+        assert spa[1] > 0
+    assert _same_state(ea, eb)
+    assert sa == sb and ta == tb
+    for (da, pa), (db, pb) in zip(ga, gb):
+        assert np.isfinite(da).all()
+        assert np.array_equal(da, db) and np.array_equal(pa, pb)
+
+
 def test_autograd_rollout(torch_cuda):
     from arp.autograd import rollout, state_tensor
     from arp.sim import FluidSim
