@@ -1941,9 +1941,14 @@ int arp_set_integrator(arp_ctx* c, const ArpTableau* t, float timestep, uint32_t
 // the host then restores the list words the step began with and runs that step with the
 // rebuild chain.  The result is the one of issuing the chain at every stage.  The call
 // returns once its steps have run, so calls of fewer than kSpecMin steps (one frame at a
-// time, say) stay asynchronous and issue the chain.
-constexpr uint32_t kSpecW = 4;
+// time, say) stay asynchronous and issue the chain.  The host waits on an event after every
+// kSpecE steps (an event between two kernels idles the GPU for about 6 us); a snapshot slot
+// is overwritten only by a later step, which returns at once when an earlier one aborted,
+// so the slot of an aborted step survives until its restore.
+constexpr uint32_t kSpecW = 8;
+constexpr uint32_t kSpecE = 4;
 constexpr uint32_t kSpecMin = 8;
+static_assert(kSpecW % kSpecE == 0 && kSpecW / kSpecE < 4, "events in flight: at most three of four");
 
 static bool spec_ok(arp_ctx* c) {
     const ArpTableau& T = c->tab;
@@ -2026,17 +2031,22 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps, const RecRange* rr =
                     hipStreamSynchronize(c->stream);
                     return rc;
                 }
-                HIPCHK(c, hipEventRecord(c->specEv[issued % kSpecW], c->stream));
                 issued++;
+                if (issued % kSpecE == 0 || issued == m)  // the end of a group of kSpecE steps
+                    HIPCHK(c, hipEventRecord(c->specEv[((issued - 1) / kSpecE) % 4], c->stream));
             }
-            HIPCHK(c, hipEventSynchronize(c->specEv[done % kSpecW]));
+            // (done is at a group's start, and issued at least kSpecW further or at m, so the
+            // group's event is recorded)
+            const uint32_t grp = done / kSpecE, grpEnd = std::min(m, (grp + 1) * kSpecE);
+            HIPCHK(c, hipEventSynchronize(c->specEv[grp % 4]));
             if (__atomic_load_n(c->hAbort, __ATOMIC_ACQUIRE) != kInvalid) {
                 aborted = true;
                 break;
             }
-            if (adv) c->time += dt;
-            c->spec_steps++;
-            done++;
+            for (; done < grpEnd; done++) {
+                if (adv) c->time += dt;
+                c->spec_steps++;
+            }
         }
         if (!aborted) {
             k += m;
