@@ -1971,65 +1971,67 @@ __device__ __forceinline__ void pair_vjp_sym(float mi, float mj, float rhoi, flo
     const float ave = Ae[0] * vx + Ae[1] * vy + Ae[2] * vz;
     const float avi = aix * vx + aiy * vy + aiz * vz;
     const float C = mi * mj;
-    {  // density rates: (wd_i m_j + wd_j m_i) gs (v.r)
-        const float c1 = wdi * mj + wdj * mi;
-        for (int c = 0; c < 3; c++) {
-            o.gv[c] += c1 * gs * r[c];
-            o.gr[c] += c1 * (gs * v[c] + 2.0f * dgs * vr * r[c]);
-        }
-    }
-    {  // pressure: C (qi + qj) grad W
-        const float Q = qi + qj;
-        for (int c = 0; c < 3; c++) {
-            o.F[c] += C * Q * gs * r[c];
-            o.gr[c] += C * Q * (gs * Ae[c] + 2.0f * dgs * are * r[c]);
-        }
-        o.gri += C * dqi * gs * are;
-    }
-    if (it.potential) {  // contact: (m_j / rho_j) on side i, (m_i / rho_i) on side j
+    // Every term lies along r, v, Ae (or the contact's Ac): scalar coefficients first, then
+    // three products per direction.  Terms:
+    //   density rates (wd_i m_j + wd_j m_i) gs (v.r):  gv += c1 gs r;  gr += c1 (gs v + 2 dgs (v.r) r)
+    //   pressure C (qi + qj) grad W:  F += CQ gs r;  gr += CQ (gs Ae + 2 dgs (Ae.r) r);  gri += C dqi gs (Ae.r)
+    //   artificial viscosity -C bf (v.r) / ((rhoi + rhoj) D) grad W, h5 = gs / D:
+    //     F += E (v.r) h5 r;  gv += E (Ae.r) h5 r;
+    //     gr += E ((Ae.r) h5 v + (v.r) h5 Ae + 2 (v.r)(Ae.r) dh5 r);  gri -= E (v.r)(Ae.r) h5 / (rhoi + rhoj)
+    //   viscosity -C f gs rr / (rhoi rhoj D) v:  F -= s4 v;  gv -= s4 Ae;  gr -= 2 (Ae.v) K4 du r
+    //   contact (m_j / rho_j) on side i, (m_i / rho_i) on side j:
+    //     F += sa cp gs r;  gr += (dcp gs (Ac.r) / L + 2 cp dgs (Ac.r)) r + cp gs Ac
+    const float c1 = wdi * mj + wdj * mi;
+    const float CQ = C * (qi + qj);
+    const float D = rr + eps_hh, iD = frcp(D), iRR = frcp(rhoi * rhoj);
+    const float iS = frcp(rhoi + rhoj);
+    const float E0 = -C * iS;
+    const float E = E0 * bf;
+    const float h5 = gs * iD, dh5 = dgs * iD - gs * (iD * iD);
+    const float Evh = E * vr * h5, Eah = E * are * h5;
+    float Fr = CQ * gs + Evh;  // F along r
+    const float Ra = Fr;       // gr along Ae
+    const float Rv = c1 * gs + Eah;  // gr along v, gv along r
+    float Rr = 2.0f * dgs * (c1 * vr + CQ * are) + 2.0f * E * vr * are * dh5;  // gr along r
+    float Fv = 0.0f, Ga = 0.0f;  // F along v, gv along Ae (viscosity)
+    float gri = C * dqi * gs * are - Evh * are * iS;
+    float cA = 0.0f, Ac[3] = {0.0f, 0.0f, 0.0f};  // gr along the contact's Ac
+    if (it.potential) {
         const float L = sqrtf(rr);
         const float cp = contact_potential(it.potential, L, it.radius, it.strength);
         if (cp != 0.0f) {
             const float dcp = contact_potential_d(it.potential, L, it.radius, it.strength);
             const float irj = frcp(rhoj), iri = frcp(rhoi), iL = frcp(L);
             const float sa = mj * irj, sb = mi * iri;
-            const float Ac[3] = {sa * aix - sb * ajx, sa * aiy - sb * ajy, sa * aiz - sb * ajz};
+            Ac[0] = sa * aix - sb * ajx;
+            Ac[1] = sa * aiy - sb * ajy;
+            Ac[2] = sa * aiz - sb * ajz;
             const float arc = Ac[0] * rx + Ac[1] * ry + Ac[2] * rz;
             const float arj = ajx * rx + ajy * ry + ajz * rz;
-            for (int c = 0; c < 3; c++) {
-                o.F[c] += sa * cp * gs * r[c];
-                o.gr[c] += dcp * r[c] * iL * gs * arc + cp * (gs * Ac[c] + 2.0f * dgs * arc * r[c]);
-            }
-            o.gri += sb * cp * gs * arj * iri;
+            Fr += sa * cp * gs;
+            Rr += dcp * iL * gs * arc + 2.0f * cp * dgs * arc;
+            cA = cp * gs;
+            gri += sb * cp * gs * arj * iri;
         }
     }
-    const float D = rr + eps_hh, iD = frcp(D), iRR = frcp(rhoi * rhoj);
-    if (f != 0.0f) {  // viscosity: -C f gs rr / (rhoi rhoj D) v
+    if (f != 0.0f) {
         const float u = gs * rr * iD;
         const float du = (dgs * rr + gs) * iD - gs * rr * (iD * iD);
         const float K4 = C * f * iRR;
         const float s4 = K4 * u;
-        for (int c = 0; c < 3; c++) {
-            o.F[c] -= s4 * v[c];
-            o.gv[c] -= s4 * Ae[c];
-            o.gr[c] -= ave * K4 * du * 2.0f * r[c];
-        }
-        o.gri += ave * s4 * frcp(rhoi);
+        Fv = -s4;
+        Ga = -s4;
+        Rr -= 2.0f * ave * K4 * du;
+        gri += ave * s4 * frcp(rhoi);
     }
-    o.dvisc += -avi * C * (gs * rr * iD) * iRR;
-    {  // artificial viscosity: -C bf (v.r)/((rhoi+rhoj) D) grad W
-        const float iS = frcp(rhoi + rhoj);
-        const float E0 = -C * iS;
-        const float E = E0 * bf;
-        const float h5 = gs * iD, dh5 = dgs * iD - gs * (iD * iD);
-        for (int c = 0; c < 3; c++) {
-            o.F[c] += E * vr * h5 * r[c];
-            o.gv[c] += E * are * h5 * r[c];
-            o.gr[c] += E * (are * h5 * v[c] + vr * h5 * Ae[c] + vr * are * dh5 * 2.0f * r[c]);
-        }
-        o.gri += -E * vr * are * h5 * iS;
-        o.dbulk += E0 * vr * ari * h5;
+    for (int c = 0; c < 3; c++) {
+        o.F[c] = Fr * r[c] + Fv * v[c];
+        o.gv[c] = Rv * r[c] + Ga * Ae[c];
+        o.gr[c] = Rr * r[c] + Rv * v[c] + Ra * Ae[c] + cA * Ac[c];
     }
+    o.gri = gri;
+    o.dvisc = -avi * C * (h5 * rr) * iRR;
+    o.dbulk = E0 * vr * ari * h5;
 }
 
 // REC: the neighbours come from the forward stage's record instead of a new sweep.
