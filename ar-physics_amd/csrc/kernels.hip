@@ -1243,6 +1243,12 @@ extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f(FluidArgs a) {
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nlr_f(FluidArgs a) {
     fluid_body<5, false, true>(a, xcd_block(), gridDim.x);
 }
+// (a batch of scenes that all have one mobile and one boundary material, each its own)
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f_batch(BatchArgs b) {
+    uint32_t bid, nb;
+    const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
+    fluid_body<3, false, true>(a, bid, nb);
+}
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA

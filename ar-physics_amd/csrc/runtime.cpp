@@ -77,6 +77,7 @@ enum Kern {
     K_FLUID_NL_F,
     K_FLUID_NLR_F,
     K_SPEC_RESTORE,
+    K_FLUID_NL_F_B,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -90,7 +91,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
                                    "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f",
-                                   "k_spec_restore"};
+                                   "k_spec_restore", "k_fluid_nl_f_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
@@ -242,6 +243,7 @@ struct arp_ctx {
     bool prof = false;
     bool prof_sel[K_COUNT];  // kernels timed while profiling
     std::vector<Launch>* capture = nullptr;  // record launches here instead of issuing them
+    bool capture_fm = false;  // a capture may take the fast-material walk (every scene of its batch can)
     uint32_t rowsort_div = 32;               // particles per row-sort workgroup (tuning knob)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -1191,9 +1193,9 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
             }
             if (rec) f.recUse = rec->fl + ARP_MAX_STAGES + s;
             // one mobile and one boundary material, nothing elastic: the walk selects a
-            // pair's materials by kind (batches keep the general form, shared by all scenes)
-            const bool fm = !c->capture && c->mobile_mat >= 0 && !c->any_elastic && !c->bnd_elastic &&
-                            c->bnd_mat != -2;
+            // pair's materials by kind (a batch when every scene of it can)
+            const bool fm = (!c->capture || c->capture_fm) && c->mobile_mat >= 0 && !c->any_elastic &&
+                            !c->bnd_elastic && c->bnd_mat != -2;
             f.fastMat = fm ? (uint32_t)c->mobile_mat : 0u;
             f.fastBnd = fm ? (uint32_t)(c->bnd_mat >= 0 ? c->bnd_mat : c->mobile_mat) : 0u;
             if (sp) {  // the walk alone, keeping the chain's bookkeeping
@@ -2775,6 +2777,7 @@ Kern batch_kernel(int k) {
         case K_FLUID: return K_FLUID_B;
         case K_FLUID_NL: return K_FLUID_NL_B;
         case K_FLUID_NLB: return K_FLUID_NLB_B;
+        case K_FLUID_NL_F: return K_FLUID_NL_F_B;
         default: return K_COUNT;
     }
 }
@@ -2841,14 +2844,21 @@ int build_plan(arp_batch* b, float dt) {
         }
     std::vector<HostState> start(ns);
     for (size_t s = 0; s < ns; s++) start[s] = save_host(b->ctx[s]);
+    // the fast-material walk when every scene has one mobile and one boundary material (each
+    // scene's own, in its arguments), so that all scenes launch the same kernels
+    bool allfm = true;
+    for (arp_ctx* c : b->ctx)
+        allfm = allfm && c->mobile_mat >= 0 && !c->any_elastic && !c->bnd_elastic && c->bnd_mat != -2;
     std::vector<BLaunch> plan[2];
     for (int p = 0; p < 2; p++) {
         std::vector<std::vector<Launch>> lists(ns);
         for (size_t s = 0; s < ns; s++) {
             arp_ctx* c = b->ctx[s];
             c->capture = &lists[s];
+            c->capture_fm = allfm;
             rc = step_once(c, dt);
             c->capture = nullptr;
+            c->capture_fm = false;
             if (rc) {
                 for (size_t t = 0; t < ns; t++) load_host(b->ctx[t], start[t]);
                 return bfail(b, rc, "scene %zu: %s", s, c->err.c_str());
@@ -3072,8 +3082,10 @@ int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chose
     uint32_t bestDiv = cand[0];
     for (uint32_t div : cand) {
         for (arp_ctx* c : b->ctx) c->rowsort_div = div;
-        // one untimed step builds the plan and warms the caches
-        if ((rc = run_batch(b, dt, 1))) break;
+        // two untimed steps build the plan, warm the caches and rebuild the lists that the
+        // restore reset, so the timed steps see the reuse that follows (where a row sort
+        // launched with many small workgroups still pays for every one of them)
+        if ((rc = run_batch(b, dt, 2))) break;
         HIPCHK(L0, hipEventRecord(e0, L0->stream));
         if ((rc = run_batch(b, dt, trial_steps))) break;
         HIPCHK(L0, hipEventRecord(e1, L0->stream));

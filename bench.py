@@ -128,7 +128,7 @@ def time_ensemble(nscenes, steps, dt):
     sims = [FluidSim(sc, integrator="rk2", timestep=dt) for sc in scenes]
     npart = sum(sc.n for sc in scenes)
     B = Batch(sims, graph=True)
-    div = B.tune(dt, 2)
+    div = B.tune(dt, 4)
     B.step(dt, 2)
     B.synchronize()
     torch.cuda.synchronize()

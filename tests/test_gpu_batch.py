@@ -68,6 +68,42 @@ def test_batch_matches_individual(torch_cuda, integ, graph, nsteps):
         s.close()
 
 
+def _two_fluids(sc):
+    """The scene with every other fluid particle of a second, lighter fluid material."""
+    import dataclasses
+
+    mats = np.concatenate([sc.materials, sc.materials[1:2]])
+    mats["target_den"][2] = mats["start_den"][2] = np.float32(800.0)
+    mats["mass"][2] = np.float32(mats["mass"][1] * 0.8)
+    p = sc.particles.copy()
+    p["mat"][1::2] = 2
+    return dataclasses.replace(sc, particles=p, materials=mats, interactions=S.resolve_interactions(mats, sc.h))
+
+
+def test_batch_of_mixed_material_forms(torch_cuda):
+    """A batch takes the fast-material walk only when every scene has one mobile and one
+    boundary material; a scene with two fluids puts the whole batch on the general walk.
+    Both batches equal the scenes stepped alone (which take their own forms)."""
+    from arp.sim import Batch
+
+    dt = 5e-5
+    base = _scenes()
+    for scenes in (base, base[:2] + [_two_fluids(base[2])] + base[3:]):
+        print("case batch material forms", [len(sc.materials) for sc in scenes], flush=True)
+        solo = _sims(scenes, "rk2", dt)
+        for s in solo:
+            s.step(dt, 6)
+        sims = _sims(scenes, "rk2", dt)
+        B = Batch(sims, graph=True)
+        B.step(dt, 6)
+        B.synchronize()
+        for k, (a, b) in enumerate(zip(sims, solo)):
+            assert _same(a.particles(), b.particles()), "scene %d differs" % k
+        B.close()
+        for s in sims + solo:
+            s.close()
+
+
 def test_batch_tune_leaves_states(torch_cuda):
     from arp.sim import Batch
 
