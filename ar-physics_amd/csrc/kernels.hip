@@ -1212,10 +1212,17 @@ ARP_ENTRIES(k_rowsort, RowSortArgs, rowsort_body)
 // the material tables' location is chosen once per launch (uniform branch)
 template <int MODE>
 __device__ __forceinline__ void fluid_entry(const FluidArgs& a, uint32_t bid, uint32_t nb) {
-    if (a.phys.nm <= (uint32_t)kMatLds)
-        fluid_body<MODE, true>(a, bid, nb);
-    else
-        fluid_body<MODE, false>(a, bid, nb);
+    // a list build (MODE 2) may be launched with fewer workgroups than blocks of 256
+    // particles (in a batch, where it mostly returns at once): each workgroup then takes
+    // every nb-th block
+    const uint32_t nvb = MODE == 2 ? (a.n + 255u) / 256u : bid + 1u;
+    for (uint32_t vb = bid; vb < nvb; vb += nb) {
+        if (a.phys.nm <= (uint32_t)kMatLds)
+            fluid_body<MODE, true>(a, vb, nb);
+        else
+            fluid_body<MODE, false>(a, vb, nb);
+        if (MODE != 2) break;
+    }
 }
 template <int MODE>
 __device__ __forceinline__ void fluid_entry_batch(const BatchArgs& b) {

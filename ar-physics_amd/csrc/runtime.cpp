@@ -357,6 +357,9 @@ int launch(arp_ctx* c, Kern k, uint32_t blocks, uint32_t threads, const A& args)
 }
 
 inline uint32_t nblk(uint64_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
+// workgroups of a grid-stride pass that a stage reusing its lists skips (its launch costs
+// about a workgroup dispatch each): four blocks of particles per workgroup, at most 1024
+inline uint32_t gated_blocks(uint64_t n) { return std::min<uint32_t>(std::max<uint32_t>(nblk(n) / 4, 1), 1024); }
 
 int fill_u32(arp_ctx* c, DBuf& b, uint32_t n, uint32_t v) {
     FillArgs a{P<uint32_t>(b), n, v};
@@ -586,7 +589,8 @@ int run_key(arp_ctx* c, const float* posd, const uint32_t* nlw = nullptr, bool c
         k.cpyDstP = P<float>(c->Yland[0]);
         k.cpyDstV = P<float>(c->Yland[1]);
     }
-    return launch(c, K_KEY, nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, k);
+    // (grid-stride: with lists kept the launch mostly returns at once, so it stays small)
+    return launch(c, K_KEY, nlw ? gated_blocks(c->n) : nblk(c->n), 256, k);
 }
 
 // Yland <- Ysort when the live state is only in the sorted buffers
@@ -642,7 +646,7 @@ int run_sort(arp_ctx* c, const SortIO& io) {
         sc.dstAuxV = io.dstAuxV;
     }
     // (grid-stride: with lists kept the launch is mostly a no-op, so it stays small)
-    if ((rc = launch(c, K_SCATTER, io.nlw ? std::min<uint32_t>(nblk(c->n), 1024) : nblk(c->n), 256, sc))) return rc;
+    if ((rc = launch(c, K_SCATTER, io.nlw ? gated_blocks(c->n) : nblk(c->n), 256, sc))) return rc;
     RowSortArgs r{};
     r.grid = c->grid;
     r.n = c->n;
@@ -1212,7 +1216,10 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
                     f.recMk = P<uint32_t>(rec->mk);
                     f.recPid = P<uint32_t>(rec->pid);
                 }
-            } else if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB, nblk(c->n), 256, f))) {
+            } else if ((rc = launch(c, rec ? K_FLUID_NLBR : K_FLUID_NLB,
+                                    // (in a batch: gated_blocks workgroups, each building
+                                    // several blocks of particles)
+                                    c->capture && !rec ? gated_blocks(c->n) : nblk(c->n), 256, f))) {
                 return rc;
             }
             const Kern walk = rec ? (fm ? K_FLUID_NLR_F : K_FLUID_NLR) : (fm ? K_FLUID_NL_F : K_FLUID_NL);
