@@ -805,12 +805,12 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             const float diff = fmaxf(hh - rr, 0.0f);
             const float gs = -6.0f * kn * diff * diff;
             const float gx = gs * rx, gy = gs * ry, gz = gs * rz;
-            if (!bnd) acc.den += m2 * (vx * gx + vy * gy + vz * gz);
+            const float vr = vx * rx + vy * ry + vz * rz;
+            if (!bnd) acc.den += m2 * (gs * vr);
             const float cp = it.potential ? contact_potential(it.potential, sqrtf(rr), it.radius, it.strength) : 0.0f;
-            const float pf = m1 * m2 * (q1 + q2);
-            acc.fx += pf * gx;
-            acc.fy += pf * gy;
-            acc.fz += pf * gz;
+            // pressure along grad W; a mobile pair's artificial viscosity lies along it too and
+            // joins the same coefficient below
+            float pf = m1 * m2 * (q1 + q2);
             if (cp != 0.0f) {
                 const float cs = fdiv(m2 * cp, d2);
                 const float cx = cs * gx, cy = cs * gy, cz = cs * gz;
@@ -819,7 +819,6 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     const float irr = fdiv(1.0f, rr);
                     const float ix = rx * irr, iy = ry * irr, iz = rz * irr;
                     const float cfr = cx * rx + cy * ry + cz * rz;
-                    const float vr = vx * rx + vy * ry + vz * rz;
                     const float nx = ix * cfr, ny = iy * cfr, nz = iz * cfr;
                     float tx = vx - ix * vr, ty = vy - iy * vr, tz = vz - iz * vr;
                     const float itl = fdiv(1.0f, sqrtf(tx * tx + ty * ty + tz * tz));
@@ -834,9 +833,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                         acc.fz += itl2 * tz;
                     }
                 }
-                acc.fx += cx;
-                acc.fy += cy;
-                acc.fz += cz;
+                pf += cs;  // (the contact force cs grad W)
             }
             if (!bnd) {
                 const float den_ = rr + a.phys.eps_hh;
@@ -847,11 +844,11 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     acc.fz -= s * vz;
                 }
                 const float ff = mat1 == mat2 ? bf1 + bulk2 : it.dampening;
-                const float s = m1 * m2 * fdiv(ff * (vx * rx + vy * ry + vz * rz), (d1 + d2) * den_);
-                acc.fx -= s * gx;
-                acc.fy -= s * gy;
-                acc.fz -= s * gz;
+                pf -= m1 * m2 * fdiv(ff * vr, (d1 + d2) * den_);
             }
+            acc.fx += pf * gx;
+            acc.fy += pf * gy;
+            acc.fz += pf * gz;
         };
         // the buffered candidates' terms; the next neighbour's loads are issued before the
         // current pair's arithmetic, into the other of two register sets (no copies, so a
