@@ -277,8 +277,8 @@ const char* arp_batch_last_error(const arp_batch* batch);
 /* nsteps RK steps of length dt on every scene (the time of each context advances). */
 int arp_batch_step(arp_batch* batch, float dt, uint32_t nsteps);
 /* Autotuner: times trial_steps batch steps (after two untimed ones) for each candidate
- * row-sort workgroup size (particles per workgroup) and keeps the fastest (*chosen_div); the
- * scenes' states are restored after every trial. */
+ * row-sort workgroup size (particles per workgroup), in two rounds, and keeps the fastest
+ * (*chosen_div); the scenes' states are restored after every trial. */
 int arp_batch_tune(arp_batch* batch, float dt, uint32_t trial_steps, uint32_t* chosen_div);
 
 /* ---- particle insertion (FluidSim::add_particles, soft_body/mod.rs:180-206) ----------- */
