@@ -2010,6 +2010,18 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps, const RecRange* rr =
     uint32_t k = 0;
     const bool adv = !rr || rr->advance;
     while (k < nsteps) {
+        if (rr && rr->K >= 2 && nsteps - k >= kSpecMin && spec_ok(c)) {
+            // a recorded step speculates from its stage buffers: bring its input there
+            // (what the chain's reusing sort copies; a rebuild sorts from the landing ones)
+            RecStep& R = c->rec[(rr->j0 + k) % rr->K];
+            if (c->rec_prefilled != R.Xp[0].p) {
+                if ((rc = sync_yland(c))) return rc;
+                const size_t bytes = 16 * (size_t)c->n;
+                HIPCHK(c, hipMemcpyAsync(R.Xp[0].p, c->Yland[0].p, bytes, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(R.Xv[0].p, c->Yland[1].p, bytes, hipMemcpyDeviceToDevice, c->stream));
+                c->rec_prefilled = R.Xp[0].p;
+            }
+        }
         if (nsteps - k < kSpecMin || !spec_ok(c) ||
             (rr && (rr->K < 2 || c->rec_prefilled != c->rec[(rr->j0 + k) % rr->K].Xp[0].p))) {
             if ((rc = issue_step(c, dt, rr, k, nullptr))) return rc;
