@@ -196,7 +196,7 @@ def main():
         log("copy kernel: %d bytes read and %d bytes written per launch" % (4 * n, 4 * n))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding); "
+        print("bench.py measures one GPU: the library has no multi-GPU path (each scene lives on one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
