@@ -1,7 +1,8 @@
 /*
  * oracle.h -- CPU restatement of ar-physics' particle step.  TEST INFRASTRUCTURE ONLY:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
- * only as the checker; the product path never calls it.
+ * only to check results against (or, in bench.py, to time the CPU); the product path never
+ * calls it.
  *
  * Every function restates the reference GLSL/Rust it cites (paths relative to the
  * reference repository).  Two precisions are built from one source (oracle_impl.h):

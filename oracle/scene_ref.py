@@ -1,6 +1,6 @@
 """Scene-file oracle: a numpy restatement of the reference example runner's TOML scene
 reader and of FluidSim::new's particle generation.  TEST INFRASTRUCTURE ONLY: imported by
-tests/ as the checker of the native scene-file reader; the product path never imports it.
+tests/ to check the native scene-file reader against; the product path never imports it.
 
 Restated from the reference (paths relative to the reference repository):
 
