@@ -80,15 +80,28 @@ def _two_fluids(sc):
     return dataclasses.replace(sc, particles=p, materials=mats, interactions=S.resolve_interactions(mats, sc.h))
 
 
+def _swapped(sc):
+    """The scene with its two materials in the other order (fluid 0, boundary 1)."""
+    import dataclasses
+
+    mats = sc.materials[::-1].copy()
+    p, b = sc.particles.copy(), sc.boundary.copy()
+    p["mat"] = 1 - p["mat"]
+    b["mat"] = 1 - b["mat"]
+    return dataclasses.replace(sc, particles=p, boundary=b, materials=mats,
+                               interactions=S.resolve_interactions(mats, sc.h))
+
+
 def test_batch_of_mixed_material_forms(torch_cuda):
     """A batch takes the fast-material walk only when every scene has one mobile and one
-    boundary material; a scene with two fluids puts the whole batch on the general walk.
-    Both batches equal the scenes stepped alone (which take their own forms)."""
+    boundary material (each scene's own material indices); a scene with two fluids puts the
+    whole batch on the general walk.  Every batch equals the scenes stepped alone (which take
+    their own forms)."""
     from arp.sim import Batch
 
     dt = 5e-5
     base = _scenes()
-    for scenes in (base, base[:2] + [_two_fluids(base[2])] + base[3:]):
+    for scenes in (base, base[:1] + [_swapped(base[1])] + base[2:], base[:2] + [_two_fluids(base[2])] + base[3:]):
         print("case batch material forms", [len(sc.materials) for sc in scenes], flush=True)
         solo = _sims(scenes, "rk2", dt)
         for s in solo:
