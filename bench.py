@@ -253,6 +253,14 @@ def main():
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
     # the walk kernel that ran (most launches)
     fluid_kernel = max(fluid_kernels, key=lambda k: kern.get(k, {}).get("launches", 0))
+    if fluid_kernel in kern:
+        # the timed region's bound on one launch (its steps hold little but these walks, back
+        # to back): the step time over the stages per step
+        from arp import tableau
+
+        stages = int(tableau.tableau(args.integrator)["stages"])
+        if stages > 0:
+            kern[fluid_kernel]["timed_region_bound_us"] = 1e6 * sec / args.steps / stages
     sim.profile(False)
     fwd_adj = None
     ens = None
