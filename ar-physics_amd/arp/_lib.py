@@ -21,7 +21,7 @@ SYMBOLS = [
     "arp_scene_parse", "arp_scene_free", "arp_scene_error", "arp_scene_settings", "arp_scene_object",
     "arp_scene_generate", "arp_scene_arrays", "arp_scene_contains", "arp_scene_apply", "arp_toml_to_json",
     "arp_kernel_radius", "arp_num_materials", "arp_add_particles", "arp_scene_spawn", "arp_num_particles",
-    "arp_num_solids",
+    "arp_num_solids", "arp_neighbor_lists_fit",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
     L.arp_adjoint_stiffness_grad.argtypes = [vp, vp, C.c_uint32]
     L.arp_set_neighbor_skin.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_set_neighbor_max_age.argtypes = [vp, C.c_uint32]
+    L.arp_neighbor_lists_fit.argtypes = [C.c_uint32, C.c_uint32]
     L.arp_neighbor_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.arp_speculation_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]

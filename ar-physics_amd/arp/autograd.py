@@ -8,6 +8,11 @@ Columns of the state tensor: density, position x/y/z, velocity x/y/z, rows in th
 particle order.  Optional ``params`` ((nmat, 2): bulk_viscosity, viscosity per material)
 receives dL/dparams; its values must equal the scene's (the forward runs on the uploaded
 materials).
+
+A sim holds one recording.  Rollouts chained on one sim (y1 = rollout(sim, x0, ...),
+y2 = rollout(sim, y1, ...)) are still differentiated correctly: a backward whose recording
+was replaced records its rollout again from the saved input, which leaves the sim's live
+state at that rollout's end.
 """
 import numpy as np
 
@@ -37,6 +42,15 @@ def state_tensor(particles, device="cuda"):
     return torch.from_numpy(a[:, list(_COLS)].copy()).to(device)
 
 
+def _record(sim, x0h, dt, nsteps, checkpoint_every):
+    """Upload x0 (n, 7 host array) into the sim's state and record nsteps steps from it."""
+    p = sim.scene.particles.copy()
+    a = p.view(np.float32).reshape(len(p), 12)
+    a[:, list(_COLS)] = x0h
+    sim.upload_state(p)
+    sim.adjoint_record(dt, nsteps, checkpoint_every)
+
+
 def _rollout_fn():
     import torch
 
@@ -48,19 +62,23 @@ def _rollout_fn():
                 want = np.stack([sc.materials["bulk_visc"], sc.materials["visc"]], 1).astype(np.float32)
                 if not np.array_equal(params.detach().cpu().numpy().astype(np.float32), want):
                     raise ValueError("params differ from the scene's materials (upload a new scene to change them)")
-            p = sc.particles.copy()
-            a = p.view(np.float32).reshape(len(p), 12)
-            a[:, list(_COLS)] = x0.detach().cpu().numpy()
-            sim.upload_state(p)
-            sim.adjoint_record(dt, nsteps, checkpoint_every)
+            x0h = x0.detach().cpu().numpy().copy()
+            _record(sim, x0h, dt, nsteps, checkpoint_every)
             y = state_tensor(sim.particles(), x0.device)
             ctx.sim = sim
             ctx.has_params = params is not None
+            # the sim holds one recording: a later rollout on it replaces this one, and the
+            # backward then records this rollout again from its saved input
+            ctx.replay = (x0h, dt, nsteps, checkpoint_every)
+            ctx.generation = sim.record_generation
             return y
 
         @staticmethod
         def backward(ctx, gy):
             sim = ctx.sim
+            if sim.record_generation != ctx.generation:
+                _record(sim, *ctx.replay)
+                ctx.generation = sim.record_generation
             n = gy.shape[0]
             g_out = records_from_state(gy.contiguous())
             g_in = torch.zeros((n, 12), dtype=torch.float32, device=gy.device)

@@ -54,10 +54,16 @@ class FluidSim:
         """FluidSim::init: explicit RK methods need no state history."""
         return self
 
+    def _touch(self):
+        """Counts the calls that replace the state or the recording (see autograd.rollout)."""
+        self.record_generation = getattr(self, "record_generation", 0) + 1
+
     def update(self, nframes=1):
+        self._touch()
         check(self.ctx, lib().arp_update(self.ctx, nframes))
 
     def step(self, dt, nsteps=1):
+        self._touch()
         check(self.ctx, lib().arp_step(self.ctx, float(dt), int(nsteps)))
 
     @property
@@ -110,6 +116,7 @@ class FluidSim:
     def upload_state(self, particles, solids=None):
         particles = np.ascontiguousarray(particles)
         so = np.ascontiguousarray(solids) if solids is not None else None
+        self._touch()
         check(self.ctx, lib().arp_upload_state(self.ctx, _p(particles), len(particles), _p(so),
                                                len(so) if so is not None else 0))
 
@@ -143,6 +150,9 @@ class FluidSim:
         """Advance nsteps steps of length dt, recording for adjoint_backward (a checkpoint
         every checkpoint_every steps; default: the whole trajectory in one segment)."""
         k = nsteps if checkpoint_every is None else checkpoint_every
+        # a new recording replaces the previous one: its generation tells holders of the old
+        # one (a rollout's backward) that they must record again
+        self._touch()
         check(self.ctx, lib().arp_adjoint_record(self.ctx, float(dt), int(nsteps), int(k)))
 
     def adjoint_backward(self, d_grad_out, d_grad_in, want_params=True):
@@ -177,10 +187,12 @@ class FluidSim:
         48 entries per particle).  always_build rebuilds at every stage (testing);
         speculate=False issues the rebuild chain at every stage (same results)."""
         flags = (1 if always_build else 0) | (0 if speculate else 2)
+        self._touch()
         check(self.ctx, lib().arp_set_neighbor_skin(self.ctx, float(skin), int(cap), flags))
 
     def set_neighbor_max_age(self, stages):
         """Rebuild kept lists older than this many RK stages (0: only when invalid)."""
+        self._touch()
         check(self.ctx, lib().arp_set_neighbor_max_age(self.ctx, int(stages)))
 
     def neighbor_stats(self):

@@ -94,6 +94,12 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_spec_restore", "k_fluid_nl_f_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
+
+// The list walks form 32-bit byte offsets of entries (entry k of particle i lies at element
+// k * (n + kPad) + i), so every list entry must lie below 4 GiB.
+static bool nl_fits_u32(uint32_t cap, uint32_t n) {
+    return (uint64_t)cap * ((uint64_t)n + kPad) * 4u < (1ull << 32);
+}
 constexpr uint32_t kRecNb = 32;  // neighbours per particle and stage kept by a recording
 
 struct DBuf {
@@ -224,6 +230,7 @@ struct arp_ctx {
     DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec;
     // adjoint recording
     bool rec_valid = false;
+    uint32_t rec_nb_cap = kRecNb;  // neighbours per particle and stage of the recording
     float rec_dt = 0.0f;
     uint32_t rec_nsteps = 0, rec_K = 0, rec_seg = 0;
     ArpTableau rec_tab{};
@@ -726,9 +733,9 @@ float box_extent(const GridP& g) {
 
 // Lists are kept for fluid-only scenes (the solid path searches the grid itself).
 bool nl_enabled(const arp_ctx* c) {
-    // (the walks address list entries with 32-bit offsets)
+    // (the walks address list entries with 32-bit byte offsets)
     return c->nl_skin > 0.0f && !c->solids_active && c->n != 0 && !c->nl_suspend &&
-           (uint64_t)c->nl_cap * ((uint64_t)c->n + kPad) < (1ull << 32);
+           nl_fits_u32(c->nl_cap, c->n);
 }
 
 int nl_prepare(arp_ctx* c) {
@@ -1184,7 +1191,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
             f.nbList = P<uint32_t>(rec->nbList[s]);
             f.nbCount = P<uint32_t>(rec->nbCount[s]);
             f.nbOver = rec->fl + s;
-            f.nbCap = rec_cap(c);
+            f.nbCap = c->rec_nb_cap;  // the capacity the record was allocated with
         }
         if (nl) {
             // the list pair: one searches and builds, the other walks the kept lists; the
@@ -1279,7 +1286,7 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
     if ((rc = dalloc(c, R.stamps, 4 * ARP_MAX_STAGES))) return rc;
     if (R.stamps.p != st0) HIPCHK(c, hipMemsetAsync(R.stamps.p, 0, 4 * ARP_MAX_STAGES, c->stream));
     for (int s = 0; s < S; s++)
-        if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)rec_cap(c) * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
+        if ((rc = dalloc(c, R.nbList[s], 4 * (size_t)c->rec_nb_cap * n)) || (rc = dalloc(c, R.nbCount[s], 4 * n))) return rc;
     for (int s = 0; s < S; s++) {
         const void* rs0 = R.rowStamp[s].p;
         if ((rc = dalloc(c, R.Xp[s], 16 * n)) || (rc = dalloc(c, R.Xv[s], 16 * n)) ||
@@ -2144,8 +2151,10 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
         }
     if (nsteps == 0 || checkpoint_every == 0 || !(dt > 0.0f) || !std::isfinite(dt))
         return fail(c, ARP_E_ARG, "adjoint_record needs nsteps >= 1, checkpoint_every >= 1 and dt > 0");
-    if ((uint64_t)rec_cap(c) * ((uint64_t)c->n + kPad) * 4u >= (1ull << 32))
+    if (!nl_fits_u32(rec_cap(c), c->n))
         return fail(c, ARP_E_LIMIT, "the neighbour record of %u particles exceeds 4 GB per stage", c->n);
+    // every step of this recording, and every replay of it, keeps this many neighbours
+    c->rec_nb_cap = rec_cap(c);
     HIPCHK(c, hipSetDevice(c->device));
     const uint32_t K = std::min(checkpoint_every, nsteps);
     const uint32_t nseg = (nsteps + K - 1) / K;
@@ -2225,6 +2234,8 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
         return fail(c, ARP_E_LIMIT, "material parameter gradients need at most %d materials", kMatLds);
     if (memcmp(&c->rec_tab, &c->tab, sizeof(ArpTableau)) != 0)
         return fail(c, ARP_E_STATE, "the integrator changed after the trajectory was recorded");
+    if (rec_cap(c) != c->rec_nb_cap)  // replays write the recorded lists
+        return fail(c, ARP_E_STATE, "the neighbour settings changed after the trajectory was recorded");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t n = std::max<uint32_t>(c->n, 1) + kPad;
     const int S = c->tab.stages;
@@ -2486,57 +2497,89 @@ int arp_compute_forces(arp_ctx* c, const ArpParticle* d_state, const ArpSolidPar
     return ARP_OK;
 }
 
-int arp_lincomb(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
-                const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out, ArpSolidParticle* d_solid_out) {
-    int rc;
-    if ((rc = need_scene(c))) return rc;
-    if (nterms == 0 || !coefs || !d_terms || !d_out) return fail(c, ARP_E_ARG, "lincomb: bad arguments");
-    // reduce_owned/reduce_ref: chunks of 5, combined again with coefficient 1
-    // (particles.rs:381-417)
-    if (nterms > 5) {
-        uint32_t nch = (nterms + 4) / 5;
-        if (nch > 5) return fail(c, ARP_E_LIMIT, "lincomb of more than 25 terms");
-        std::vector<DBuf> tmp(nch), stmp(nch);
-        std::vector<const ArpParticle*> tp(nch);
-        std::vector<const ArpSolidParticle*> sp(nch);
-        std::vector<float> ones(nch, 1.0f);
-        for (uint32_t k = 0; k < nch; k++) {
-            if ((rc = dalloc(c, tmp[k], sizeof(ArpParticle) * (size_t)std::max(c->n, 1u)))) return rc;
-            if (d_solid_terms && d_solid_out)
-                if ((rc = dalloc(c, stmp[k], sizeof(ArpSolidParticle) * (size_t)std::max(c->ns, 1u)))) return rc;
-            uint32_t k0 = 5 * k, nk = std::min(5u, nterms - k0);
-            rc = arp_lincomb(c, nk, coefs + k0, d_terms + k0, d_solid_terms ? d_solid_terms + k0 : nullptr,
-                             P<ArpParticle>(tmp[k]), d_solid_out ? P<ArpSolidParticle>(stmp[k]) : nullptr);
-            if (rc) return rc;
-            tp[k] = P<ArpParticle>(tmp[k]);
-            sp[k] = d_solid_out ? P<ArpSolidParticle>(stmp[k]) : nullptr;
-        }
-        rc = arp_lincomb(c, nch, ones.data(), tp.data(), d_solid_out ? sp.data() : nullptr, d_out, d_solid_out);
-        hipStreamSynchronize(c->stream);
-        for (uint32_t k = 0; k < nch; k++) {
-            if (tmp[k].p) hipFree(tmp[k].p);
-            if (stmp[k].p) hipFree(stmp[k].p);
-        }
-        return rc;
-    }
+// One launch of lin_comb_N / s_lin_comb_N over at most 5 terms; solids take part only when
+// both their terms and their output are given.
+static int lincomb5(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
+                    const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out, ArpSolidParticle* d_solid_out) {
+    const bool sol = d_solid_terms && d_solid_out;
     LincombArgs a{};
     a.n = c->n;
-    a.ns = (d_solid_terms && d_solid_out) ? c->ns : 0;
+    a.ns = sol ? c->ns : 0;
     a.nterms = nterms;
     for (uint32_t k = 0; k < nterms; k++) {
         a.coef[k] = coefs[k];
         a.terms[k] = d_terms[k];
-        a.sterms[k] = d_solid_terms ? d_solid_terms[k] : nullptr;
+        a.sterms[k] = sol ? d_solid_terms[k] : nullptr;
     }
     a.out = d_out;
-    a.sout = a.ns ? d_solid_out : nullptr;
+    a.sout = sol ? d_solid_out : nullptr;
     return launch(c, K_LINCOMB, nblk(std::max(c->n, a.ns)), 256, a);
+}
+
+// reduce_owned/reduce_ref (particles.rs:381-417): terms in chunks of 5, each chunk combined
+// into a partial, and the partials combined again with coefficient 1.  part[k] / spart[k]
+// receive chunk k (spart is used only when solids take part).
+static int lincomb_chunked(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
+                           const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out,
+                           ArpSolidParticle* d_solid_out, ArpParticle* const* part, ArpSolidParticle* const* spart) {
+    const bool sol = d_solid_terms && d_solid_out;
+    const uint32_t nch = (nterms + 4) / 5;
+    const ArpParticle* tp[5];
+    const ArpSolidParticle* sp[5];
+    const float ones[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+    int rc;
+    for (uint32_t k = 0; k < nch; k++) {
+        const uint32_t k0 = 5 * k, nk = std::min(5u, nterms - k0);
+        if ((rc = lincomb5(c, nk, coefs + k0, d_terms + k0, sol ? d_solid_terms + k0 : nullptr, part[k],
+                           sol ? spart[k] : nullptr)))
+            return rc;
+        tp[k] = part[k];
+        sp[k] = sol ? spart[k] : nullptr;
+    }
+    return lincomb5(c, nch, ones, tp, sol ? sp : nullptr, d_out, sol ? d_solid_out : nullptr);
+}
+
+static int lincomb_check(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
+                         ArpParticle* d_out) {
+    int rc;
+    if ((rc = need_scene(c))) return rc;
+    if (nterms == 0 || !coefs || !d_terms || !d_out) return fail(c, ARP_E_ARG, "lincomb: bad arguments");
+    for (uint32_t k = 0; k < nterms; k++)
+        if (!d_terms[k]) return fail(c, ARP_E_ARG, "lincomb: term %u is null", k);
+    if (nterms > 25) return fail(c, ARP_E_LIMIT, "lincomb of more than 25 terms");
+    return ARP_OK;
+}
+
+int arp_lincomb(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
+                const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out, ArpSolidParticle* d_solid_out) {
+    int rc;
+    if ((rc = lincomb_check(c, nterms, coefs, d_terms, d_out))) return rc;
+    if (nterms <= 5) return lincomb5(c, nterms, coefs, d_terms, d_solid_terms, d_out, d_solid_out);
+    const bool sol = d_solid_terms && d_solid_out;
+    const uint32_t nch = (nterms + 4) / 5;
+    DBuf tmp[5], stmp[5];
+    ArpParticle* part[5] = {};
+    ArpSolidParticle* spart[5] = {};
+    for (uint32_t k = 0; k < nch && rc == ARP_OK; k++) {
+        rc = dalloc(c, tmp[k], sizeof(ArpParticle) * (size_t)std::max(c->n, 1u));
+        if (rc == ARP_OK && sol) rc = dalloc(c, stmp[k], sizeof(ArpSolidParticle) * (size_t)std::max(c->ns, 1u));
+        part[k] = P<ArpParticle>(tmp[k]);
+        spart[k] = P<ArpSolidParticle>(stmp[k]);
+    }
+    if (rc == ARP_OK) rc = lincomb_chunked(c, nterms, coefs, d_terms, d_solid_terms, d_out, d_solid_out, part, spart);
+    hipStreamSynchronize(c->stream);  // the partials are freed below
+    for (uint32_t k = 0; k < nch; k++) {
+        if (tmp[k].p) hipFree(tmp[k].p);
+        if (stmp[k].p) hipFree(stmp[k].p);
+    }
+    return rc;
 }
 
 int arp_velocity(arp_ctx* c, float r, const ArpParticle* d_in, const ArpSolidParticle* d_sin, ArpParticle* d_out,
                  ArpSolidParticle* d_sout) {
     int rc;
     if ((rc = need_scene(c))) return rc;
+    if (!d_in || !d_out) return fail(c, ARP_E_ARG, "velocity: null particle buffer");
     VelocityArgs a{c->n, (d_sin && d_sout) ? c->ns : 0, r, d_in, d_sin, d_out, d_sout};
     return launch(c, K_VELOCITY, nblk(std::max(c->n, a.ns)), 256, a);
 }
@@ -2552,6 +2595,8 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
     c->nl_always = (flags & ARP_NL_ALWAYS_BUILD) != 0;
     c->nl_nospec = (flags & ARP_NL_NO_SPECULATION) != 0;
     c->nl_dirty = true;
+    // a replay must walk the recorded steps with the settings they were recorded with
+    c->rec_valid = false;
     if (c->nlWords.p) {
         HIPCHK(c, hipSetDevice(c->device));
         FillArgs fa{P<uint32_t>(c->nlWords) + 8, 2, 0};  // statistics restart
@@ -2561,10 +2606,16 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
     return ARP_OK;
 }
 
+int arp_neighbor_lists_fit(uint32_t cap, uint32_t n) {
+    if (cap == 0) cap = 48;
+    return nl_fits_u32(cap, n) ? 1 : 0;
+}
+
 int arp_set_neighbor_max_age(arp_ctx* c, uint32_t stages) {
     if (!c) return ARP_E_ARG;
     c->nl_max_age = stages;
     c->nl_dirty = true;
+    c->rec_valid = false;
     return ARP_OK;
 }
 

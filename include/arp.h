@@ -183,6 +183,10 @@ int arp_upload_state(arp_ctx* ctx, const ArpParticle* particles, uint32_t n,
  * results are the same; for comparison and debugging). */
 #define ARP_NL_NO_SPECULATION 2u
 int arp_set_neighbor_skin(arp_ctx* ctx, float skin, uint32_t cap, uint32_t flags);
+/* 1 when lists of cap entries (0: 48) can be kept for n particles: every entry must lie
+ * below 4 GiB (cap * (n + 16) * 4 bytes); otherwise the steps search at every stage.  No
+ * device needed; no counterpart in the reference. */
+int arp_neighbor_lists_fit(uint32_t cap, uint32_t n);
 /* Rebuild lists older than this many RK stages even when they are still valid (0: only when
  * invalid).  A build re-sorts the particles into bucket order, which keeps the neighbour
  * gathers local as the particles move. */

@@ -283,3 +283,58 @@ def test_autograd_rollout(torch_cuda):
     assert np.allclose(x0.grad.cpu().numpy(), gi, rtol=1e-6, atol=1e-6 * np.abs(gi).max())
     assert np.allclose(params.grad.cpu().numpy(), gp[:, :2], rtol=1e-6, atol=1e-9)
     sim.close()
+
+
+def test_autograd_chained_rollouts_on_one_sim(torch_cuda):
+    """Two rollouts chained on one sim: the first one's backward must not use the second
+    one's recording (it records its own rollout again).  Checked against the same chain
+    on two separate sims."""
+    from arp.autograd import rollout, state_tensor
+    from arp.sim import FluidSim
+
+    torch = torch_cuda
+    sc = _scene(True)
+    dt = 5e-5
+
+    def chain(sim_a, sim_b):
+        x0 = state_tensor(sc.particles).requires_grad_(True)
+        y1 = rollout(sim_a, x0, dt, 2)
+        y2 = rollout(sim_b, y1, dt, 2)
+        loss = (y1[:, 4:7] ** 2).sum() + (y2[:, 4:7] ** 3).sum() + y2[:, 1].sum()
+        loss.backward()
+        return x0.grad.cpu().numpy()
+
+    shared = FluidSim(sc, integrator="rk2", timestep=dt)
+    g_shared = chain(shared, shared)
+    a = FluidSim(sc, integrator="rk2", timestep=dt)
+    b = FluidSim(sc, integrator="rk2", timestep=dt)
+    g_sep = chain(a, b)
+    assert np.isfinite(g_sep).all() and np.abs(g_sep).max() > 0
+    assert np.allclose(g_shared, g_sep, rtol=1e-6, atol=1e-6 * np.abs(g_sep).max())
+    for s in (shared, a, b):
+        s.close()
+
+
+def test_backward_refuses_changed_neighbor_settings(torch_cuda):
+    """A replay must walk the recorded steps with the list capacity they were recorded with:
+    changing the list settings after a recording drops it."""
+    from arp._lib import ArpError
+    from arp.sim import FluidSim
+
+    torch = torch_cuda
+    sc = _scene(False)
+    dt = 5e-5
+    sim = FluidSim(sc, integrator="rk2", timestep=dt)
+    sim.set_neighbor_skin(0.2)
+    sim.adjoint_record(dt, 4, 2)
+    sim.set_neighbor_skin(0.2, cap=1024)
+    d_out = torch.ones((sc.n, 12), dtype=torch.float32, device="cuda")
+    d_in = torch.zeros_like(d_out)
+    with pytest.raises(ArpError):
+        sim.adjoint_backward(d_out, d_in)
+    # recorded again with the new capacity, the sweep runs (with replays)
+    sim.adjoint_record(dt, 4, 2)
+    sim.adjoint_backward(d_out, d_in)
+    sim.synchronize()
+    assert torch.isfinite(d_in).all()
+    sim.close()

@@ -335,3 +335,56 @@ def test_solid_step_parity(torch_cuda, oracle_mod, integ):
     print("plastic strain rel err %.2e" % e, flush=True)
     assert e <= TOL
     assert np.array_equal(so["ref_pos"], sc.solids["ref_pos"])
+
+
+@pytest.mark.parametrize("nterms", [3, 9])
+def test_lincomb_and_velocity_with_solids(torch_cuda, oracle_mod, nterms):
+    """s_lin_comb_N (particles.rs:309-313) and vel_solid (particles.rs:206-250) on a scene with
+    solids: plastic strain and reference positions combine like den/pos/vel; 9 terms take the
+    chunk-of-5 path.  Solid outputs are left alone when no solid terms are given."""
+    from arp.sim import from_device, to_device
+
+    sc = _solid_scene(dim=3, plastic=True)
+    rng = np.random.default_rng(11)
+    sim = _sim(sc)
+    o = oracle_mod.Oracle(sc)
+    terms_h, sterms_h, terms_d, sterms_d = [], [], [], []
+    for _ in range(nterms):
+        p = sc.particles.copy()
+        p["den"] = rng.normal(size=sc.n)
+        p["pos"][:, :3] = rng.normal(size=(sc.n, 3))
+        p["vel"][:, :3] = rng.normal(size=(sc.n, 3))
+        s = sc.solids.copy()
+        s["ref_pos"][:, :3] = rng.normal(size=(sc.ns, 3))
+        s["stress"][:] = rng.normal(size=s["stress"].shape)
+        terms_h.append(p)
+        sterms_h.append(s)
+        terms_d.append(to_device(p))
+        sterms_d.append(to_device(s))
+    co = rng.normal(size=nterms).astype(np.float32)
+    out, sout = to_device(wire.particles(sc.n)), to_device(wire.solids(sc.ns))
+    sim.lincomb(co, terms_d, out, solid_terms=sterms_d, solid_out=sout)
+    torch_cuda.cuda.synchronize()
+    got, gs = from_device(out, wire.PARTICLE), from_device(sout, wire.SOLID)
+    ref = o.lincomb(co, [o.state("f", particles=p, solids=s) for p, s in zip(terms_h, sterms_h)])
+    for k in ("den", "pos", "vel"):
+        assert oracle_mod.rel_err(got[k], ref[k]) <= 1e-6, k
+    assert oracle_mod.rel_err(gs["ref_pos"], ref["ref"]) <= 1e-6
+    assert oracle_mod.rel_err(gs["stress"].reshape(-1, 16), ref["stress"]) <= 1e-6
+    assert np.array_equal(gs["part_id"], sterms_h[0]["part_id"])
+    # no solid terms: the solid output is not touched (and nothing reads a null term)
+    sout2 = to_device(wire.solids(sc.ns))
+    sim.lincomb(co, terms_d, out, solid_terms=None, solid_out=sout2)
+    torch_cuda.cuda.synchronize()
+    assert not sout2.any().item()
+    # vel_solid
+    vout, svout = to_device(wire.particles(sc.n)), to_device(wire.solids(sc.ns))
+    sim.velocity(0.25, terms_d[0], vout, d_solid_in=sterms_d[0], d_solid_out=svout)
+    torch_cuda.cuda.synchronize()
+    gv, gsv = from_device(vout, wire.PARTICLE), from_device(svout, wire.SOLID)
+    rv = o.velocity(0.25, o.state("f", particles=terms_h[0], solids=sterms_h[0]))
+    for k in ("den", "pos", "vel"):
+        assert np.array_equal(gv[k], rv[k]), k
+    assert np.array_equal(gsv["ref_pos"], rv["ref"]) and np.array_equal(gsv["stress"].reshape(-1, 16), rv["stress"])
+    assert np.array_equal(gsv["part_id"], sterms_h[0]["part_id"])
+    sim.close()

@@ -80,3 +80,15 @@ def test_graft_entry_build_is_importable():
     ge = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(ge)
     assert callable(ge.build) and callable(ge.smoke)
+
+
+def test_neighbor_list_size_limit():
+    """The list walks address entries with 32-bit byte offsets: lists are kept only while
+    cap * (n + 16) * 4 bytes stay below 4 GiB (entry k of particle i at k * (n + 16) + i)."""
+    L = _lib.lib()
+    for cap in (8, 48, 1024):
+        nmax = -(-(1 << 32) // (4 * cap)) - 16  # first n whose entries reach 2^32 bytes
+        assert L.arp_neighbor_lists_fit(cap, nmax - 1) == 1
+        assert L.arp_neighbor_lists_fit(cap, nmax) == 0
+    assert L.arp_neighbor_lists_fit(0, 1 << 20) == 1  # default 48 entries at the 1 M column
+    assert L.arp_neighbor_lists_fit(1024, 1 << 20) == 0  # 4 GiB of lists: not kept
