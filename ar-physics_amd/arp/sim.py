@@ -132,14 +132,17 @@ class FluidSim:
     def neighbors_update(self, d_state):
         check(self.ctx, lib().arp_neighbors_update(self.ctx, self._dp(d_state)))
 
-    def lincomb(self, coefs, terms, out, solid_terms=None, solid_out=None):
+    def lincomb(self, coefs, terms, out, solid_terms=None, solid_out=None, noalloc=False):
+        """out = sum_t coefs[t] * terms[t] (device std430 buffers); noalloc: the variant that
+        keeps its chunk partials in the library's fixed per-GPU scratch array."""
         n = len(terms)
         co = np.asarray(coefs, np.float32)
         tp = (C.c_void_p * n)(*[t.data_ptr() for t in terms])
         sp = (C.c_void_p * n)(*[t.data_ptr() for t in solid_terms]) if solid_terms else None
-        check(self.ctx, lib().arp_lincomb(self.ctx, n, _p(co), C.cast(tp, C.c_void_p),
-                                          C.cast(sp, C.c_void_p) if sp is not None else None,
-                                          self._dp(out), self._dp(solid_out)))
+        fn = lib().arp_lincomb_noalloc if noalloc else lib().arp_lincomb
+        check(self.ctx, fn(self.ctx, n, _p(co), C.cast(tp, C.c_void_p),
+                           C.cast(sp, C.c_void_p) if sp is not None else None,
+                           self._dp(out), self._dp(solid_out)))
 
     def velocity(self, r, d_in, d_out, d_solid_in=None, d_solid_out=None):
         check(self.ctx, lib().arp_velocity(self.ctx, float(r), self._dp(d_in), self._dp(d_solid_in),

@@ -5,7 +5,8 @@
 1. hipcc --genco --offload-arch=gfx950 csrc/kernels.hip  -> csrc/_build/kernels.hsaco
 2. the code object is embedded (csrc/embed.S, .incbin) into the host runtime
    csrc/runtime.cpp, compiled with g++ against the HIP runtime API, and linked with the
-   host-only scene-file reader (csrc/toml.cpp, csrc/scene_file.cpp)    -> arp/libarp.so
+   host-only scene-file reader (csrc/toml.cpp, csrc/scene_file.cpp) and the per-GPU scratch
+   array (csrc/scratch.hip, hipcc)                                        -> arp/libarp.so
 """
 import os
 import shutil
@@ -54,8 +55,14 @@ def build(verbose=False):
         run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math",
              "-I" + os.path.join(ROCM, "include"), "-c", "-o", o, os.path.join(CSRC, name + ".cpp")])
         host_objs.append(o)
+    # the fixed per-GPU scratch array of the allocation-free entry points: a HIP translation
+    # unit of its own, so the HIP runtime registers the array and places it on each device
+    scr = os.path.join(BUILD, "scratch.o")
+    run([hipcc, "-c", "-fPIC", "--offload-arch=gfx950", "-O2", "-std=c++17",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-o", scr, os.path.join(CSRC, "scratch.hip")])
     tmp = OUT + ".tmp"
-    run(["g++", "-shared", "-o", tmp, rt] + host_objs + [emb, "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+    run(["g++", "-shared", "-o", tmp, rt, scr] + host_objs + [emb, "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
          "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-Wl,--no-undefined"])
     shutil.move(tmp, OUT)
     if verbose:

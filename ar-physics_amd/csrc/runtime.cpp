@@ -16,6 +16,7 @@
 
 #include "../../include/arp.h"
 #include "common.h"
+#include "scratch.h"
 
 using namespace arp;
 
@@ -229,6 +230,11 @@ struct arp_ctx {
     const void* rec_prefilled = nullptr;
     DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec;
     // adjoint recording
+    // this GPU's fixed scratch array (scratch.hip) for the entry points that allocate nothing:
+    // its device address, its size, and the HIP error of the lookup (0: found)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int scratch_err = 0;
     bool rec_valid = false;
     uint32_t rec_nb_cap = kRecNb;  // neighbours per particle and stage of the recording
     float rec_dt = 0.0f;
@@ -1597,6 +1603,8 @@ int arp_create(int device, uint32_t flags, arp_ctx** out) {
         return ARP_E_HIP;
     }
     c->stream = c->own_stream;
+    // with the device current: this GPU's scratch array (looked up once per device)
+    c->scratch_err = arp_scratch_acquire(device, &c->scratch, &c->scratch_bytes);
     arp_tableau_by_name("rk4", &c->tab);
     *out = c;
     return ARP_OK;
@@ -2573,6 +2581,35 @@ int arp_lincomb(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpPartic
         if (stmp[k].p) hipFree(stmp[k].p);
     }
     return rc;
+}
+
+int arp_lincomb_noalloc(arp_ctx* c, uint32_t nterms, const float* coefs, const ArpParticle* const* d_terms,
+                        const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out,
+                        ArpSolidParticle* d_solid_out) {
+    int rc;
+    if ((rc = lincomb_check(c, nterms, coefs, d_terms, d_out))) return rc;
+    if (nterms <= 5) return lincomb5(c, nterms, coefs, d_terms, d_solid_terms, d_out, d_solid_out);
+    if (c->scratch_err || !c->scratch)
+        return fail(c, ARP_E_HIP, "no device scratch on device %d (lookup error %d)", c->device, c->scratch_err);
+    // the chunk partials live in this GPU's scratch array: particle partials, then solid ones,
+    // each 256-byte aligned; every byte read is written first in the same call
+    const bool sol = d_solid_terms && d_solid_out;
+    const uint32_t nch = (nterms + 4) / 5;
+    const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pb = al(sizeof(ArpParticle) * (size_t)c->n), sb = sol ? al(sizeof(ArpSolidParticle) * (size_t)c->ns) : 0;
+    if (nch * (pb + sb) > c->scratch_bytes)
+        return fail(c, ARP_E_LIMIT,
+                    "lincomb_noalloc: %u terms of %u particles and %u solids need %zu bytes of partials, the "
+                    "fixed device scratch holds %zu",
+                    nterms, c->n, sol ? c->ns : 0, nch * (pb + sb), c->scratch_bytes);
+    char* base = static_cast<char*>(c->scratch);
+    ArpParticle* part[5] = {};
+    ArpSolidParticle* spart[5] = {};
+    for (uint32_t k = 0; k < nch; k++) {
+        part[k] = reinterpret_cast<ArpParticle*>(base + k * pb);
+        spart[k] = sol ? reinterpret_cast<ArpSolidParticle*>(base + nch * pb + k * sb) : nullptr;
+    }
+    return lincomb_chunked(c, nterms, coefs, d_terms, d_solid_terms, d_out, d_solid_out, part, spart);
 }
 
 int arp_velocity(arp_ctx* c, float r, const ArpParticle* d_in, const ArpSolidParticle* d_sin, ArpParticle* d_out,

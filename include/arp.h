@@ -211,6 +211,18 @@ int arp_compute_forces(arp_ctx* ctx, const ArpParticle* d_state,
 int arp_lincomb(arp_ctx* ctx, uint32_t nterms, const float* coefs,
                 const ArpParticle* const* d_terms, const ArpSolidParticle* const* d_solid_terms,
                 ArpParticle* d_out, ArpSolidParticle* d_solid_out);
+/* arp_lincomb that allocates nothing on the device.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same results as arp_lincomb, bit for bit.
+ * With more than 5 terms the chunk partials live in a fixed 128 MiB scratch array that each
+ * GPU holds from the library's first use until the process ends: nchunks * (48 B * n
+ * [+ 96 B * ns with solids]), each part rounded up to 256 B, must fit (ARP_E_LIMIT
+ * otherwise), where nchunks = ceil(nterms / 5).  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams or threads) race. */
+int arp_lincomb_noalloc(arp_ctx* ctx, uint32_t nterms, const float* coefs,
+                        const ArpParticle* const* d_terms,
+                        const ArpSolidParticle* const* d_solid_terms, ArpParticle* d_out,
+                        ArpSolidParticle* d_solid_out);
 /* ArithShaders::velocity (particles.rs:438-446, GLSL vel/vel_solid). */
 int arp_velocity(arp_ctx* ctx, float r, const ArpParticle* d_in,
                  const ArpSolidParticle* d_solid_in, ArpParticle* d_out,
