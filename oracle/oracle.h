@@ -13,10 +13,12 @@
  * order per kind (the reference appends with atomicAdd, neighbor_list.rs:101), so the
  * particles refused by the 128-slot cap are the highest ids of their cell.
  *
- * Parity status: every function is checked by tests/test_oracle.py against golden vectors,
- * closed forms, brute force or the FP64 build, except the ones marked "parity unpinned":
- *   orc_solid_forces_* (elastic force, hourglass, self contact, plastic flow): parity unpinned
- *     beyond FP32-vs-FP64 agreement and the zero-strain cases;
+ * Parity status: every function is checked by tests/test_oracle.py and
+ * tests/test_oracle_solid.py against closed forms, invariants, brute force or the FP64
+ * build (the reference holds no golden vectors); orc_solid_forces_* by rigid motion, pair
+ * antisymmetry, the St Venant-Kirchhoff energy gradient, the Hooke traction of a uniform
+ * stretch and shear, the hourglass restoring force, the plastic threshold and the
+ * self-contact closed form.  Marked "parity unpinned":
  *   the tableau taken for the name "rk2" (the explicit midpoint rule) and velocity Verlet
  *     (numerical-integration crate absent from the reference): parity unpinned.
  */
