@@ -33,6 +33,8 @@ def box_scene(pos, vel=None, den=1000.0, bpos=None, bounds=((0, 0, 0), (1, 1, 1)
             sm["mass"] = fm["mass"]
         mats.append(sm)
         p["mat"][np.asarray(solids_mask)] = 2
+    for m in mats_extra or ():  # further materials after these (ids 2, 3, ... or 3, ...)
+        mats.append(m)
     mats = np.array(mats, wire.MATERIAL)
     if bpos is None:
         b = wire.particles(0)
@@ -155,3 +157,44 @@ def uniform_block(nside, spacing, dim=3, jitter=0.0, seed=0, origin=(0, 0, 0)):
 
 def kernel_closed_form(dim, h):
     return 4.0 / (math.pi * h**8) if dim == 2 else 315.0 / (64.0 * math.pi * h**9)
+
+
+def deformed_solid_scene(dim=3, plastic=False, with_fluid=True, seed=0, order=2):
+    """An elastic (optionally plastic) block of strain order `order` beside a jittered fluid
+    on a boundary floor, the block deformed by a random affine map plus noise, every particle
+    moving."""
+    sp = 0.01
+    h = 1.3 * sp
+    nside = 6
+    X = uniform_block(nside, sp, dim, origin=(0.3, 0.3, 0.3))
+    parts = [X]
+    mask = [np.ones(len(X), bool)]
+    if with_fluid:
+        Fl = uniform_block(8, sp, dim, jitter=0.1, seed=seed, origin=(0.25, 0.2, 0.25))
+        Fl = Fl[~np.all(np.abs(Fl - X.mean(0)) < (nside / 2 + 1) * sp, axis=1)]
+        parts.append(Fl)
+        mask.append(np.zeros(len(Fl), bool))
+    P = np.concatenate(parts)
+    mask = np.concatenate(mask)
+    sm = S.solid_material(1000.0, -2e5, -1e5, normal_damp=-20.0, shear_damp=-30.0, strain_order=order,
+                          plastic=dict(yield_strength=50.0, relaxation_time=0.5) if plastic else None)
+    sm["mass"] = 1000.0 * sp**dim
+    fm = S.fluid_material(h, 0.02, density=1000.0, speed_of_sound=20.0, viscosity=0.01)
+    fm["mass"] = 1000.0 * sp**dim
+    B = uniform_block(12, sp, dim, origin=(0.2, 0.17, 0.2))
+    B = B[B[:, 1] < 0.2]
+    dims = (0.8, 0.8, 0.8) if dim == 3 else (0.8, 0.8, 0)
+    sc = box_scene(P, bpos=B, bounds=((0.0, 0.0, 0.0), dims), h=h, fluid=fm, solids_mask=mask, solid_mat=sm)
+    rng = np.random.default_rng(seed)
+    # deform the solid: affine stretch + noise, and give everything a velocity field
+    sid = sc.particles["solid_id"] != wire.INVALID
+    c = sc.particles["pos"][sid, :dim].mean(0)
+    A = np.eye(dim) + rng.normal(0, 0.03, (dim, dim))
+    sc.particles["pos"][sid, :dim] = (sc.particles["pos"][sid, :dim] - c) @ A.T + c
+    sc.particles["pos"][:, :dim] += rng.normal(0, 0.0005, (sc.n, dim)).astype(np.float32)
+    sc.particles["vel"][:, :dim] = rng.normal(0, 0.05, (sc.n, dim)).astype(np.float32)
+    sc.particles["den"] *= (1 + rng.uniform(-2e-3, 2e-3, sc.n)).astype(np.float32)
+    if plastic:
+        sc.solids["stress"][:, 0] = 1e-3
+        sc.solids["stress"][:, 5] = -1e-3
+    return sc

@@ -9,7 +9,7 @@ import pytest
 from arp import scene as S
 from arp import tableau as T
 from arp import wire
-from helpers import box_scene, random_grid_scene, split_cand, uniform_block
+from helpers import box_scene, deformed_solid_scene as _solid_scene, random_grid_scene, split_cand, uniform_block
 
 pytestmark = pytest.mark.gpu
 
@@ -250,44 +250,6 @@ def test_full_size_column_properties(torch_cuda):
 
 
 # ---------------- solid path (strain.rs, solid.rs) ------------------------------------
-def _solid_scene(dim=3, plastic=False, with_fluid=True, seed=0):
-    sp = 0.01
-    h = 1.3 * sp
-    nside = 6
-    X = uniform_block(nside, sp, dim, origin=(0.3, 0.3, 0.3))
-    parts = [X]
-    mask = [np.ones(len(X), bool)]
-    if with_fluid:
-        Fl = uniform_block(8, sp, dim, jitter=0.1, seed=seed, origin=(0.25, 0.2, 0.25))
-        Fl = Fl[~np.all(np.abs(Fl - X.mean(0)) < (nside / 2 + 1) * sp, axis=1)]
-        parts.append(Fl)
-        mask.append(np.zeros(len(Fl), bool))
-    P = np.concatenate(parts)
-    mask = np.concatenate(mask)
-    sm = S.solid_material(1000.0, -2e5, -1e5, normal_damp=-20.0, shear_damp=-30.0,
-                          plastic=dict(yield_strength=50.0, relaxation_time=0.5) if plastic else None)
-    sm["mass"] = 1000.0 * sp**dim
-    fm = S.fluid_material(h, 0.02, density=1000.0, speed_of_sound=20.0, viscosity=0.01)
-    fm["mass"] = 1000.0 * sp**dim
-    B = uniform_block(12, sp, dim, origin=(0.2, 0.17, 0.2))
-    B = B[B[:, 1] < 0.2]
-    dims = (0.8, 0.8, 0.8) if dim == 3 else (0.8, 0.8, 0)
-    sc = box_scene(P, bpos=B, bounds=((0.0, 0.0, 0.0), dims), h=h, fluid=fm, solids_mask=mask, solid_mat=sm)
-    rng = np.random.default_rng(seed)
-    # deform the solid: affine stretch + noise, and give everything a velocity field
-    sid = sc.particles["solid_id"] != wire.INVALID
-    c = sc.particles["pos"][sid, :dim].mean(0)
-    A = np.eye(dim) + rng.normal(0, 0.03, (dim, dim))
-    sc.particles["pos"][sid, :dim] = (sc.particles["pos"][sid, :dim] - c) @ A.T + c
-    sc.particles["pos"][:, :dim] += rng.normal(0, 0.0005, (sc.n, dim)).astype(np.float32)
-    sc.particles["vel"][:, :dim] = rng.normal(0, 0.05, (sc.n, dim)).astype(np.float32)
-    sc.particles["den"] *= (1 + rng.uniform(-2e-3, 2e-3, sc.n)).astype(np.float32)
-    if plastic:
-        sc.solids["stress"][:, 0] = 1e-3
-        sc.solids["stress"][:, 5] = -1e-3
-    return sc
-
-
 @pytest.mark.parametrize("case", ["3d", "2d", "3d_plastic", "3d_solid_only"])
 def test_solid_compute_forces_parity(torch_cuda, oracle_mod, case):
     from arp.sim import from_device, to_device
