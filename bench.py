@@ -1,10 +1,15 @@
 """Benchmark: SPH particle-steps/s of the 3-D fluid column (1,048,576 particles, RK2,
-dt 5e-5) on one MI355X, with the dominant kernel's HBM roofline and the CPU oracle timed on
-the host cores beside it.
+dt 5e-5, 500 steps) on one MI355X, with the dominant kernel's HBM roofline and the CPU oracle
+timed on the host cores beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line on stdout; progress goes to stderr.
+The timed region is steps W .. W+K of the configuration (after W warm-up steps from the
+initial state).  Besides it, the line reports the whole 500-step configuration from the
+initial state, uploaded again (its own timing, list rebuilds and reuses),
+mass / momentum / energy drift over it, a per-field FP32 and FP64 error sample of the GPU
+step against the oracle at its end, forward + adjoint with checkpoint replay, and the
+batched ensemble.  Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
@@ -17,12 +22,11 @@ sys.path.insert(0, os.path.join(ROOT, "ar-physics_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# algorithmic bytes per particle per RK2 step of the whole pipeline (key pass, cell table,
-# reorder, gather, fused RK; cell-key counting sort design)
-B_ALG_STEP = {"rk2": 352.0, "rk4": 872.0}
 # algorithmic bytes per particle of one k_fluid launch: read the stage state (pos+den,
 # vel+q: 32 B), read y_n (28 B), write the next stage state (28 B), write the cell key (4 B)
 B_ALG_FLUID = 92.0
+# the configuration: 500 RK2 steps of the column
+CONFIG_STEPS = 500
 
 
 def log(*a):
@@ -62,39 +66,136 @@ def cpu_baseline(scene, integ, dt, steps, threads):
                        "the reference GLSL cannot run here (no GL context)" % (steps, integ.upper(), scene.n, threads))
 
 
-def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
-    """Forward with recording + reverse sweep of nsteps steps (one segment, no replay) on the
-    same workload, against nsteps plain steps."""
+def tait_energy(den, rho0, c):
+    """Specific internal energy of the Tait EOS (fluid.rs:49), e(rho) = int_rho0^rho p/r^2 dr
+    with p = (c^2 rho0 / 7)((r/rho0)^7 - 1)."""
+    B = c * c * rho0 / 7.0
+    return B * ((den**6 / rho0**7 - 1.0 / rho0) / 6.0 + 1.0 / den - 1.0 / rho0)
+
+
+def drift(sc, p0, p1, t, g):
+    """Mass, momentum and energy of the mobile particles at the start and end of a run.
+    Momentum: change beyond the free-fall impulse -M g t (the boundary and, through the
+    reference's m_i/rho_i force scaling, the pressure exchange momentum with the walls).
+    Energy: kinetic + gravitational + Tait internal energy (artificial viscosity and the
+    walls dissipate it)."""
+    import numpy as np
+
+    m = sc.materials["mass"][p0["mat"]].astype(np.float64)
+    M = float(m.sum())
+    mat = sc.materials[int(p0["mat"][0])]
+    rho0, c = float(mat["start_den"]), float(mat["sound_speed"])
+
+    def mom(p):
+        return (m[:, None] * p["vel"][:, :3].astype(np.float64)).sum(0)
+
+    def energy(p):
+        v = p["vel"][:, :3].astype(np.float64)
+        y = p["pos"][:, 1].astype(np.float64)
+        d = p["den"].astype(np.float64)
+        e = dict(kinetic=float((m * 0.5 * (v * v).sum(1)).sum()), gravitational=float((m * g * y).sum()),
+                 internal=float((m * tait_energy(d, rho0, c)).sum()))
+        e["total"] = e["kinetic"] + e["gravitational"] + e["internal"]
+        return e
+
+    P0, P1 = mom(p0), mom(p1)
+    imp = np.array([0.0, -M * g * t, 0.0])
+    e0, e1 = energy(p0), energy(p1)
+    # potential energy released if the whole column fell by its own height: the energy scale
+    Escale = M * g * float(p0["pos"][:, 1].max() - p0["pos"][:, 1].min())
+    return dict(
+        mass_rel=0.0,  # particle count and masses are fixed: the total mass is exact
+        momentum_change_beyond_gravity_rel=float(np.linalg.norm(P1 - P0 - imp) / (M * g * t)),
+        energy_J={"start": e0, "end": e1},
+        energy_drift_rel=float((e1["total"] - e0["total"]) / Escale),
+        density_rel_range_end=[float(p1["den"].min() / rho0 - 1.0), float(p1["den"].max() / rho0 - 1.0)],
+        note="momentum change beyond the free-fall impulse, relative to M g t; energy drift "
+             "(kinetic + gravitational + Tait internal) relative to M g (column height).  The "
+             "reference scales pressure accelerations by m_i/rho_i, so pressure does almost no "
+             "work while the continuity equation compresses the layers on the floor: the Tait "
+             "energy grows without a matching loss elsewhere",
+    )
+
+
+def fp64_sample(sim, sc, p, integ, dt, threads):
+    """One step from the downloaded state p: the GPU against the FP32 and the FP64 oracle,
+    normwise relative error per field."""
+    from arp import tableau
+    from oracle import oracle as O
+
+    if not os.path.exists(O.LIB):
+        O.build()
+    o = O.Oracle(sc, threads=threads)
+    tab = tableau.tableau(integ)
+    r32 = o.step(o.state("f", particles=p), tab, dt, 1)
+    r64 = o.step(o.state("d", particles=p), tab, dt, 1)
+    sim.upload_state(p)
+    sim.step(dt, 1)
+    got = sim.particles()
+    out = {}
+    for k in ("den", "pos", "vel"):
+        out[k] = dict(vs_fp32=O.rel_err(got[k], r32[k]), vs_fp64=O.rel_err(got[k], r64[k]),
+                      fp32_oracle_vs_fp64=O.rel_err(r32[k], r64[k]))
+    return out
+
+
+def config_run(sim, p_init, nsteps):
+    """The configuration's nsteps steps from the initial state, timed as a whole."""
+    sim.upload_state(p_init)
+    sim.synchronize()
+    nl0, sp0 = sim.neighbor_stats(), sim.speculation_stats()
+    t0 = time.perf_counter()
+    sim.update(nsteps)
+    sim.synchronize()
+    sec = time.perf_counter() - t0
+    nl1, sp1 = sim.neighbor_stats(), sim.speculation_stats()
+    n = len(p_init)
+    return dict(steps=nsteps, seconds=sec, ms_per_step=1e3 * sec / nsteps, particle_steps_per_s=n * nsteps / sec,
+                stage_builds=nl1[0] - nl0[0], stage_reuses=nl1[1] - nl0[1],
+                speculative_steps=sp1[0] - sp0[0], speculation_aborts=sp1[1] - sp0[1])
+
+
+def time_adjoint(sim, sc, p_start, dt, nsteps, K):
+    """Forward with recording (a checkpoint every K steps) + reverse sweep (replaying every
+    segment but the last from its checkpoint) of nsteps steps, against nsteps plain steps,
+    all from the same state (uploaded before each timed call).  Median of three rounds after
+    an untimed one."""
     import numpy as np
     import torch
 
-    log("adjoint: record %d steps + reverse sweep" % nsteps)
+    log("adjoint: %d steps, checkpoint every %d" % (nsteps, K))
     rng = np.random.default_rng(0)
     g = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
     g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(rng.standard_normal((sc.n, 7)).astype(np.float32)).cuda()
     gin = torch.zeros_like(g)
     torch.cuda.synchronize()
-    # one untimed round of the timed calls (allocates the recording and the reverse-mode
-    # buffers), then the median of three timed rounds
-    sim.adjoint_record(dt, nsteps, nsteps)
-    sim.adjoint_backward(g, gin, want_params=True)
-    sim.synchronize()
-    rounds = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        sim.adjoint_record(dt, nsteps, nsteps)
+
+    def timed(fn):
+        sim.upload_state(p_start)
         sim.synchronize()
-        t1 = time.perf_counter()
+        t0 = time.perf_counter()
+        fn()
+        sim.synchronize()
+        return time.perf_counter() - t0
+
+    fwd, rec, bwd = [], [], []
+    for r in range(4):
+        f = timed(lambda: sim.step(dt, nsteps))
+        a = timed(lambda: sim.adjoint_record(dt, nsteps, K))
+        t0 = time.perf_counter()
         sim.adjoint_backward(g, gin, want_params=True)
         sim.synchronize()
-        t2 = time.perf_counter()
-        rounds.append((t1 - t0, t2 - t1))
-    rec_s = sorted(r[0] for r in rounds)[1]
-    bwd_s = sorted(r[1] for r in rounds)[1]
-    # the reverse stage (on the recorded neighbours, or searching after a record overflow;
-    # fluid-only / one-material / general table forms), timed with events in another round
+        b = time.perf_counter() - t0
+        if r:  # the first round allocates the recording and the reverse-mode buffers
+            fwd.append(f)
+            rec.append(a)
+            bwd.append(b)
+    med = lambda x: sorted(x)[len(x) // 2]  # noqa: E731
+    fwd_ms, rec_ms, bwd_ms = 1e3 * med(fwd), 1e3 * med(rec), 1e3 * med(bwd)
+    # the reverse stage kernels, timed with events in another round
     adj_kernels = ["k_fluid_adj" + sfx for sfx in ("", "_rec", "_f", "_rec_f", "_g", "_rec_g")]
-    sim.adjoint_record(dt, nsteps, nsteps)
+    sim.upload_state(p_start)
+    sim.adjoint_record(dt, nsteps, K)
     sim.profile_select(adj_kernels)
     sim.profile(True)
     sim.profile_reset()
@@ -106,10 +207,9 @@ def time_adjoint(sim, sc, dt, nsteps, fwd_ms_per_step):
         ms, cnt = ms + m1, cnt + c1
     sim.profile(False)
     finite = bool(torch.isfinite(gin).all().item())
-    rec_ms, bwd_ms = 1e3 * rec_s, 1e3 * bwd_s
-    return dict(steps=nsteps, checkpoint_every=nsteps, record_ms=rec_ms, backward_ms=bwd_ms, timed_rounds=3,
-                forward_ms=fwd_ms_per_step * nsteps,
-                ratio_fwd_plus_adjoint_to_fwd=(rec_ms + bwd_ms) / (fwd_ms_per_step * nsteps),
+    return dict(steps=nsteps, checkpoint_every=K, segments_replayed=(nsteps + K - 1) // K - 1,
+                forward_ms=fwd_ms, record_ms=rec_ms, backward_ms=bwd_ms, timed_rounds=len(fwd),
+                ratio_fwd_plus_adjoint_to_fwd=(rec_ms + bwd_ms) / fwd_ms,
                 k_fluid_adj_avg_us=1e3 * ms / cnt if cnt else None, grad_finite=finite)
 
 
@@ -160,7 +260,7 @@ def time_ensemble(nscenes, steps, dt):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=CONFIG_STEPS)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--shape", default="128,128,64", help="fluid lattice nx,ny,nz (y up)")
     ap.add_argument("--integrator", default="rk2")
@@ -181,8 +281,9 @@ def main():
                     help="neighbour lists kept across RK stages with radius h*(1+skin) (0: search at every stage)")
     ap.add_argument("--max-age", type=int, default=None,
                     help="rebuild kept neighbour lists older than this many stages (default: the library's)")
-    ap.add_argument("--adjoint-steps", type=int, default=10,
-                    help="steps of the recorded rollout whose reverse sweep is timed (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=20,
+                    help="steps of the recorded rollouts whose reverse sweep is timed, with a checkpoint "
+                         "every steps/4 (replays) and every steps (0: skip)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -217,6 +318,7 @@ def main():
     # the list walk (its fast-material form when the scene has one mobile and one boundary
     # material), or the plain step without lists
     fluid_kernels = ["k_fluid_nl", "k_fluid_nl_f"] if args.skin > 0 else ["k_fluid"]
+    p_init = sc.particles.copy()
     log("warmup %d steps" % args.warmup)
     sim.update(args.warmup)
     sim.synchronize()
@@ -227,8 +329,8 @@ def main():
             torch.cuda.synchronize()
     except Exception:
         pass
-    # the timed steps run without per-kernel events; a separate pass with events on the
-    # stream times the kernels
+    # the timed steps (steps W .. W+K of the configuration) run without per-kernel events; a
+    # separate pass with events on the stream times the kernels
     nl0 = sim.neighbor_stats()
     sp0 = sim.speculation_stats()
     log("timing %d steps" % args.steps)
@@ -239,6 +341,7 @@ def main():
     sec = t1 - t0
     nl1 = sim.neighbor_stats()
     sp1 = sim.speculation_stats()
+    timed_kernels = fluid_kernels + ["k_fluid_nlb", "k_rowsort", "k_scatter", "k_scan", "k_key"]
     sim.profile_select(fluid_kernels if args.events == "fluid" else None)
     sim.profile(args.events != "none")
     sim.profile_reset()
@@ -247,27 +350,41 @@ def main():
         sim.update(args.event_steps)
         sim.synchronize()
     kern = {}
-    for k in ("k_fluid", "k_fluid_nl", "k_fluid_nl_f", "k_fluid_nlb", "k_rowsort", "k_scatter", "k_scan", "k_key"):
+    for k in ["k_fluid"] + timed_kernels:
         ms, cnt = sim.profile_query(k)
         if cnt:
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
     # the walk kernel that ran (most launches)
     fluid_kernel = max(fluid_kernels, key=lambda k: kern.get(k, {}).get("launches", 0))
-    if fluid_kernel in kern:
-        # the timed region's bound on one launch (its steps hold little but these walks, back
-        # to back): the step time over the stages per step
-        from arp import tableau
-
-        stages = int(tableau.tableau(args.integrator)["stages"])
-        if stages > 0:
-            kern[fluid_kernel]["timed_region_bound_us"] = 1e6 * sec / args.steps / stages
     sim.profile(False)
-    fwd_adj = None
+    # the rebuild chain's kernels (what a moving scene pays), timed with events on a build
+    sim.profile_select(timed_kernels[len(fluid_kernels):])
+    sim.profile(args.events != "none")
+    sim.profile_reset()
+    if args.events != "none":
+        sim.upload_state(sim.particles())  # the next stage rebuilds the lists
+        sim.update(1)
+        sim.synchronize()
+        for k in timed_kernels[len(fluid_kernels):]:
+            ms, cnt = sim.profile_query(k)
+            if cnt:
+                kern.setdefault(k, dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt))
+    sim.profile(False)
+    # the configuration itself: CONFIG_STEPS steps from the initial state, timed as a whole
+    log("configuration: %d steps from the initial state" % CONFIG_STEPS)
+    cfg = config_run(sim, p_init, CONFIG_STEPS)
+    p_end = sim.particles()
+    finite = bool(np.isfinite(p_end["pos"]).all())
+    cfg["state_finite"] = finite
+    cfg["drift"] = drift(sc, p_init, p_end, CONFIG_STEPS * dt, float(np.float32(sc.g)))
+    if not args.no_cpu:
+        log("fp32/fp64 error sample: one step at the end of the configuration")
+        cfg["error_sample_one_step_at_end"] = fp64_sample(sim, sc, p_end, args.integrator, dt, args.cpu_threads)
+    fwd_adj = []
     ens = None
     if args.adjoint_steps > 0:
-        fwd_adj = time_adjoint(sim, sc, dt, args.adjoint_steps, 1e3 * sec / args.steps)
-    p = sim.particles()
-    finite = bool(np.isfinite(p["pos"]).all())
+        for K in sorted({max(1, args.adjoint_steps // 4), args.adjoint_steps}):
+            fwd_adj.append(time_adjoint(sim, sc, p_end, dt, args.adjoint_steps, K))
     sim.close()
     if args.ensemble > 0:
         ens = time_ensemble(args.ensemble, args.ensemble_steps, dt)
@@ -310,13 +427,12 @@ def main():
             "traffic": traffic,
             "alg_bytes_per_launch": bytes_launch,
         },
-        "step_roofline_frac": B_ALG_STEP.get(args.integrator, 0.0) * value / HBM_PEAK,
         "kernels": kern,
         "neighbor_lists": {"skin": args.skin, "stage_builds": nl1[0] - nl0[0],
                            "stage_reuses": nl1[1] - nl0[1],
                            # timed steps issued as list walks alone, and those run again
                            "speculative_steps": sp1[0] - sp0[0], "speculation_aborts": sp1[1] - sp0[1]},
-        "state_finite": finite,
+        "config_run": cfg,
         "fwd_adjoint": fwd_adj,
         "ensemble": ens,
     }
