@@ -368,11 +368,18 @@ def small_fluid_3d(n_side=12, spacing=0.005, seed=1, jitter=0.05, box=(0.12, 0.1
 
 
 def cantilever_3d(nx=40, ny=40, nz=400, spacing=0.0025, E=1e6, nu=0.3, density=1000.0,
-                  h_factor=1.3, clamp_layers=3, dt=1e-5):
-    """3-D elastic cantilever: an nx*ny*nz block of solid particles along z; one end held by
-    clamp_layers of fixed boundary particles; linear elastic (St Venant-Kirchhoff on the
-    Green-Lagrange strain, strain_order 2) with lambda, mu from E and nu in the reference's
-    negative sign convention; gravity loading."""
+                  h_factor=1.3, clamp_layers=3, clamp_length=8, clamp_scale=3e6, dt=1e-5):
+    """3-D elastic cantilever: an nx*ny*nz block of solid particles along z; linear elastic
+    (St Venant-Kirchhoff on the Green-Lagrange strain, strain_order 2) with lambda, mu from E
+    and nu in the reference's negative sign convention; gravity loading.  The end z < 0 is
+    clamped by fixed boundary particles, `clamp_layers` thick: a wall behind the end face and
+    plates above, below and beside the first `clamp_length` layers of the beam.  The
+    reference has no clamp; the beam-clamp pair is given the Tait contact potential
+    (fluid.rs:58), which repels closer than r0 and attracts beyond it, with r0 = spacing (the
+    gap between the beam and the plates) and scale clamp_scale, so the plates grip the root
+    from every side.  (fluid.rs:177 passes an interaction's radius as the potential's scale
+    and its strength as r0.)  The default linear contact (interaction.rs:42-50) pushes with a
+    nearly constant pressure and cannot hold the beam."""
     h = f32(h_factor * spacing)
     P = _cubic_block((nx, ny, nz), spacing, (0.0, 0.0, 0.0), 0.0, 0)
     lam, mu = lame_from_young(E, nu)
@@ -381,8 +388,14 @@ def cantilever_3d(nx=40, ny=40, nz=400, spacing=0.0025, E=1e6, nu=0.3, density=1
     wall = immobile_material(0.0)
     wall["mass"] = f32(1.0 * spacing**3)
     mats = np.array([wall, solid], wire.MATERIAL)
-    # clamp: boundary block behind z < 0 covering the beam cross-section
-    C = _cubic_block((nx, ny, clamp_layers), spacing, (0.0, 0.0, -clamp_layers * spacing), 0.0, 0)
+    # the clamp: the lattice cells of the box around the root, grown by clamp_layers, minus
+    # the beam itself
+    L = clamp_layers
+    ax = [np.arange(-L, nx + L), np.arange(-L, ny + L), np.arange(-L, clamp_length)]
+    I, J, K = np.meshgrid(*ax, indexing="ij")
+    Cg = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    beam = (Cg[:, 0] >= 0) & (Cg[:, 0] < nx) & (Cg[:, 1] >= 0) & (Cg[:, 1] < ny) & (Cg[:, 2] >= 0)
+    C = (Cg[~beam].astype(np.float64) + 0.5) * spacing
     flu = wire.particles(len(P))
     flu["pos"][:, :3] = P
     flu["den"] = density
@@ -395,29 +408,95 @@ def cantilever_3d(nx=40, ny=40, nz=400, spacing=0.0025, E=1e6, nu=0.3, density=1
     ext = (nx * spacing, ny * spacing, nz * spacing)
     bounds = wire.aabb([-margin, -margin - 0.5 * ext[2], -margin],
                        [ext[0] + 2 * margin, ext[1] + 2 * margin + 0.5 * ext[2], ext[2] + 2 * margin])
-    inter = resolve_interactions(mats, h)
+    clamp = wire.interaction(wire.POT_TAIT, float(f32(spacing)), clamp_scale, 0.0, 0.0)
+    inter = resolve_interactions(mats, h, {(0, 1): clamp})
     return Scene(flu, bnd, mats, inter, bounds, float(h), 9.81, 0.0, dt=dt, subticks=1,
                  integrator="rk4", name="cantilever_3d")
 
 
+def supported_ball(n, spacing):
+    """Exactly n points of a cubic lattice (spacing `spacing`) forming a ball, every one with at
+    least one lattice neighbour along each axis, so that the corrected-gradient matrix
+    L = sum V grad W(dX) dX^T of every solid particle (strain.rs:101-136) is invertible for a
+    support radius h in (spacing, sqrt(2) spacing].  For radii from the ball of n points up
+    in quarter spacings: the lattice ball with the unsupported points dropped (repeatedly, a
+    drop can unsupport a neighbour) is trimmed to n by removing outermost points (by distance,
+    then lattice index) whose removal leaves every neighbour supported; the first radius that
+    reaches exactly n is taken.  Returns (n, 3) offsets from the centre."""
+    r = (3.0 * n / (4.0 * math.pi)) ** (1.0 / 3.0)
+    for _ in range(40):
+        m = int(math.ceil(r)) + 1
+        g1 = np.arange(-m, m + 1)
+        I, J, K = np.meshgrid(g1, g1, g1, indexing="ij")
+        G = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+        d2 = (G.astype(np.float64) ** 2).sum(1)
+        keep = d2 <= r * r
+        G, d2 = G[keep], d2[keep]
+        order = np.lexsort((np.arange(len(G)), d2))
+        G = G[order]
+        idx = {tuple(g): k for k, g in enumerate(G)}
+        nb = np.full((len(G), 6), -1, np.int64)  # neighbour along +x -x +y -y +z -z
+        for k, g in enumerate(G):
+            for q in range(6):
+                h = [int(g[0]), int(g[1]), int(g[2])]
+                h[q // 2] += 1 if q % 2 == 0 else -1
+                nb[k, q] = idx.get(tuple(h), -1)
+        inside = np.ones(len(G), bool)
+
+        def has(k, q):
+            j = nb[k, q]
+            return j >= 0 and inside[j]
+
+        def covered(k):
+            return all(has(k, 2 * ax) or has(k, 2 * ax + 1) for ax in range(3))
+
+        changed = True
+        while changed:
+            changed = False
+            for k in np.nonzero(inside)[0]:
+                if not covered(k):
+                    inside[k] = False
+                    changed = True
+        count = int(inside.sum())
+        progress = True
+        while count > n and progress:
+            progress = False
+            for k in range(len(G) - 1, -1, -1):  # outermost first
+                if not inside[k]:
+                    continue
+                # removable if every neighbour keeps a neighbour along the axis it shares
+                # with k (k sits at the neighbour's side q ^ 1)
+                if all(not (nb[k, q] >= 0 and inside[nb[k, q]]) or has(nb[k, q], q) for q in range(6)):
+                    inside[k] = False
+                    count -= 1
+                    progress = True
+                    break  # again from the outermost
+        if count == n:
+            assert all(covered(k) for k in np.nonzero(inside)[0])
+            return G[inside].astype(np.float64) * spacing
+        r += 0.25
+    raise ValueError("no supported lattice ball of %d points" % n)
+
+
 def contact_3d(n_fluid_side=64, n_sphere=65536, spacing=None, E=5e5, nu=0.4, density=1000.0,
                v0=(0.0, -2.0, 0.0), contact_strength=1e4, alpha=0.02, h_factor=1.3, dt=2e-5,
-               seed=0):
-    """Fluid-solid contact: an n_fluid_side^3 jittered fluid block and an elastic sphere of
-    about n_sphere particles above it moving at v0; linear contact potential strength
-    contact_strength (radius 2h, the loader's defaults)."""
+               seed=0, gap=1.0):
+    """Fluid-solid contact: an n_fluid_side^3 jittered fluid block and an elastic ball of
+    exactly n_sphere particles (cubic lattice, every particle with a full 3-D reference
+    support, supported_ball) whose lowest layer is `gap` spacings above the fluid's top layer,
+    moving at v0; linear contact potential of strength contact_strength (radius 2h, the
+    loader's defaults).  Pairs interact within h only (the reference's 27-bucket sweep), so
+    with gap = 1 < h / spacing the contact acts from the first step."""
     if spacing is None:
         spacing = 0.005
     h = f32(h_factor * spacing)
     Pf = _cubic_block((n_fluid_side,) * 3, spacing, (0.0, 0.0, 0.0), 0.05, seed)
     L = n_fluid_side * spacing
-    r = (3.0 * n_sphere / (4.0 * math.pi)) ** (1.0 / 3.0) * spacing
-    c = np.array([0.5 * L, L + r + 2 * spacing, 0.5 * L])
-    m = int(math.ceil(r / spacing)) + 1
-    g1 = np.arange(-m, m + 1)
-    I, J, K = np.meshgrid(g1, g1, g1, indexing="ij")
-    Q = np.stack([I.ravel(), J.ravel(), K.ravel()], 1).astype(np.float64) * spacing
-    Q = Q[(Q**2).sum(1) <= r * r] + c
+    Qo = supported_ball(n_sphere, spacing)
+    r = float(np.abs(Qo).max())
+    # the ball's lowest layer `gap` spacings above the fluid's top layer (at L - spacing / 2)
+    c = np.array([0.5 * L, L + r + (gap - 0.5) * spacing, 0.5 * L])
+    Q = Qo + c
     box = (L, L + 2 * r + 4 * spacing, L)
     B = _walls(box, spacing, 3)
     water = fluid_material(h, alpha, density=1000.0, speed_of_sound=20.0, viscosity=0.0)

@@ -43,29 +43,16 @@ def _cantilever():
 
 
 def _contact():
-    """The contact scene's fluid block and materials with a 4x4x4 elastic cube (a small
-    lattice sphere has particles whose reference neighbours are coplanar, so their
-    corrected-gradient matrix is singular) one spacing above the fluid surface, moving down."""
-    from arp import wire
-
-    base = S.contact_3d(n_fluid_side=6, n_sphere=60)
-    fl = base.particles[base.particles["mat"] == 2]
-    sp = 0.005
-    top = 6 * sp
-    idx = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(4), indexing="ij"), -1).reshape(-1, 3)
-    Q = (idx + 0.5) * sp + np.array([1.0 * sp, top + 0.5 * sp, 1.0 * sp])
-    p = wire.particles(len(fl) + len(Q))
-    p[: len(fl)] = fl
-    p["pos"][len(fl):, :3] = Q
-    p["vel"][len(fl):, :3] = (0.0, -2.0, 0.0)
-    p["den"][len(fl):] = 1000.0
-    p["mat"][len(fl):] = 1
-    p["solid_id"] = wire.INVALID
+    """The contact scene at small size: a 6^3 fluid block and a 68-particle elastic ball (every
+    particle with a full reference support) one spacing above it, moving down; the fluid
+    stirred and the ball's positions perturbed."""
+    sc = S.contact_3d(n_fluid_side=6, n_sphere=68)
+    p = sc.particles.copy()
+    fl = p["mat"] == 2
     rng = np.random.default_rng(2)
-    p["vel"][: len(fl), :3] = (0.05 * rng.standard_normal((len(fl), 3))).astype(np.float32)
-    p["pos"][len(fl):, :3] += (2e-4 * rng.standard_normal((len(Q), 3))).astype(np.float32)
-    return S.Scene(p, base.boundary, base.materials, base.interactions, base.bounds, base.h, base.g, base.alpha,
-                   dt=base.dt, subticks=1, integrator="rk2", name="contact_cube")
+    p["vel"][fl, :3] = (0.05 * rng.standard_normal((int(fl.sum()), 3))).astype(np.float32)
+    p["pos"][~fl, :3] += (2e-4 * rng.standard_normal((int((~fl).sum()), 3))).astype(np.float32)
+    return dataclasses.replace(sc, particles=p, name="contact_ball")
 
 
 def _x_of(parts):

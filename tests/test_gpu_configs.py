@@ -1,0 +1,103 @@
+"""The fluid-solid contact scene and the clamped cantilever at their full sizes, and the contact
+scene at a size the oracle steps, on the GPU through the C ABI (needs a GPU).
+
+- contact, 262,144 fluid particles (64^3, jittered) and a 65,536-particle elastic ball,
+  RK2, dt 2e-5: five steps stay finite and the contact slows the ball's lowest layer compared
+  with free fall (the reference doubles an elastic particle's gravity); a 10-step recording with a
+  checkpoint every 5 steps gives finite gradients with respect to the ball's initial
+  velocity and the fluid's bulk viscosity;
+- the same scene with an 8^3 fluid block and a 100-particle ball: two steps within 1e-5 of
+  the FP32 oracle per field;
+- cantilever, 40 x 40 x 400 = 640,000 solid particles, RK4, dt 1e-5: 800 steps stay finite
+  and the clamped root falls less than the free end (the clamp grips the beam's surface
+  through a contact potential; in 8 ms its hold reaches little of a 40 x 40 section)."""
+import numpy as np
+import pytest
+
+from arp import scene as S
+from arp import tableau as T
+from arp import wire
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_contact_full_size(torch_cuda):
+    from arp.sim import FluidSim
+
+    torch = torch_cuda
+    sc = S.contact_3d()
+    ball = sc.particles["mat"] == 1
+    assert sc.n == 64**3 + 65536 and ball.sum() == 65536
+    print("case contact full size", sc.n, sc.nb, sc.ns, flush=True)
+    sim = FluidSim(sc, integrator="rk2", timestep=2e-5)
+    sim.update(5)
+    p = sim.particles()
+    assert np.isfinite(p["pos"]).all() and np.isfinite(p["vel"]).all()
+    t = 5 * 2e-5
+    g = float(np.float32(9.81))
+    y0 = sc.particles["pos"][:, 1]
+    low = ball & (y0 < y0[ball].min() + 0.5 * 0.005)  # the ball's lowest layer
+    vy = float(p["vel"][low, 1].mean())
+    free = -2.0 - 2.0 * g * t
+    print("lowest layer (%d particles) mean vy %.5f, free fall %.5f" % (low.sum(), vy, free), flush=True)
+    assert vy > free + 0.1 * (2.0 * g * t)  # the contact takes a tenth of gravity's pull at least
+    # reverse mode at full size: 10 steps, a checkpoint every 5 (one segment replayed)
+    sim.upload_state(sc.particles)
+    sim.adjoint_record(2e-5, 10, 5)
+    gout = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
+    idx = torch.from_numpy(np.nonzero(ball)[0]).cuda()
+    gout[idx, 4:7] = 1.0 / 65536  # loss: the ball's centroid
+    gin = torch.zeros_like(gout)
+    gp = sim.adjoint_backward(gout, gin)
+    sim.synchronize()
+    gv = gin[idx, 8:11].sum(0).cpu().numpy()  # dL/d(initial ball velocity)
+    print("d centroid / d v0:", gv, "d centroid / d bulk_visc(water):", gp[2, 0], flush=True)
+    assert torch.isfinite(gin).all() and np.isfinite(gp).all()
+    # the centroid moves with the initial velocity: about t (times the doubled pos' rate)
+    assert 0.5 * 10 * 2e-5 < gv[1] < 4 * 10 * 2e-5
+    sim.close()
+
+
+def test_contact_small_vs_oracle(torch_cuda, oracle_mod):
+    from arp.sim import FluidSim
+
+    sc = S.contact_3d(n_fluid_side=8, n_sphere=100)
+    print("case contact small", sc.n, sc.ns, flush=True)
+    o = oracle_mod.Oracle(sc)
+    ref = o.step(o.state("f"), T.tableau("rk2"), 2e-5, 2)
+    sim = FluidSim(sc, integrator="rk2", timestep=2e-5)
+    sim.update(2)
+    got = sim.particles()
+    for k in ("den", "pos", "vel"):
+        e = oracle_mod.rel_err(got[k], ref[k])
+        print("  %s %.2e" % (k, e), flush=True)
+        assert e <= 1e-5, (k, e)
+    sim.close()
+
+
+def test_cantilever_full_size(torch_cuda):
+    from arp.sim import FluidSim
+
+    sc = S.cantilever_3d()
+    assert sc.n == 640000 and sc.ns == 640000
+    print("case cantilever full size", sc.n, sc.nb, flush=True)
+    sim = FluidSim(sc, integrator="rk4", timestep=1e-5)
+    sim.update(800)
+    p = sim.particles()
+    assert np.isfinite(p["pos"]).all() and np.isfinite(p["vel"]).all()
+    z = sc.particles["pos"][:, 2]
+    dy = p["pos"][:, 1] - sc.particles["pos"][:, 1]
+    root = z < 8 * 0.0025
+    tip = z > z.max() - 0.003
+    print("root dy %.3e tip dy %.3e" % (dy[root].mean(), dy[tip].mean()), flush=True)
+    assert dy[tip].mean() < 0 and abs(dy[root].mean()) < abs(dy[tip].mean())
+    sim.close()
