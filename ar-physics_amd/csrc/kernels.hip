@@ -600,6 +600,11 @@ __shared__ DInter s_inter[kMatLds * kMatLds];
 #ifndef ARP_NLB
 #define ARP_NLB 4  // list entries whose neighbour loads are issued together (list walk)
 #endif
+#ifndef ARP_WALK_DIRECT
+// list walk without LDS compaction: the pair follows the test in registers (fewer gather
+// requests: no reload of the position, no velocity request for entries beyond h)
+#define ARP_WALK_DIRECT 1
+#endif
 #ifndef ARP_PP_ALL
 #define ARP_PP_ALL 0  // two neighbour register sets in the search kernels' drains too
 #endif
@@ -880,40 +885,86 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         uint32_t nrec = 0;  // neighbours recorded (reverse sweep)
         uint32_t nfront = 0, nback = 0;  // list entries within h / within h + skin (MODE 2)
         if constexpr (WALK) {
-            // walk the kept list: the entries now within h are buffered in LDS (branch-free
-            // test, ARP_NLB entries per batch) and drained as in the search
+            // walk the kept list, ARP_NLB entries per batch: the direct walk evaluates the
+            // entries now within h as they are tested; the recording walk buffers them in LDS
+            // (branch-free test), records them and drains them as the search does
             const uint32_t cw = a.nl.count[i];
             const uint32_t fa = cw & 0xFFFFu, fb = cw >> 16, m = fa + fb, cap = a.nl.cap;
             const uint32_t* L = a.nl.list;
             // 32-bit element offsets (the host keeps cap x n below 2^32): entry k at k n + i,
             // the back entries (k >= fa) cap - fb - fa further on
             const uint32_t n32 = a.n, jump = (cap - fb - fa) * n32;
-            uint32_t offk = i;
             uint32_t k = 0, nl = 0;
-            for (;;) {
-                while (nl + ARP_NLB <= (uint32_t)kList && k < m) {
-                    uint32_t e[ARP_NLB], ok[ARP_NLB];
+            // the entries k0 .. k0 + ARP_NLB - 1 (unconditional loads: a lane past its count
+            // reads its entry 0; a 32-bit byte offset from the uniform base, an SGPR-based load)
+            auto fetch = [&](uint32_t k0, uint32_t (&eo)[ARP_NLB], uint32_t (&oo)[ARP_NLB]) {
 #pragma unroll
-                    for (int u = 0; u < ARP_NLB; u++) {
-                        const uint32_t kk = k + u;
-                        ok[u] = kk < m ? 1u : 0u;
-                        // unconditional loads: a lane past its count reads its entry 0
-                        const uint32_t off = ok[u] ? offk + (uint32_t)u * n32 + (kk >= fa ? jump : 0u) : i;
-                        // (a 32-bit byte offset from the uniform base: an SGPR-based load)
-                        const uint32_t v = *reinterpret_cast<const uint32_t*>(
-                            reinterpret_cast<const char*>(L) + off * 4u);
-                        e[u] = ok[u] ? v : 0u;
-                    }
-                    offk += (uint32_t)ARP_NLB * n32;
+                for (int u = 0; u < ARP_NLB; u++) {
+                    const uint32_t kk = k0 + u;
+                    oo[u] = kk < m ? 1u : 0u;
+                    const uint32_t off = oo[u] ? i + kk * n32 + (kk >= fa ? jump : 0u) : i;
+                    const uint32_t v = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(L) + off * 4u);
+                    eo[u] = oo[u] ? v : 0u;
+                }
+            };
+            // the entries of the next batch are loaded while the current batch's neighbour
+            // positions are in flight, so each batch waits for one round trip, not two
+            uint32_t e[ARP_NLB], ok[ARP_NLB];
+            fetch(0, e, ok);
+#if ARP_WALK_DIRECT
+            if constexpr (!REC) {
+                // direct walk: each entry's position is tested in registers and, for the
+                // entries within h, the velocity is loaded (lanes outside h issue no request)
+                // and the pair evaluated at once, in list order (the buffered walk's order)
+                for (; k < m; k += ARP_NLB) {
                     float4 xj[ARP_NLB];
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) xj[u] = ((e[u] & kBnd) ? B : X)[e[u] & ~kBnd];
+                    uint32_t en[ARP_NLB], okn[ARP_NLB];
+                    fetch(k + ARP_NLB, en, okn);
+                    bool inh[ARP_NLB];
+                    PairIn in[ARP_NLB];
+#pragma unroll
+                    for (int u = 0; u < ARP_NLB; u++) {
+                        const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
+                        inh[u] = ok[u] && hh - (rx * rx + ry * ry + rz * rz) > 0.0f;
+                        in[u].bnd = (e[u] & kBnd) != 0;
+                        in[u].xj = xj[u];
+                        const uint32_t j = e[u] & ~kBnd;
+                        if (inh[u]) {
+                            in[u].vj = (in[u].bnd ? BV : XV)[j];
+                            in[u].mk2 = FM ? (in[u].bnd ? a.fastBnd : a.fastMat) : (in[u].bnd ? a.bmk : a.mk)[j];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < ARP_NLB; u++)
+                        if (inh[u]) pair(in[u]);
+#pragma unroll
+                    for (int u = 0; u < ARP_NLB; u++) {
+                        e[u] = en[u];
+                        ok[u] = okn[u];
+                    }
+                }
+            } else
+#endif
+            for (;;) {
+                while (nl + ARP_NLB <= (uint32_t)kList && k < m) {
+                    float4 xj[ARP_NLB];
+#pragma unroll
+                    for (int u = 0; u < ARP_NLB; u++) xj[u] = ((e[u] & kBnd) ? B : X)[e[u] & ~kBnd];
+                    uint32_t en[ARP_NLB], okn[ARP_NLB];
+                    fetch(k + ARP_NLB, en, okn);
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) {
                         const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
                         const uint32_t in = ok[u] & (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
                         list[nl * 256 + tid] = e[u];
                         nl += in;
+                    }
+#pragma unroll
+                    for (int u = 0; u < ARP_NLB; u++) {
+                        e[u] = en[u];
+                        ok[u] = okn[u];
                     }
                     k += ARP_NLB;
                 }
@@ -1235,20 +1286,23 @@ extern "C" __global__ void __launch_bounds__(256) k_fluid_rec(FluidArgs a) { flu
 // the list pair of kernels: k_fluid_nlb searches and builds, k_fluid_nl walks the lists
 extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb(FluidArgs a) { fluid_entry<2>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256, 4) k_fluid_nlb_batch(BatchArgs b) { fluid_entry_batch<2>(b); }
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl(FluidArgs a) { fluid_entry<3>(a, xcd_block(), gridDim.x); }
+#ifndef ARP_NL_WAVES
+#define ARP_NL_WAVES 4  // waves per SIMD the list walks are compiled for (the direct walk's registers)
+#endif
+extern "C" __global__ void __launch_bounds__(256, ARP_NL_WAVES) k_fluid_nl(FluidArgs a) { fluid_entry<3>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nl_batch(BatchArgs b) { fluid_entry_batch<3>(b); }
 // the same pair recording each stage's neighbours within h (the reverse sweep reads them)
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nlbr(FluidArgs a) { fluid_entry<4>(a, xcd_block(), gridDim.x); }
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nlr(FluidArgs a) { fluid_entry<5>(a, xcd_block(), gridDim.x); }
 // the walks of scenes with one mobile and one boundary material and no elastic particles
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f(FluidArgs a) {
+extern "C" __global__ void __launch_bounds__(256, ARP_NL_WAVES) k_fluid_nl_f(FluidArgs a) {
     fluid_body<3, false, true>(a, xcd_block(), gridDim.x);
 }
 extern "C" __global__ void __launch_bounds__(256) k_fluid_nlr_f(FluidArgs a) {
     fluid_body<5, false, true>(a, xcd_block(), gridDim.x);
 }
 // (a batch of scenes that all have one mobile and one boundary material, each its own)
-extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f_batch(BatchArgs b) {
+extern "C" __global__ void __launch_bounds__(256, ARP_NL_WAVES) k_fluid_nl_f_batch(BatchArgs b) {
     uint32_t bid, nb;
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
     fluid_body<3, false, true>(a, bid, nb);
