@@ -261,6 +261,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=CONFIG_STEPS)
+    ap.add_argument("--config-steps", type=int, default=CONFIG_STEPS,
+                    help="steps of the configuration run from the initial state (the column's 500; fewer only "
+                         "to shorten profiling runs)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--shape", default="128,128,64", help="fluid lattice nx,ny,nz (y up)")
     ap.add_argument("--integrator", default="rk2")
@@ -371,12 +374,12 @@ def main():
                 kern.setdefault(k, dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt))
     sim.profile(False)
     # the configuration itself: CONFIG_STEPS steps from the initial state, timed as a whole
-    log("configuration: %d steps from the initial state" % CONFIG_STEPS)
-    cfg = config_run(sim, p_init, CONFIG_STEPS)
+    log("configuration: %d steps from the initial state" % args.config_steps)
+    cfg = config_run(sim, p_init, args.config_steps)
     p_end = sim.particles()
     finite = bool(np.isfinite(p_end["pos"]).all())
     cfg["state_finite"] = finite
-    cfg["drift"] = drift(sc, p_init, p_end, CONFIG_STEPS * dt, float(np.float32(sc.g)))
+    cfg["drift"] = drift(sc, p_init, p_end, args.config_steps * dt, float(np.float32(sc.g)))
     if not args.no_cpu:
         log("fp32/fp64 error sample: one step at the end of the configuration")
         cfg["error_sample_one_step_at_end"] = fp64_sample(sim, sc, p_end, args.integrator, dt, args.cpu_threads)
