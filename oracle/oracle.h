@@ -5,7 +5,9 @@
  * calls it.
  *
  * Every function restates the reference GLSL/Rust it cites (paths relative to the
- * reference repository).  Two precisions are built from one source (oracle_impl.h):
+ * reference repository).  Three arithmetics are built from one source (oracle_impl.h): a
+ * forward-mode derivative build (oracle_dual.cpp, liboracle_dual.so: REAL = an FP64 dual
+ * number, orc_jvp_step gives exact directional derivatives of the step) and two precisions:
  *   *_f  FP32, with the reference's operation order inside each expression and the
  *        81-lane sweep + base-3 LDS reduction order of the compute shaders;
  *   *_d  FP64, the same algorithm in double, used to state FP32 rounding error.

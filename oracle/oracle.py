@@ -90,6 +90,27 @@ def lib():
     return _lib
 
 
+LIB_DUAL = os.path.join(HERE, "liboracle_dual.so")
+_lib_dual = None
+# parameter codes of the derivative build (the order of the adjoint's parameter gradients)
+PARAM_FIELDS = {"bulk_visc": 1, "visc": 2, "normal_stiffness": 3, "shear_stiffness": 4,
+                "normal_damp": 5, "shear_damp": 6}
+
+
+def lib_dual():
+    """The forward-mode derivative build (oracle_dual.cpp: REAL = an FP64 dual number)."""
+    global _lib_dual
+    if _lib_dual is None:
+        if not os.path.exists(LIB_DUAL):
+            build()
+        L = C.CDLL(LIB_DUAL)
+        L.orc_jvp_step.argtypes = [C.POINTER(_Scene), C.c_void_p, C.c_double, C.c_int, C.POINTER(_State),
+                                   C.POINTER(_State), C.c_int, C.c_int]
+        L.orc_jvp_step.restype = C.c_int
+        _lib_dual = L
+    return _lib_dual
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -180,6 +201,19 @@ class Oracle:
             rc = lib().orc_step_d(C.byref(self.sc), _p(tab), C.c_double(dt), nsteps, C.byref(self._cs(st)))
         assert rc == 0, rc
         return st
+
+    def jvp(self, st, tangent, tab, dt, nsteps=1, param=None):
+        """Exact forward-mode derivative of nsteps steps (FP64): returns (y, ydot) with y the
+        state after the steps from st and ydot = J tangent, plus dy/dp when param =
+        (material index, field name of PARAM_FIELDS) seeds that parameter with tangent 1.
+        st and tangent are FP64 states ("d")."""
+        y = {k: np.ascontiguousarray(v, np.float64).copy() for k, v in st.items()}
+        t = {k: np.ascontiguousarray(v, np.float64).copy() for k, v in tangent.items()}
+        pm, pf = (-1, 0) if param is None else (int(param[0]), PARAM_FIELDS[param[1]])
+        rc = lib_dual().orc_jvp_step(C.byref(self.sc), _p(tab), C.c_double(dt), int(nsteps), C.byref(self._cs(y)),
+                                     C.byref(self._cs(t)), pm, pf)
+        assert rc == 0, rc
+        return y, t
 
     def verlet(self, st, dt, nsteps=1, D=None):
         """Velocity Verlet over this oracle's force function, in the state's precision: the

@@ -66,6 +66,9 @@ static int F(ctx_init)(F(orc_ctx) * c, const orc_scene* sc) {
     c->g = sc->g;
     c->dim = orc_dim(&sc->bounds);
     c->k = F(orc_norm_const)(c->dim, (REAL)sc->h);
+#ifdef ORC_CTX_HOOK
+    ORC_CTX_HOOK(c); /* the derivative build seeds a material parameter's tangent here */
+#endif
     return 0;
 }
 

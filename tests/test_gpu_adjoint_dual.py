@@ -1,0 +1,119 @@
+"""Reverse mode against the exact FP64 derivative of the oracle (needs a GPU).
+
+The oracle's forward-mode build (oracle/oracle_dual.cpp, REAL = an FP64 dual number) gives
+J u exactly for any direction u, and dy/dp for any material parameter p.  The GPU's gradient
+G = dL/dx0 of L = <g, y_N - x0> is checked through <G, u> = <g, J u - u> for many random
+directions at once (no finite-difference step), and its parameter gradients against
+<g, dy/dp>.  g and u lie on disjoint fields or particles, so the identity block drops out.
+Tolerance: 1e-3 relative (an FP32 adjoint on an FP32 trajectory against exact FP64)."""
+import numpy as np
+import pytest
+
+from arp import tableau as T
+from test_gpu_adjoint import _records, _scene
+from test_gpu_adjoint_solid import _cantilever, _contact
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+FIELDS = {"den": (0, 1), "pos": (1, 4), "vel": (4, 7)}
+SCALE = {"den": 1.0, "pos": 1e-4, "vel": 1e-2}
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _state_from(o, parts):
+    return o.state("d", particles=parts)
+
+
+def _tangent(o, st, u7):
+    t = {k: np.zeros_like(v) for k, v in st.items()}
+    t["den"][:] = u7[:, 0]
+    t["pos"][:, :3] = u7[:, 1:4]
+    t["vel"][:, :3] = u7[:, 4:7]
+    return t
+
+
+def _flat(y):
+    return np.concatenate([y["den"][:, None], y["pos"][:, :3], y["vel"][:, :3]], 1)
+
+
+def _gpu_grad(torch, sc, integ, dt, nsteps, K, g):
+    from arp.sim import FluidSim
+
+    sim = FluidSim(sc, integrator=integ, timestep=dt)
+    sim.upload_state(sc.particles)
+    sim.adjoint_record(dt, nsteps, K)
+    d_in = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
+    gp = sim.adjoint_backward(_records(torch, g.astype(np.float32)), d_in)
+    sim.synchronize()
+    gi = d_in.cpu().numpy()[:, [0, 4, 5, 6, 8, 9, 10]].astype(np.float64)
+    sim.close()
+    return gi, gp
+
+
+def _masked(rng, n, field, scale, parity=None):
+    a = np.zeros((n, 7))
+    lo, hi = FIELDS[field]
+    a[:, lo:hi] = rng.standard_normal((n, hi - lo)) * scale
+    if parity is not None:
+        a[np.arange(n) % 2 != parity] = 0.0
+    return a
+
+
+CASES = {
+    "fluid_rk2_replay": (lambda: _scene(True), "rk2", 5e-5, 3, 2, {1: ("bulk_visc", "visc")}),
+    "fluid_rk4": (lambda: _scene(False), "rk4", 5e-5, 2, 2, {1: ("bulk_visc",)}),
+    "cantilever_rk2": (_cantilever, "rk2", 1e-5, 2, 1,
+                       {1: ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp")}),
+    "contact_rk2": (_contact, "rk2", 1e-5, 2, 2, {1: ("normal_stiffness", "shear_stiffness"), 2: ("bulk_visc",)}),
+}
+PAIRS = [("vel", "pos"), ("den", "vel"), ("pos", "vel"), ("vel", "den"), ("vel", "vel"), ("pos", "pos")]
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_adjoint_vs_exact_derivative(torch_cuda, oracle_mod, case):
+    make, integ, dt, nsteps, K, params = CASES[case]
+    sc = make()
+    tab = T.tableau(integ)
+    o = oracle_mod.Oracle(sc)
+    st = _state_from(o, sc.particles)
+    rng = np.random.default_rng(17)
+    worst = 0.0
+    for gf, uf in PAIRS:
+        same = gf == uf
+        g = _masked(rng, sc.n, gf, 1.0, 0 if same else None)
+        gi, gp = _gpu_grad(torch_cuda, sc, integ, dt, nsteps, K, g)
+        for rep in range(3):  # several directions for one gradient
+            u = _masked(rng, sc.n, uf, SCALE[uf], 1 if same else None)
+            y, yd = o.jvp(st, _tangent(o, st, u), tab, dt, nsteps)
+            exact = float(np.sum(g * (_flat(yd) - u)))
+            ad = float(np.sum(gi * u))
+            rel = abs(ad - exact) / max(abs(exact), 1e-300)
+            worst = max(worst, rel)
+            print("case %s g %s u %s #%d ad %.9e exact %.9e rel %.2e" % (case, gf, uf, rep, ad, exact, rel),
+                  flush=True)
+            assert exact != 0.0
+            assert rel < TOL
+        if gf == "vel" and uf == "pos":  # parameter gradients of this loss
+            zero = _tangent(o, st, np.zeros((sc.n, 7)))
+            cols = {"bulk_visc": 0, "visc": 1, "normal_stiffness": 2, "shear_stiffness": 3, "normal_damp": 4,
+                    "shear_damp": 5}
+            for m, names in params.items():
+                for name in names:
+                    _, yd = o.jvp(st, zero, tab, dt, nsteps, param=(m, name))
+                    exact = float(np.sum(g * _flat(yd)))
+                    ad = float(gp[m, cols[name]])
+                    rel = abs(ad - exact) / max(abs(exact), 1e-300)
+                    print("case %s param %d %s ad %.9e exact %.9e rel %.2e" % (case, m, name, ad, exact, rel),
+                          flush=True)
+                    assert exact != 0.0
+                    assert rel < TOL
+    print("case %s worst directional rel %.2e" % (case, worst), flush=True)
