@@ -10,6 +10,7 @@ Per-stage drop-ins (``compute_forces``, ``lincomb``, ``velocity``, ``neighbors_u
 take torch tensors holding std430 records on the context's device.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -253,6 +254,10 @@ class FluidSim:
         return ms.value, cnt.value
 
     def close(self):
+        # a batch holding this context is destroyed first (garbage collection of a cycle may
+        # finalize the sim before the batch that refers to it)
+        for b in list(getattr(self, "_batches", ())):
+            b.close()
         if self.ctx:
             lib().arp_destroy(self.ctx)
             self.ctx = C.c_void_p()
@@ -265,8 +270,8 @@ class FluidSim:
 
 
 class Batch:
-    """Independent fluid scenes stepped together (arp_batch_*): one launch per kernel and RK
-    stage for all of them, replayed from a hipGraph when graph=True."""
+    """Independent scenes stepped together (arp_batch_*): one launch per kernel and RK stage
+    for all of them, replayed from a hipGraph when graph=True."""
 
     def __init__(self, sims, graph=True):
         self.sims = list(sims)
@@ -276,6 +281,10 @@ class Batch:
         if rc != 0:
             msg = lib().arp_last_error(self.sims[0].ctx)
             raise ArpError("arp_batch_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
+        for s in self.sims:
+            if not hasattr(s, "_batches"):
+                s._batches = weakref.WeakSet()
+            s._batches.add(self)
 
     def _check(self, rc):
         if rc != 0:
@@ -289,6 +298,26 @@ class Batch:
         d = C.c_uint32()
         self._check(lib().arp_batch_tune(self.b, float(dt), int(trial_steps), C.byref(d)))
         return d.value
+
+    def adjoint_record(self, dt, nsteps, checkpoint_every=None):
+        """Record nsteps steps of every scene for adjoint_backward (a checkpoint every
+        checkpoint_every steps; default: one segment)."""
+        k = nsteps if checkpoint_every is None else checkpoint_every
+        for s in self.sims:
+            s._touch()
+        self._check(lib().arp_batch_adjoint_record(self.b, float(dt), int(nsteps), int(k)))
+
+    def adjoint_backward(self, d_grad_out, d_grad_in, want_params=True):
+        """Per scene: device tensors of n_s Particle records (dL/d state after / before the
+        recorded steps).  Returns one (nmat, 6) parameter-gradient array per scene, or None."""
+        n = len(self.sims)
+        go = (C.c_void_p * n)(*[t.data_ptr() for t in d_grad_out])
+        gi = (C.c_void_p * n)(*[t.data_ptr() for t in d_grad_in])
+        gps = [np.zeros(6 * lib().arp_num_materials(s.ctx), np.float32) for s in self.sims] if want_params else None
+        gpp = (C.c_void_p * n)(*[g.ctypes.data for g in gps]) if want_params else None
+        self._check(lib().arp_batch_adjoint_backward(self.b, C.cast(go, C.c_void_p), C.cast(gi, C.c_void_p),
+                                                     C.cast(gpp, C.c_void_p) if gpp is not None else None))
+        return [g.reshape(-1, 6) for g in gps] if want_params else None
 
     def synchronize(self):
         for s in self.sims:

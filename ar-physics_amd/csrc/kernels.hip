@@ -1607,8 +1607,8 @@ __device__ __forceinline__ void store_rows3(float* dst, const M4& A) {
 // One thread per solid: F and F' from the static reference neighbours, the PK2 stress
 // hooke(E(F)+eps_p) + hooke(E'), A = (F S^T) L^-1 for the elastic force, and the plastic
 // flow rate (solid.rs:334-341, 497-518).
-extern "C" __global__ void __launch_bounds__(256) k_strain(SolidArgs a) {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, uint32_t) {
+    const uint32_t s = bid * 256u + threadIdx.x;
     if (s >= a.ns) return;
     float* ke = a.keps + 16 * (size_t)s;
     const uint32_t p = a.part[s];
@@ -1680,6 +1680,8 @@ extern "C" __global__ void __launch_bounds__(256) k_strain(SolidArgs a) {
     }
 }
 
+ARP_ENTRIES(k_strain, SolidArgs, strain_body)
+
 __device__ __forceinline__ float3 rows3_mul(const float4* R, float x, float y, float z) {
     return make_float3(R[0].x * x + R[0].y * y + R[0].z * z, R[1].x * x + R[1].y * y + R[1].z * z,
                        R[2].x * x + R[2].y * y + R[2].z * z);
@@ -1689,8 +1691,8 @@ __device__ __forceinline__ float3 rows3_mul(const float4* R, float x, float y, f
 // restoring force for those inside the current 27-cell stencil, self contact of
 // reference-distant pairs (solid.rs:343-451); then dest + dest2 (shaders/mod.rs:79) and
 // the stage epilogue for the particle and its plastic strain.
-extern "C" __global__ void __launch_bounds__(256) k_solid(SolidArgs a) {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uint32_t) {
+    const uint32_t s = bid * 256u + threadIdx.x;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
     const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
     const float4* REF = reinterpret_cast<const float4*>(a.ref);
@@ -1826,6 +1828,8 @@ extern "C" __global__ void __launch_bounds__(256) k_solid(SolidArgs a) {
         }
     }
 }
+
+ARP_ENTRIES(k_solid, SolidArgs, solid_body)
 
 extern "C" __global__ void __launch_bounds__(256) k_eps_in(EpsInArgs a) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;

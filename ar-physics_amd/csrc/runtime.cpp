@@ -79,6 +79,8 @@ enum Kern {
     K_FLUID_NLR_F,
     K_SPEC_RESTORE,
     K_FLUID_NL_F_B,
+    K_STRAIN_B,
+    K_SOLID_B,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -92,7 +94,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
                                    "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f",
-                                   "k_spec_restore", "k_fluid_nl_f_batch"};
+                                   "k_spec_restore", "k_fluid_nl_f_batch", "k_strain_batch", "k_solid_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -255,7 +257,8 @@ struct arp_ctx {
     // profiling: event pairs around launches
     bool prof = false;
     bool prof_sel[K_COUNT];  // kernels timed while profiling
-    std::vector<Launch>* capture = nullptr;  // record launches here instead of issuing them
+    std::vector<Launch>* capture = nullptr;
+    bool capture_solid = false;  // a batch holds scenes with solids: emit the solid path's launches  // record launches here instead of issuing them
     bool capture_fm = false;  // a capture may take the fast-material walk (every scene of its batch can)
     uint32_t rowsort_div = 32;               // particles per row-sort workgroup (tuning knob)
     std::vector<hipEvent_t> ev_pool;
@@ -343,13 +346,13 @@ hipEvent_t next_event(arp_ctx* c) {
 
 template <class A>
 int launch(arp_ctx* c, Kern k, uint32_t blocks, uint32_t threads, const A& args) {
-    if (blocks == 0) return ARP_OK;
-    if (c->capture) {
+    if (c->capture) {  // (a launch of no workgroups too: the scene takes no part in it)
         Launch L{(int)k, blocks, threads, std::vector<uint8_t>(sizeof(A))};
         memcpy(L.args.data(), &args, sizeof(A));
         c->capture->push_back(std::move(L));
         return ARP_OK;
     }
+    if (blocks == 0) return ARP_OK;
     void* params[] = {const_cast<A*>(&args)};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof && c->prof_sel[k]) {
@@ -1251,6 +1254,12 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
             sa.rk = r;
             if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
             if ((rc = launch(c, K_SOLID, nblk(c->ns), 256, sa))) return rc;
+        } else if (c->capture && c->capture_solid) {
+            // a fluid scene in a batch with solid scenes: the solid path's launches with no
+            // workgroups of this scene, so that every scene has the same launch sequence
+            const SolidArgs sa{};
+            if ((rc = launch(c, K_STRAIN, 0, 256, sa))) return rc;
+            if ((rc = launch(c, K_SOLID, 0, 256, sa))) return rc;
         }
         if (!last) {
             Xp = r.Oposd;
@@ -2885,6 +2894,8 @@ Kern batch_kernel(int k) {
         case K_FLUID_NL: return K_FLUID_NL_B;
         case K_FLUID_NLB: return K_FLUID_NLB_B;
         case K_FLUID_NL_F: return K_FLUID_NL_F_B;
+        case K_STRAIN: return K_STRAIN_B;
+        case K_SOLID: return K_SOLID_B;
         default: return K_COUNT;
     }
 }
@@ -2953,9 +2964,11 @@ int build_plan(arp_batch* b, float dt) {
     for (size_t s = 0; s < ns; s++) start[s] = save_host(b->ctx[s]);
     // the fast-material walk when every scene has one mobile and one boundary material (each
     // scene's own, in its arguments), so that all scenes launch the same kernels
-    bool allfm = true;
-    for (arp_ctx* c : b->ctx)
+    bool allfm = true, anysolid = false;
+    for (arp_ctx* c : b->ctx) {
         allfm = allfm && c->mobile_mat >= 0 && !c->any_elastic && !c->bnd_elastic && c->bnd_mat != -2;
+        anysolid = anysolid || c->solids_active;
+    }
     std::vector<BLaunch> plan[2];
     for (int p = 0; p < 2; p++) {
         std::vector<std::vector<Launch>> lists(ns);
@@ -2963,9 +2976,11 @@ int build_plan(arp_batch* b, float dt) {
             arp_ctx* c = b->ctx[s];
             c->capture = &lists[s];
             c->capture_fm = allfm;
+            c->capture_solid = anysolid;
             rc = step_once(c, dt);
             c->capture = nullptr;
             c->capture_fm = false;
+            c->capture_solid = false;
             if (rc) {
                 for (size_t t = 0; t < ns; t++) load_host(b->ctx[t], start[t]);
                 return bfail(b, rc, "scene %zu: %s", s, c->err.c_str());
@@ -2983,6 +2998,9 @@ int build_plan(arp_batch* b, float dt) {
             }
         }
         for (size_t j = 0; j < nl; j++) {
+            uint32_t total = 0;
+            for (size_t s = 0; s < ns; s++) total += lists[s][j].blocks;
+            if (total == 0) continue;  // no scene takes part
             BLaunch B;
             B.k = batch_kernel(lists[0][j].k);
             if (B.k == K_COUNT) {
@@ -3130,7 +3148,6 @@ int arp_batch_create(arp_ctx* const* ctxs, uint32_t nscenes, uint32_t flags, arp
         if (!c) return ARP_E_ARG;
         if (!c->have_scene) return fail(L0, ARP_E_STATE, "scene %u has no scene uploaded", s);
         if (c->device != L0->device) return fail(L0, ARP_E_ARG, "scene %u is on another device", s);
-        if (c->solids_active) return fail(L0, ARP_E_ARG, "scene %u runs the solid path (batches hold fluid scenes)", s);
         if (c->tab.stages == ARP_VERLET)
             return fail(L0, ARP_E_ARG, "scene %u uses velocity Verlet (batches run Runge-Kutta tableaus)", s);
         if (memcmp(&c->tab, &L0->tab, sizeof(ArpTableau)) != 0)
@@ -3163,6 +3180,26 @@ int arp_batch_step(arp_batch* b, float dt, uint32_t nsteps) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return bfail(b, ARP_E_ARG, "dt must be positive");
     HIPCHK(b->lead, hipSetDevice(b->lead->device));
     return run_batch(b, dt, nsteps);
+}
+
+int arp_batch_adjoint_record(arp_batch* b, float dt, uint32_t nsteps, uint32_t checkpoint_every) {
+    if (!b) return ARP_E_ARG;
+    for (size_t s = 0; s < b->ctx.size(); s++) {
+        const int rc = arp_adjoint_record(b->ctx[s], dt, nsteps, checkpoint_every);
+        if (rc) return bfail(b, rc, "scene %zu: %s", s, b->ctx[s]->err.c_str());
+    }
+    return ARP_OK;
+}
+
+int arp_batch_adjoint_backward(arp_batch* b, const void* const* d_grad_out, void* const* d_grad_in,
+                               float* const* grad_params) {
+    if (!b) return ARP_E_ARG;
+    if (!d_grad_out || !d_grad_in) return bfail(b, ARP_E_ARG, "null cotangent array");
+    for (size_t s = 0; s < b->ctx.size(); s++) {
+        const int rc = arp_adjoint_backward(b->ctx[s], d_grad_out[s], d_grad_in[s], grad_params ? grad_params[s] : nullptr);
+        if (rc) return bfail(b, rc, "scene %zu: %s", s, b->ctx[s]->err.c_str());
+    }
+    return ARP_OK;
 }
 
 int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chosen_div) {

@@ -279,7 +279,8 @@ int arp_set_stiffness_field(arp_ctx* ctx, const float* scale, uint32_t ns);
 int arp_adjoint_stiffness_grad(arp_ctx* ctx, float* out, uint32_t ns);
 
 /* ---- batched ensembles (no counterpart in the reference: one FluidSim per scene) ------- */
-/* Independent fluid scenes, one context each (uploaded as usual, same device, same
+/* Independent scenes (fluid, or with solids: the fluid scenes take no part in the solid
+ * path's launches), one context each (uploaded as usual, same device, same
  * integrator), stepped together: every kernel of an RK stage runs once for the whole batch
  * (each workgroup looks up its scene), and with ARP_BATCH_GRAPH the two-step launch cycle is
  * captured in a hipGraph and replayed.  The contexts keep working on their own between
@@ -292,6 +293,14 @@ void arp_batch_destroy(arp_batch* batch);
 const char* arp_batch_last_error(const arp_batch* batch);
 /* nsteps RK steps of length dt on every scene (the time of each context advances). */
 int arp_batch_step(arp_batch* batch, float dt, uint32_t nsteps);
+/* Reverse mode over a batch (no counterpart in the reference): arp_adjoint_record and
+ * arp_adjoint_backward of every scene, scene after scene, each on its context's stream.
+ * d_grad_out[s] / d_grad_in[s] are scene s's device cotangent arrays (n_s Particle records),
+ * grad_params (may be NULL) holds one host pointer per scene, each NULL or 6 * nmat_s floats
+ * (arp_adjoint_backward's layout). */
+int arp_batch_adjoint_record(arp_batch* batch, float dt, uint32_t nsteps, uint32_t checkpoint_every);
+int arp_batch_adjoint_backward(arp_batch* batch, const void* const* d_grad_out, void* const* d_grad_in,
+                               float* const* grad_params);
 /* Autotuner: times trial_steps batch steps (after two untimed ones) for each candidate
  * row-sort workgroup size (particles per workgroup), in two rounds, and keeps the fastest
  * (*chosen_div); the scenes' states are restored after every trial. */

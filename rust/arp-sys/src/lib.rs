@@ -249,6 +249,8 @@ extern "C" {
     pub fn arp_batch_destroy(batch: *mut ArpBatch);
     pub fn arp_batch_last_error(batch: *const ArpBatch) -> *const c_char;
     pub fn arp_batch_step(batch: *mut ArpBatch, dt: f32, nsteps: u32) -> c_int;
+    pub fn arp_batch_adjoint_record(batch: *mut ArpBatch, dt: f32, nsteps: u32, checkpoint_every: u32) -> c_int;
+    pub fn arp_batch_adjoint_backward(batch: *mut ArpBatch, d_grad_out: *const *const c_void, d_grad_in: *const *mut c_void, grad_params: *const *mut f32) -> c_int;
     pub fn arp_batch_tune(batch: *mut ArpBatch, dt: f32, trial_steps: u32, chosen_div: *mut u32) -> c_int;
     pub fn arp_kernel_radius(ctx: *const ArpCtx) -> f32;
     pub fn arp_num_materials(ctx: *const ArpCtx) -> u32;
@@ -266,7 +268,6 @@ extern "C" {
     pub fn arp_scene_apply(ctx: *mut ArpCtx, f: *const ArpSceneFile) -> c_int;
     pub fn arp_scene_spawn(ctx: *mut ArpCtx, f: *const ArpSceneFile, object: u32, offset: *const f32, material_added: *mut u32) -> c_int;
     pub fn arp_toml_to_json(text: *const c_char, len: usize, out: *mut c_char, cap: usize, needed: *mut usize) -> c_int;
-
 }
 
 /// The message of the last failed call on `ctx`.

@@ -159,3 +159,144 @@ def test_batch_rejects_mixed_integrators(torch_cuda):
         Batch(sims)
     for s in sims:
         s.close()
+
+
+def test_mixed_fluid_solid_batch_vs_oracle(torch_cuda, oracle_mod):
+    """A batch of fluid scenes and scenes with elastic (one plastic) solids and fluid-solid
+    contact: every kernel of a stage runs once for all of them (the fluid scenes take no
+    part in the solid path's launches).  Each scene is bitwise the same scene stepped alone
+    and within 1e-5 of the FP32 oracle per field, plastic strain included."""
+    from arp import tableau as T
+    from arp.sim import Batch
+    from helpers import deformed_solid_scene
+
+    dt, nsteps = 1e-5, 3
+    scenes = [S.fluid_block_3d(700, alpha=0.02, seed=11, name="f0"),
+              deformed_solid_scene(dim=3, seed=4),
+              S.fluid_block_3d(300, alpha=0.06, seed=12, name="f1"),
+              deformed_solid_scene(dim=3, seed=5, plastic=True),
+              S.contact_3d(n_fluid_side=8, n_sphere=100)]
+    print("case mixed batch", [(sc.n, sc.ns) for sc in scenes], flush=True)
+    solo = _sims(scenes, "rk2", dt)
+    for s in solo:
+        # a batch with solid scenes searches at every stage (its fluid scenes keep no lists):
+        # so do the solo runs it is compared with bitwise
+        s.set_neighbor_skin(0.0)
+        s.step(dt, nsteps)
+    sims = _sims(scenes, "rk2", dt)
+    B = Batch(sims, graph=True)
+    B.step(dt, nsteps)
+    B.synchronize()
+    tab = T.tableau("rk2")
+    for k, (a, b, sc) in enumerate(zip(sims, solo, scenes)):
+        pa = a.particles()
+        assert _same(pa, b.particles()), "scene %d differs from its solo run" % k
+        o = oracle_mod.Oracle(sc)
+        ref = o.step(o.state("f"), tab, dt, nsteps)
+        for f in ("den", "pos", "vel"):
+            e = oracle_mod.rel_err(pa[f], ref[f])
+            print("  scene %d %s %.2e" % (k, f, e), flush=True)
+            assert e <= 1e-5, (k, f, e)
+        if sc.ns > 1:
+            so = a.solids()
+            assert np.array_equal(np.ascontiguousarray(so).view(np.uint8), np.ascontiguousarray(b.solids()).view(np.uint8))
+            e = oracle_mod.rel_err(so["stress"].reshape(-1, 16), ref["stress"])
+            assert e <= 1e-5 or not np.abs(ref["stress"]).max() > 0, e
+    B.close()
+    for s in sims + solo:
+        s.close()
+
+
+def test_fluid_batch_vs_oracle(torch_cuda, oracle_mod):
+    """Every scene of a fluid batch (lists kept per scene, graph replay) within 1e-4 of the
+    oracle after 12 RK2 steps (the kept lists change the order of the per-particle sums, as
+    in tests/test_gpu_nl.py)."""
+    from arp import tableau as T
+    from arp.sim import Batch
+
+    dt, nsteps = 5e-5, 12
+    scenes = _scenes()
+    sims = _sims(scenes, "rk2", dt)
+    B = Batch(sims, graph=True)
+    B.step(dt, nsteps)
+    B.synchronize()
+    tab = T.tableau("rk2")
+    for k, (a, sc) in enumerate(zip(sims, scenes)):
+        o = oracle_mod.Oracle(sc)
+        ref = o.step(o.state("f"), tab, dt, nsteps)
+        pa = a.particles()
+        for f in ("den", "pos", "vel"):
+            e = oracle_mod.rel_err(pa[f], ref[f])
+            assert e <= 1e-4, (k, f, e)
+    B.close()
+    for s in sims:
+        s.close()
+
+
+def test_batched_adjoint_kinetic_energy(torch_cuda, oracle_mod):
+    """Batched reverse mode (arp_batch_adjoint_*) of a per-scene kinetic-energy loss
+    L_s = sum_i m_i |v_i|^2 / 2 at the end, for fluid scenes and a solid scene: each scene's
+    gradient is bitwise its solo sweep's, and matches the oracle's exact FP64 derivative along
+    random position and density directions (within 1e-3)."""
+    import torch
+
+    from arp import tableau as T
+    from arp.sim import Batch, FluidSim, to_device
+    from helpers import deformed_solid_scene
+
+    dt, nsteps, K = 1e-5, 4, 2
+    scenes = [S.fluid_block_3d(500, alpha=0.05, seed=21, name="f0"), deformed_solid_scene(dim=3, seed=6),
+              S.fluid_block_3d(900, alpha=0.02, seed=22, name="f1")]
+    for sc in scenes:
+        rng = np.random.default_rng(sc.n)
+        sc.particles["vel"][:, :3] += (0.05 * rng.standard_normal((sc.n, 3))).astype(np.float32)
+    sims = [FluidSim(sc, integrator="rk2", timestep=dt) for sc in scenes]
+    B = Batch(sims, graph=False)
+    B.adjoint_record(dt, nsteps, K)
+
+    def ke_cotangent(sim, sc):
+        p = sim.particles()
+        m = sc.materials["mass"][p["mat"]].astype(np.float64)
+        g = np.zeros((sc.n, 7))
+        g[:, 4:7] = m[:, None] * p["vel"][:, :3]
+        return g
+
+    gs = [ke_cotangent(s, sc) for s, sc in zip(sims, scenes)]
+    from test_gpu_adjoint import _records
+
+    gout = [_records(torch, g.astype(np.float32)) for g in gs]
+    gin = [torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda") for sc in scenes]
+    gps = B.adjoint_backward(gout, gin)
+    B.synchronize()
+    tab = T.tableau("rk2")
+    rng = np.random.default_rng(5)
+    for k, (sc, g, gi_t, gp) in enumerate(zip(scenes, gs, gin, gps)):
+        gi = gi_t.cpu().numpy()[:, [0, 4, 5, 6, 8, 9, 10]].astype(np.float64)
+        # the same scene alone
+        solo = FluidSim(sc, integrator="rk2", timestep=dt)
+        solo.adjoint_record(dt, nsteps, K)
+        gin1 = torch.zeros_like(gi_t)
+        gp1 = solo.adjoint_backward(_records(torch, g.astype(np.float32)), gin1)
+        solo.synchronize()
+        assert torch.equal(gin1, gi_t) and np.array_equal(gp1, gp), k
+        solo.close()
+        o = oracle_mod.Oracle(sc)
+        st = o.state("d")
+        for rep in range(4):
+            u = np.zeros((sc.n, 7))
+            if rep % 2:
+                u[:, 0] = rng.standard_normal(sc.n)
+            else:
+                u[:, 1:4] = 1e-4 * rng.standard_normal((sc.n, 3))
+            t = {kk: np.zeros_like(v) for kk, v in st.items()}
+            t["den"][:] = u[:, 0]
+            t["pos"][:, :3] = u[:, 1:4]
+            _, yd = o.jvp(st, t, tab, dt, nsteps)
+            exact = float(np.sum(g[:, 4:7] * yd["vel"][:, :3]))  # <g, J u> (g on velocities, u not)
+            ad = float(np.sum(gi * u))
+            rel = abs(ad - exact) / max(abs(exact), 1e-300)
+            print("batch adjoint scene %d dir %d ad %.9e exact %.9e rel %.2e" % (k, rep, ad, exact, rel), flush=True)
+            assert exact != 0.0 and rel < 1e-3
+    B.close()
+    for s in sims:
+        s.close()
