@@ -22,7 +22,7 @@ SYMBOLS = [
     "arp_scene_generate", "arp_scene_arrays", "arp_scene_contains", "arp_scene_apply", "arp_toml_to_json",
     "arp_kernel_radius", "arp_num_materials", "arp_add_particles", "arp_scene_spawn", "arp_num_particles",
     "arp_num_solids", "arp_neighbor_lists_fit", "arp_lincomb_noalloc", "arp_batch_adjoint_record",
-    "arp_batch_adjoint_backward",
+    "arp_batch_adjoint_backward", "arp_batch_tuning",
 ]
 
 
@@ -76,6 +76,7 @@ def lib():
     L.arp_lincomb_noalloc.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp]
     L.arp_batch_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_batch_adjoint_backward.argtypes = [vp, vp, vp, vp]
+    L.arp_batch_tuning.argtypes = [vp, vp, vp]
     L.arp_velocity.argtypes = [vp, C.c_float, vp, vp, vp, vp]
     L.arp_neighbors_update.argtypes = [vp, vp]
     L.arp_grid_info.argtypes = [vp, vp]

@@ -295,9 +295,17 @@ class Batch:
         self._check(lib().arp_batch_step(self.b, float(dt), int(nsteps)))
 
     def tune(self, dt, trial_steps=2):
+        """Autotune the batch's knobs; returns the chosen particles per row-sort workgroup
+        (tuning() gives every choice)."""
         d = C.c_uint32()
         self._check(lib().arp_batch_tune(self.b, float(dt), int(trial_steps), C.byref(d)))
         return d.value
+
+    def tuning(self):
+        """(particles per row-sort workgroup, list walks buffered in LDS: 0 or 1)."""
+        d, w = C.c_uint32(), C.c_uint32()
+        self._check(lib().arp_batch_tuning(self.b, C.byref(d), C.byref(w)))
+        return d.value, w.value
 
     def adjoint_record(self, dt, nsteps, checkpoint_every=None):
         """Record nsteps steps of every scene for adjoint_backward (a checkpoint every

@@ -623,7 +623,7 @@ __shared__ DInter s_inter[kMatLds * kMatLds];
 // FM (walks only): every mobile particle has material fastMat, every boundary particle
 // fastBnd, and none is elastic; a pair's material parameters are then selected by whether
 // the neighbour is a boundary particle, without per-neighbour material loads.
-template <int MODE, bool LT, bool FM = false>
+template <int MODE, bool LT, bool FM = false, bool DIRECT = (ARP_WALK_DIRECT != 0)>
 __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uint32_t) {
     static_assert(!FM || MODE == 3 || MODE == 5, "the fast-material form walks kept lists");
     constexpr bool REC = MODE == 1 || MODE >= 4;
@@ -911,8 +911,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             // positions are in flight, so each batch waits for one round trip, not two
             uint32_t e[ARP_NLB], ok[ARP_NLB];
             fetch(0, e, ok);
-#if ARP_WALK_DIRECT
-            if constexpr (!REC) {
+            if constexpr (!REC && DIRECT) {
                 // direct walk: each entry's position is tested in registers and, for the
                 // entries within h, the velocity is loaded (lanes outside h issue no request)
                 // and the pair evaluated at once, in list order (the buffered walk's order)
@@ -946,7 +945,6 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     }
                 }
             } else
-#endif
             for (;;) {
                 while (nl + ARP_NLB <= (uint32_t)kList && k < m) {
                     float4 xj[ARP_NLB];
@@ -1258,7 +1256,7 @@ ARP_ENTRIES(k_rowsort, RowSortArgs, rowsort_body)
 // cell-sorted particles (of the whole batch)
 // (five waves per SIMD: measured faster than four without the few spilled registers)
 // the material tables' location is chosen once per launch (uniform branch)
-template <int MODE>
+template <int MODE, bool DIRECT = (ARP_WALK_DIRECT != 0)>
 __device__ __forceinline__ void fluid_entry(const FluidArgs& a, uint32_t bid, uint32_t nb) {
     // a list build (MODE 2) may be launched with fewer workgroups than blocks of 256
     // particles (in a batch, where it mostly returns at once): each workgroup then takes
@@ -1266,17 +1264,17 @@ __device__ __forceinline__ void fluid_entry(const FluidArgs& a, uint32_t bid, ui
     const uint32_t nvb = MODE == 2 ? (a.n + 255u) / 256u : bid + 1u;
     for (uint32_t vb = bid; vb < nvb; vb += nb) {
         if (a.phys.nm <= (uint32_t)kMatLds)
-            fluid_body<MODE, true>(a, vb, nb);
+            fluid_body<MODE, true, false, DIRECT>(a, vb, nb);
         else
-            fluid_body<MODE, false>(a, vb, nb);
+            fluid_body<MODE, false, false, DIRECT>(a, vb, nb);
         if (MODE != 2) break;
     }
 }
-template <int MODE>
+template <int MODE, bool DIRECT = (ARP_WALK_DIRECT != 0)>
 __device__ __forceinline__ void fluid_entry_batch(const BatchArgs& b) {
     uint32_t bid, nb;
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
-    fluid_entry<MODE>(a, bid, nb);
+    fluid_entry<MODE, DIRECT>(a, bid, nb);
 }
 
 // (five waves per SIMD: measured faster than four without the few spilled registers)
@@ -1307,6 +1305,14 @@ extern "C" __global__ void __launch_bounds__(256, ARP_NL_WAVES) k_fluid_nl_f_bat
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
     fluid_body<3, false, true>(a, bid, nb);
 }
+// the batch walks' other form, a choice of the batch autotuner: the entries within h
+// buffered in LDS and drained with their positions loaded again (five waves per SIMD)
+extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f_batch_buf(BatchArgs b) {
+    uint32_t bid, nb;
+    const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
+    fluid_body<3, false, true, false>(a, bid, nb);
+}
+extern "C" __global__ void __launch_bounds__(256) k_fluid_nl_batch_buf(BatchArgs b) { fluid_entry_batch<3, false>(b); }
 
 // ------------------------------------------------------------------------------------
 // std430 AoS <-> internal SoA

@@ -81,6 +81,8 @@ enum Kern {
     K_FLUID_NL_F_B,
     K_STRAIN_B,
     K_SOLID_B,
+    K_FLUID_NL_F_B_BUF,
+    K_FLUID_NL_B_BUF,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -94,7 +96,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_nlb", "k_fluid_nlb_batch",
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
                                    "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f",
-                                   "k_spec_restore", "k_fluid_nl_f_batch", "k_strain_batch", "k_solid_batch"};
+                                   "k_spec_restore", "k_fluid_nl_f_batch", "k_strain_batch", "k_solid_batch",
+                                   "k_fluid_nl_f_batch_buf", "k_fluid_nl_batch_buf"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -2921,6 +2924,7 @@ struct arp_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> ev;  // one per context + one for the batch stream
     std::string err;
+    uint32_t walk_buf = 0;  // the list walks' LDS-buffered form (an autotuner choice)
 };
 
 namespace {
@@ -3003,6 +3007,8 @@ int build_plan(arp_batch* b, float dt) {
             if (total == 0) continue;  // no scene takes part
             BLaunch B;
             B.k = batch_kernel(lists[0][j].k);
+            if (b->walk_buf && B.k == K_FLUID_NL_F_B) B.k = K_FLUID_NL_F_B_BUF;
+            if (b->walk_buf && B.k == K_FLUID_NL_B) B.k = K_FLUID_NL_B_BUF;
             if (B.k == K_COUNT) {
                 for (size_t t = 0; t < ns; t++) load_host(b->ctx[t], start[t]);
                 return bfail(b, ARP_E_ARG, "kernel %s has no batched form (scenes with solids are not batched)",
@@ -3182,6 +3188,13 @@ int arp_batch_step(arp_batch* b, float dt, uint32_t nsteps) {
     return run_batch(b, dt, nsteps);
 }
 
+int arp_batch_tuning(const arp_batch* b, uint32_t* rowsort_div, uint32_t* walk_buffered) {
+    if (!b) return ARP_E_ARG;
+    if (rowsort_div) *rowsort_div = b->lead->rowsort_div;
+    if (walk_buffered) *walk_buffered = b->walk_buf;
+    return ARP_OK;
+}
+
 int arp_batch_adjoint_record(arp_batch* b, float dt, uint32_t nsteps, uint32_t checkpoint_every) {
     if (!b) return ARP_E_ARG;
     for (size_t s = 0; s < b->ctx.size(); s++) {
@@ -3218,8 +3231,10 @@ int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chose
         hs[s] = save_host(b->ctx[s]);
         tm[s] = b->ctx[s]->time;
     }
-    const uint32_t cand[] = {16, 32, 64, 128, 512};
-    constexpr int kCand = 5;
+    // the knobs: particles per row-sort workgroup, and the list walks' form (registers only,
+    // four waves per SIMD; or the entries within h buffered in LDS, five waves per SIMD)
+    const uint32_t divs[] = {16, 32, 64, 128, 512};
+    constexpr int kDiv = 5, kCand = 2 * kDiv;
     hipEvent_t e0, e1;
     HIPCHK(L0, hipEventCreate(&e0));
     HIPCHK(L0, hipEventCreate(&e1));
@@ -3229,7 +3244,8 @@ int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chose
     for (float& t : tbest) t = 1e30f;
     for (int round = 0; round < 2 && !rc; round++) {
         for (int ci = 0; ci < kCand; ci++) {
-            const uint32_t div = cand[ci];
+            const uint32_t div = divs[ci % kDiv];
+            b->walk_buf = (uint32_t)(ci / kDiv);
             for (arp_ctx* c : b->ctx) c->rowsort_div = div;
             // two untimed steps build the plan, warm the caches and rebuild the lists that the
             // restore reset, so the timed steps see the reuse that follows (where a row sort
@@ -3252,14 +3268,16 @@ int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chose
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    uint32_t bestDiv = cand[0];
+    uint32_t bestDiv = divs[0], bestWalk = 0;
     float best = 1e30f;
     for (int ci = 0; ci < kCand; ci++)
         if (tbest[ci] < best) {
             best = tbest[ci];
-            bestDiv = cand[ci];
+            bestDiv = divs[ci % kDiv];
+            bestWalk = (uint32_t)(ci / kDiv);
         }
     for (arp_ctx* c : b->ctx) c->rowsort_div = bestDiv;
+    b->walk_buf = bestWalk;
     for (Checkpoint& k : ck)
         for (DBuf* d : {&k.P, &k.V, &k.mk, &k.pid}) free_buf(*d);
     if (chosen_div) *chosen_div = bestDiv;

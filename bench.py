@@ -229,6 +229,7 @@ def time_ensemble(nscenes, steps, dt):
     npart = sum(sc.n for sc in scenes)
     B = Batch(sims, graph=True)
     div = B.tune(dt, 4)
+    _, walk_buffered = B.tuning()
     B.step(dt, 2)
     B.synchronize()
     torch.cuda.synchronize()
@@ -254,7 +255,8 @@ def time_ensemble(nscenes, steps, dt):
     return dict(scenes=nscenes, particles=npart, steps=steps, integrator="rk2", dt=dt,
                 ms_per_step=1e3 * sec / steps, particle_steps_per_s=npart * steps / sec,
                 sequential_particle_steps_per_s=npart * seq_steps / seq, graph=True,
-                tuned_rowsort_particles_per_workgroup=div, state_finite=finite)
+                tuned_rowsort_particles_per_workgroup=div,
+                tuned_list_walk="LDS-buffered" if walk_buffered else "registers", state_finite=finite)
 
 
 def main():

@@ -301,10 +301,15 @@ int arp_batch_step(arp_batch* batch, float dt, uint32_t nsteps);
 int arp_batch_adjoint_record(arp_batch* batch, float dt, uint32_t nsteps, uint32_t checkpoint_every);
 int arp_batch_adjoint_backward(arp_batch* batch, const void* const* d_grad_out, void* const* d_grad_in,
                                float* const* grad_params);
-/* Autotuner: times trial_steps batch steps (after two untimed ones) for each candidate
- * row-sort workgroup size (particles per workgroup), in two rounds, and keeps the fastest
- * (*chosen_div); the scenes' states are restored after every trial. */
+/* Autotuner: times trial_steps batch steps (after two untimed ones) for each candidate pair
+ * of row-sort workgroup size (particles per workgroup: 16 .. 512) and list-walk form
+ * (registers only, or the neighbours within h buffered in LDS), in two rounds, and keeps
+ * the fastest (*chosen_div: the row-sort size); the scenes' states are restored after
+ * every trial.  The results are the same for every choice. */
 int arp_batch_tune(arp_batch* batch, float dt, uint32_t trial_steps, uint32_t* chosen_div);
+/* The batch's current choices: particles per row-sort workgroup, and 1 when the list walks
+ * buffer the neighbours within h in LDS. */
+int arp_batch_tuning(const arp_batch* batch, uint32_t* rowsort_div, uint32_t* walk_buffered);
 
 /* ---- particle insertion (FluidSim::add_particles, soft_body/mod.rs:180-206) ----------- */
 /* FluidSim::kernel_radius (soft_body/mod.rs:166) and num_materials (mod.rs:176-178). */

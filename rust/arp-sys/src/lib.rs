@@ -252,6 +252,7 @@ extern "C" {
     pub fn arp_batch_adjoint_record(batch: *mut ArpBatch, dt: f32, nsteps: u32, checkpoint_every: u32) -> c_int;
     pub fn arp_batch_adjoint_backward(batch: *mut ArpBatch, d_grad_out: *const *const c_void, d_grad_in: *const *mut c_void, grad_params: *const *mut f32) -> c_int;
     pub fn arp_batch_tune(batch: *mut ArpBatch, dt: f32, trial_steps: u32, chosen_div: *mut u32) -> c_int;
+    pub fn arp_batch_tuning(batch: *const ArpBatch, rowsort_div: *mut u32, walk_buffered: *mut u32) -> c_int;
     pub fn arp_kernel_radius(ctx: *const ArpCtx) -> f32;
     pub fn arp_num_materials(ctx: *const ArpCtx) -> u32;
     pub fn arp_num_particles(ctx: *const ArpCtx) -> u32;
