@@ -213,6 +213,96 @@ def time_adjoint(sim, sc, p_start, dt, nsteps, K):
                 k_fluid_adj_avg_us=1e3 * ms / cnt if cnt else None, grad_finite=finite)
 
 
+def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss):
+    """A configuration with solids at its full size: fwd_steps plain steps from the initial
+    state (timed as a whole), then forward + adjoint of adj_steps steps with a checkpoint
+    every K (replaying every segment but the last) against the same steps plain, and the
+    gradient norms of the loss.  loss(particles) -> (value, cotangent (n, 7))."""
+    import numpy as np
+    import torch
+
+    from arp.sim import FluidSim
+
+    log("%s: %d particles (%d solid), %d steps" % (name, sc.n, sc.ns, fwd_steps))
+    sim = FluidSim(sc, integrator=integ, timestep=dt)
+    sim.update(2)
+    sim.upload_state(sc.particles)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(dt, fwd_steps)
+    sim.synchronize()
+    sec = time.perf_counter() - t0
+    p = sim.particles()
+    out = dict(particles=sc.n, solids=sc.ns, integrator=integ, dt=dt, steps=fwd_steps, seconds=sec,
+               ms_per_step=1e3 * sec / fwd_steps, particle_steps_per_s=sc.n * fwd_steps / sec,
+               state_finite=bool(np.isfinite(p["pos"]).all()))
+    if adj_steps:
+        log("%s: forward + adjoint of %d steps, checkpoint every %d" % (name, adj_steps, K))
+        sim.upload_state(sc.particles)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(dt, adj_steps)
+        sim.synchronize()
+        fwd = time.perf_counter() - t0
+        sim.upload_state(sc.particles)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.adjoint_record(dt, adj_steps, K)
+        sim.synchronize()
+        rec = time.perf_counter() - t0
+        val, g7 = loss(sim.particles())
+        g = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
+        g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(g7.astype(np.float32)).cuda()
+        gin = torch.zeros_like(g)
+        t0 = time.perf_counter()
+        gp = sim.adjoint_backward(g, gin, want_params=True)
+        sim.synchronize()
+        bwd = time.perf_counter() - t0
+        gi = gin.cpu().numpy()
+        out["fwd_adjoint"] = dict(steps=adj_steps, checkpoint_every=K, segments_replayed=(adj_steps + K - 1) // K - 1,
+                                  forward_ms=1e3 * fwd, record_ms=1e3 * rec, backward_ms=1e3 * bwd,
+                                  ratio_fwd_plus_adjoint_to_fwd=(rec + bwd) / fwd, loss=val,
+                                  grad_x0_norm=float(np.linalg.norm(gi[:, [4, 5, 6]])),
+                                  grad_v0_norm=float(np.linalg.norm(gi[:, [8, 9, 10]])),
+                                  grad_params={"normal_stiffness": float(gp[1, 2]), "shear_stiffness": float(gp[1, 3]),
+                                               "bulk_visc_last_material": float(gp[-1, 0])},
+                                  grad_finite=bool(np.isfinite(gi).all() and np.isfinite(gp).all()))
+    sim.close()
+    return out
+
+
+def cantilever_loss(sc):
+    """Squared tip-displacement error to a seeded target (the free end's particles)."""
+    import numpy as np
+
+    z = sc.particles["pos"][:, 2]
+    tip = z > z.max() - 0.003
+    rng = np.random.default_rng(0)
+    target = sc.particles["pos"][tip, :3].astype(np.float64) + 1e-5 * rng.standard_normal((int(tip.sum()), 3))
+
+    def loss(p):
+        d = p["pos"][tip, :3].astype(np.float64) - target
+        g = np.zeros((len(p), 7))
+        g[tip, 1:4] = 2.0 * d
+        return float((d * d).sum()), g
+
+    return loss
+
+
+def contact_loss(sc):
+    """The ball's final centroid height."""
+    import numpy as np
+
+    ball = sc.particles["mat"] == 1
+
+    def loss(p):
+        g = np.zeros((len(p), 7))
+        g[ball, 2] = 1.0 / ball.sum()
+        return float(p["pos"][ball, 1].mean()), g
+
+    return loss
+
+
 def time_ensemble(nscenes, steps, dt):
     """Independent fluid scenes (log-uniform 2,048-65,536 particles, alpha uniform in
     [0.01, 0.1], seeded) stepped as one batch (graph replay, tuned), against the same scenes
@@ -286,6 +376,10 @@ def main():
                     help="neighbour lists kept across RK stages with radius h*(1+skin) (0: search at every stage)")
     ap.add_argument("--max-age", type=int, default=None,
                     help="rebuild kept neighbour lists older than this many stages (default: the library's)")
+    ap.add_argument("--solid-configs", type=int, default=1,
+                    help="also run the cantilever (640,000 solids: 200 RK4 steps; forward + adjoint with a "
+                         "checkpoint every 20) and the fluid-ball contact (327,680 particles: 1000 RK2 steps; "
+                         "forward + adjoint of 200 steps, a checkpoint every 50) at full size (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="steps of the recorded rollouts whose reverse sweep is timed, with a checkpoint "
                          "every steps/4 (replays) and every steps (0: skip)")
@@ -391,6 +485,12 @@ def main():
         for K in sorted({max(1, args.adjoint_steps // 4), args.adjoint_steps}):
             fwd_adj.append(time_adjoint(sim, sc, p_end, dt, args.adjoint_steps, K))
     sim.close()
+    solid = None
+    if args.solid_configs:
+        c3 = S.cantilever_3d()
+        c4 = S.contact_3d()
+        solid = {"cantilever_3d": time_solid_config("cantilever", c3, "rk4", 1e-5, 200, 200, 20, cantilever_loss(c3)),
+                 "contact_3d": time_solid_config("contact", c4, "rk2", 2e-5, 1000, 200, 50, contact_loss(c4))}
     if args.ensemble > 0:
         ens = time_ensemble(args.ensemble, args.ensemble_steps, dt)
     value = sc.n * args.steps / sec
@@ -439,6 +539,7 @@ def main():
                            "speculative_steps": sp1[0] - sp0[0], "speculation_aborts": sp1[1] - sp0[1]},
         "config_run": cfg,
         "fwd_adjoint": fwd_adj,
+        "solid_configs": solid,
         "ensemble": ens,
     }
     if not args.no_cpu:
