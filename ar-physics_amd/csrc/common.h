@@ -362,6 +362,9 @@ struct SolidArgs {
     const uint32_t* sflag;    // per storage index: 1 when refused by the cap / non-finite
     const uint32_t* pid;      // storage -> particle id
     const uint32_t* sidByPid;
+    // per grid row, written by k_strain and read by the self-contact sweeps: min and max of
+    // u = x - X over the row's solids (.w of the first: 1 when one lies outside the grid)
+    float* rowBox;
     const float* kscale;      // per solid stiffness scale of (lambda, mu), or null (1)
     // fluid part of elastic particles (written by k_fluid)
     const float* KFp;

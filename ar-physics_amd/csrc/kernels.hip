@@ -1520,9 +1520,13 @@ __device__ M4 m4_inv(const M4& M) {
     t.m[15] = (a20 * b03 - a21 * b01 + a22 * b00) * id;
     return t;
 }
+// (loops over the dimension are unrolled with constant indices, so that matrices stay in
+// registers)
 __device__ __forceinline__ float m4_trace(const M4& A, uint32_t dim) {
     float t = A.m[0];
-    for (uint32_t i = 1; i < dim; i++) t += A.m[5 * i];
+#pragma unroll
+    for (uint32_t i = 1; i < 4; i++)
+        if (i < dim) t += A.m[5 * i];
     return t;
 }
 __device__ __forceinline__ float m4_dot(const M4& x, const M4& y) {
@@ -1545,7 +1549,9 @@ __device__ __forceinline__ M4 m4_hooke(const M4& S, float lambda, float mu, uint
 }
 __device__ M4 m4_cauchy(const M4& F, uint32_t dim) {
     M4 C = m4_mul(m4_trans(F), F);
-    for (uint32_t i = dim; i < 4; i++) C.m[5 * i] = 1.0f;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+        if (i >= dim) C.m[5 * i] = 1.0f;
     return C;
 }
 // Seth-Hill strain of order two_m/2 (solid.rs:105-168): log series, Denman-Beavers root
@@ -1592,7 +1598,12 @@ __device__ M4 m4_seth_hill_rate(const M4& F, const M4& Fd, int two_m, uint32_t d
     M4 sr = m4_zero();
     for (uint32_t i = 0; i < m; i++) {
         M4 term = m4_ident();
-        for (uint32_t j = 0; j < m; j++) term = m4_mul(term, j == i ? dC : C);
+        for (uint32_t j = 0; j < m; j++) {
+            if (j == i)
+                term = m4_mul(term, dC);
+            else
+                term = m4_mul(term, C);
+        }
         sr = m4_add(sr, term);
     }
     sr = m4_scale(1.0f / (float)two_m, sr);
@@ -1603,6 +1614,16 @@ __device__ M4 m4_seth_hill_rate(const M4& F, const M4& Fd, int two_m, uint32_t d
     return sr;
 }
 
+__device__ __forceinline__ M4 ld_m4(const float* base, uint32_t s) {
+    const float4* p = reinterpret_cast<const float4*>(base) + 4 * (size_t)s;
+    M4 r;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 v = p[q];
+        r.m[4 * q] = v.x, r.m[4 * q + 1] = v.y, r.m[4 * q + 2] = v.z, r.m[4 * q + 3] = v.w;
+    }
+    return r;
+}
 __device__ __forceinline__ void store_rows3(float* dst, const M4& A) {
     float4* d = reinterpret_cast<float4*>(dst);
     d[0] = make_float4(A.m[0], A.m[4], A.m[8], 0.f);
@@ -1610,22 +1631,103 @@ __device__ __forceinline__ void store_rows3(float* dst, const M4& A) {
     d[2] = make_float4(A.m[2], A.m[6], A.m[10], 0.f);
 }
 
+__device__ __forceinline__ bool in_grid(const GridP& g, float4 p) {
+    return p.x >= g.min[0] && p.x <= g.min[0] + g.dim[0] && p.y >= g.min[1] && p.y <= g.min[1] + g.dim[1] &&
+           p.z >= g.min[2] && p.z <= g.min[2] + g.dim[2];
+}
+
+// Self-contact row boxes: for each row of the current grid (one wave per row), the box of
+// the displacements u = x - X of the row's solids and a flag when one of them lies outside
+// the grid (its cell is clamped).  A contact pair needs |X_j - X_i| > 8h (solid.rs:426-450);
+// with X = x - u, |X_j - X_i| <= |x_j - x_i| + |u_j - u_i|, and for j in the stencil of i
+// (both inside the grid) |x_j - x_i| < 2 |cell|.  contact_row_clear bounds |u_j - u_i| by
+// the box, so a sweep skips only rows none of whose solids can pass the test.
+__device__ __forceinline__ void contact_row_boxes(const SolidArgs& a, uint32_t bid, uint32_t nb) {
+    if (!a.rowBox) return;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t stampv = *a.stamp;
+    const float4* X = reinterpret_cast<const float4*>(a.Xposd);
+    const float4* REF = reinterpret_cast<const float4*>(a.ref);
+    float4* box = reinterpret_cast<float4*>(a.rowBox);
+    const float inf = __builtin_inff();
+    for (uint32_t row = bid * 4u + w; row < a.grid.nrows; row += nb * 4u) {
+        if (a.rowStamp[row] != stampv) continue;
+        const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
+        const uint32_t lo = ro[0], hi = ro[a.grid.sub[0]];
+        float mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
+        uint32_t out = 0;
+        for (uint32_t q = lo + lane; q < hi; q += 64u) {
+            const uint32_t pe = a.perm[q];
+            if (pe & kFlag) continue;
+            if (!(a.mk[pe] & kMkSolid)) continue;
+            const float4 X2 = REF[a.sidByPid[a.pid[pe]]];
+            const float4 xj = X[pe];
+            const float u0 = xj.x - X2.x, u1 = xj.y - X2.y, u2 = xj.z - X2.z;
+            mn0 = fminf(mn0, u0), mn1 = fminf(mn1, u1), mn2 = fminf(mn2, u2);
+            mx0 = fmaxf(mx0, u0), mx1 = fmaxf(mx1, u1), mx2 = fmaxf(mx2, u2);
+            if (!in_grid(a.grid, xj)) out = 1u;
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            mn0 = fminf(mn0, __shfl_xor(mn0, o)), mn1 = fminf(mn1, __shfl_xor(mn1, o));
+            mn2 = fminf(mn2, __shfl_xor(mn2, o));
+            mx0 = fmaxf(mx0, __shfl_xor(mx0, o)), mx1 = fmaxf(mx1, __shfl_xor(mx1, o));
+            mx2 = fmaxf(mx2, __shfl_xor(mx2, o));
+            out |= __shfl_xor(out, o);
+        }
+        if (lane == 0) {
+            box[2 * (size_t)row] = make_float4(mn0, mn1, mn2, __uint_as_float(out));
+            box[2 * (size_t)row + 1] = make_float4(mx0, mx1, mx2, 0.0f);
+        }
+    }
+}
+
+// How far u_j - u_i may reach for a pair of the stencil of a solid at x (reference X) to
+// stay within r_cut in the reference state: r_cut - 2 |cell| less a rounding margin, or -1
+// (no row is clear) outside the grid or without boxes.
+__device__ __forceinline__ float contact_reach(const SolidArgs& a, float4 x, float r_cut) {
+    if (!a.rowBox || !in_grid(a.grid, x)) return -1.0f;
+    const GridP& g = a.grid;
+    float cell2 = 0.0f, ext = 0.0f;
+    for (int d = 0; d < 3; d++) {
+        const float e = g.dim[d] > 0.0f ? g.dim[d] / (float)g.sub[d] : 0.0f;
+        cell2 += e * e;
+        ext += fabsf(g.min[d]) + g.dim[d];
+    }
+    return r_cut - 2.0f * sqrtf(cell2) - (1e-5f * ext + 1e-3f * r_cut);
+}
+
+// True when no solid of the row can pass the reference-distance test against the solid of
+// displacement u (see contact_row_boxes).
+__device__ __forceinline__ bool contact_row_clear(const SolidArgs& a, uint32_t row, float u0, float u1, float u2,
+                                                  float reach) {
+    if (!(reach > 0.0f)) return false;
+    const float4 lo = reinterpret_cast<const float4*>(a.rowBox)[2 * (size_t)row];
+    const float4 hi = reinterpret_cast<const float4*>(a.rowBox)[2 * (size_t)row + 1];
+    if (__float_as_uint(lo.w) != 0u) return false;
+    if (!(lo.x <= hi.x)) return true;  // no solid in the row
+    const float d0 = fmaxf(hi.x - u0, u0 - lo.x), d1 = fmaxf(hi.y - u1, u1 - lo.y), d2 = fmaxf(hi.z - u2, u2 - lo.z);
+    return d0 * d0 + d1 * d1 + d2 * d2 < reach * reach;
+}
+
 // One thread per solid: F and F' from the static reference neighbours, the PK2 stress
 // hooke(E(F)+eps_p) + hooke(E'), A = (F S^T) L^-1 for the elastic force, and the plastic
 // flow rate (solid.rs:334-341, 497-518).
-__device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, uint32_t) {
+// The workgroups first compute the self-contact row boxes of the stage.
+__device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, uint32_t nb) {
+    contact_row_boxes(a, bid, nb);
     const uint32_t s = bid * 256u + threadIdx.x;
     if (s >= a.ns) return;
-    float* ke = a.keps + 16 * (size_t)s;
+    float4* ke = reinterpret_cast<float4*>(a.keps) + 4 * (size_t)s;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t p = a.part[s];
     if (p == kInvalid) {
-        for (int q = 0; q < 16; q++) ke[q] = 0.0f;
+        for (int q = 0; q < 4; q++) ke[q] = z4;
         return;
     }
     const uint32_t i = a.inv[p];
     const uint32_t mk = a.mk[i];
     if (!(mk & kMkElastic)) {
-        for (int q = 0; q < 16; q++) ke[q] = 0.0f;
+        for (int q = 0; q < 4; q++) ke[q] = z4;
         return;
     }
     const DMat& M = a.mats[mk & kMatMask];
@@ -1635,20 +1737,40 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
     const float ks = a.kscale ? a.kscale[s] : 1.0f;
     float F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [row*3+col]
     const float4* G = reinterpret_cast<const float4*>(a.sG);
-    for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
-        const uint32_t j = a.inv[a.sPid[q]];
-        const float4 g = G[q];
-        const float4 xj = X[j], vj = XV[j];
-        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-        const float dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
-        const float gg[3] = {g.x, g.y, g.z};
+    const uint32_t q0 = a.sOff[s], q1 = a.sOff[s + 1];
+    // four entries at a time: each level of the entry -> storage index -> state gathers is
+    // issued for all four before the next (a tail entry reloads the last one, unused)
+    for (uint32_t qb = q0; qb < q1; qb += 4) {
+        uint32_t jj[4];
+        float4 gq[4], xq[4], vq[4];
 #pragma unroll
-        for (int r = 0; r < 3; r++)
+        for (int k = 0; k < 4; k++) {
+            const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
+            jj[k] = a.sPid[q];
+            gq[k] = G[q];
+        }
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                F[3 * r + c] += g.w * (dx[r] * gg[c]);
-                Fd[3 * r + c] += g.w * (dv[r] * gg[c]);
-            }
+        for (int k = 0; k < 4; k++) jj[k] = a.inv[jj[k]];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xq[k] = X[jj[k]];
+            vq[k] = XV[jj[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (qb + (uint32_t)k >= q1) break;
+            const float4 g = gq[k], xj = xq[k], vj = vq[k];
+            const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+            const float dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+            const float gg[3] = {g.x, g.y, g.z};
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    F[3 * r + c] += g.w * (dx[r] * gg[c]);
+                    Fd[3 * r + c] += g.w * (dv[r] * gg[c]);
+                }
+        }
     }
     M4 F4 = m4_zero(), Fd4 = m4_zero();
     for (int r = 0; r < 3; r++)
@@ -1656,20 +1778,16 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
             F4.m[4 * c + r] = F[3 * r + c];
             Fd4.m[4 * c + r] = Fd[3 * r + c];
         }
-    M4 eps;
-    const float* ep = a.Xeps + 16 * (size_t)s;
-    for (int q = 0; q < 16; q++) eps.m[q] = ep[q];
+    const M4 eps = ld_m4(a.Xeps, s);
     const uint32_t dim = a.grid.dim_n;
     const M4 E = m4_add(m4_seth_hill(F4, M.strain_order, dim), eps);
     const M4 Ed = m4_seth_hill_rate(F4, Fd4, M.strain_order, dim);
     const M4 S = m4_add(m4_hooke(E, ks * M.lambda, ks * M.mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
     const M4 P = m4_mul(F4, m4_trans(S));
-    M4 Li;
-    for (int q = 0; q < 16; q++) Li.m[q] = a.Linv[16 * (size_t)s + q];
-    const M4 A = m4_mul(P, Li);
+    const M4 A = m4_mul(P, ld_m4(a.Linv, s));
     store_rows3(a.A + 12 * (size_t)s, A);
     store_rows3(a.Fm + 12 * (size_t)s, F4);
-    for (int q = 0; q < 16; q++) ke[q] = 0.0f;
+    M4 rate = m4_zero();
     if (M.plastic) {
         // eq_plastic_strain_sq carries the integer factor (dim-1)/dim == 0 (solid.rs:212-215)
         M4 dev = eps;
@@ -1681,9 +1799,10 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
         if (ys * ys <= sn) {
             sn = sqrtf(sn);
             const float sc = (sn - ys) / M.relax;
-            for (int q = 0; q < 16; q++) ke[q] = sc * (S.m[q] / sn);
+            for (int q = 0; q < 16; q++) rate.m[q] = sc * (S.m[q] / sn);
         }
     }
+    for (int q = 0; q < 4; q++) ke[q] = make_float4(rate.m[4 * q], rate.m[4 * q + 1], rate.m[4 * q + 2], rate.m[4 * q + 3]);
 }
 
 ARP_ENTRIES(k_strain, SolidArgs, strain_body)
@@ -1775,6 +1894,8 @@ __device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uin
             const float r_cut = 8.0f * h, r_geom = h * h / r_cut;
             const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
             const uint32_t xl = ci[0] > 0 ? ci[0] - 1 : 0, xh = min(ci[0] + 1, sx - 1);
+            const float reach = contact_reach(a, xi, r_cut);
+            const float u0 = xi.x - Xs.x, u1 = xi.y - Xs.y, u2 = xi.z - Xs.z;
             for (int dz = -1; dz <= 1; dz++) {
                 const int nz = (int)ci[2] + dz;
                 if (nz < 0 || nz >= (int)sz) continue;
@@ -1783,6 +1904,7 @@ __device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uin
                     if (ny < 0 || ny >= (int)sy) continue;
                     const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
                     if (a.rowStamp[row] != *a.stamp) continue;
+                    if (contact_row_clear(a, row, u0, u1, u2, reach)) continue;
                     const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
                     for (uint32_t q = ro[xl]; q < ro[xh + 1]; q++) {
                         const uint32_t pe = a.perm[q];
@@ -2770,6 +2892,8 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
         const uint32_t sx = a.grid.sub[0], sy = a.grid.sub[1], sz = a.grid.sub[2];
         const uint32_t xl = ci[0] > 0 ? ci[0] - 1 : 0, xh = min(ci[0] + 1, sx - 1);
         const uint32_t stampv = *a.stamp;
+        const float reach = contact_reach(a, xi, r_cut);
+        const float u0 = xi.x - Xs.x, u1 = xi.y - Xs.y, u2 = xi.z - Xs.z;
         for (int dz = -1; dz <= 1; dz++) {
             const int nz = (int)ci[2] + dz;
             if (nz < 0 || nz >= (int)sz) continue;
@@ -2778,6 +2902,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
                 if (ny < 0 || ny >= (int)sy) continue;
                 const uint32_t row = (uint32_t)ny + sy * (uint32_t)nz;
                 if (a.rowStamp[row] != stampv) continue;
+                if (contact_row_clear(a, row, u0, u1, u2, reach)) continue;
                 const uint32_t* ro = a.rowOff + (size_t)row * a.grid.rowstride;
                 for (uint32_t q = ro[xl]; q < ro[xh + 1]; q++) {
                     const uint32_t pe = a.perm[q];

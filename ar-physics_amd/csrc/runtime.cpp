@@ -204,7 +204,7 @@ struct arp_ctx {
     // solids: static reference-neighbour data and plastic-strain registers (solid-id order)
     bool solids_active = false;
     DBuf part_d, ref_d, Linv_d, sOff_d, sPid_d, sG_d, eOff_d, ePid_d, eSid_d, eG_d;
-    DBuf Yeps, Xeps[2], AccE, Ke[kMaxK], keps, A_d, Fm_d, ps_eps;
+    DBuf Yeps, Xeps[2], AccE, Ke[kMaxK], keps, A_d, Fm_d, ps_eps, rowBox;
     // per-stage API buffers
     DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys, saos_tmp;
     // grid
@@ -291,7 +291,7 @@ struct arp_ctx {
             bufs.push_back(&Ke[k]);
         }
         for (DBuf* b : {&sflag, &KF[0], &KF[1], &part_d, &ref_d, &Linv_d, &sOff_d, &sPid_d, &sG_d, &eOff_d,
-                        &ePid_d, &eSid_d, &eG_d, &Yeps, &Xeps[0], &Xeps[1], &AccE, &keps, &A_d, &Fm_d, &ps_eps,
+                        &ePid_d, &eSid_d, &eG_d, &Yeps, &Xeps[0], &Xeps[1], &AccE, &keps, &A_d, &Fm_d, &ps_eps, &rowBox,
                         &saos_tmp})
             bufs.push_back(b);
     }
@@ -860,6 +860,8 @@ int alloc_state(arp_ctx* c) {
             if ((rc = dalloc(c, *b, 64 * ns))) return rc;
         for (DBuf* b : {&c->A_d, &c->Fm_d})
             if ((rc = dalloc(c, *b, 48 * ns))) return rc;
+        // per grid row: the self-contact row test's displacement box (two float4)
+        if ((rc = dalloc(c, c->rowBox, 32 * (size_t)std::max<uint32_t>(c->grid.nrows, 1)))) return rc;
     }
     for (DBuf* b : {&c->keys, &c->ranks, &c->tmpKey, &c->tmpSrc, &c->perm, &c->gPid, &c->gGrp})
         if ((rc = dalloc(c, *b, 4 * n))) return rc;
@@ -937,6 +939,7 @@ SolidArgs solid_base(arp_ctx* c) {
     a.A = P<float>(c->A_d);
     a.Fm = P<float>(c->Fm_d);
     a.keps = P<float>(c->keps);
+    a.rowBox = P<float>(c->rowBox);
     a.perm = c->slot.perm;
     a.rowOff = c->slot.rowOff;
     a.rowStamp = c->slot.rowStamp;

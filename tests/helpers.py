@@ -198,3 +198,27 @@ def deformed_solid_scene(dim=3, plastic=False, with_fluid=True, seed=0, order=2)
         sc.solids["stress"][:, 0] = 1e-3
         sc.solids["stress"][:, 5] = -1e-3
     return sc
+
+
+def self_contact_scene(seed=0, gap=0.3, outside=False):
+    """Two 5^3 blocks of one elastic material, 0.4 m apart in the reference configuration
+    (12.5 h, beyond the 8h self-contact cut) and side by side now with a gap of `gap`
+    spacings along x, so self contact (solid.rs:426-450) acts between them; deformed and
+    moving.  With `outside`, the grid ends inside the second block, whose outer particles
+    then sit in clamped cells."""
+    sp = 0.02
+    h = 1.6 * sp
+    A = uniform_block(5, sp, 3, jitter=0.1, seed=seed, origin=(0.1, 0.3, 0.3)).astype(np.float64)
+    B = uniform_block(5, sp, 3, jitter=0.1, seed=seed + 1, origin=(0.1, 0.3, 0.3)).astype(np.float64)
+    X = np.concatenate([A, B + np.array([0.0, 0.0, 0.4])])
+    x = np.concatenate([A, B + np.array([(5 + gap) * sp, 0.0, 0.0])])
+    rng = np.random.default_rng(seed)
+    x = x + (x - x.mean(0)) @ (0.02 * rng.standard_normal((3, 3))).T + 0.03 * sp * rng.standard_normal(x.shape)
+    sm = S.solid_material(1000.0, -4e5, -1e5, normal_damp=-30.0, shear_damp=-20.0)
+    sm["mass"] = 1000.0 * sp**3
+    bounds = ((0.0, 0.0, 0.0), (0.25 if outside else 1.0, 1.0, 1.0))
+    sc = box_scene(X, bounds=bounds, h=h, solids_mask=np.ones(len(X), bool), solid_mat=sm)
+    sc.particles["pos"][:, :3] = x.astype(np.float32)
+    sc.particles["vel"][:, :3] = (0.1 * rng.standard_normal(x.shape)).astype(np.float32)
+    sc.particles["den"] *= (1 + rng.uniform(-2e-3, 2e-3, sc.n)).astype(np.float32)
+    return sc

@@ -12,6 +12,7 @@ import pytest
 from arp import tableau as T
 from test_gpu_adjoint import _records, _scene
 from test_gpu_adjoint_solid import _cantilever, _contact
+from helpers import self_contact_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +75,7 @@ CASES = {
     "cantilever_rk2": (_cantilever, "rk2", 1e-5, 2, 1,
                        {1: ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp")}),
     "contact_rk2": (_contact, "rk2", 1e-5, 2, 2, {1: ("normal_stiffness", "shear_stiffness"), 2: ("bulk_visc",)}),
+    "self_contact_rk2": (self_contact_scene, "rk2", 1e-5, 2, 1, {2: ("normal_stiffness", "shear_damp")}),
 }
 PAIRS = [("vel", "pos"), ("den", "vel"), ("pos", "vel"), ("vel", "den"), ("vel", "vel"), ("pos", "pos")]
 

@@ -9,7 +9,8 @@ import pytest
 from arp import scene as S
 from arp import tableau as T
 from arp import wire
-from helpers import box_scene, deformed_solid_scene as _solid_scene, random_grid_scene, split_cand, uniform_block
+from helpers import box_scene, deformed_solid_scene as _solid_scene, random_grid_scene, self_contact_scene, split_cand, \
+    uniform_block
 
 pytestmark = pytest.mark.gpu
 
@@ -297,6 +298,45 @@ def test_solid_step_parity(torch_cuda, oracle_mod, integ):
     print("plastic strain rel err %.2e" % e, flush=True)
     assert e <= TOL
     assert np.array_equal(so["ref_pos"], sc.solids["ref_pos"])
+
+
+@pytest.mark.parametrize("case", ["inside", "outside"])
+def test_self_contact_parity(torch_cuda, oracle_mod, case):
+    """Self contact (solid.rs:426-450) of two blocks of one material, far apart in the
+    reference state and touching now: one force evaluation and two RK2 steps against the
+    oracle.  "outside": the grid ends inside one block, whose outer particles sit in clamped
+    cells of the stencil."""
+    from arp.sim import from_device, to_device
+
+    sc = self_contact_scene(outside=case == "outside")
+    print("case self contact", case, sc.n, sc.ns, flush=True)
+    o = oracle_mod.Oracle(sc)
+    d32 = o.compute_forces(o.state("f"))
+    d64 = o.compute_forces(o.state("d"))
+    # the contact is part of the forces checked: without it block A's forces change
+    st = o.state("d")
+    st["pos"][125:, 0] += 0.3
+    far = o.compute_forces(st)
+    effect = float(np.abs(far["vel"][:125] - d64["vel"][:125]).max())
+    assert effect > 0.5, effect  # m/s^2 (gravity's term is 2g here)
+    sim = _sim(sc)
+    din, sin = to_device(sc.particles), to_device(sc.solids)
+    dout, sout = to_device(wire.particles(sc.n)), to_device(wire.solids(sc.ns))
+    sim.compute_forces(din, sin, dout, sout)
+    torch_cuda.cuda.synchronize()
+    errs = _report("self contact forces " + case, _state_of(from_device(dout, wire.PARTICLE)), d32, d64)
+    for k, (e32, _) in errs.items():
+        assert e32 <= TOL, (k, e32)
+    sim.close()
+    sim = _sim(sc, integrator="rk2", timestep=1e-5)
+    tab = T.tableau("rk2")
+    y32 = o.step(o.state("f"), tab, 1e-5, 2)
+    y64 = o.step(o.state("d"), tab, 1e-5, 2)
+    sim.update(2)
+    errs = _report("self contact step " + case, _state_of(sim.particles()), y32, y64)
+    for k, (e32, _) in errs.items():
+        assert e32 <= TOL, (k, e32)
+    sim.close()
 
 
 @pytest.mark.parametrize("nterms", [3, 9])
