@@ -2035,12 +2035,15 @@ static int issue_step(arp_ctx* c, float dt, const RecRange* rr, uint32_t k, cons
 // Runs nsteps steps; a run of kSpecMin steps or more on kept lists is issued speculatively,
 // kSpecW steps in flight, and a step whose lists turn out invalid runs again with the chain.
 // A recorded step speculates when the previous one left its input in its stage buffers.
-static int run_steps(arp_ctx* c, float dt, uint32_t nsteps, const RecRange* rr = nullptr) {
+// (spec_min: the fewest steps issued speculatively; the reverse-mode calls, which wait for
+// their steps anyway, speculate on runs of two)
+static int run_steps(arp_ctx* c, float dt, uint32_t nsteps, const RecRange* rr = nullptr,
+                     uint32_t spec_min = kSpecMin) {
     int rc;
     uint32_t k = 0;
     const bool adv = !rr || rr->advance;
     while (k < nsteps) {
-        if (rr && rr->K >= 2 && nsteps - k >= kSpecMin && spec_ok(c)) {
+        if (rr && rr->K >= 2 && nsteps - k >= spec_min && spec_ok(c)) {
             // a recorded step speculates from its stage buffers: bring its input there
             // (what the chain's reusing sort copies; a rebuild sorts from the landing ones)
             RecStep& R = c->rec[(rr->j0 + k) % rr->K];
@@ -2052,7 +2055,7 @@ static int run_steps(arp_ctx* c, float dt, uint32_t nsteps, const RecRange* rr =
                 c->rec_prefilled = R.Xp[0].p;
             }
         }
-        if (nsteps - k < kSpecMin || !spec_ok(c) ||
+        if (nsteps - k < spec_min || !spec_ok(c) ||
             (rr && (rr->K < 2 || c->rec_prefilled != c->rec[(rr->j0 + k) % rr->K].Xp[0].p))) {
             if ((rc = issue_step(c, dt, rr, k, nullptr))) return rc;
             if (adv) c->time += dt;
@@ -2198,8 +2201,10 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
         // it too), as its replay from the checkpoint does, so that every replay repeats the
         // recorded steps exactly
         if (nseg > 1) c->nl_dirty = true;
+        // only the last segment is kept; the others run as plain steps (the same arithmetic)
+        // for their checkpoints, and the reverse sweep replays them
         const RecRange rr{j0, nsteps, K, true};
-        if ((rc = run_steps(c, dt, j1 - j0, &rr))) return rc;
+        if ((rc = run_steps(c, dt, j1 - j0, seg + 1 == nseg ? &rr : nullptr, 2))) return rc;
     }
     c->rec_prefilled = nullptr;
     if ((rc = ck_save(c, c->ck[nseg], nseg > 1))) return rc;
@@ -2292,7 +2297,7 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
             replayed = true;
             if ((rc = ck_restore(c, c->ck[seg]))) return rc;
             const RecRange rr{j0, j1, K, false};
-            if ((rc = run_steps(c, c->rec_dt, j1 - j0, &rr))) return rc;
+            if ((rc = run_steps(c, c->rec_dt, j1 - j0, &rr, 2))) return rc;
             c->rec_prefilled = nullptr;
             if ((rc = fetch_over(c, j1 - j0))) return rc;
             c->rec_seg = (uint32_t)seg;
