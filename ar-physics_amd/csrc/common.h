@@ -351,6 +351,10 @@ struct SolidArgs {
     const float* Xvelq;
     const uint32_t* mk;
     const float* Xeps;        // stage input plastic strain per solid (16)
+    // when set: y_n's plastic strain, which a solid of a non-plastic material keeps through
+    // the step (its rate is zero): its strain reads it here and its RK update of the plastic
+    // strain is skipped (null: every solid updates it)
+    const float* epsY;
     float* A;                 // per solid: (P L^-1) rows, 3 float4
     float* Fm;                // per solid: F rows, 3 float4
     float* keps;              // per solid: plastic strain rate (16)
@@ -559,6 +563,7 @@ struct EpsInArgs {  // AoS solids -> plastic strain registers
     uint32_t ns;
     const void* aos;
     float* eps;
+    const uint32_t* order;  // internal solid -> its record in aos
 };
 
 struct EpsOutArgs {  // plastic strain (or its rate) -> AoS solids {part_id, ref_pos, stress}
@@ -567,6 +572,7 @@ struct EpsOutArgs {  // plastic strain (or its rate) -> AoS solids {part_id, ref
     const float* ref;  // null: write ref_pos = 0 (derivative)
     const uint32_t* part;
     void* aos;
+    const uint32_t* order;  // internal solid -> its record in aos
 };
 
 struct ClearSolidsArgs {

@@ -588,6 +588,13 @@ struct PairIn {
     bool bnd;
 };
 
+#ifndef ARP_ST_GROUP
+#define ARP_ST_GROUP 4  // strain list entries whose gathers are issued together
+#endif
+#ifndef ARP_EL_GROUP
+#define ARP_EL_GROUP 4  // elastic-force list entries whose gathers are issued together
+#endif
+
 #ifndef ARP_BATCH
 #define ARP_BATCH 4  // candidates whose loads are issued together (arrays are padded by this)
 #endif
@@ -1721,16 +1728,19 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t p = a.part[s];
     if (p == kInvalid) {
-        for (int q = 0; q < 4; q++) ke[q] = z4;
+        if (!a.epsY)
+            for (int q = 0; q < 4; q++) ke[q] = z4;
         return;
     }
     const uint32_t i = a.inv[p];
     const uint32_t mk = a.mk[i];
     if (!(mk & kMkElastic)) {
-        for (int q = 0; q < 4; q++) ke[q] = z4;
+        if (!a.epsY)
+            for (int q = 0; q < 4; q++) ke[q] = z4;
         return;
     }
     const DMat& M = a.mats[mk & kMatMask];
+    const bool fixedEps = a.epsY && !M.plastic;
     const float4* X = reinterpret_cast<const float4*>(a.Xposd);
     const float4* XV = reinterpret_cast<const float4*>(a.Xvelq);
     const float4 xi = X[i], vi = XV[i];
@@ -1738,26 +1748,27 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
     float F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [row*3+col]
     const float4* G = reinterpret_cast<const float4*>(a.sG);
     const uint32_t q0 = a.sOff[s], q1 = a.sOff[s + 1];
-    // four entries at a time: each level of the entry -> storage index -> state gathers is
-    // issued for all four before the next (a tail entry reloads the last one, unused)
-    for (uint32_t qb = q0; qb < q1; qb += 4) {
-        uint32_t jj[4];
-        float4 gq[4], xq[4], vq[4];
+    // ARP_ST_GROUP entries at a time: each level of the entry -> storage index -> state
+    // gathers is issued for the whole group before the next (a tail entry reloads the last
+    // one, unused)
+    for (uint32_t qb = q0; qb < q1; qb += ARP_ST_GROUP) {
+        uint32_t jj[ARP_ST_GROUP];
+        float4 gq[ARP_ST_GROUP], xq[ARP_ST_GROUP], vq[ARP_ST_GROUP];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < ARP_ST_GROUP; k++) {
             const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
             jj[k] = a.sPid[q];
             gq[k] = G[q];
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) jj[k] = a.inv[jj[k]];
+        for (int k = 0; k < ARP_ST_GROUP; k++) jj[k] = a.inv[jj[k]];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < ARP_ST_GROUP; k++) {
             xq[k] = X[jj[k]];
             vq[k] = XV[jj[k]];
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < ARP_ST_GROUP; k++) {
             if (qb + (uint32_t)k >= q1) break;
             const float4 g = gq[k], xj = xq[k], vj = vq[k];
             const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
@@ -1778,7 +1789,7 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
             F4.m[4 * c + r] = F[3 * r + c];
             Fd4.m[4 * c + r] = Fd[3 * r + c];
         }
-    const M4 eps = ld_m4(a.Xeps, s);
+    const M4 eps = ld_m4(fixedEps ? a.epsY : a.Xeps, s);
     const uint32_t dim = a.grid.dim_n;
     const M4 E = m4_add(m4_seth_hill(F4, M.strain_order, dim), eps);
     const M4 Ed = m4_seth_hill_rate(F4, Fd4, M.strain_order, dim);
@@ -1802,7 +1813,8 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
             for (int q = 0; q < 16; q++) rate.m[q] = sc * (S.m[q] / sn);
         }
     }
-    for (int q = 0; q < 4; q++) ke[q] = make_float4(rate.m[4 * q], rate.m[4 * q + 1], rate.m[4 * q + 2], rate.m[4 * q + 3]);
+    if (!fixedEps)
+        for (int q = 0; q < 4; q++) ke[q] = make_float4(rate.m[4 * q], rate.m[4 * q + 1], rate.m[4 * q + 2], rate.m[4 * q + 3]);
 }
 
 ARP_ENTRIES(k_strain, SolidArgs, strain_body)
@@ -1845,46 +1857,71 @@ __device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uin
         cell_of(a.grid, xi.x, xi.y, xi.z, ci[0], ci[1], ci[2]);
         float fx = 0.f, fy = 0.f, fz = 0.f;
         const float4* EG = reinterpret_cast<const float4*>(a.eG);
-        for (uint32_t q = a.eOff[s]; q < a.eOff[s + 1]; q++) {
-            const uint32_t s2 = a.eSid[q];
-            const uint32_t j = a.inv[a.ePid[q]];
-            const float4 g = EG[q];
-            const float4 xj = X[j];
-            const float V2 = m1 / xj.w;
-            const float4* A2 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s2;
-            const float c = -V1 * V2;
-            float4 Asum[3];
-            for (int r = 0; r < 3; r++)
-                Asum[r] = make_float4(c * (A1[r].x + A2[r].x), c * (A1[r].y + A2[r].y), c * (A1[r].z + A2[r].z), 0.f);
-            const float3 el = rows3_mul(Asum, g.x, g.y, g.z);
-            fx += el.x;
-            fy += el.y;
-            if (a.grid.dim_n > 2) fz += el.z;
-            // hourglass (solid.rs:432-443): same material, W(dX) != 0, inside the stencil
-            if (g.w != 0.0f && valid_i && a.sflag[j] == 0u) {
-                uint32_t cj[3] = {0, 0, 0};
-                cell_of(a.grid, xj.x, xj.y, xj.z, cj[0], cj[1], cj[2]);
-                bool adj = true;
-                for (int d = 0; d < 3; d++) adj &= (ci[d] + 1 >= cj[d]) && (cj[d] + 1 >= ci[d]);
-                if (adj) {
-                    const float4 X2 = REF[s2];
-                    const float dX[3] = {X2.x - Xs.x, X2.y - Xs.y, X2.z - Xs.z};
-                    const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-                    const float l = sqrtf(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
-                    const float4* F2 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s2;
-                    float err[3];
-                    for (int r = 0; r < 3; r++) {
-                        const float4 fa = make_float4(0.5f * (F1[r].x + F2[r].x), 0.5f * (F1[r].y + F2[r].y),
-                                                      0.5f * (F1[r].z + F2[r].z), 0.f);
-                        err[r] = fa.x * dX[0] + fa.y * dX[1] + fa.z * dX[2] - dx[r];
+        const float4* AA = reinterpret_cast<const float4*>(a.A);
+        const float4* FF = reinterpret_cast<const float4*>(a.Fm);
+        const uint32_t q0 = a.eOff[s], q1 = a.eOff[s + 1];
+        // ARP_EL_GROUP entries at a time: each level of the gathers (entry -> storage index
+        // -> state) is issued for the whole group before the next; the group's pairs are then
+        // summed in entry order (a tail entry reloads the last one, unused)
+        for (uint32_t qb = q0; qb < q1; qb += ARP_EL_GROUP) {
+            uint32_t sq[ARP_EL_GROUP], jq[ARP_EL_GROUP];
+            float4 gq[ARP_EL_GROUP], xq[ARP_EL_GROUP], Aq[ARP_EL_GROUP][3];
+#pragma unroll
+            for (int k = 0; k < ARP_EL_GROUP; k++) {
+                const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
+                sq[k] = a.eSid[q];
+                jq[k] = a.ePid[q];
+                gq[k] = EG[q];
+            }
+#pragma unroll
+            for (int k = 0; k < ARP_EL_GROUP; k++) {
+                jq[k] = a.inv[jq[k]];
+#pragma unroll
+                for (int r = 0; r < 3; r++) Aq[k][r] = AA[3 * (size_t)sq[k] + r];
+            }
+#pragma unroll
+            for (int k = 0; k < ARP_EL_GROUP; k++) xq[k] = X[jq[k]];
+#pragma unroll
+            for (int k = 0; k < ARP_EL_GROUP; k++) {
+                if (qb + (uint32_t)k >= q1) break;
+                const uint32_t s2 = sq[k], j = jq[k];
+                const float4 g = gq[k], xj = xq[k];
+                const float V2 = m1 / xj.w;
+                const float c = -V1 * V2;
+                float4 Asum[3];
+                for (int r = 0; r < 3; r++)
+                    Asum[r] = make_float4(c * (A1[r].x + Aq[k][r].x), c * (A1[r].y + Aq[k][r].y),
+                                          c * (A1[r].z + Aq[k][r].z), 0.f);
+                const float3 el = rows3_mul(Asum, g.x, g.y, g.z);
+                fx += el.x;
+                fy += el.y;
+                if (a.grid.dim_n > 2) fz += el.z;
+                // hourglass (solid.rs:432-443): same material, W(dX) != 0, inside the stencil
+                if (g.w != 0.0f && valid_i && a.sflag[j] == 0u) {
+                    uint32_t cj[3] = {0, 0, 0};
+                    cell_of(a.grid, xj.x, xj.y, xj.z, cj[0], cj[1], cj[2]);
+                    bool adj = true;
+                    for (int d = 0; d < 3; d++) adj &= (ci[d] + 1 >= cj[d]) && (cj[d] + 1 >= ci[d]);
+                    if (adj) {
+                        const float4 X2 = REF[s2];
+                        const float dX[3] = {X2.x - Xs.x, X2.y - Xs.y, X2.z - Xs.z};
+                        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+                        const float l = sqrtf(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+                        const float4* F2 = FF + 3 * (size_t)s2;
+                        float err[3];
+                        for (int r = 0; r < 3; r++) {
+                            const float4 fa = make_float4(0.5f * (F1[r].x + F2[r].x), 0.5f * (F1[r].y + F2[r].y),
+                                                          0.5f * (F1[r].z + F2[r].z), 0.f);
+                            err[r] = fa.x * dX[0] + fa.y * dX[1] + fa.z * dX[2] - dx[r];
+                        }
+                        const float m2 = m1, d2 = xj.w;
+                        const float sc = (100000000.0f * m1 * m2 / (d1 * d2)) *
+                                         (0.5f * (err[0] * dx[0] + err[1] * dx[1] + err[2] * dx[2]) / (l + 10.0f * EPS * h)) *
+                                         (1.0f / (l * l + EPS * h * h)) * g.w;
+                        fx -= sc * dx[0] / (l + EPS * h);
+                        fy -= sc * dx[1] / (l + EPS * h);
+                        fz -= sc * dx[2] / (l + EPS * h);
                     }
-                    const float m2 = m1, d2 = xj.w;
-                    const float sc = (100000000.0f * m1 * m2 / (d1 * d2)) *
-                                     (0.5f * (err[0] * dx[0] + err[1] * dx[1] + err[2] * dx[2]) / (l + 10.0f * EPS * h)) *
-                                     (1.0f / (l * l + EPS * h * h)) * g.w;
-                    fx -= sc * dx[0] / (l + EPS * h);
-                    fy -= sc * dx[1] / (l + EPS * h);
-                    fz -= sc * dx[2] / (l + EPS * h);
                 }
             }
         }
@@ -1940,9 +1977,9 @@ __device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uin
         } else {
             float4 op;
             rk_update_particle(a.rk, a.grid, M, i, kP, kV, haveKey, nextKey, nextBin, op);
-            rk_update_eps(a.rk, s, a.keps + 16 * (size_t)s);
+            if (!(a.epsY && !M.plastic)) rk_update_eps(a.rk, s, a.keps + 16 * (size_t)s);
         }
-    } else if (s < a.ns && a.mode == 0) {
+    } else if (s < a.ns && a.mode == 0 && !a.epsY) {
         // solids of non-elastic materials keep their plastic strain (zero rate)
         float z[16];
         for (int q = 0; q < 16; q++) z[q] = 0.0f;
@@ -1962,7 +1999,7 @@ ARP_ENTRIES(k_solid, SolidArgs, solid_body)
 extern "C" __global__ void __launch_bounds__(256) k_eps_in(EpsInArgs a) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= a.ns) return;
-    const AosSolid so = reinterpret_cast<const AosSolid*>(a.aos)[s];
+    const AosSolid so = reinterpret_cast<const AosSolid*>(a.aos)[a.order[s]];
     float4* e = reinterpret_cast<float4*>(a.eps + 16 * (size_t)s);
     for (int q = 0; q < 4; q++) e[q] = so.stress[q];
 }
@@ -1976,7 +2013,7 @@ extern "C" __global__ void __launch_bounds__(256) k_eps_out(EpsOutArgs a) {
     o.ref_pos = a.ref ? reinterpret_cast<const float4*>(a.ref)[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4* e = reinterpret_cast<const float4*>(a.eps + 16 * (size_t)s);
     for (int q = 0; q < 4; q++) o.stress[q] = e[q];
-    reinterpret_cast<AosSolid*>(a.aos)[s] = o;
+    reinterpret_cast<AosSolid*>(a.aos)[a.order[s]] = o;
 }
 
 // ------------------------------------------------------------------------------------

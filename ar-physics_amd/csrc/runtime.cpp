@@ -173,6 +173,11 @@ struct arp_ctx {
     PhysP phys{};
     std::vector<ArpMaterial> mats_h;
     std::vector<uint32_t> mat_by_pid, sid_by_pid, mk_by_pid;
+    // Solids are numbered internally in the cell order of their reference positions, so that
+    // a thread per solid reads its particle's state (stored cell-sorted) coalesced:
+    // s_ord[t] = external solid id of internal solid t, s_rank = the inverse.  Every per-solid
+    // device array is in the internal order; the solid records the caller sees are not.
+    std::vector<uint32_t> s_ord, s_rank;
     int64_t mobile_mat = -1;  // the material of every mobile particle, or -1 (none or several)
     int64_t bnd_mat = -1;     // the material of every boundary particle (none: -1, several: -2)
     bool bnd_elastic = false; // some boundary word carries the elastic flag
@@ -186,7 +191,7 @@ struct arp_ctx {
     std::vector<uint32_t> b_orig;  // sorted boundary slot -> original boundary id
     std::vector<uint32_t> bP_h, rP_h;
 
-    DBuf mats, inter, bposd, bvelq, bmk, bP, rP, mkStatic, matByPid, sidByPid;
+    DBuf mats, inter, bposd, bvelq, bmk, bP, rP, mkStatic, matByPid, sidByPid, sidExtByPid, sOrd;
     // state
     DBuf Ysort[2], Yland[2];  // [0] posd, [1] velq
     // the live state is only in Ysort (same storage order): a list-reusing RK step of several
@@ -277,7 +282,7 @@ struct arp_ctx {
     arp_ctx() {
         for (bool& b : prof_sel) b = true;
         bufs = {&mats,     &inter,     &bposd,      &bvelq,    &bmk,    &bP,       &rP,
-                &mkStatic, &matByPid,  &sidByPid,   &Ysort[0], &Ysort[1], &Yland[0], &Yland[1],
+                &mkStatic, &matByPid,  &sidByPid,   &sidExtByPid, &sOrd, &Ysort[0], &Ysort[1], &Yland[0], &Yland[1],
                 &meta_mk[0], &meta_mk[1], &meta_pid[0], &meta_pid[1], &Xb[0][0], &Xb[0][1],
                 &Xb[1][0], &Xb[1][1],  &Acc[0],     &Acc[1],   &inv,    &ps_mk[0], &ps_mk[1],
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
@@ -491,11 +496,16 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
     std::vector<uint32_t> refcell(n, ARP_INVALID_INDEX);
     for (uint64_t cc = 0; cc < c->ncells; cc++)
         for (uint32_t q = c->rP_h[cc]; q < c->rP_h[cc + 1]; q++) refcell[kept[q]] = (uint32_t)cc;
+    // (arrays in the internal solid order; refx by external id)
+    const std::vector<uint32_t>& ord = c->s_ord;
+    const std::vector<uint32_t>& rank = c->s_rank;
     std::vector<uint32_t> part(ns), sOff(ns + 1, 0), eOff(ns + 1, 0), sPid, ePid, eSid;
-    std::vector<float> ref(4 * (size_t)ns), Linv(16 * (size_t)ns, 0.0f), sG, eG;
-    for (uint32_t s = 0; s < ns; s++) {
-        part[s] = solids[s].part_id;
-        for (int q = 0; q < 4; q++) ref[4 * s + q] = solids[s].ref_pos[q];
+    std::vector<float> refx(4 * (size_t)ns), ref(4 * (size_t)ns), Linv(16 * (size_t)ns, 0.0f), sG, eG;
+    for (uint32_t s = 0; s < ns; s++)
+        for (int q = 0; q < 4; q++) refx[4 * s + q] = solids[s].ref_pos[q];
+    for (uint32_t t = 0; t < ns; t++) {
+        part[t] = solids[ord[t]].part_id;
+        for (int q = 0; q < 4; q++) ref[4 * t + q] = refx[4 * (size_t)ord[t] + q];
     }
     auto gradw = [&](const float* r, float* out) -> float {
         float rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3];
@@ -504,10 +514,11 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
         for (int q = 0; q < 4; q++) out[q] = gs * r[q];
         return diff;
     };
-    for (uint32_t s = 0; s < ns; s++) {
-        sOff[s] = (uint32_t)sPid.size();
-        eOff[s] = (uint32_t)ePid.size();
-        const uint32_t p = part[s];
+    for (uint32_t t = 0; t < ns; t++) {
+        const uint32_t s = ord[t];
+        sOff[t] = (uint32_t)sPid.size();
+        eOff[t] = (uint32_t)ePid.size();
+        const uint32_t p = part[t];
         if (p == ARP_INVALID_INDEX || p >= n) continue;
         const uint32_t mat = particles[p].mat;
         if (!is_stiff(mats[mat])) continue;
@@ -528,7 +539,7 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
                             const uint32_t s2 = particles[p2].solid_id;
                             const uint32_t mat2 = particles[p2].mat;
                             float dX[4], gw[4];
-                            for (int r = 0; r < 4; r++) dX[r] = ref[4 * s2 + r] - ref[4 * s + r];
+                            for (int r = 0; r < 4; r++) dX[r] = refx[4 * (size_t)s2 + r] - refx[4 * (size_t)s + r];
                             const float diff = gradw(dX, gw);
                             if (is_stiff(mats[mat2])) {
                                 const float V2 = mats[mat2].mass / mats[mat2].start_den;
@@ -542,7 +553,7 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
                             }
                             if (mat2 == mat && diff > 0.0f) {
                                 ePid.push_back(p2);
-                                eSid.push_back(s2);
+                                eSid.push_back(rank[s2]);
                                 eG.push_back(gw[0]);
                                 eG.push_back(gw[1]);
                                 eG.push_back(gw[2]);
@@ -554,7 +565,7 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
         for (uint32_t i = dim; i < 4; i++) L[5 * i] = 1.0f;
         float Li[16];
         invert4(L, Li);
-        for (int q = 0; q < 16; q++) Linv[16 * (size_t)s + q] = Li[q];
+        for (int q = 0; q < 16; q++) Linv[16 * (size_t)t + q] = Li[q];
         // strain pairs carry L^-1 gradW(dX) (strain.rs:156)
         for (size_t q = s_first, t = 0; q < sPid.size(); q++, t++) {
             const float* gw = &gws[3 * t];
@@ -915,7 +926,7 @@ int aos_to_soa(arp_ctx* c, const void* d_aos, float* posd, float* velq, uint32_t
 }
 
 int soa_to_aos(arp_ctx* c, const float* posd, const float* velq, const uint32_t* pid, void* d_aos) {
-    AosOutArgs a{c->n, posd, velq, pid, P<uint32_t>(c->sidByPid), P<uint32_t>(c->matByPid), d_aos};
+    AosOutArgs a{c->n, posd, velq, pid, P<uint32_t>(c->sidExtByPid), P<uint32_t>(c->matByPid), d_aos};
     return launch(c, K_AOS_OUT, nblk(c->n), 256, a);
 }
 
@@ -1256,6 +1267,8 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
             sa.mk = P<uint32_t>(c->meta_mk[c->cur]);
             sa.pid = P<uint32_t>(c->meta_pid[c->cur]);
             sa.Xeps = Xe;
+            // solids of non-plastic materials keep y_n's plastic strain (zero rate)
+            sa.epsY = verlet ? nullptr : P<float>(c->Yeps);
             sa.mode = 0;
             sa.rk = r;
             if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
@@ -1847,7 +1860,7 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
         if (c->mk_by_pid[i] & kMkElastic) c->any_elastic = true;
     for (uint32_t i = 1; i < n && c->mobile_mat >= 0; i++)
         if (c->mat_by_pid[i] != (uint32_t)c->mobile_mat) c->mobile_mat = -1;
-    if ((rc = upload(c, c->sidByPid, c->sid_by_pid.data(), 4 * (size_t)n))) return rc;
+    if ((rc = upload(c, c->sidExtByPid, c->sid_by_pid.data(), 4 * (size_t)n))) return rc;
 
     // static boundary grid (mode 0 fill, once: neighbor_list.rs:247-264)
     {
@@ -1911,6 +1924,31 @@ int arp_scene_upload(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
         std::vector<uint32_t> kept;
         cell_sort(ncells, key, room, c->rP_h, kept);
         if ((rc = upload(c, c->rP, c->rP_h.data(), c->rP_h.size() * 4))) return rc;
+        // the internal solid order: the solids of the reference grid in its cell order, then
+        // the others (refused by a full cell, non-finite, unused) by id
+        c->s_ord.clear();
+        c->s_rank.assign(ns, ARP_INVALID_INDEX);
+        for (uint32_t p : kept) {
+            const uint32_t sid = particles[p].solid_id;
+            if (sid < ns && c->s_rank[sid] == ARP_INVALID_INDEX) {
+                c->s_rank[sid] = (uint32_t)c->s_ord.size();
+                c->s_ord.push_back(sid);
+            }
+        }
+        for (uint32_t sid = 0; sid < ns; sid++)
+            if (c->s_rank[sid] == ARP_INVALID_INDEX) {
+                c->s_rank[sid] = (uint32_t)c->s_ord.size();
+                c->s_ord.push_back(sid);
+            }
+        {
+            std::vector<uint32_t> si(std::max<uint32_t>(n, 1), ARP_INVALID_INDEX);
+            for (uint32_t i = 0; i < n; i++) {
+                const uint32_t sid = c->sid_by_pid[i];
+                si[i] = sid < ns ? c->s_rank[sid] : sid;
+            }
+            if ((rc = upload(c, c->sidByPid, si.data(), 4 * (size_t)n))) return rc;
+            if ((rc = upload(c, c->sOrd, c->s_ord.data(), 4 * (size_t)std::max<uint32_t>(ns, 1)))) return rc;
+        }
         // the solid path runs only when there is more than one solid (shaders/mod.rs:51)
         c->solids_active = ns > 1;
         if (c->solids_active)
@@ -1943,7 +1981,7 @@ int arp_upload_state(arp_ctx* c, const ArpParticle* particles, uint32_t n, const
         const ArpSolidParticle* src = solids ? solids : c->solids_h.data();
         if (solids && ns != c->ns) return fail(c, ARP_E_ARG, "state has %u solids, scene %u", ns, c->ns);
         if ((rc = upload(c, c->saos_tmp, src, sizeof(ArpSolidParticle) * (size_t)c->ns))) return rc;
-        EpsInArgs e{c->ns, c->saos_tmp.p, P<float>(c->Yeps)};
+        EpsInArgs e{c->ns, c->saos_tmp.p, P<float>(c->Yeps), P<uint32_t>(c->sOrd)};
         if ((rc = launch(c, K_EPS_IN, nblk(c->ns), 256, e))) return rc;
     }
     if (c->keys_valid)
@@ -2229,7 +2267,9 @@ int arp_set_stiffness_field(arp_ctx* c, const float* scale, uint32_t ns) {
     if (ns != c->ns) return fail(c, ARP_E_ARG, "stiffness field has %u entries, scene %u solids", ns, c->ns);
     c->kscale_h.assign(scale, scale + ns);
     c->rec_valid = false;
-    return upload(c, c->kscale, scale, 4 * (size_t)std::max<uint32_t>(ns, 1));
+    std::vector<float> ks(std::max<uint32_t>(ns, 1), 1.0f);
+    for (uint32_t t = 0; t < ns; t++) ks[t] = scale[c->s_ord[t]];
+    return upload(c, c->kscale, ks.data(), 4 * ks.size());
 }
 
 int arp_adjoint_stiffness_grad(arp_ctx* c, float* out, uint32_t ns) {
@@ -2248,7 +2288,8 @@ int arp_adjoint_stiffness_grad(arp_ctx* c, float* out, uint32_t ns) {
             continue;
         }
         const ArpMaterial& M = c->mats_h[c->mat_by_pid[p]];
-        out[s] = M.normal_stiffness * sp[4 * s] + M.shear_stiffness * sp[4 * s + 1];
+        const size_t t = c->s_rank[s];
+        out[s] = M.normal_stiffness * sp[4 * t] + M.shear_stiffness * sp[4 * t + 1];
     }
     return ARP_OK;
 }
@@ -2330,15 +2371,16 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
             acc[6 * m + 1] = pg[2 * m + 1];
         }
         // per-solid partials w.r.t. the effective (scaled) constants -> per material
-        for (uint32_t s = 0; s < c->ns && !sp.empty(); s++) {
+        for (uint32_t t = 0; t < c->ns && !sp.empty(); t++) {  // (internal order)
+            const uint32_t s = c->s_ord[t];
             const uint32_t p = c->solids_h[s].part_id;
             if (p >= c->n) continue;
             const uint32_t m = c->mat_by_pid[p];
             const float ks = c->kscale_h.empty() ? 1.0f : c->kscale_h[s];
-            acc[6 * m + 2] += (double)sp[4 * s] * ks;
-            acc[6 * m + 3] += (double)sp[4 * s + 1] * ks;
-            acc[6 * m + 4] += sp[4 * s + 2];
-            acc[6 * m + 5] += sp[4 * s + 3];
+            acc[6 * m + 2] += (double)sp[4 * t] * ks;
+            acc[6 * m + 3] += (double)sp[4 * t + 1] * ks;
+            acc[6 * m + 4] += sp[4 * t + 2];
+            acc[6 * m + 5] += sp[4 * t + 3];
         }
         for (size_t q = 0; q < acc.size(); q++) grad_params[q] = (float)acc[q];
     }
@@ -2359,7 +2401,8 @@ int arp_download(arp_ctx* c, ArpParticle* particles, uint32_t n, ArpSolidParticl
         if (ns != c->ns) return fail(c, ARP_E_ARG, "download of %u solids, scene has %u", ns, c->ns);
         if (c->solids_active) {
             if ((rc = dalloc(c, c->saos_tmp, sizeof(ArpSolidParticle) * (size_t)ns))) return rc;
-            EpsOutArgs e{ns, P<float>(c->Yeps), P<float>(c->ref_d), P<uint32_t>(c->part_d), c->saos_tmp.p};
+            EpsOutArgs e{ns, P<float>(c->Yeps), P<float>(c->ref_d), P<uint32_t>(c->part_d), c->saos_tmp.p,
+                         P<uint32_t>(c->sOrd)};
             if ((rc = launch(c, K_EPS_OUT, nblk(ns), 256, e))) return rc;
             HIPCHK(c, hipMemcpyAsync(solids, c->saos_tmp.p, sizeof(ArpSolidParticle) * (size_t)ns,
                                      hipMemcpyDeviceToHost, c->stream));
@@ -2497,7 +2540,7 @@ int arp_compute_forces(arp_ctx* c, const ArpParticle* d_state, const ArpSolidPar
     f.KdV = P<float>(kd[1]);
     if ((rc = launch(c, K_FLUID, nblk(c->n), 256, f))) return rc;
     if (c->solids_active) {
-        EpsInArgs e{c->ns, d_solids, P<float>(c->ps_eps)};
+        EpsInArgs e{c->ns, d_solids, P<float>(c->ps_eps), P<uint32_t>(c->sOrd)};
         if ((rc = launch(c, K_EPS_IN, nblk(c->ns), 256, e))) return rc;
         SolidArgs sa = solid_base(c);
         sa.Xposd = P<float>(c->Xb[1][0]);
@@ -2515,7 +2558,8 @@ int arp_compute_forces(arp_ctx* c, const ArpParticle* d_state, const ArpSolidPar
     if (d_solid_deriv && d_solids) {
         if (c->solids_active) {
             // dest.solids (cleared) + dest2.solids {part_id, 0, plastic rate}
-            EpsOutArgs e{c->ns, P<float>(c->keps), nullptr, P<uint32_t>(c->part_d), d_solid_deriv};
+            EpsOutArgs e{c->ns, P<float>(c->keps), nullptr, P<uint32_t>(c->part_d), d_solid_deriv,
+                         P<uint32_t>(c->sOrd)};
             if ((rc = launch(c, K_EPS_OUT, nblk(c->ns), 256, e))) return rc;
         } else {
             ClearSolidsArgs a{c->ns, d_solids, d_solid_deriv};
