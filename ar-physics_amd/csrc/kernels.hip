@@ -588,6 +588,11 @@ struct PairIn {
     bool bnd;
 };
 
+#ifndef ARP_ADJ_GROUP
+#define ARP_ADJ_GROUP 4  // solid adjoint list entries whose gathers are issued together
+#endif
+// per-solid F and F' cotangents of the solid reverse pass: 18 floats, padded to 20 (float4 rows)
+constexpr uint32_t kFbarStride = 20;
 #ifndef ARP_ST_GROUP
 #define ARP_ST_GROUP 4  // strain list entries whose gathers are issued together
 #endif
@@ -2842,7 +2847,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     const SolidArgs& a = aa.f;
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= a.ns) return;
-    float* fb = aa.Fbar + 18 * (size_t)s;
+    float* fb = aa.Fbar + kFbarStride * (size_t)s;
     const uint32_t p = a.part[s];
     const uint32_t i = p == kInvalid ? 0u : a.inv[p];
     if (p == kInvalid || !(a.mk[i] & kMkElastic)) {
@@ -2870,13 +2875,38 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     cell_of(a.grid, xi.x, xi.y, xi.z, ci[0], ci[1], ci[2]);
     float Ab[9] = {0}, Fhb[9] = {0}, xb[3] = {0, 0, 0}, rb = 0.0f, fs[3] = {0, 0, 0};
     const float4* EG = reinterpret_cast<const float4*>(a.eG);
-    for (uint32_t q = a.eOff[s]; q < a.eOff[s + 1]; q++) {
-        const uint32_t s2 = a.eSid[q];
-        const uint32_t j = a.inv[a.ePid[q]];
-        const float4 g = EG[q];
-        const float4 xj = X[j];
+    const float4* AA = reinterpret_cast<const float4*>(a.A);
+    const float4* WVv = reinterpret_cast<const float4*>(aa.Wv);
+    const uint32_t q0 = a.eOff[s], q1 = a.eOff[s + 1];
+    // ARP_ADJ_GROUP entries at a time, each gather level issued for the group (as k_solid)
+    for (uint32_t qb = q0; qb < q1; qb += ARP_ADJ_GROUP) {
+        uint32_t sq[ARP_ADJ_GROUP], jq[ARP_ADJ_GROUP];
+        float4 gq[ARP_ADJ_GROUP], xq[ARP_ADJ_GROUP], wq[ARP_ADJ_GROUP], Aq[ARP_ADJ_GROUP][3];
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+            const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
+            sq[k] = a.eSid[q];
+            jq[k] = a.ePid[q];
+            gq[k] = EG[q];
+        }
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+            jq[k] = a.inv[jq[k]];
+#pragma unroll
+            for (int r = 0; r < 3; r++) Aq[k][r] = AA[3 * (size_t)sq[k] + r];
+        }
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+            xq[k] = X[jq[k]];
+            wq[k] = WVv[jq[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+        if (qb + (uint32_t)k >= q1) break;
+        const uint32_t s2 = sq[k], j = jq[k];
+        const float4 g = gq[k], xj = xq[k];
         const float dj = xj.w, V2 = m1 / dj, c = -V1 * V2;
-        const float4* A2 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s2;
+        const float4* A2 = Aq[k];
         const float gg[3] = {g.x, g.y, g.z};
         float e[3];
         for (int r = 0; r < 3; r++) {
@@ -2886,7 +2916,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
         fs[0] += c * e[0];
         fs[1] += c * e[1];
         fs[2] += zm * c * e[2];
-        const float4 wvj = ld_w(aa.Wv, j, ws);
+        const float4 wvj = make_float4(ws * wq[k].x, ws * wq[k].y, ws * wq[k].z, ws * wq[k].w);
         const float aj[3] = {wvj.x / dj, wvj.y / dj, wvj.z / dj};
         const float aim[3] = {ai[0], ai[1], zm * ai[2]}, ajm[3] = {aj[0], aj[1], zm * aj[2]};
         // own force c (A1 + A2) g and the partner's c (A2 + A1)(-g): both hold A1
@@ -2921,6 +2951,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
                 rb += o2.gd2;
                 for (int q2 = 0; q2 < 9; q2++) Fhb[q2] += 0.5f * o2.gfa[q2];
             }
+        }
         }
     }
     // self contact (solid.rs:426-450): own force and the partners' forces on this particle
@@ -2989,18 +3020,36 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     const float4 vi = XVv[i];
     float F[9] = {0}, Fd[9] = {0};
     const float4* G = reinterpret_cast<const float4*>(a.sG);
-    for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
-        const uint32_t j = a.inv[a.sPid[q]];
-        const float4 g = G[q];
-        const float4 xj = X[j], vj = XVv[j];
-        const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-        const float dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
-        const float gg[3] = {g.x, g.y, g.z};
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++) {
-                F[3 * r + c] += g.w * (dx[r] * gg[c]);
-                Fd[3 * r + c] += g.w * (dv[r] * gg[c]);
-            }
+    const uint32_t p0 = a.sOff[s], p1 = a.sOff[s + 1];
+    for (uint32_t qb = p0; qb < p1; qb += ARP_ST_GROUP) {  // (grouped as in k_strain)
+        uint32_t jj[ARP_ST_GROUP];
+        float4 gq[ARP_ST_GROUP], xq[ARP_ST_GROUP], vq[ARP_ST_GROUP];
+#pragma unroll
+        for (int k = 0; k < ARP_ST_GROUP; k++) {
+            const uint32_t q = min(qb + (uint32_t)k, p1 - 1u);
+            jj[k] = a.sPid[q];
+            gq[k] = G[q];
+        }
+#pragma unroll
+        for (int k = 0; k < ARP_ST_GROUP; k++) jj[k] = a.inv[jj[k]];
+#pragma unroll
+        for (int k = 0; k < ARP_ST_GROUP; k++) {
+            xq[k] = X[jj[k]];
+            vq[k] = XVv[jj[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < ARP_ST_GROUP; k++) {
+            if (qb + (uint32_t)k >= p1) break;
+            const float4 g = gq[k], xj = xq[k], vj = vq[k];
+            const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+            const float dv[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+            const float gg[3] = {g.x, g.y, g.z};
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    F[3 * r + c] += g.w * (dx[r] * gg[c]);
+                    Fd[3 * r + c] += g.w * (dv[r] * gg[c]);
+                }
+        }
     }
     M4 F4 = m4_zero(), Fd4 = m4_zero(), Ab4 = m4_zero(), Fh4 = m4_zero();
     for (int r = 0; r < 3; r++)
@@ -3010,16 +3059,13 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
             Ab4.m[4 * c + r] = Ab[3 * r + c];
             Fh4.m[4 * c + r] = Fhb[3 * r + c];
         }
-    M4 eps;
-    const float* ep = a.Xeps + 16 * (size_t)s;
-    for (int q = 0; q < 16; q++) eps.m[q] = ep[q];
+    const M4 eps = ld_m4(a.Xeps, s);
     const float ks = a.kscale ? a.kscale[s] : 1.0f;
     const float lam = ks * M.lambda, mu = ks * M.mu;
     const M4 E = m4_add(m4_seth_hill(F4, 2, dim), eps);
     const M4 Ed = m4_seth_hill_rate(F4, Fd4, 2, dim);
     const M4 S = m4_add(m4_hooke(E, lam, mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
-    M4 Li;
-    for (int q = 0; q < 16; q++) Li.m[q] = a.Linv[16 * (size_t)s + q];
+    const M4 Li = ld_m4(a.Linv, s);
     const M4 Pb = m4_mul(Ab4, m4_trans(Li));
     const M4 Sb = m4_mul(m4_trans(Pb), F4);
     M4 Fb = m4_add(m4_mul(Pb, S), Fh4);
@@ -3058,36 +3104,78 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
     const float4* REF = reinterpret_cast<const float4*>(a.ref);
     const float h = a.phys.h, hh = h * h, kn = a.phys.k;
     const float Vp = M.mass / M.start_den;  // this particle's volume in its neighbours' F
-    const float* fb = aa.Fbar + 18 * (size_t)s;
+    const float* fb = aa.Fbar + kFbarStride * (size_t)s;
     const float4 Xs = REF[s];
     float xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
     const float4* G = reinterpret_cast<const float4*>(a.sG);
-    for (uint32_t q = a.sOff[s]; q < a.sOff[s + 1]; q++) {
-        const uint32_t pid2 = a.sPid[q];
-        const uint32_t s2 = a.sidByPid[pid2];
-        const float4 g = G[q];
-        const float gg[3] = {g.x, g.y, g.z};
-        // own F: dx = x_q - x_s enters with weight V2 L_s^-1 gradW(dX)
-        for (int r = 0; r < 3; r++) {
-            const float t = g.w * (fb[3 * r] * gg[0] + fb[3 * r + 1] * gg[1] + fb[3 * r + 2] * gg[2]);
-            const float td = g.w * (fb[9 + 3 * r] * gg[0] + fb[9 + 3 * r + 1] * gg[1] + fb[9 + 3 * r + 2] * gg[2]);
-            xb[r] -= t;
-            vb[r] -= td;
+    float f1[18];  // this solid's F, F' cotangents
+    {
+        const float4* fb4 = reinterpret_cast<const float4*>(fb);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 v = fb4[q];
+            f1[4 * q] = v.x, f1[4 * q + 1] = v.y, f1[4 * q + 2] = v.z, f1[4 * q + 3] = v.w;
         }
-        // the neighbour's F holds this particle with weight Vp L_s2^-1 gradW(X_s - X_s2)
-        const float4 X2 = REF[s2];
-        const float r4[4] = {Xs.x - X2.x, Xs.y - X2.y, Xs.z - X2.z, Xs.w - X2.w};
-        const float rr = r4[0] * r4[0] + r4[1] * r4[1] + r4[2] * r4[2] + r4[3] * r4[3];
-        const float diff = fmaxf(hh - rr, 0.0f);
-        const float gs = -6.0f * kn * diff * diff;
-        const float gw[3] = {gs * r4[0], gs * r4[1], gs * r4[2]};
-        const float* L2 = a.Linv + 16 * (size_t)s2;
-        float g2[3];
-        for (int r = 0; r < 3; r++) g2[r] = L2[r] * gw[0] + L2[4 + r] * gw[1] + L2[8 + r] * gw[2];
-        const float* fb2 = aa.Fbar + 18 * (size_t)s2;
-        for (int r = 0; r < 3; r++) {
-            xb[r] += Vp * (fb2[3 * r] * g2[0] + fb2[3 * r + 1] * g2[1] + fb2[3 * r + 2] * g2[2]);
-            vb[r] += Vp * (fb2[9 + 3 * r] * g2[0] + fb2[9 + 3 * r + 1] * g2[1] + fb2[9 + 3 * r + 2] * g2[2]);
+        const float2 v = reinterpret_cast<const float2*>(fb)[8];
+        f1[16] = v.x, f1[17] = v.y;
+    }
+    const uint32_t q0 = a.sOff[s], q1 = a.sOff[s + 1];
+    for (uint32_t qb = q0; qb < q1; qb += ARP_ADJ_GROUP) {
+        uint32_t sq[ARP_ADJ_GROUP];
+        float4 gq[ARP_ADJ_GROUP], Xq[ARP_ADJ_GROUP], Lq[ARP_ADJ_GROUP][3], Fq[ARP_ADJ_GROUP][5];
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+            const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
+            sq[k] = a.sPid[q];
+            gq[k] = G[q];
+        }
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) sq[k] = a.sidByPid[sq[k]];
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+            Xq[k] = REF[sq[k]];
+            const float4* L4 = reinterpret_cast<const float4*>(a.Linv) + 4 * (size_t)sq[k];
+#pragma unroll
+            for (int r = 0; r < 3; r++) Lq[k][r] = L4[r];
+            const float4* F4 = reinterpret_cast<const float4*>(aa.Fbar + kFbarStride * (size_t)sq[k]);
+#pragma unroll
+            for (int r = 0; r < 5; r++) Fq[k][r] = F4[r];
+        }
+        // (every entry of the group is evaluated and a tail entry's sums are discarded by
+        // selection, so that the group's loads stay ahead of the arithmetic)
+#pragma unroll
+        for (int k = 0; k < ARP_ADJ_GROUP; k++) {
+            const bool live = qb + (uint32_t)k < q1;
+            const float4 g = gq[k];
+            const float gg[3] = {g.x, g.y, g.z};
+            // own F: dx = x_q - x_s enters with weight V2 L_s^-1 gradW(dX)
+            for (int r = 0; r < 3; r++) {
+                const float t = g.w * (f1[3 * r] * gg[0] + f1[3 * r + 1] * gg[1] + f1[3 * r + 2] * gg[2]);
+                const float td = g.w * (f1[9 + 3 * r] * gg[0] + f1[9 + 3 * r + 1] * gg[1] + f1[9 + 3 * r + 2] * gg[2]);
+                xb[r] = live ? xb[r] - t : xb[r];
+                vb[r] = live ? vb[r] - td : vb[r];
+            }
+            // the neighbour's F holds this particle with weight Vp L_s2^-1 gradW(X_s - X_s2)
+            const float4 X2 = Xq[k];
+            const float r4[4] = {Xs.x - X2.x, Xs.y - X2.y, Xs.z - X2.z, Xs.w - X2.w};
+            const float rr = r4[0] * r4[0] + r4[1] * r4[1] + r4[2] * r4[2] + r4[3] * r4[3];
+            const float diff = fmaxf(hh - rr, 0.0f);
+            const float gs = -6.0f * kn * diff * diff;
+            const float gw[3] = {gs * r4[0], gs * r4[1], gs * r4[2]};
+            // columns 0..2 of L_s2^-1 (column-major), rows 0..2
+            const float L2[12] = {Lq[k][0].x, Lq[k][0].y, Lq[k][0].z, Lq[k][0].w, Lq[k][1].x, Lq[k][1].y,
+                                  Lq[k][1].z, Lq[k][1].w, Lq[k][2].x, Lq[k][2].y, Lq[k][2].z, Lq[k][2].w};
+            float g2[3];
+            for (int r = 0; r < 3; r++) g2[r] = L2[r] * gw[0] + L2[4 + r] * gw[1] + L2[8 + r] * gw[2];
+            const float fb2[20] = {Fq[k][0].x, Fq[k][0].y, Fq[k][0].z, Fq[k][0].w, Fq[k][1].x, Fq[k][1].y, Fq[k][1].z,
+                                   Fq[k][1].w, Fq[k][2].x, Fq[k][2].y, Fq[k][2].z, Fq[k][2].w, Fq[k][3].x, Fq[k][3].y,
+                                   Fq[k][3].z, Fq[k][3].w, Fq[k][4].x, Fq[k][4].y, Fq[k][4].z, Fq[k][4].w};
+            for (int r = 0; r < 3; r++) {
+                const float t = Vp * (fb2[3 * r] * g2[0] + fb2[3 * r + 1] * g2[1] + fb2[3 * r + 2] * g2[2]);
+                const float td = Vp * (fb2[9 + 3 * r] * g2[0] + fb2[9 + 3 * r + 1] * g2[1] + fb2[9 + 3 * r + 2] * g2[2]);
+                xb[r] = live ? xb[r] + t : xb[r];
+                vb[r] = live ? vb[r] + td : vb[r];
+            }
         }
     }
     float4* GP = reinterpret_cast<float4*>(aa.Gp) + i;

@@ -139,6 +139,9 @@ struct RecStep {
     DBuf nbList[ARP_MAX_STAGES], nbCount[ARP_MAX_STAGES];
     DBuf Xp[ARP_MAX_STAGES], Xv[ARP_MAX_STAGES];
     DBuf perm[ARP_MAX_STAGES], sflag[ARP_MAX_STAGES], rowOff[ARP_MAX_STAGES], rowStamp[ARP_MAX_STAGES];
+    // solid stages: the strain kernel's A = P L^-1, F and self-contact row boxes, written by
+    // the recorded stage itself (so that the reverse stage does not recompute them)
+    DBuf sA[ARP_MAX_STAGES], sF[ARP_MAX_STAGES], sBox[ARP_MAX_STAGES];
     GridSlot slot[ARP_MAX_STAGES];
 };
 
@@ -1269,6 +1272,11 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
             sa.Xeps = Xe;
             // solids of non-plastic materials keep y_n's plastic strain (zero rate)
             sa.epsY = verlet ? nullptr : P<float>(c->Yeps);
+            if (rec) {  // the stage's strain results stay in the record
+                sa.A = P<float>(rec->sA[s]);
+                sa.Fm = P<float>(rec->sF[s]);
+                sa.rowBox = P<float>(rec->sBox[s]);
+            }
             sa.mode = 0;
             sa.rk = r;
             if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
@@ -1336,6 +1344,12 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
         g.rowStamp = P<uint32_t>(R.rowStamp[s]);
         g.stamp = P<uint32_t>(R.stamps) + s;
         g.full = false;
+        if (c->solids_active) {
+            const size_t ns = std::max<uint32_t>(c->ns, 1);
+            if ((rc = dalloc(c, R.sA[s], 48 * ns)) || (rc = dalloc(c, R.sF[s], 48 * ns)) ||
+                (rc = dalloc(c, R.sBox[s], 32 * nr)))
+                return rc;
+        }
     }
     return ARP_OK;
 }
@@ -1354,7 +1368,9 @@ void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
         for (DBuf* b : {&R.mk, &R.pid, &R.src, &R.stamps}) free_buf(*b);
         for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
-            for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s]}) free_buf(*b);
+            for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s], &R.sA[s],
+                            &R.sF[s], &R.sBox[s]})
+                free_buf(*b);
     }
     for (size_t i = nck; i < c->ck.size(); i++)
         for (DBuf* b : {&c->ck[i].P, &c->ck[i].V, &c->ck[i].mk, &c->ck[i].pid}) free_buf(*b);
@@ -1529,7 +1545,7 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
                                   : (fl ? K_FLUID_ADJ_REC_F : fast ? K_FLUID_ADJ_REC : K_FLUID_ADJ_REC_G);
         if ((rc = launch(c, ka, nblk(c->n), 256, a))) return rc;
         if (c->solids_active) {
-            // the stage's stress and F again, then the two solid passes
+            // the two solid passes
             SolidArgs sa = solid_base(c);
             sa.perm = g.perm;
             sa.rowOff = g.rowOff;
@@ -1541,7 +1557,10 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             sa.mk = P<uint32_t>(R.mk);
             sa.pid = P<uint32_t>(R.pid);
             sa.Xeps = P<float>(c->Yeps);
-            if ((rc = launch(c, K_STRAIN, nblk(c->ns), 256, sa))) return rc;
+            // the stage's A, F and row boxes as its recorded forward pass left them
+            sa.A = P<float>(R.sA[s]);
+            sa.Fm = P<float>(R.sF[s]);
+            sa.rowBox = P<float>(R.sBox[s]);
             SolidAdjArgs sj{};
             sj.f = sa;
             sj.Wp = wP;
@@ -2319,7 +2338,8 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
     if ((rc = fill_u32(c, c->pgradPart, nblk(c->n) * 2 * kMatLds, 0))) return rc;  // accumulated per block
     if (c->solids_active) {
         const size_t ns = std::max<uint32_t>(c->ns, 1);
-        if ((rc = dalloc(c, c->Fbar, 4 * 18 * ns)) || (rc = dalloc(c, c->spar, 4 * 4 * ns))) return rc;
+        // (20 floats per solid: the kernels' padded rows of 18 F, F' cotangents)
+        if ((rc = dalloc(c, c->Fbar, 4 * 20 * ns)) || (rc = dalloc(c, c->spar, 4 * 4 * ns))) return rc;
         if ((rc = fill_u32(c, c->spar, 4 * c->ns, 0))) return rc;
     }
     if ((rc = dalloc(c, c->pgradSum, 4 * 2 * kMatLds))) return rc;
