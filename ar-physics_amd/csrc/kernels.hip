@@ -923,10 +923,12 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             // positions are in flight, so each batch waits for one round trip, not two
             uint32_t e[ARP_NLB], ok[ARP_NLB];
             fetch(0, e, ok);
-            if constexpr (!REC && DIRECT) {
+            if constexpr (DIRECT) {
                 // direct walk: each entry's position is tested in registers and, for the
                 // entries within h, the velocity is loaded (lanes outside h issue no request)
-                // and the pair evaluated at once, in list order (the buffered walk's order)
+                // and the pair evaluated at once, in list order (the buffered walk's order);
+                // a recording walk records those entries in the same order
+                char* nbw = REC ? reinterpret_cast<char*>(a.nbList) : nullptr;
                 for (; k < m; k += ARP_NLB) {
                     float4 xj[ARP_NLB];
 #pragma unroll
@@ -949,7 +951,15 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     }
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++)
-                        if (inh[u]) pair(in[u]);
+                        if (inh[u]) {
+                            if (REC) {
+                                // (32-bit byte offsets: the host keeps the record below 4 GB)
+                                if (nrec < a.nbCap)
+                                    *reinterpret_cast<uint32_t*>(nbw + (nrec * n32 + i) * 4u) = e[u];
+                                nrec++;
+                            }
+                            pair(in[u]);
+                        }
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) {
                         e[u] = en[u];
@@ -2340,7 +2350,8 @@ __device__ __forceinline__ void fluid_adj_body(const AdjArgs& a) {
                 const uint32_t j = e & ~kBnd;
                 in.xj = bnd ? B[j] : X[j];
                 in.vj = bnd ? BV[j] : XV[j];
-                in.mk2 = bnd ? a.bmk[j] : a.mk[j];
+                // (FLUID with FAST: every mobile particle is of material fastMat)
+                in.mk2 = bnd ? a.bmk[j] : ((FLUID && FAST) ? a.fastMat : a.mk[j]);
                 const float4 wvj = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : WV[j];
                 in.wvx = wvj.x;
                 in.wvy = wvj.y;

@@ -2253,11 +2253,12 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     for (uint32_t seg = 0; seg < nseg; seg++) {
         const uint32_t j0 = seg * K, j1 = std::min(nsteps, j0 + K);
         if ((rc = ck_save(c, c->ck[seg], nseg > 1))) return rc;
-        // kept neighbour lists restart with every segment of a recording that the reverse
-        // sweep replays (one of several segments, the last included: a second sweep replays
-        // it too), as its replay from the checkpoint does, so that every replay repeats the
-        // recorded steps exactly
-        if (nseg > 1) c->nl_dirty = true;
+        // kept neighbour lists restart with the recorded segment when the reverse sweep
+        // replays segments (a second sweep replays it too), as every replay from a checkpoint
+        // does, so that every replay repeats the recorded steps exactly.  The segments run
+        // only for their checkpoints keep their lists (a replay then differs from them only in
+        // the order of the per-particle sums)
+        if (nseg > 1 && seg + 1 == nseg) c->nl_dirty = true;
         // only the last segment is kept; the others run as plain steps (the same arithmetic)
         // for their checkpoints, and the reverse sweep replays them
         const RecRange rr{j0, nsteps, K, true};
