@@ -394,6 +394,10 @@ struct SolidAdjArgs {
     float* LamV;
     float* Fbar;        // per solid 18: cotangent rows of F (3x3) then of F' (3x3)
     float* spar;        // per solid 4 (+=): dL/d (lambda, mu, lambda_d, mu_d) as the solid sees them
+    // per solid of a material with strain_order != 2: the cotangents of E and E' (16 each,
+    // column-major) and F' (9, row-major), 48 floats, completed into Fbar by
+    // k_solid_adj_strain (null when the scene has no such material)
+    float* Ebuf;
 };
 
 struct InvArgs {

@@ -3073,8 +3073,9 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     const M4 eps = ld_m4(a.Xeps, s);
     const float ks = a.kscale ? a.kscale[s] : 1.0f;
     const float lam = ks * M.lambda, mu = ks * M.mu;
-    const M4 E = m4_add(m4_seth_hill(F4, 2, dim), eps);
-    const M4 Ed = m4_seth_hill_rate(F4, Fd4, 2, dim);
+    const int order = M.strain_order;
+    const M4 E = m4_add(m4_seth_hill(F4, order, dim), eps);
+    const M4 Ed = m4_seth_hill_rate(F4, Fd4, order, dim);
     const M4 S = m4_add(m4_hooke(E, lam, mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
     const M4 Li = ld_m4(a.Linv, s);
     const M4 Pb = m4_mul(Ab4, m4_trans(Li));
@@ -3088,9 +3089,23 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
             Eb.m[4 * c + r] = lam * dg + 2.0f * mu * Sb.m[4 * c + r];
             Edb.m[4 * c + r] = M.lambda_d * dg + 2.0f * M.mu_d * Sb.m[4 * c + r];
         }
-    const M4 EbS = m4_add(Eb, m4_trans(Eb)), EdbS = m4_add(Edb, m4_trans(Edb));
-    Fb = m4_add(Fb, m4_add(m4_scale(0.5f, m4_mul(F4, EbS)), m4_scale(0.5f, m4_mul(Fd4, EdbS))));
-    const M4 Fdb = m4_scale(0.5f, m4_mul(F4, EdbS));
+    M4 Fdb = m4_zero();
+    if (order == 2) {  // E = (F^T F - I)/2, E' = (F'^T F + F^T F')/2
+        const M4 EbS = m4_add(Eb, m4_trans(Eb)), EdbS = m4_add(Edb, m4_trans(Edb));
+        Fb = m4_add(Fb, m4_add(m4_scale(0.5f, m4_mul(F4, EbS)), m4_scale(0.5f, m4_mul(Fd4, EdbS))));
+        Fdb = m4_scale(0.5f, m4_mul(F4, EdbS));
+    } else if (aa.Ebuf) {
+        // other orders: E, E' cotangents and F' for k_solid_adj_strain, which adds the
+        // Seth-Hill terms to F-bar and forms F'-bar
+        float4* eb = reinterpret_cast<float4*>(aa.Ebuf) + 12 * (size_t)s;
+        for (int q = 0; q < 4; q++) {
+            eb[q] = make_float4(Eb.m[4 * q], Eb.m[4 * q + 1], Eb.m[4 * q + 2], Eb.m[4 * q + 3]);
+            eb[4 + q] = make_float4(Edb.m[4 * q], Edb.m[4 * q + 1], Edb.m[4 * q + 2], Edb.m[4 * q + 3]);
+        }
+        eb[8] = make_float4(Fd[0], Fd[1], Fd[2], Fd[3]);
+        eb[9] = make_float4(Fd[4], Fd[5], Fd[6], Fd[7]);
+        eb[10] = make_float4(Fd[8], 0.f, 0.f, 0.f);
+    }
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) {
             fb[3 * r + c] = Fb.m[4 * c + r];
@@ -3101,6 +3116,145 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     sp[1] += 2.0f * m4_dot(Sb, E);
     sp[2] += m4_trace(Ed, dim) * trS;
     sp[3] += 2.0f * m4_dot(Sb, Ed);
+}
+
+// ---- forward-mode tangents of the Seth-Hill strain and its rate (strain orders != 2) ----
+// A matrix and its tangent; each operation carries the derivative of the forward one
+// (d(AB) = dA B + A dB, d(A^-1) = -A^-1 dA A^-1), so that the functions below give the exact
+// directional derivative of m4_seth_hill / m4_seth_hill_rate as they compute.
+struct DM4 {
+    M4 v, d;
+};
+__device__ __forceinline__ DM4 dm4_mul(const DM4& A, const DM4& B) {
+    DM4 r;
+    r.v = m4_mul(A.v, B.v);
+    r.d = m4_add(m4_mul(A.d, B.v), m4_mul(A.v, B.d));
+    return r;
+}
+__device__ __forceinline__ DM4 dm4_add(const DM4& A, const DM4& B) {
+    return DM4{m4_add(A.v, B.v), m4_add(A.d, B.d)};
+}
+__device__ __forceinline__ DM4 dm4_scale(float c, const DM4& A) { return DM4{m4_scale(c, A.v), m4_scale(c, A.d)}; }
+__device__ __forceinline__ DM4 dm4_trans(const DM4& A) { return DM4{m4_trans(A.v), m4_trans(A.d)}; }
+__device__ DM4 dm4_inv(const DM4& A) {
+    DM4 r;
+    r.v = m4_inv(A.v);
+    r.d = m4_scale(-1.0f, m4_mul(m4_mul(r.v, A.d), r.v));
+    return r;
+}
+__device__ DM4 dm4_cauchy(const DM4& F, uint32_t dim) {
+    DM4 C = dm4_mul(dm4_trans(F), F);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+        if (i >= dim) {
+            C.v.m[5 * i] = 1.0f;
+            C.d.m[5 * i] = 0.0f;
+        }
+    return C;
+}
+// the tangent of m4_seth_hill (solid.rs:105-168)
+__device__ M4 dm4_seth_hill(const DM4& F, int two_m, uint32_t dim) {
+    const DM4 C = dm4_cauchy(F, dim);
+    const DM4 I{m4_ident(), m4_zero()};
+    if (two_m == 0) {
+        DM4 sum{m4_zero(), m4_zero()}, term = dm4_scale(-1.0f, I), AmI;
+        for (int q = 0; q < 16; q++) {
+            AmI.v.m[q] = -1.0f * (C.v.m[q] - I.v.m[q]);
+            AmI.d.m[q] = -C.d.m[q];
+        }
+        for (int i = 1; i <= 5; i++) {
+            term = dm4_mul(term, AmI);
+            for (int q = 0; q < 16; q++) {
+                sum.v.m[q] += term.v.m[q] / (float)i;
+                sum.d.m[q] += term.d.m[q] / (float)i;
+            }
+        }
+        return m4_scale(0.5f, sum.d);
+    }
+    const uint32_t m = (uint32_t)abs(two_m) >> 1;
+    DM4 Cm = I;
+    for (uint32_t i = 0; i < m; i++) Cm = dm4_mul(Cm, C);
+    if ((abs(two_m) & 1) == 1) {
+        DM4 Y = C, Z = I;
+        for (int i = 0; i < 15; i++) {
+            const DM4 iZ = dm4_inv(Z), iY = dm4_inv(Y);
+            const DM4 Yn = dm4_scale(0.5f, dm4_add(Y, iZ)), Zn = dm4_scale(0.5f, dm4_add(Z, iY));
+            Y = Yn;
+            Z = Zn;
+        }
+        Cm = dm4_mul(Cm, Y);
+    }
+    if (two_m < 0) Cm = dm4_inv(Cm);
+    return m4_scale(1.0f / (float)two_m, Cm.d);
+}
+// the tangent of m4_seth_hill_rate
+__device__ M4 dm4_seth_hill_rate(const DM4& F, const DM4& Fd, int two_m, uint32_t dim) {
+    DM4 dC = dm4_mul(dm4_trans(Fd), F);
+    dC = dm4_add(dC, dm4_trans(dC));
+    const DM4 C = dm4_cauchy(F, dim);
+    if (two_m == 0) return dm4_mul(dm4_scale(0.5f, dC), dm4_inv(C)).d;
+    const uint32_t m = (uint32_t)abs(two_m) >> 1;
+    DM4 sr{m4_zero(), m4_zero()};
+    for (uint32_t i = 0; i < m; i++) {
+        DM4 term{m4_ident(), m4_zero()};
+        for (uint32_t j = 0; j < m; j++) {
+            if (j == i)
+                term = dm4_mul(term, dC);
+            else
+                term = dm4_mul(term, C);
+        }
+        sr = dm4_add(sr, term);
+    }
+    sr = dm4_scale(1.0f / (float)two_m, sr);
+    if (two_m < 0) {
+        const DM4 iC = dm4_inv(C);
+        return dm4_mul(dm4_mul(dm4_scale(-1.0f, iC), sr), iC).d;
+    }
+    return sr.d;
+}
+
+// Completes F-bar and forms F'-bar for solids of strain_order != 2 from the E, E' cotangents
+// k_solid_adj1 left: F-bar += (dE/dF)^T E-bar + (dE'/dF)^T E'-bar, F'-bar = (dE'/dF')^T E'-bar,
+// one forward-mode tangent per entry of F (and of F').
+extern "C" __global__ void __launch_bounds__(256) k_solid_adj_strain(SolidAdjArgs aa) {
+    const SolidArgs& a = aa.f;
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= a.ns || !aa.Ebuf) return;
+    const uint32_t p = a.part[s];
+    if (p == kInvalid) return;
+    const uint32_t i = a.inv[p];
+    if (!(a.mk[i] & kMkElastic)) return;
+    const DMat& M = a.mats[a.mk[i] & kMatMask];
+    const int order = M.strain_order;
+    if (order == 2) return;
+    const uint32_t dim = a.grid.dim_n;
+    const float4* eb = reinterpret_cast<const float4*>(aa.Ebuf) + 12 * (size_t)s;
+    M4 Eb, Edb;
+    for (int q = 0; q < 4; q++) {
+        const float4 x = eb[q], y = eb[4 + q];
+        Eb.m[4 * q] = x.x, Eb.m[4 * q + 1] = x.y, Eb.m[4 * q + 2] = x.z, Eb.m[4 * q + 3] = x.w;
+        Edb.m[4 * q] = y.x, Edb.m[4 * q + 1] = y.y, Edb.m[4 * q + 2] = y.z, Edb.m[4 * q + 3] = y.w;
+    }
+    const float fd9[9] = {eb[8].x, eb[8].y, eb[8].z, eb[8].w, eb[9].x, eb[9].y, eb[9].z, eb[9].w, eb[10].x};
+    const float4* F1 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s;
+    M4 F4 = m4_zero(), Fd4 = m4_zero();
+    for (int r = 0; r < 3; r++) {
+        const float4 row = F1[r];
+        F4.m[r] = row.x, F4.m[4 + r] = row.y, F4.m[8 + r] = row.z;
+        for (int c = 0; c < 3; c++) Fd4.m[4 * c + r] = fd9[3 * r + c];
+    }
+    float* fb = aa.Fbar + kFbarStride * (size_t)s;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            DM4 F{F4, m4_zero()}, Fd{Fd4, m4_zero()};
+            F.d.m[4 * c + r] = 1.0f;
+            const M4 dE = dm4_seth_hill(F, order, dim);
+            const M4 dEd = dm4_seth_hill_rate(F, Fd, order, dim);
+            fb[3 * r + c] += m4_dot(Eb, dE) + m4_dot(Edb, dEd);
+            DM4 G{F4, m4_zero()}, Gd{Fd4, m4_zero()};
+            Gd.d.m[4 * c + r] = 1.0f;
+            fb[9 + 3 * r + c] = m4_dot(Edb, dm4_seth_hill_rate(G, Gd, order, dim));
+        }
 }
 
 extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) {

@@ -83,6 +83,7 @@ enum Kern {
     K_SOLID_B,
     K_FLUID_NL_F_B_BUF,
     K_FLUID_NL_B_BUF,
+    K_SOLID_ADJ_STRAIN,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -97,7 +98,7 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
                                    "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f",
                                    "k_spec_restore", "k_fluid_nl_f_batch", "k_strain_batch", "k_solid_batch",
-                                   "k_fluid_nl_f_batch_buf", "k_fluid_nl_batch_buf"};
+                                   "k_fluid_nl_f_batch_buf", "k_fluid_nl_batch_buf", "k_solid_adj_strain"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -257,6 +258,7 @@ struct arp_ctx {
     std::vector<RecStep> rec;
     DBuf LamIn[2], LamAcc[2], LamNext[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
     DBuf Fbar, spar;  // solid reverse mode: per-solid F cotangents, per-solid stiffness partials
+    DBuf Ebuf;        // solid reverse mode, strain orders != 2: E, E' cotangents and F' per solid
     DBuf kscale;      // per-solid stiffness scale (empty: 1)
     DBuf recFlags;    // the recorded steps' stage flags, 2 * ARP_MAX_STAGES words per step
     bool keys_valid = false;
@@ -1427,6 +1429,14 @@ int fetch_over(arp_ctx* c, uint32_t nrec) {
 // Reverse sweep over one recorded step: lam (P,V) holds dL/dy_{n+1} in the step's sorted
 // order on entry and dL/dy_n in the order of the state before the step on exit.
 //   w_s = dt b_s lam + sum_{t>s} dt a_ts G_t,   G_s = J_f(X_s)^T w_s,   dL/dy_n = lam + sum_s G_s
+// an elastic material whose Seth-Hill order is not 2 (its strain adjoint needs
+// k_solid_adj_strain)
+static bool any_other_order(const arp_ctx* c) {
+    for (uint32_t m = 0; m < c->nm; m++)
+        if (is_stiff(c->mats_h[m]) && c->mats_h[m].strain_order != 2) return true;
+    return false;
+}
+
 int reverse_step(arp_ctx* c, RecStep& R, float dt) {
     const ArpTableau& T = c->tab;
     const int S = T.stages;
@@ -1572,7 +1582,10 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             sj.LamV = P<float>(c->LamAcc[1]);
             sj.Fbar = P<float>(c->Fbar);
             sj.spar = P<float>(c->spar);
+            const bool other = any_other_order(c);
+            sj.Ebuf = other ? P<float>(c->Ebuf) : nullptr;
             if ((rc = launch(c, K_SOLID_ADJ1, nblk(c->ns), 256, sj))) return rc;
+            if (other && (rc = launch(c, K_SOLID_ADJ_STRAIN, nblk(c->ns), 256, sj))) return rc;
             if ((rc = launch(c, K_SOLID_ADJ2, nblk(c->ns), 256, sj))) return rc;
         }
     }
@@ -1665,7 +1678,7 @@ void arp_destroy(arp_ctx* c) {
         if (b->p) hipFree(b->p);
     free_rec(c);
     for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->LamNext[0], &c->LamNext[1],
-                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->kscale, &c->gaos, &c->recFlags, &c->nlSnap})
+                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->Ebuf, &c->kscale, &c->gaos, &c->recFlags, &c->nlSnap})
         free_buf(*b);
     if (c->hAbort) hipHostFree(c->hAbort);
     for (hipEvent_t e : c->specEv)
@@ -2228,9 +2241,8 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     if (c->solids_active)
         for (uint32_t m = 0; m < c->nm; m++) {
             const ArpMaterial& M = c->mats_h[m];
-            if (is_stiff(M) && (M.strain_order != 2 || M.plastic))
-                return fail(c, ARP_E_STATE,
-                            "the solid adjoint covers strain_order 2 without plastic flow (material %u)", m);
+            if (is_stiff(M) && M.plastic)
+                return fail(c, ARP_E_STATE, "the solid adjoint does not cover plastic flow (material %u)", m);
         }
     if (nsteps == 0 || checkpoint_every == 0 || !(dt > 0.0f) || !std::isfinite(dt))
         return fail(c, ARP_E_ARG, "adjoint_record needs nsteps >= 1, checkpoint_every >= 1 and dt > 0");
@@ -2341,6 +2353,7 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
         const size_t ns = std::max<uint32_t>(c->ns, 1);
         // (20 floats per solid: the kernels' padded rows of 18 F, F' cotangents)
         if ((rc = dalloc(c, c->Fbar, 4 * 20 * ns)) || (rc = dalloc(c, c->spar, 4 * 4 * ns))) return rc;
+        if (any_other_order(c) && (rc = dalloc(c, c->Ebuf, 4 * 48 * ns))) return rc;
         if ((rc = fill_u32(c, c->spar, 4 * c->ns, 0))) return rc;
     }
     if ((rc = dalloc(c, c->pgradSum, 4 * 2 * kMatLds))) return rc;

@@ -69,6 +69,22 @@ def _masked(rng, n, field, scale, parity=None):
     return a
 
 
+def _cantilever_order(order):
+    """The small cantilever with another Seth-Hill order (solid.rs:105-168): 0 the log
+    series, odd orders the Denman-Beavers root, negative orders the inverse.  Stiffness and
+    damping 1e5 times the cantilever's (stepped with dt 1e-7), so that the stress carries
+    about 60 % of <g, J u> and the orders differ in it by up to 22 % (measured on the
+    oracle): a wrong strain derivative cannot hide behind the hourglass term."""
+    import dataclasses
+
+    sc = _cantilever()
+    mats = sc.materials.copy()
+    mats["strain_order"][1] = order
+    for f in ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp"):
+        mats[f][1] = np.float32(mats[f][1] * 1e5)
+    return dataclasses.replace(sc, materials=mats)
+
+
 CASES = {
     "fluid_rk2_replay": (lambda: _scene(True), "rk2", 5e-5, 3, 2, {1: ("bulk_visc", "visc")}),
     "fluid_rk4": (lambda: _scene(False), "rk4", 5e-5, 2, 2, {1: ("bulk_visc",)}),
@@ -76,6 +92,11 @@ CASES = {
                        {1: ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp")}),
     "contact_rk2": (_contact, "rk2", 1e-5, 2, 2, {1: ("normal_stiffness", "shear_stiffness"), 2: ("bulk_visc",)}),
     "self_contact_rk2": (self_contact_scene, "rk2", 1e-5, 2, 1, {2: ("normal_stiffness", "shear_damp")}),
+    "cantilever_order2": (lambda: _cantilever_order(2), "rk2", 1e-7, 2, 1, {1: ("normal_stiffness",)}),
+    "cantilever_order0": (lambda: _cantilever_order(0), "rk2", 1e-7, 2, 1, {1: ("normal_stiffness", "shear_damp")}),
+    "cantilever_order1": (lambda: _cantilever_order(1), "rk2", 1e-7, 2, 1, {1: ("shear_stiffness",)}),
+    "cantilever_order3": (lambda: _cantilever_order(3), "rk2", 1e-7, 2, 1, {1: ("normal_stiffness", "normal_damp")}),
+    "cantilever_order-2": (lambda: _cantilever_order(-2), "rk2", 1e-7, 2, 1, {1: ("shear_stiffness", "shear_damp")}),
 }
 PAIRS = [("vel", "pos"), ("den", "vel"), ("pos", "vel"), ("vel", "den"), ("vel", "vel"), ("pos", "pos")]
 
