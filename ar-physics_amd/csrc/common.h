@@ -21,6 +21,7 @@ constexpr uint32_t kBucketCap = 128u;      // slots per reference bucket (neighb
 constexpr int kMaxSX = 2048;               // cells along x supported by the row sort
 constexpr int kRowCap = 2048;              // row elements sorted in LDS (larger rows: HBM)
 constexpr int kMaxK = 4;                   // stored stage derivatives for non-subdiagonal RK
+constexpr int kMaxStages = 6;              // stages of a tableau (ARP_MAX_STAGES)
 
 // mk word of a particle: material id | flags
 constexpr uint32_t kMatMask = 0xFFFFu;
@@ -379,10 +380,11 @@ struct SolidArgs {
     RkArgs rk;
 };
 
-// Adjoint of the solid part of one stage (strain_order 2, no plastic flow): pass 1 per solid
-// gathers the cotangents of its stress (elastic force), its F (hourglass) and the direct
-// position / density terms, then runs the strain adjoint; pass 2 turns the F, F' cotangents
-// of a solid and its reference neighbours into position / velocity cotangents.
+// Adjoint of the solid part of one stage: pass 1 per solid gathers the cotangents of its
+// stress (elastic force, plastic rate), its F (hourglass) and the direct position / density
+// terms, then runs the strain adjoint (strain orders other than 2 complete it in a pass of
+// their own); pass 2 turns the F, F' cotangents of a solid and its reference neighbours into
+// position / velocity cotangents.
 struct SolidAdjArgs {
     SolidArgs f;        // the stage's forward arguments (A, Fm recomputed by k_strain)
     const float* Wp;    // stage cotangent of k, times wscale
@@ -398,6 +400,15 @@ struct SolidAdjArgs {
     // column-major) and F' (9, row-major), 48 floats, completed into Fbar by
     // k_solid_adj_strain (null when the scene has no such material)
     float* Ebuf;
+    // plastic flow (null lamE: no plastic material).  Per solid 16 floats each (column-major):
+    // lamE = dL/d (the step's plastic strain), gE[t] = dL/d (stage t's input plastic strain).
+    // The stage's plastic-rate cotangent is wcl lamE + sum over t > stage of wcg[t] gE[t];
+    // the last reverse stage (stage 0) adds every gE[t] to lamE (dL/d y_n).
+    float* lamE;
+    float* gE[kMaxStages];
+    float wcl;
+    float wcg[kMaxStages];
+    int stage, nstages;
 };
 
 struct InvArgs {

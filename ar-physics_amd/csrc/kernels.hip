@@ -2781,8 +2781,8 @@ extern "C" __global__ void __launch_bounds__(256) k_reduce(ReduceArgs a) {
 }
 
 // ------------------------------------------------------------------------------------
-// adjoint of the solid part (reverse mode of strain.rs:139-160, solid.rs:334-450 for
-// strain_order 2 without plastic flow)
+// adjoint of the solid part (reverse mode of strain.rs:139-160, solid.rs:105-168, 334-450 and
+// the plastic flow of solid.rs:499-518)
 // ------------------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(256) k_inv(InvArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -3079,8 +3079,29 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     const M4 S = m4_add(m4_hooke(E, lam, mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
     const M4 Li = ld_m4(a.Linv, s);
     const M4 Pb = m4_mul(Ab4, m4_trans(Li));
-    const M4 Sb = m4_mul(m4_trans(Pb), F4);
+    M4 Sb = m4_mul(m4_trans(Pb), F4);
     M4 Fb = m4_add(m4_mul(Pb, S), Fh4);
+    // plastic flow (solid.rs:499-518): rate = (|S| - A)/T S/|S| when |S|^2 >= A^2, with A the
+    // yield stress, constant since eq_plastic_strain_sq carries the factor (dim-1)/dim == 0;
+    // its cotangent is this stage's share of the plastic strain's cotangents
+    const bool plast = M.plastic && aa.lamE;
+    if (plast) {
+        M4 rbar = m4_scale(aa.wcl, ld_m4(aa.lamE, s));
+        for (int t = aa.stage + 1; t < aa.nstages; t++) rbar = m4_add(rbar, m4_scale(aa.wcg[t], ld_m4(aa.gE[t], s)));
+        M4 dev = eps;
+        const float tr = m4_trace(eps, dim);
+        for (int c = 0; c < 4; c++) dev.m[5 * c] = eps.m[5 * c] - tr / (float)dim;
+        const float eq = (float)((dim - 1) / dim) * m4_dot(dev, dev);
+        const float ys = (M.yield + M.hard * powf(eq, M.hard_exp / 2.0f)) * (1.0f + M.kin_hard * logf(1.0f));
+        const float sn2 = m4_dot(S, S);
+        if (ys * ys <= sn2) {
+            // d rate = c1 dS + c2 (S . dS) S
+            const float n = sqrtf(sn2);
+            const float c1 = (n - ys) / (M.relax * n), c2 = ys / (M.relax * n * n * n);
+            const float sr = m4_dot(S, rbar);
+            for (int q = 0; q < 16; q++) Sb.m[q] += c1 * rbar.m[q] + c2 * sr * S.m[q];
+        }
+    }
     const float trS = Sb.m[0] + Sb.m[5] + Sb.m[10] + Sb.m[15];
     M4 Eb, Edb;
     for (int c = 0; c < 4; c++)
@@ -3116,6 +3137,18 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     sp[1] += 2.0f * m4_dot(Sb, E);
     sp[2] += m4_trace(Ed, dim) * trS;
     sp[3] += 2.0f * m4_dot(Sb, Ed);
+    if (plast) {
+        // the stage input's plastic strain enters E (E = seth_hill(F) + eps): its cotangent
+        // is E-bar; the last reverse stage folds every stage's into dL/d y_n
+        float4* g = reinterpret_cast<float4*>(aa.gE[aa.stage]) + 4 * (size_t)s;
+        for (int q = 0; q < 4; q++) g[q] = make_float4(Eb.m[4 * q], Eb.m[4 * q + 1], Eb.m[4 * q + 2], Eb.m[4 * q + 3]);
+        if (aa.stage == 0) {
+            M4 lam = m4_add(ld_m4(aa.lamE, s), Eb);
+            for (int t = 1; t < aa.nstages; t++) lam = m4_add(lam, ld_m4(aa.gE[t], s));
+            float4* l = reinterpret_cast<float4*>(aa.lamE) + 4 * (size_t)s;
+            for (int q = 0; q < 4; q++) l[q] = make_float4(lam.m[4 * q], lam.m[4 * q + 1], lam.m[4 * q + 2], lam.m[4 * q + 3]);
+        }
+    }
 }
 
 // ---- forward-mode tangents of the Seth-Hill strain and its rate (strain orders != 2) ----

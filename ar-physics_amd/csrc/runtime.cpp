@@ -132,6 +132,7 @@ struct GridSlot {
 
 // One recorded step of a checkpoint segment: everything the stage adjoints read.
 static_assert(kRecFlags == 2 * ARP_MAX_STAGES, "recorded step flags: two per stage");
+static_assert(kMaxStages == ARP_MAX_STAGES, "stage limit of the kernels' argument blocks");
 
 struct RecStep {
     DBuf mk, pid, src, stamps;
@@ -143,11 +144,13 @@ struct RecStep {
     // solid stages: the strain kernel's A = P L^-1, F and self-contact row boxes, written by
     // the recorded stage itself (so that the reverse stage does not recompute them)
     DBuf sA[ARP_MAX_STAGES], sF[ARP_MAX_STAGES], sBox[ARP_MAX_STAGES];
+    DBuf sEps[ARP_MAX_STAGES];  // each stage's input plastic strain (scenes with plastic flow)
     GridSlot slot[ARP_MAX_STAGES];
 };
 
 struct Checkpoint {
     DBuf P, V, mk, pid;
+    DBuf E;  // the plastic strain (scenes with a plastic material)
 };
 
 // A launch recorded instead of issued (the batch path collects one per scene and issues
@@ -259,6 +262,7 @@ struct arp_ctx {
     DBuf LamIn[2], LamAcc[2], LamNext[2], Wb[2], Gb[ARP_MAX_STAGES][2], pgradPart, pgradSum, gaos;
     DBuf Fbar, spar;  // solid reverse mode: per-solid F cotangents, per-solid stiffness partials
     DBuf Ebuf;        // solid reverse mode, strain orders != 2: E, E' cotangents and F' per solid
+    DBuf LamE, GbE[ARP_MAX_STAGES];  // plastic flow: the plastic strain's step and stage cotangents
     DBuf kscale;      // per-solid stiffness scale (empty: 1)
     DBuf recFlags;    // the recorded steps' stage flags, 2 * ARP_MAX_STAGES words per step
     bool keys_valid = false;
@@ -433,6 +437,22 @@ float host_pressure(const ArpMaterial& m, float den) {
 }
 
 bool is_stiff(const ArpMaterial& m) { return m.normal_stiffness != 0.0f || m.shear_stiffness != 0.0f; }
+
+// an elastic material whose Seth-Hill order is not 2 (its strain adjoint needs
+// k_solid_adj_strain)
+static bool any_other_order(const arp_ctx* c) {
+    for (uint32_t m = 0; m < c->nm; m++)
+        if (is_stiff(c->mats_h[m]) && c->mats_h[m].strain_order != 2) return true;
+    return false;
+}
+
+// an elastic material with plastic flow (its plastic strain is state the reverse sweep
+// carries)
+static bool any_plastic(const arp_ctx* c) {
+    for (uint32_t m = 0; m < c->nm; m++)
+        if (is_stiff(c->mats_h[m]) && c->mats_h[m].plastic) return true;
+    return false;
+}
 
 // stable counting sort of items by cell with a per-cell cap; returns the prefix over cells
 // of kept items and the kept items in (cell, id) order
@@ -1132,6 +1152,10 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
     const float* Xp = verlet ? P<float>(c->XF[1][0]) : Y0p;  // the first stage's input
     const float* Xv = verlet ? P<float>(c->XF[1][1]) : Y0v;
     const float* Xe = verlet ? P<float>(c->XFe) : P<float>(c->Yeps);
+    if (rec && rec->sEps[0].p) {  // the step's plastic strain, the first stage's input
+        HIPCHK(c, hipMemcpyAsync(rec->sEps[0].p, c->Yeps.p, 64 * (size_t)c->ns, hipMemcpyDeviceToDevice, c->stream));
+        Xe = P<float>(rec->sEps[0]);
+    }
     for (int s = 0; s < S; s++) {
         if (s > 0 && rec) c->slot = rec->slot[s];
         if (s > 0 && !sp) {
@@ -1172,7 +1196,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
                 }
             r.Oposd = rec ? P<float>(rec->Xp[s + 1]) : P<float>(c->Xb[s % 2][0]);
             r.Ovelq = rec ? P<float>(rec->Xv[s + 1]) : P<float>(c->Xb[s % 2][1]);
-            r.Oeps = P<float>(c->Xeps[s % 2]);
+            r.Oeps = (rec && rec->sEps[s + 1].p) ? P<float>(rec->sEps[s + 1]) : P<float>(c->Xeps[s % 2]);
         } else {
             r.Oposd = P<float>(c->Yland[0]);
             r.Ovelq = P<float>(c->Yland[1]);
@@ -1293,7 +1317,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
         if (!last) {
             Xp = r.Oposd;
             Xv = r.Ovelq;
-            Xe = P<float>(c->Xeps[s % 2]);
+            Xe = r.Oeps;
         }
     }
     if (rec) c->slot = own;
@@ -1351,6 +1375,7 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
             if ((rc = dalloc(c, R.sA[s], 48 * ns)) || (rc = dalloc(c, R.sF[s], 48 * ns)) ||
                 (rc = dalloc(c, R.sBox[s], 32 * nr)))
                 return rc;
+            if (any_plastic(c) && (rc = dalloc(c, R.sEps[s], 64 * ns))) return rc;
         }
     }
     return ARP_OK;
@@ -1371,11 +1396,11 @@ void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
         for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s], &R.sA[s],
-                            &R.sF[s], &R.sBox[s]})
+                            &R.sF[s], &R.sBox[s], &R.sEps[s]})
                 free_buf(*b);
     }
     for (size_t i = nck; i < c->ck.size(); i++)
-        for (DBuf* b : {&c->ck[i].P, &c->ck[i].V, &c->ck[i].mk, &c->ck[i].pid}) free_buf(*b);
+        for (DBuf* b : {&c->ck[i].P, &c->ck[i].V, &c->ck[i].mk, &c->ck[i].pid, &c->ck[i].E}) free_buf(*b);
     if (c->rec.size() > nrec) c->rec.resize(nrec);
     if (c->ck.size() > nck) c->ck.resize(nck);
     c->rec_valid = false;
@@ -1393,6 +1418,7 @@ int ck_save(arp_ctx* c, Checkpoint& k, bool full = true) {
     if (full && ((rc = d2d(c, k.P, c->Yland[0], 16 * n)) || (rc = d2d(c, k.V, c->Yland[1], 16 * n)) ||
                  (rc = d2d(c, k.mk, c->meta_mk[c->cur], 4 * n))))
         return rc;
+    if (full && c->solids_active && any_plastic(c) && (rc = d2d(c, k.E, c->Yeps, 64 * (size_t)c->ns))) return rc;
     return ARP_OK;
 }
 
@@ -1409,6 +1435,8 @@ int ck_restore(arp_ctx* c, Checkpoint& k) {
     HIPCHK(c, hipMemcpyAsync(c->Yland[1].p, k.V.p, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->meta_mk[c->cur].p, k.mk.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->meta_pid[c->cur].p, k.pid.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+    if (k.E.p && c->solids_active && any_plastic(c))
+        HIPCHK(c, hipMemcpyAsync(c->Yeps.p, k.E.p, 64 * (size_t)c->ns, hipMemcpyDeviceToDevice, c->stream));
     int rc = fill_u32(c, c->binCount, c->grid.nrows + 1, 0);
     if (rc) return rc;
     c->keys_valid = false;
@@ -1429,14 +1457,6 @@ int fetch_over(arp_ctx* c, uint32_t nrec) {
 // Reverse sweep over one recorded step: lam (P,V) holds dL/dy_{n+1} in the step's sorted
 // order on entry and dL/dy_n in the order of the state before the step on exit.
 //   w_s = dt b_s lam + sum_{t>s} dt a_ts G_t,   G_s = J_f(X_s)^T w_s,   dL/dy_n = lam + sum_s G_s
-// an elastic material whose Seth-Hill order is not 2 (its strain adjoint needs
-// k_solid_adj_strain)
-static bool any_other_order(const arp_ctx* c) {
-    for (uint32_t m = 0; m < c->nm; m++)
-        if (is_stiff(c->mats_h[m]) && c->mats_h[m].strain_order != 2) return true;
-    return false;
-}
-
 int reverse_step(arp_ctx* c, RecStep& R, float dt) {
     const ArpTableau& T = c->tab;
     const int S = T.stages;
@@ -1566,7 +1586,8 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             sa.Xvelq = P<float>(R.Xv[s]);
             sa.mk = P<uint32_t>(R.mk);
             sa.pid = P<uint32_t>(R.pid);
-            sa.Xeps = P<float>(c->Yeps);
+            // the stage's input plastic strain (constant, y_n's, without plastic flow)
+            sa.Xeps = R.sEps[s].p ? P<float>(R.sEps[s]) : P<float>(c->Yeps);
             // the stage's A, F and row boxes as its recorded forward pass left them
             sa.A = P<float>(R.sA[s]);
             sa.Fm = P<float>(R.sF[s]);
@@ -1584,6 +1605,16 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             sj.spar = P<float>(c->spar);
             const bool other = any_other_order(c);
             sj.Ebuf = other ? P<float>(c->Ebuf) : nullptr;
+            if (c->LamE.p && any_plastic(c)) {
+                sj.lamE = P<float>(c->LamE);
+                for (int t = 0; t < S; t++) {
+                    sj.gE[t] = P<float>(c->GbE[t]);
+                    sj.wcg[t] = dt * T.a[t][s];
+                }
+                sj.wcl = dt * T.b[s];
+                sj.stage = s;
+                sj.nstages = S;
+            }
             if ((rc = launch(c, K_SOLID_ADJ1, nblk(c->ns), 256, sj))) return rc;
             if (other && (rc = launch(c, K_SOLID_ADJ_STRAIN, nblk(c->ns), 256, sj))) return rc;
             if ((rc = launch(c, K_SOLID_ADJ2, nblk(c->ns), 256, sj))) return rc;
@@ -1678,12 +1709,12 @@ void arp_destroy(arp_ctx* c) {
         if (b->p) hipFree(b->p);
     free_rec(c);
     for (DBuf* b : {&c->LamIn[0], &c->LamIn[1], &c->LamAcc[0], &c->LamAcc[1], &c->LamNext[0], &c->LamNext[1],
-                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->Ebuf, &c->kscale, &c->gaos, &c->recFlags, &c->nlSnap})
+                    &c->Wb[0], &c->Wb[1], &c->pgradPart, &c->pgradSum, &c->Fbar, &c->spar, &c->Ebuf, &c->LamE, &c->kscale, &c->gaos, &c->recFlags, &c->nlSnap})
         free_buf(*b);
     if (c->hAbort) hipHostFree(c->hAbort);
     for (hipEvent_t e : c->specEv)
         if (e) hipEventDestroy(e);
-    for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(c->Gb[s][0]), free_buf(c->Gb[s][1]);
+    for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(c->Gb[s][0]), free_buf(c->Gb[s][1]), free_buf(c->GbE[s]);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->mod) hipModuleUnload(c->mod);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -2238,12 +2269,6 @@ int arp_adjoint_record(arp_ctx* c, float dt, uint32_t nsteps, uint32_t checkpoin
     if ((rc = need_scene(c))) return rc;
     if (c->tab.stages == ARP_VERLET)
         return fail(c, ARP_E_STATE, "reverse mode covers the Runge-Kutta tableaus (not velocity Verlet)");
-    if (c->solids_active)
-        for (uint32_t m = 0; m < c->nm; m++) {
-            const ArpMaterial& M = c->mats_h[m];
-            if (is_stiff(M) && M.plastic)
-                return fail(c, ARP_E_STATE, "the solid adjoint does not cover plastic flow (material %u)", m);
-        }
     if (nsteps == 0 || checkpoint_every == 0 || !(dt > 0.0f) || !std::isfinite(dt))
         return fail(c, ARP_E_ARG, "adjoint_record needs nsteps >= 1, checkpoint_every >= 1 and dt > 0");
     if (!nl_fits_u32(rec_cap(c), c->n))
@@ -2355,6 +2380,12 @@ int arp_adjoint_backward(arp_ctx* c, const void* d_grad_out, void* d_grad_in, fl
         if ((rc = dalloc(c, c->Fbar, 4 * 20 * ns)) || (rc = dalloc(c, c->spar, 4 * 4 * ns))) return rc;
         if (any_other_order(c) && (rc = dalloc(c, c->Ebuf, 4 * 48 * ns))) return rc;
         if ((rc = fill_u32(c, c->spar, 4 * c->ns, 0))) return rc;
+        if (any_plastic(c)) {  // the plastic strain's cotangents: dL/d y_N = 0 (the loss is on particles)
+            if ((rc = dalloc(c, c->LamE, 64 * ns))) return rc;
+            for (int s = 0; s < S; s++)
+                if ((rc = dalloc(c, c->GbE[s], 64 * ns))) return rc;
+            if ((rc = fill_u32(c, c->LamE, 16 * c->ns, 0))) return rc;
+        }
     }
     if ((rc = dalloc(c, c->pgradSum, 4 * 2 * kMatLds))) return rc;
     if ((rc = fill_u32(c, c->pgradSum, 2 * kMatLds, 0))) return rc;

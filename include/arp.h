@@ -268,7 +268,9 @@ int arp_adjoint_record(arp_ctx* ctx, float dt, uint32_t nsteps, uint32_t checkpo
  * output; d_grad_in receives dL/d of the state before them (mat/solid_id fields zero).
  * grad_params (host, may be NULL): 6*nmat floats per material m: [6m] bulk_viscosity,
  * [6m+1] viscosity, [6m+2] normal_stiffness, [6m+3] shear_stiffness, [6m+4] normal_damp,
- * [6m+5] shear_damp (at most 16 materials).  Solids: strain_order 2 without plastic flow.
+ * [6m+5] shear_damp (at most 16 materials).  Solids: every strain order, plastic flow
+ * included (the plastic strain is carried as state; dL/d of the final plastic strain is
+ * taken as zero and that of the initial one is not returned).
  * The live state is left at the end of the recorded trajectory. */
 int arp_adjoint_backward(arp_ctx* ctx, const void* d_grad_out, void* d_grad_in, float* grad_params);
 /* Per-solid stiffness scale k_s multiplying the solid's (normal, shear) stiffness in the

@@ -12,7 +12,7 @@ import pytest
 from arp import tableau as T
 from test_gpu_adjoint import _records, _scene
 from test_gpu_adjoint_solid import _cantilever, _contact
-from helpers import self_contact_scene
+from helpers import deformed_solid_scene, self_contact_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -85,6 +85,22 @@ def _cantilever_order(order):
     return dataclasses.replace(sc, materials=mats)
 
 
+def _plastic(seed):
+    """The deformed plastic block (plastic flow active at every solid, |S| > A) with its
+    stiffness and damping 1e5 times larger and relaxation time 4e3 s, stepped with dt 1e-7: the
+    plastic strain then moves the stress within the two steps, and <g, J u> differs by 11-14 %
+    from the same scene without plastic flow (measured on the oracle), so a wrong plastic-rate
+    derivative cannot hide behind the hourglass term."""
+    import dataclasses
+
+    sc = deformed_solid_scene(dim=3, plastic=True, seed=seed)
+    m = sc.materials.copy()
+    for f in ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp"):
+        m[f][2] = np.float32(m[f][2] * 1e5)
+    m["relaxation_time"][2] = 4e3
+    return dataclasses.replace(sc, materials=m)
+
+
 CASES = {
     "fluid_rk2_replay": (lambda: _scene(True), "rk2", 5e-5, 3, 2, {1: ("bulk_visc", "visc")}),
     "fluid_rk4": (lambda: _scene(False), "rk4", 5e-5, 2, 2, {1: ("bulk_visc",)}),
@@ -92,6 +108,15 @@ CASES = {
                        {1: ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp")}),
     "contact_rk2": (_contact, "rk2", 1e-5, 2, 2, {1: ("normal_stiffness", "shear_stiffness"), 2: ("bulk_visc",)}),
     "self_contact_rk2": (self_contact_scene, "rk2", 1e-5, 2, 1, {2: ("normal_stiffness", "shear_damp")}),
+    # checkpoints every step in plastic_rk2 (a replay restores the plastic strain).  Its
+    # dL/d normal_damp is 430 times smaller than dL/d shear_damp (a sum that nearly cancels;
+    # the oracle's value agrees with its central differences to 3e-5, the FP32 sweep's to
+    # 6e-3), so the normal damping is checked on the unscaled block (plastic_soft_rk2)
+    "plastic_rk2": (lambda: _plastic(0), "rk2", 1e-7, 2, 1,
+                    {2: ("normal_stiffness", "shear_stiffness", "shear_damp"), 1: ("bulk_visc",)}),
+    "plastic_soft_rk2": (lambda: deformed_solid_scene(dim=3, plastic=True), "rk2", 1e-6, 2, 1,
+                         {2: ("normal_damp", "normal_stiffness")}),
+    "plastic_rk4": (lambda: _plastic(3), "rk4", 1e-7, 2, 2, {2: ("shear_stiffness", "shear_damp")}),
     "cantilever_order2": (lambda: _cantilever_order(2), "rk2", 1e-7, 2, 1, {1: ("normal_stiffness",)}),
     "cantilever_order0": (lambda: _cantilever_order(0), "rk2", 1e-7, 2, 1, {1: ("normal_stiffness", "shear_damp")}),
     "cantilever_order1": (lambda: _cantilever_order(1), "rk2", 1e-7, 2, 1, {1: ("shear_stiffness",)}),
