@@ -7,6 +7,12 @@
 
 #include "common.h"
 
+// Multiply-adds are fused only inside one source expression (as C's FP_CONTRACT ON), so the
+// rounding of every expression is fixed by its text and not by what the optimiser sees
+// around it: the walk forms specialised for one material pair, the general walks and the
+// batch entries then round alike.
+#pragma clang fp contract(on)
+
 using namespace arp;
 
 struct AosParticle {  // ArpParticle, std430 48 B
@@ -93,19 +99,27 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsu
 __device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
-// EOS (fluid.rs:45-52)
+// the seventh power by multiplication: x^7 = x^4 x^2 x
+__device__ __forceinline__ float pow7(float x) {
+    const float x2 = x * x, x4 = x2 * x2;
+    return x4 * x2 * x;
+}
+
+// EOS (fluid.rs:45-52).  Tait: the seventh power by multiplication and the quotients through
+// v_rcp_f32, a few ulp from powf and the divisions (far below the parity tolerance) at a tenth
+// of their instructions: every stage epilogue evaluates it for every particle.
 __device__ __forceinline__ float eos_pressure(uint32_t eq, float density, float c, float d0) {
     switch (eq) {
         case 1: return c * c;
         case 2: return c * c * (density - d0);
-        case 3: return (c * c * d0 / 7.0f) * (powf(density / d0, 7.0f) - 1.0f);
+        case 3: return (c * c * d0 * (1.0f / 7.0f)) * (pow7(density * frcp(d0)) - 1.0f);
         default: return 0.0f;
     }
 }
 
 // p / (den*den) of a state, the per-particle half of the pressure term (fluid.rs:183-186)
 __device__ __forceinline__ float pressure_q(const DMat& m, float den) {
-    return eos_pressure(m.eq, den, m.c, m.target_den) / (den * den);
+    return eos_pressure(m.eq, den, m.c, m.target_den) * frcp(den * den);
 }
 
 // contact_potential (fluid.rs:54-67)
@@ -113,14 +127,14 @@ __device__ __forceinline__ float contact_potential(uint32_t eq, float r, float s
     switch (eq) {
         case 1: return r <= r0 ? strength : 0.0f;
         case 2: return strength * (r0 - r);
-        case 3: return (strength * r / 7.0f) * (powf(r0 / r, 7.0f) - 1.0f);
+        case 3: return (strength * r * (1.0f / 7.0f)) * (pow7(r0 * frcp(r)) - 1.0f);
         case 4: {
             float f = r0 / r;
             f *= f * f;
             f *= f;
             return strength * (f * f - 2.0f * f);
         }
-        case 5: return (strength * r0 / 7.0f) * (powf(r / r0, 7.0f) - 1.0f);
+        case 5: return (strength * r0 * (1.0f / 7.0f)) * (pow7(r * frcp(r0)) - 1.0f);
         default: return 0.0f;
     }
 }
@@ -2040,8 +2054,8 @@ __device__ __forceinline__ float contact_potential_d(uint32_t eq, float r, float
     switch (eq) {
         case 2: return -strength;
         case 3: {
-            const float f = powf(r0 / r, 7.0f);
-            return (strength / 7.0f) * (f - 1.0f) - strength * f;
+            const float f = pow7(r0 * frcp(r));
+            return (strength * (1.0f / 7.0f)) * (f - 1.0f) - strength * f;
         }
         case 4: {
             float f = r0 / r;
@@ -2049,7 +2063,10 @@ __device__ __forceinline__ float contact_potential_d(uint32_t eq, float r, float
             f *= f;
             return strength * (2.0f * f - 2.0f) * (-6.0f * f / r);
         }
-        case 5: return strength * powf(r / r0, 6.0f);
+        case 5: {
+            const float x = r * frcp(r0), x2 = x * x;
+            return strength * (x2 * x2 * x2);
+        }
         default: return 0.0f;
     }
 }
@@ -2058,7 +2075,10 @@ __device__ __forceinline__ float contact_potential_d(uint32_t eq, float r, float
 __device__ __forceinline__ float pressure_q_d(const DMat& m, float den) {
     float p = eos_pressure(m.eq, den, m.c, m.target_den), dp = 0.0f;
     if (m.eq == 2) dp = m.c * m.c;
-    if (m.eq == 3) dp = m.c * m.c * powf(den / m.target_den, 6.0f);
+    if (m.eq == 3) {
+        const float x = den * frcp(m.target_den), x2 = x * x;
+        dp = m.c * m.c * (x2 * x2 * x2);
+    }
     return dp / (den * den) - 2.0f * p / (den * den * den);
 }
 
