@@ -199,6 +199,19 @@ class FluidSim:
         self._touch()
         check(self.ctx, lib().arp_set_neighbor_max_age(self.ctx, int(stages)))
 
+    def set_neighbor_short_skin(self, short_skin):
+        """Two-level lists: the walks use a short list of the candidates within
+        h*(1+short_skin), refreshed from the long list (no grid search) when a particle moves
+        short_skin*h/2 from its position at the refresh (0 disables; below the skin)."""
+        self._touch()
+        check(self.ctx, lib().arp_set_neighbor_short_skin(self.ctx, float(short_skin)))
+
+    def neighbor_refreshes(self):
+        """Short-list refreshes since the list settings last changed."""
+        r = C.c_uint32(0)
+        check(self.ctx, lib().arp_neighbor_refreshes(self.ctx, C.byref(r)))
+        return int(r.value)
+
     def neighbor_stats(self):
         """(stages that searched and built lists, stages that reused them)."""
         b, r = C.c_uint32(0), C.c_uint32(0)
@@ -253,6 +266,14 @@ class FluidSim:
         check(self.ctx, lib().arp_profile_query(self.ctx, kernel.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
 
+
+    def profile_durations(self, kernel):
+        """Event time (ms) of each launch of one kernel since the last reset, in launch order."""
+        n = C.c_uint64(0)
+        check(self.ctx, lib().arp_profile_durations(self.ctx, kernel.encode(), None, 0, C.byref(n)))
+        buf = (C.c_float * max(int(n.value), 1))()
+        check(self.ctx, lib().arp_profile_durations(self.ctx, kernel.encode(), buf, n.value, C.byref(n)))
+        return np.array(buf[: int(n.value)], dtype=np.float64)
     def close(self):
         # a batch holding this context is destroyed first (garbage collection of a cycle may
         # finalize the sim before the batch that refers to it)

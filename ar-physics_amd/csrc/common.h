@@ -226,13 +226,25 @@ struct RkArgs {
 //   words[10]            the stage counter
 //   words[12 + (g&1)]    stages since the lists in use at stage g were built
 //   words[14]            rebuild lists older than this many stages (0: never for age)
+//   words[16 + (g&1)] = g  when the stage at counter g found a particle beyond the reach of
+//                          the short list (two-level lists, below)
+//   words[18]            number of short-list refreshes
 // Stage g reuses iff words[(g-1)&1] == g-1 and words[2+((g-1)&1)] != g-1.
 // cellCnt holds two count arrays; the build at counter g counts into array g&1.
+// Two-level lists (srad2 > 0): the walks use a short list of the candidates within
+// h + short skin (slist/scount, the same layout), kept while every particle stays within slim
+// of its position sref at the short list's build; a stage whose short list is stale walks
+// the long list instead and writes a new short list from it (a refresh: no grid search).
+// sref.w holds the particle's remaining reach of the long list, lim - |sref - ref|, so the one
+// read of sref checks both lists.  A build writes both lists.
 struct NlArgs {
     uint32_t* words;
     uint32_t* list;
     uint32_t* count;
     float* ref;         // float4 per storage index: position at the build
+    uint32_t* slist;    // two-level lists: the short list (as list), its counts and
+    uint32_t* scount;   // build positions (float4: position, remaining long-list reach)
+    float* sref;
     uint32_t* cellCnt;  // 2 * ncells
     uint32_t* cellRec;  // 2 per storage index: first bucket counted at the build, axis mask
     const uint32_t* rP; // reference prefix per cell (ncells+1), with FluidArgs::bP the static entries
@@ -241,6 +253,9 @@ struct NlArgs {
     uint32_t capThresh;  // counts above this are checked against the bucket's static entries
     float rad, rad2;     // h + skin and its square (build radius)
     float lim2;          // allowed squared displacement from the build position (< (skin/2)^2)
+    float lim;           // its square root (negative: rebuild at every stage)
+    float srad2;         // two-level lists: (h + short skin)^2 (0: one level)
+    float slim2;         // allowed squared displacement from the short list's build position
     float ext;           // half-width of the box around a particle whose buckets count it
     float grey2;         // pairs farther apart than this (squared) are checked to lie in
                          // neighbouring buckets, as the reference's 27-bucket stencil requires
@@ -316,7 +331,7 @@ struct FluidArgs {
 // words of the neighbour-list state beyond the decision words
 constexpr int kNlAbort = 15;  // the aborting speculative step (0xFFFFFFFF: none)
 constexpr int kNlWords = 32;
-constexpr int kSpecSnap = 20;  // words per speculative step snapshot
+constexpr int kSpecSnap = 24;  // words per speculative step snapshot
 constexpr int kRecFlags = 12;  // a recorded step's flags: overflow and list use per stage (2 * 6 stages)
 
 // Restore of the list words after an aborted speculative step (one thread).

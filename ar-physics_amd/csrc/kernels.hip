@@ -659,6 +659,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
     // MODES 2, 3: the stage's build counter and whether it walks the kept lists
     uint32_t gen = 0;
     bool use = false;
+    bool swalk = false, refresh = false;  // two-level lists: short walk, short-list refresh
     const bool spec = WALK && a.spec != 0;
     if (NL) {
         if (spec) {
@@ -685,6 +686,7 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                     for (int q = 0; q < 15; q++) a.specSnap[q] = a.nl.words[q];
                     a.specSnap[15] = *a.stampW;
                     a.specSnap[16] = a.statsW[0];
+                    for (int q = 16; q < 19; q++) a.specSnap[q + 1] = a.nl.words[q];
                 }
                 // the bookkeeping of the rebuild chain's reusing stage (no block reads these)
                 a.nl.words[10] = gen;
@@ -708,8 +710,15 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
             use = nl_reuse(a.nl.words);
             if (use != WALK) return;  // the other kernel of the pair runs this stage
         }
+        // two-level lists: a walking stage uses the short list unless the previous stage found
+        // a particle beyond its reach; then it walks the long list and writes a new short one
+        if (use && a.nl.srad2 > 0.0f) {
+            swalk = a.nl.words[16 + ((gen - 1u) & 1u)] != gen - 1u;
+            refresh = !swalk;
+        }
         if (bid == 0 && tid == 0) {
             if (REC) *a.recUse = use ? 1u : 0u;
+            if (refresh) a.nl.words[18] += 1u;
             a.nl.words[gen & 1u] = gen;
             a.nl.words[6] = gen;
             a.nl.words[12 + (gen & 1u)] = use ? a.nl.words[12 + ((gen - 1u) & 1u)] + 1u : 0u;  // age
@@ -910,13 +919,22 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
 
         uint32_t nrec = 0;  // neighbours recorded (reverse sweep)
         uint32_t nfront = 0, nback = 0;  // list entries within h / within h + skin (MODE 2)
+        // two-level lists: the short list written by a build or a refresh, entries within h
+        // at its front, the others from its end (as the long list)
+        uint32_t sfront = 0, sback = 0;
+        auto short_push = [&](uint32_t v, bool inh) {
+            const uint32_t cap = a.nl.cap;
+            uint32_t& c = inh ? sfront : sback;
+            if (c < cap) a.nl.slist[(size_t)(inh ? c : cap - 1u - c) * a.n + i] = v;
+            c++;
+        };
         if constexpr (WALK) {
             // walk the kept list, ARP_NLB entries per batch: the direct walk evaluates the
             // entries now within h as they are tested; the recording walk buffers them in LDS
             // (branch-free test), records them and drains them as the search does
-            const uint32_t cw = a.nl.count[i];
+            const uint32_t cw = (swalk ? a.nl.scount : a.nl.count)[i];
             const uint32_t fa = cw & 0xFFFFu, fb = cw >> 16, m = fa + fb, cap = a.nl.cap;
-            const uint32_t* L = a.nl.list;
+            const uint32_t* L = swalk ? a.nl.slist : a.nl.list;
             // 32-bit element offsets (the host keeps cap x n below 2^32): entry k at k n + i,
             // the back entries (k >= fa) cap - fb - fa further on
             const uint32_t n32 = a.n, jump = (cap - fb - fa) * n32;
@@ -954,7 +972,9 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) {
                         const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                        inh[u] = ok[u] && hh - (rx * rx + ry * ry + rz * rz) > 0.0f;
+                        const float rr = rx * rx + ry * ry + rz * rz;
+                        inh[u] = ok[u] && hh - rr > 0.0f;
+                        if (refresh && ok[u] && a.nl.srad2 - rr > 0.0f) short_push(e[u], inh[u]);
                         in[u].bnd = (e[u] & kBnd) != 0;
                         in[u].xj = xj[u];
                         const uint32_t j = e[u] & ~kBnd;
@@ -991,7 +1011,9 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
 #pragma unroll
                     for (int u = 0; u < ARP_NLB; u++) {
                         const float rx = xj[u].x - xi.x, ry = xj[u].y - xi.y, rz = xj[u].z - xi.z;
-                        const uint32_t in = ok[u] & (hh - (rx * rx + ry * ry + rz * rz) > 0.0f ? 1u : 0u);
+                        const float rr = rx * rx + ry * ry + rz * rz;
+                        const uint32_t in = ok[u] & (hh - rr > 0.0f ? 1u : 0u);
+                        if (refresh && ok[u] && a.nl.srad2 - rr > 0.0f) short_push(e[u], in != 0u);
                         list[nl * 256 + tid] = e[u];
                         nl += in;
                     }
@@ -1140,7 +1162,9 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
                         const uint32_t j = v & ~kBnd;
                         const float4 xj = ((v & kBnd) ? B : X)[j];
                         const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                        if (hh - (rx * rx + ry * ry + rz * rz) > 0.0f) {
+                        const float rr = rx * rx + ry * ry + rz * rz;
+                        if (a.nl.srad2 - rr > 0.0f) short_push(v, hh - rr > 0.0f);
+                        if (hh - rr > 0.0f) {
                             if (nfront < cap) a.nl.list[(size_t)nfront * a.n + i] = v;
                             nfront++;
                             if (REC) {
@@ -1232,7 +1256,30 @@ __device__ __forceinline__ void fluid_body(const FluidArgs& a, uint32_t bid, uin
         } else {
             float4 op;
             rk_update_particle(a.rk, a.grid, Me, i, kP, kV, haveKey, nextKey, nextBin, op);
-            if (NL) {
+            if (NL && a.nl.srad2 > 0.0f) {
+                // two-level lists: the short list's build position and the particle's remaining
+                // reach of the long list, read after a short walk and written after a build
+                // or a refresh (which is where the short list was built)
+                float4 r0;
+                if (swalk) {
+                    r0 = reinterpret_cast<const float4*>(a.nl.sref)[i];
+                } else {
+                    float reach = a.nl.lim;
+                    if (use) {
+                        const float4 rl = reinterpret_cast<const float4*>(a.nl.ref)[i];
+                        const float ex = xi.x - rl.x, ey = xi.y - rl.y, ez = xi.z - rl.z;
+                        reach = a.nl.lim - sqrtf(ex * ex + ey * ey + ez * ez);
+                    }
+                    r0 = make_float4(xi.x, xi.y, xi.z, reach);
+                    reinterpret_cast<float4*>(a.nl.sref)[i] = r0;
+                    a.nl.scount[i] = sfront + sback > a.nl.cap ? 0u : sfront | (sback << 16);
+                }
+                const float dx = op.x - r0.x, dy = op.y - r0.y, dz = op.z - r0.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                // (a particle past either reach, or a non-finite one, fails both tests)
+                if (!(d2 <= a.nl.slim2)) a.nl.words[16 + (gen & 1u)] = gen;
+                if (!(r0.w >= 0.0f && d2 <= r0.w * r0.w)) a.nl.words[2 + (gen & 1u)] = gen;
+            } else if (NL) {
                 // the next stage may keep the lists only if this particle stays within lim of
                 // its position at the build
                 const float4 r0 = use ? reinterpret_cast<const float4*>(a.nl.ref)[i] : X[i];
@@ -1257,6 +1304,7 @@ extern "C" __global__ void __launch_bounds__(64) k_spec_restore(SpecRestoreArgs 
         for (int q = 0; q < 15; q++) a.words[q] = a.snap[q];
         *a.stamp = a.snap[15];
         a.stats[0] = a.snap[16];
+        for (int q = 16; q < 19; q++) a.words[q] = a.snap[q + 1];
         for (uint32_t q = 0; q < a.ndec; q++) a.stampDec[q] -= 1u;
     }
     a.words[kNlAbort] = kInvalid;

@@ -226,6 +226,7 @@ struct arp_ctx {
     GridSlot slot;          // where builds go (the buffers above unless recording)
     // neighbour lists kept across stages (fluid-only scenes, the plain step)
     float nl_skin = 0.2f;   // list radius h * (1 + skin); 0 disables
+    float nl_sskin = 0.0f;  // two-level lists: the short list's skin (0: one level)
     uint32_t nl_cap = 48;   // entries per particle
     bool nl_always = false; // rebuild at every stage (testing)
     bool nl_nospec = false; // issue the rebuild chain at every stage (no speculative steps)
@@ -245,7 +246,7 @@ struct arp_ctx {
     // the current step), and the stage-input buffer that already holds the live state
     RecStep* rec_next = nullptr;
     const void* rec_prefilled = nullptr;
-    DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec;
+    DBuf nlWords, nlList, nlCount, nlRef, nlCell, nlCellRec, nlSList, nlSCount, nlSRef;
     // adjoint recording
     // this GPU's fixed scratch array (scratch.hip) for the entry points that allocate nothing:
     // its device address, its size, and the HIP error of the lookup (0: found)
@@ -297,7 +298,7 @@ struct arp_ctx {
                 &ps_pid[0], &ps_pid[1], &aos_tmp,   &ps_keys,  &keys,   &ranks,    &binCount,
                 &binStart, &activeRows, &numActive, &tmpKey,   &tmpSrc, &perm,     &rowOff,
                 &gPid,     &gGrp,      &stats,    &scanState, &scanTicket, &scanEpoch, &rowStamp,
-                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell, &nlCellRec,
+                &stampDev, &nlWords,   &nlList,   &nlCount,  &nlRef,  &nlCell, &nlCellRec, &nlSList, &nlSCount, &nlSRef,
                 &Dv[0],    &Dv[1],     &De,       &XF[0][0], &XF[0][1], &XF[1][0], &XF[1][1], &XFe};
         for (int k = 0; k < kMaxK; k++) {
             bufs.push_back(&K[k][0]);
@@ -794,6 +795,10 @@ int nl_prepare(arp_ctx* c) {
         (rc = dalloc(c, c->nlCount, 4 * n)) || (rc = dalloc(c, c->nlRef, 16 * n)) ||
         (rc = dalloc(c, c->nlCell, 8 * (size_t)c->ncells)) || (rc = dalloc(c, c->nlCellRec, 8 * n)))
         return rc;
+    if (c->nl_sskin > 0.0f &&
+        ((rc = dalloc(c, c->nlSList, 4 * n * c->nl_cap)) || (rc = dalloc(c, c->nlSCount, 4 * n)) ||
+         (rc = dalloc(c, c->nlSRef, 16 * n))))
+        return rc;
     if (fresh) {
         if ((rc = fill_u32(c, c->nlWords, kNlWords, 0))) return rc;
         SpecRestoreArgs sr{P<uint32_t>(c->nlWords), nullptr, nullptr, nullptr};  // no abort pending
@@ -807,6 +812,8 @@ int nl_prepare(arp_ctx* c) {
         if ((rc = fill_u32(c, c->nlWords, 7, kInvalid))) return rc;
         FillArgs fa{P<uint32_t>(c->nlWords) + 14, 1, c->nl_max_age};
         if ((rc = launch(c, K_FILL, 1, 256, fa))) return rc;
+        FillArgs fs{P<uint32_t>(c->nlWords) + 16, 2, kInvalid};  // no short list found stale
+        if ((rc = launch(c, K_FILL, 1, 256, fs))) return rc;
         SpecRestoreArgs sr{P<uint32_t>(c->nlWords), nullptr, nullptr, nullptr};  // no abort pending
         if ((rc = launch(c, K_SPEC_RESTORE, 1, 64, sr))) return rc;
         if ((rc = fill_u32(c, c->nlCell, (uint32_t)(2 * c->ncells), 0))) return rc;
@@ -821,6 +828,9 @@ NlArgs nl_args(arp_ctx* c) {
     a.list = P<uint32_t>(c->nlList);
     a.count = P<uint32_t>(c->nlCount);
     a.ref = P<float>(c->nlRef);
+    a.slist = P<uint32_t>(c->nlSList);
+    a.scount = P<uint32_t>(c->nlSCount);
+    a.sref = P<float>(c->nlSRef);
     a.cellCnt = P<uint32_t>(c->nlCell);
     a.cellRec = P<uint32_t>(c->nlCellRec);
     a.rP = P<uint32_t>(c->rP);
@@ -841,6 +851,14 @@ NlArgs nl_args(arp_ctx* c) {
     // stay within h + skin with a relative margin far above the rounding of the tests
     const float lim = 0.5f * c->nl_skin * h * (1.0f - 1e-3f);
     a.lim2 = c->nl_always ? -1.0f : lim * lim;
+    a.lim = c->nl_always ? -1.0f : lim;
+    // two-level lists: pairs within h stay within h + short skin while both particles stay
+    // within slim of their positions at the short list's refresh (the same margin as lim)
+    if (c->nl_sskin > 0.0f) {
+        const float Rs = h * (1.0f + c->nl_sskin), slim = 0.5f * c->nl_sskin * h * (1.0f - 1e-3f);
+        a.srad2 = Rs * Rs;
+        a.slim2 = slim * slim;
+    }
     // a particle within lim of its build position lies in a bucket within ext of that
     // position, with room for the rounding of the bucket function
     a.ext = lim * (1.0f + 1e-3f) + 8.0f * kEps32 * box_extent(c->grid);
@@ -2758,6 +2776,7 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
     if (cap < 8 || cap > 1024) return fail(c, ARP_E_ARG, "list capacity must be in [8, 1024]");
     c->nl_skin = skin;
     c->nl_cap = cap;
+    if (!(c->nl_sskin < skin)) c->nl_sskin = 0.0f;  // a short skin must lie below the skin
     c->nl_always = (flags & ARP_NL_ALWAYS_BUILD) != 0;
     c->nl_nospec = (flags & ARP_NL_NO_SPECULATION) != 0;
     c->nl_dirty = true;
@@ -2768,6 +2787,8 @@ int arp_set_neighbor_skin(arp_ctx* c, float skin, uint32_t cap, uint32_t flags) 
         FillArgs fa{P<uint32_t>(c->nlWords) + 8, 2, 0};  // statistics restart
         int rc = launch(c, K_FILL, 1, 256, fa);
         if (rc) return rc;
+        FillArgs fr{P<uint32_t>(c->nlWords) + 18, 1, 0};
+        if ((rc = launch(c, K_FILL, 1, 256, fr))) return rc;
     }
     return ARP_OK;
 }
@@ -2795,6 +2816,38 @@ int arp_neighbor_stats(arp_ctx* c, uint32_t* builds, uint32_t* reused) {
     }
     if (builds) *builds = w[8];
     if (reused) *reused = w[9];
+    return ARP_OK;
+}
+
+int arp_set_neighbor_short_skin(arp_ctx* c, float sskin) {
+    if (!c) return ARP_E_ARG;
+    if (!(sskin >= 0.0f && sskin <= 1.0f)) return fail(c, ARP_E_ARG, "short skin must be in [0, 1] (a fraction of h)");
+    if (sskin > 0.0f && sskin < 0.01f) return fail(c, ARP_E_ARG, "short skin below 0.01 h leaves no room for rounding");
+    if (sskin > 0.0f && !(sskin < c->nl_skin))
+        return fail(c, ARP_E_ARG, "the short skin must lie below the list skin");
+    c->nl_sskin = sskin;
+    c->nl_dirty = true;
+    c->rec_valid = false;  // a replay must walk the recorded steps with the lists they had
+    if (c->nlWords.p) {
+        HIPCHK(c, hipSetDevice(c->device));
+        FillArgs fa{P<uint32_t>(c->nlWords) + 8, 2, 0};  // statistics restart
+        int rc = launch(c, K_FILL, 1, 256, fa);
+        if (rc) return rc;
+        FillArgs fr{P<uint32_t>(c->nlWords) + 18, 1, 0};
+        if ((rc = launch(c, K_FILL, 1, 256, fr))) return rc;
+    }
+    return ARP_OK;
+}
+
+int arp_neighbor_refreshes(arp_ctx* c, uint32_t* refreshes) {
+    if (!c) return ARP_E_ARG;
+    uint32_t w[kNlWords] = {};
+    if (c->nlWords.p) {
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipMemcpyAsync(w, c->nlWords.p, sizeof w, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (refreshes) *refreshes = w[18];
     return ARP_OK;
 }
 
@@ -2858,6 +2911,19 @@ int arp_profile_query(arp_ctx* c, const char* kernel, double* total_ms, uint64_t
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = cnt;
+    return ARP_OK;
+}
+
+int arp_profile_durations(arp_ctx* c, const char* kernel, float* ms, uint64_t capacity, uint64_t* count) {
+    if (!c || !kernel || (capacity && !ms)) return ARP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint64_t cnt = 0;
+    for (auto& m : c->ev_marks) {
+        if (strcmp(kKernNames[m.first], kernel) != 0) continue;
+        if (cnt < capacity) HIPCHK(c, hipEventElapsedTime(&ms[cnt], c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+        cnt++;
+    }
+    if (count) *count = cnt;
     return ARP_OK;
 }
 

@@ -143,15 +143,15 @@ def config_run(sim, p_init, nsteps):
     """The configuration's nsteps steps from the initial state, timed as a whole."""
     sim.upload_state(p_init)
     sim.synchronize()
-    nl0, sp0 = sim.neighbor_stats(), sim.speculation_stats()
+    nl0, sp0, rf0 = sim.neighbor_stats(), sim.speculation_stats(), sim.neighbor_refreshes()
     t0 = time.perf_counter()
     sim.update(nsteps)
     sim.synchronize()
     sec = time.perf_counter() - t0
-    nl1, sp1 = sim.neighbor_stats(), sim.speculation_stats()
+    nl1, sp1, rf1 = sim.neighbor_stats(), sim.speculation_stats(), sim.neighbor_refreshes()
     n = len(p_init)
     return dict(steps=nsteps, seconds=sec, ms_per_step=1e3 * sec / nsteps, particle_steps_per_s=n * nsteps / sec,
-                stage_builds=nl1[0] - nl0[0], stage_reuses=nl1[1] - nl0[1],
+                stage_builds=nl1[0] - nl0[0], stage_reuses=nl1[1] - nl0[1], short_list_refreshes=rf1 - rf0,
                 speculative_steps=sp1[0] - sp0[0], speculation_aborts=sp1[1] - sp0[1])
 
 
@@ -372,8 +372,13 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256,
                     help="scenes of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
-    ap.add_argument("--skin", type=float, default=0.2,
+    # (the list settings measured fastest on this workload: skins 0.2-0.5 with short skins
+    # 0.03-0.06 gave 1.10e10-1.16e10 particle-steps/s, one level at 0.2 9.8e9)
+    ap.add_argument("--skin", type=float, default=0.4,
                     help="neighbour lists kept across RK stages with radius h*(1+skin) (0: search at every stage)")
+    ap.add_argument("--short-skin", type=float, default=0.04,
+                    help="two-level lists: the walks use a short list of radius h*(1+short_skin), refreshed from "
+                         "the long list (0: one level)")
     ap.add_argument("--max-age", type=int, default=None,
                     help="rebuild kept neighbour lists older than this many stages (default: the library's)")
     ap.add_argument("--solid-configs", type=int, default=1,
@@ -412,6 +417,8 @@ def main():
     log("particles %d boundary %d" % (sc.n, sc.nb))
     sim = FluidSim(sc, integrator=args.integrator, timestep=dt)
     sim.set_neighbor_skin(args.skin)
+    if args.short_skin > 0:
+        sim.set_neighbor_short_skin(args.short_skin)
     if args.max_age is not None:
         sim.set_neighbor_max_age(args.max_age)
     # the list walk (its fast-material form when the scene has one mobile and one boundary
@@ -432,6 +439,7 @@ def main():
     # separate pass with events on the stream times the kernels
     nl0 = sim.neighbor_stats()
     sp0 = sim.speculation_stats()
+    rf0 = sim.neighbor_refreshes()
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
     sim.update(args.steps)
@@ -440,14 +448,18 @@ def main():
     sec = t1 - t0
     nl1 = sim.neighbor_stats()
     sp1 = sim.speculation_stats()
+    rf1 = sim.neighbor_refreshes()
     timed_kernels = fluid_kernels + ["k_fluid_nlb", "k_rowsort", "k_scatter", "k_scan", "k_key"]
     sim.profile_select(fluid_kernels if args.events == "fluid" else None)
     sim.profile(args.events != "none")
     sim.profile_reset()
+    ev_nl0, ev_rf0 = sim.neighbor_stats(), sim.neighbor_refreshes()
     if args.events != "none":
         log("kernel timing: %d steps with events" % args.event_steps)
         sim.update(args.event_steps)
         sim.synchronize()
+    ev_walks = sim.neighbor_stats()[1] - ev_nl0[1]
+    ev_refreshes = sim.neighbor_refreshes() - ev_rf0
     kern = {}
     for k in ["k_fluid"] + timed_kernels:
         ms, cnt = sim.profile_query(k)
@@ -455,6 +467,21 @@ def main():
             kern[k] = dict(total_ms=ms, launches=cnt, avg_us=1e3 * ms / cnt)
     # the walk kernel that ran (most launches)
     fluid_kernel = max(fluid_kernels, key=lambda k: kern.get(k, {}).get("launches", 0))
+    if args.skin > 0 and fluid_kernel in kern:
+        # A walk launch returns at once when its stage searched instead, or when an earlier
+        # speculative stage found the lists stale: only the stages that walked (one launch
+        # each, the longest ones) are the kernel's work.  With two-level lists the longest of
+        # those are the short-list refreshes.
+        d = np.sort(sim.profile_durations(fluid_kernel))[::-1]
+        work = d[:ev_walks] if ev_walks else d
+        f = kern[fluid_kernel]
+        f["walking_launches"] = int(len(work))
+        f["avg_us_all_launches"] = f["avg_us"]
+        f["avg_us"] = 1e3 * float(work.mean())
+        if ev_refreshes and ev_walks > ev_refreshes:
+            f["refreshes"] = int(ev_refreshes)
+            f["avg_us_short_walks"] = 1e3 * float(work[ev_refreshes:].mean())
+            f["avg_us_refreshes"] = 1e3 * float(work[:ev_refreshes].mean())
     sim.profile(False)
     # the rebuild chain's kernels (what a moving scene pays), timed with events on a build
     sim.profile_select(timed_kernels[len(fluid_kernels):])
@@ -533,8 +560,8 @@ def main():
             "alg_bytes_per_launch": bytes_launch,
         },
         "kernels": kern,
-        "neighbor_lists": {"skin": args.skin, "stage_builds": nl1[0] - nl0[0],
-                           "stage_reuses": nl1[1] - nl0[1],
+        "neighbor_lists": {"skin": args.skin, "short_skin": args.short_skin, "stage_builds": nl1[0] - nl0[0],
+                           "stage_reuses": nl1[1] - nl0[1], "short_list_refreshes": rf1 - rf0,
                            # timed steps issued as list walks alone, and those run again
                            "speculative_steps": sp1[0] - sp0[0], "speculation_aborts": sp1[1] - sp0[1]},
         "config_run": cfg,

@@ -194,6 +194,17 @@ int arp_set_neighbor_max_age(arp_ctx* ctx, uint32_t stages);
 /* Stages that searched and built lists, and stages that reused them, since the last
  * arp_set_neighbor_skin. */
 int arp_neighbor_stats(arp_ctx* ctx, uint32_t* builds, uint32_t* reused);
+/* Two-level lists (no counterpart in the reference, whose stages always search): the list
+ * walks use a short list of the candidates within h * (1 + short_skin), taken from the long
+ * list (skin above) while every particle stays within short_skin*h/2 of its position at the
+ * short list's refresh; a stage whose short list is stale refreshes it from the long list
+ * (no sort, no grid search), and only a stale long list costs a search.  short_skin = 0
+ * (default) walks the long list; otherwise 0.01 <= short_skin < skin.  The neighbours within
+ * h are the same.  Resets the statistics. */
+int arp_set_neighbor_short_skin(arp_ctx* ctx, float short_skin);
+/* Short-list refreshes since the last arp_set_neighbor_skin or arp_set_neighbor_short_skin
+ * (each is a stage counted as reused by arp_neighbor_stats). */
+int arp_neighbor_refreshes(arp_ctx* ctx, uint32_t* refreshes);
 /* Steps issued speculatively (each stage as its list walk alone) that completed, and those
  * whose lists turned out invalid and ran again with the rebuild chain, since the context was
  * created.  No counterpart in the reference (whose step always searches); diagnostics. */
@@ -250,6 +261,11 @@ int arp_cell_counts(arp_ctx* ctx, uint64_t first, uint64_t count, uint32_t* out)
  * event time and the number of launches since the last reset. */
 int arp_profile(arp_ctx* ctx, int enable);
 int arp_profile_query(arp_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+/* The event times (ms) of each launch of one kernel since the last reset, in launch order:
+ * up to capacity into ms (may be NULL with capacity 0), the number of launches into count.
+ * (A list walk's launch returns at once when its stage searched instead, or when an earlier
+ * speculative stage found the lists stale: these launches are short.) */
+int arp_profile_durations(arp_ctx* ctx, const char* kernel, float* ms, uint64_t capacity, uint64_t* count);
 int arp_profile_reset(arp_ctx* ctx);
 /* Time only the named kernels (comma-separated, e.g. "k_fluid,k_fluid_adj"); NULL or ""
  * selects all.  Untimed kernels launch without events. */

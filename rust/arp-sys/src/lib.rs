@@ -228,6 +228,8 @@ extern "C" {
     pub fn arp_neighbor_lists_fit(cap: u32, n: u32) -> c_int;
     pub fn arp_set_neighbor_max_age(ctx: *mut ArpCtx, stages: u32) -> c_int;
     pub fn arp_neighbor_stats(ctx: *mut ArpCtx, builds: *mut u32, reused: *mut u32) -> c_int;
+    pub fn arp_set_neighbor_short_skin(ctx: *mut ArpCtx, short_skin: f32) -> c_int;
+    pub fn arp_neighbor_refreshes(ctx: *mut ArpCtx, refreshes: *mut u32) -> c_int;
     pub fn arp_speculation_stats(ctx: *const ArpCtx, steps: *mut u64, aborted: *mut u64) -> c_int;
     pub fn arp_compute_forces(ctx: *mut ArpCtx, d_state: *const ArpParticle, d_solids: *const ArpSolidParticle, d_deriv: *mut ArpParticle, d_solid_deriv: *mut ArpSolidParticle) -> c_int;
     pub fn arp_lincomb(ctx: *mut ArpCtx, nterms: u32, coefs: *const f32, d_terms: *const *const ArpParticle, d_solid_terms: *const *const ArpSolidParticle, d_out: *mut ArpParticle, d_solid_out: *mut ArpSolidParticle) -> c_int;
@@ -239,6 +241,7 @@ extern "C" {
     pub fn arp_cell_counts(ctx: *mut ArpCtx, first: u64, count: u64, out: *mut u32) -> c_int;
     pub fn arp_profile(ctx: *mut ArpCtx, enable: c_int) -> c_int;
     pub fn arp_profile_query(ctx: *mut ArpCtx, kernel: *const c_char, total_ms: *mut f64, launches: *mut u64) -> c_int;
+    pub fn arp_profile_durations(ctx: *mut ArpCtx, kernel: *const c_char, ms: *mut f32, capacity: u64, count: *mut u64) -> c_int;
     pub fn arp_profile_reset(ctx: *mut ArpCtx) -> c_int;
     pub fn arp_profile_select(ctx: *mut ArpCtx, kernels: *const c_char) -> c_int;
     pub fn arp_adjoint_record(ctx: *mut ArpCtx, dt: f32, nsteps: u32, checkpoint_every: u32) -> c_int;
