@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# (ARP_ORACLE_LIB: another build of the same sources, e.g. the sanitized one of `make sanitize`)
+LIB = os.environ.get("ARP_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 VERLET = -1  # the stages field of a tableau that selects velocity Verlet
 
 u32p = C.POINTER(C.c_uint32)

@@ -993,7 +993,7 @@ int F(orc_lincomb)(const orc_scene* sc, int nterms, const REAL* coef,
     int nch = (nterms + 4) / 5;
     F(orc_state)* tmp = (F(orc_state)*)calloc(nch, sizeof(F(orc_state)));
     const F(orc_state)** tp = (const F(orc_state)**)calloc(nch, sizeof(void*));
-    REAL* ones = (REAL*)malloc(nch * sizeof(REAL));
+    REAL* ones = (REAL*)calloc(nch, sizeof(REAL));
     for (int c = 0; c < nch; c++) {
         F(alloc_state)(sc, &tmp[c]);
         int k0 = 5 * c, nk = nterms - k0 < 5 ? nterms - k0 : 5;
