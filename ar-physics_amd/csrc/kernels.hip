@@ -1320,7 +1320,13 @@ __device__ __forceinline__ const A& batch_scene(const BatchArgs& b, uint32_t L, 
     const uint32_t s = b.blockScene[L];
     bid = L - b.blockFirst[s];
     nb = b.blockFirst[s + 1] - b.blockFirst[s];
-    return *reinterpret_cast<const A*>(reinterpret_cast<const char*>(b.args) + (size_t)s * b.stride);
+    // the blocks are never written while a kernel runs: read them through the constant
+    // address space, so that their fields come in with scalar loads (through a global
+    // pointer the compiler must assume the kernel's own stores may change them, and loads
+    // every field the hot loops use with vector loads)
+    typedef const __attribute__((address_space(4))) A CA;
+    typedef const __attribute__((address_space(4))) char CC;
+    return *(const A*)(CA*)((CC*)(const void*)b.args + (size_t)s * b.stride);
 }
 
 #define ARP_ENTRIES(name, Args, body)                                                     \
