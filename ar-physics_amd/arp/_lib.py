@@ -23,7 +23,7 @@ SYMBOLS = [
     "arp_kernel_radius", "arp_num_materials", "arp_add_particles", "arp_scene_spawn", "arp_num_particles",
     "arp_num_solids", "arp_neighbor_lists_fit", "arp_lincomb_noalloc", "arp_batch_adjoint_record",
     "arp_batch_adjoint_backward", "arp_batch_tuning", "arp_set_neighbor_short_skin", "arp_neighbor_refreshes",
-    "arp_profile_durations",
+    "arp_profile_durations", "arp_tune_lists", "arp_batch_tune_lists",
 ]
 
 
@@ -103,6 +103,8 @@ def lib():
     L.arp_set_neighbor_short_skin.argtypes = [vp, C.c_float]
     L.arp_profile_durations.argtypes = [vp, C.c_char_p, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint64)]
     L.arp_neighbor_refreshes.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.arp_tune_lists.argtypes = [vp, C.c_float, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.arp_batch_tune_lists.argtypes = [vp, C.c_float, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.arp_speculation_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.arp_adjoint_record.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32]
     L.arp_adjoint_backward.argtypes = [vp, vp, vp, vp]

@@ -206,6 +206,15 @@ class FluidSim:
         self._touch()
         check(self.ctx, lib().arp_set_neighbor_short_skin(self.ctx, float(short_skin)))
 
+    def tune_lists(self, dt, trial_steps=100):
+        """Autotune the neighbour lists: times trial_steps steps from the current state for
+        each (skin, short skin) pair of a fixed table (one and two levels), restoring the
+        state after every trial, and sets the fastest; returns (skin, short_skin)."""
+        self._touch()
+        a, b = C.c_float(), C.c_float()
+        check(self.ctx, lib().arp_tune_lists(self.ctx, float(dt), int(trial_steps), C.byref(a), C.byref(b)))
+        return float(a.value), float(b.value)
+
     def neighbor_refreshes(self):
         """Short-list refreshes since the list settings last changed."""
         r = C.c_uint32(0)
@@ -321,6 +330,13 @@ class Batch:
         d = C.c_uint32()
         self._check(lib().arp_batch_tune(self.b, float(dt), int(trial_steps), C.byref(d)))
         return d.value
+
+    def tune_lists(self, dt, trial_steps=100):
+        """Autotune the neighbour lists of every scene (one (skin, short skin) pair for the
+        batch, FluidSim.tune_lists's table); returns (skin, short_skin)."""
+        a, b = C.c_float(), C.c_float()
+        self._check(lib().arp_batch_tune_lists(self.b, float(dt), int(trial_steps), C.byref(a), C.byref(b)))
+        return float(a.value), float(b.value)
 
     def tuning(self):
         """(particles per row-sort workgroup, list walks buffered in LDS: 0 or 1)."""

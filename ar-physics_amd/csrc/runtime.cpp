@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -3464,6 +3465,105 @@ int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chose
     for (Checkpoint& k : ck)
         for (DBuf* d : {&k.P, &k.V, &k.mk, &k.pid}) free_buf(*d);
     if (chosen_div) *chosen_div = bestDiv;
+    return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// (list skin, short skin) pairs the list tuner times, as fractions of h: one level at three
+// skins, two levels at five
+constexpr float kListCand[][2] = {{0.1f, 0.0f},  {0.2f, 0.0f},  {0.3f, 0.0f},  {0.2f, 0.04f},
+                                  {0.3f, 0.03f}, {0.3f, 0.05f}, {0.4f, 0.04f}, {0.5f, 0.05f}};
+constexpr int kNListCand = (int)(sizeof(kListCand) / sizeof(kListCand[0]));
+
+int set_lists(arp_ctx* c, float skin, float sskin) {
+    const uint32_t flags = (c->nl_always ? ARP_NL_ALWAYS_BUILD : 0u) | (c->nl_nospec ? ARP_NL_NO_SPECULATION : 0u);
+    int rc = arp_set_neighbor_skin(c, skin, c->nl_cap, flags);
+    if (!rc) rc = arp_set_neighbor_short_skin(c, sskin);
+    return rc;
+}
+
+// Times trial steps of the contexts (as the batch b when given, else the one context) from
+// their current states for every candidate pair, in two rounds keeping each candidate's
+// faster trial, restoring the states after every trial; sets the fastest pair on every
+// context.  Wall time around the whole call: it holds the list builds the pair causes and
+// the host's share (the same for every pair).
+int tune_lists(const std::vector<arp_ctx*>& cs, arp_batch* b, float dt, uint32_t trial, float* best_skin,
+               float* best_sskin) {
+    arp_ctx* L0 = cs[0];
+    for (arp_ctx* c : cs)
+        if (c->solids_active || c->n == 0)
+            return fail(L0, ARP_E_STATE, "the list tuner needs fluid-only scenes with particles");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(L0, ARP_E_ARG, "dt must be positive");
+    if (trial == 0) trial = 100;
+    const size_t ns = cs.size();
+    std::vector<Checkpoint> ck(ns);
+    std::vector<HostState> hs(ns);
+    std::vector<double> tm(ns);
+    std::vector<uint64_t> sp(ns), sa(ns);
+    int rc = ARP_OK;
+    for (size_t s = 0; s < ns && !rc; s++) {
+        rc = ck_save(cs[s], ck[s]);
+        hs[s] = save_host(cs[s]);
+        tm[s] = cs[s]->time;
+        sp[s] = cs[s]->spec_steps;
+        sa[s] = cs[s]->spec_aborts;
+    }
+    double tbest[kNListCand];
+    for (double& t : tbest) t = 1e300;
+    for (int round = 0; round < 2 && !rc; round++) {
+        for (int ci = 0; ci < kNListCand && !rc; ci++) {
+            for (arp_ctx* c : cs)
+                if ((rc = set_lists(c, kListCand[ci][0], kListCand[ci][1]))) break;
+            if (rc) break;
+            for (arp_ctx* c : cs) HIPCHK(c, hipStreamSynchronize(c->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = b ? run_batch(b, dt, trial) : run_steps(L0, dt, trial);
+            for (arp_ctx* c : cs) HIPCHK(c, hipStreamSynchronize(c->stream));
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (!rc) tbest[ci] = std::min(tbest[ci], sec);
+            for (size_t s = 0; s < ns && !rc; s++) {
+                load_host(cs[s], hs[s]);  // first: the restore writes the meta of this parity
+                rc = ck_restore(cs[s], ck[s]);
+                cs[s]->time = tm[s];
+            }
+        }
+    }
+    int bi = 1;  // (the library's default pair when every trial failed)
+    for (int ci = 0; ci < kNListCand; ci++)
+        if (tbest[ci] < tbest[bi]) bi = ci;
+    for (size_t s = 0; s < ns; s++) {
+        int rc2 = set_lists(cs[s], kListCand[bi][0], kListCand[bi][1]);
+        if (!rc) rc = rc2;
+        cs[s]->spec_steps = sp[s];
+        cs[s]->spec_aborts = sa[s];
+    }
+    for (Checkpoint& k : ck)
+        for (DBuf* d : {&k.P, &k.V, &k.mk, &k.pid, &k.E}) free_buf(*d);
+    if (best_skin) *best_skin = kListCand[bi][0];
+    if (best_sskin) *best_sskin = kListCand[bi][1];
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int arp_tune_lists(arp_ctx* c, float dt, uint32_t trial_steps, float* skin, float* short_skin) {
+    int rc;
+    if (!c) return ARP_E_ARG;
+    if ((rc = need_scene(c))) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    return tune_lists(std::vector<arp_ctx*>{c}, nullptr, dt, trial_steps, skin, short_skin);
+}
+
+int arp_batch_tune_lists(arp_batch* b, float dt, uint32_t trial_steps, float* skin, float* short_skin) {
+    if (!b) return ARP_E_ARG;
+    HIPCHK(b->lead, hipSetDevice(b->lead->device));
+    int rc = tune_lists(b->ctx, b, dt, trial_steps, skin, short_skin);
+    if (rc) b->err = b->lead->err;
     return rc;
 }
 

@@ -216,8 +216,9 @@ def time_adjoint(sim, sc, p_start, dt, nsteps, K):
 def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss):
     """A configuration with solids at its full size: fwd_steps plain steps from the initial
     state (timed as a whole), then forward + adjoint of adj_steps steps with a checkpoint
-    every K (replaying every segment but the last) against the same steps plain, and the
-    gradient norms of the loss.  loss(particles) -> (value, cotangent (n, 7))."""
+    every K (replaying every segment but the last) against the same steps plain (median of
+    three rounds after an untimed one), and the gradient norms of the loss.
+    loss(particles) -> (value, cotangent (n, 7))."""
     import numpy as np
     import torch
 
@@ -238,29 +239,36 @@ def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss):
                state_finite=bool(np.isfinite(p["pos"]).all()))
     if adj_steps:
         log("%s: forward + adjoint of %d steps, checkpoint every %d" % (name, adj_steps, K))
-        sim.upload_state(sc.particles)
-        sim.synchronize()
-        t0 = time.perf_counter()
-        sim.step(dt, adj_steps)
-        sim.synchronize()
-        fwd = time.perf_counter() - t0
-        sim.upload_state(sc.particles)
-        sim.synchronize()
-        t0 = time.perf_counter()
-        sim.adjoint_record(dt, adj_steps, K)
-        sim.synchronize()
-        rec = time.perf_counter() - t0
-        val, g7 = loss(sim.particles())
-        g = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
-        g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(g7.astype(np.float32)).cuda()
-        gin = torch.zeros_like(g)
-        t0 = time.perf_counter()
-        gp = sim.adjoint_backward(g, gin, want_params=True)
-        sim.synchronize()
-        bwd = time.perf_counter() - t0
+
+        def timed(fn):
+            sim.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            sim.synchronize()
+            return time.perf_counter() - t0, r
+
+        # one untimed round (it allocates the recording and the reverse-mode buffers), then
+        # the median of three, as for the fluid column
+        fwds, recs, bwds = [], [], []
+        for r in range(4):
+            sim.upload_state(sc.particles)
+            fwd, _ = timed(lambda: sim.step(dt, adj_steps))
+            sim.upload_state(sc.particles)
+            rec, _ = timed(lambda: sim.adjoint_record(dt, adj_steps, K))
+            val, g7 = loss(sim.particles())
+            g = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
+            g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(g7.astype(np.float32)).cuda()
+            gin = torch.zeros_like(g)
+            bwd, gp = timed(lambda: sim.adjoint_backward(g, gin, want_params=True))
+            if r:
+                fwds.append(fwd)
+                recs.append(rec)
+                bwds.append(bwd)
+        med = lambda x: sorted(x)[len(x) // 2]  # noqa: E731
+        fwd, rec, bwd = med(fwds), med(recs), med(bwds)
         gi = gin.cpu().numpy()
         out["fwd_adjoint"] = dict(steps=adj_steps, checkpoint_every=K, segments_replayed=(adj_steps + K - 1) // K - 1,
-                                  forward_ms=1e3 * fwd, record_ms=1e3 * rec, backward_ms=1e3 * bwd,
+                                  forward_ms=1e3 * fwd, record_ms=1e3 * rec, backward_ms=1e3 * bwd, timed_rounds=len(fwds),
                                   ratio_fwd_plus_adjoint_to_fwd=(rec + bwd) / fwd, loss=val,
                                   grad_x0_norm=float(np.linalg.norm(gi[:, [4, 5, 6]])),
                                   grad_v0_norm=float(np.linalg.norm(gi[:, [8, 9, 10]])),
@@ -305,8 +313,9 @@ def contact_loss(sc):
 
 def time_ensemble(nscenes, steps, dt):
     """Independent fluid scenes (log-uniform 2,048-65,536 particles, alpha uniform in
-    [0.01, 0.1], seeded) stepped as one batch (graph replay, tuned), against the same scenes
-    stepped one after the other."""
+    [0.01, 0.1], seeded) stepped as one batch (graph replay; list skins, row-sort workgroup
+    size and list-walk form autotuned), against the same scenes stepped one after the
+    other."""
     import numpy as np
     import torch
 
@@ -318,6 +327,7 @@ def time_ensemble(nscenes, steps, dt):
     sims = [FluidSim(sc, integrator="rk2", timestep=dt) for sc in scenes]
     npart = sum(sc.n for sc in scenes)
     B = Batch(sims, graph=True)
+    skin, short_skin = B.tune_lists(dt, 40)
     div = B.tune(dt, 4)
     _, walk_buffered = B.tuning()
     B.step(dt, 2)
@@ -345,7 +355,7 @@ def time_ensemble(nscenes, steps, dt):
     return dict(scenes=nscenes, particles=npart, steps=steps, integrator="rk2", dt=dt,
                 ms_per_step=1e3 * sec / steps, particle_steps_per_s=npart * steps / sec,
                 sequential_particle_steps_per_s=npart * seq_steps / seq, graph=True,
-                tuned_rowsort_particles_per_workgroup=div,
+                tuned_list_skin=skin, tuned_short_skin=short_skin, tuned_rowsort_particles_per_workgroup=div,
                 tuned_list_walk="LDS-buffered" if walk_buffered else "registers", state_finite=finite)
 
 

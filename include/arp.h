@@ -205,6 +205,14 @@ int arp_set_neighbor_short_skin(arp_ctx* ctx, float short_skin);
 /* Short-list refreshes since the last arp_set_neighbor_skin or arp_set_neighbor_short_skin
  * (each is a stage counted as reused by arp_neighbor_stats). */
 int arp_neighbor_refreshes(arp_ctx* ctx, uint32_t* refreshes);
+/* Autotuner of the lists (no counterpart in the reference, whose step always searches): times
+ * trial_steps steps (0: 100) from the current state for each (skin, short skin) pair of a
+ * fixed table -- one level at 0.1, 0.2 and 0.3 h; two levels at 0.2/0.04, 0.3/0.03,
+ * 0.3/0.05, 0.4/0.04 and 0.5/0.05 h -- in two rounds, restores the state and the time after
+ * every trial, and sets the fastest pair (written to *skin and *short_skin, either may be
+ * NULL); capacity and flags stay.  The pairs give the same neighbours within h, so results
+ * differ between them only in the rounding of the sums.  Fluid-only scenes. */
+int arp_tune_lists(arp_ctx* ctx, float dt, uint32_t trial_steps, float* skin, float* short_skin);
 /* Steps issued speculatively (each stage as its list walk alone) that completed, and those
  * whose lists turned out invalid and ran again with the rebuild chain, since the context was
  * created.  No counterpart in the reference (whose step always searches); diagnostics. */
@@ -328,6 +336,9 @@ int arp_batch_tune(arp_batch* batch, float dt, uint32_t trial_steps, uint32_t* c
 /* The batch's current choices: particles per row-sort workgroup, and 1 when the list walks
  * buffer the neighbours within h in LDS. */
 int arp_batch_tuning(const arp_batch* batch, uint32_t* rowsort_div, uint32_t* walk_buffered);
+/* The list autotuner (arp_tune_lists) over the batch: one pair for every scene, each trial
+ * trial_steps batched steps (0: 100). */
+int arp_batch_tune_lists(arp_batch* batch, float dt, uint32_t trial_steps, float* skin, float* short_skin);
 
 /* ---- particle insertion (FluidSim::add_particles, soft_body/mod.rs:180-206) ----------- */
 /* FluidSim::kernel_radius (soft_body/mod.rs:166) and num_materials (mod.rs:176-178). */

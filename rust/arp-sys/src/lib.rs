@@ -230,6 +230,7 @@ extern "C" {
     pub fn arp_neighbor_stats(ctx: *mut ArpCtx, builds: *mut u32, reused: *mut u32) -> c_int;
     pub fn arp_set_neighbor_short_skin(ctx: *mut ArpCtx, short_skin: f32) -> c_int;
     pub fn arp_neighbor_refreshes(ctx: *mut ArpCtx, refreshes: *mut u32) -> c_int;
+    pub fn arp_tune_lists(ctx: *mut ArpCtx, dt: f32, trial_steps: u32, skin: *mut f32, short_skin: *mut f32) -> c_int;
     pub fn arp_speculation_stats(ctx: *const ArpCtx, steps: *mut u64, aborted: *mut u64) -> c_int;
     pub fn arp_compute_forces(ctx: *mut ArpCtx, d_state: *const ArpParticle, d_solids: *const ArpSolidParticle, d_deriv: *mut ArpParticle, d_solid_deriv: *mut ArpSolidParticle) -> c_int;
     pub fn arp_lincomb(ctx: *mut ArpCtx, nterms: u32, coefs: *const f32, d_terms: *const *const ArpParticle, d_solid_terms: *const *const ArpSolidParticle, d_out: *mut ArpParticle, d_solid_out: *mut ArpSolidParticle) -> c_int;
@@ -256,6 +257,7 @@ extern "C" {
     pub fn arp_batch_adjoint_backward(batch: *mut ArpBatch, d_grad_out: *const *const c_void, d_grad_in: *const *mut c_void, grad_params: *const *mut f32) -> c_int;
     pub fn arp_batch_tune(batch: *mut ArpBatch, dt: f32, trial_steps: u32, chosen_div: *mut u32) -> c_int;
     pub fn arp_batch_tuning(batch: *const ArpBatch, rowsort_div: *mut u32, walk_buffered: *mut u32) -> c_int;
+    pub fn arp_batch_tune_lists(batch: *mut ArpBatch, dt: f32, trial_steps: u32, skin: *mut f32, short_skin: *mut f32) -> c_int;
     pub fn arp_kernel_radius(ctx: *const ArpCtx) -> f32;
     pub fn arp_num_materials(ctx: *const ArpCtx) -> u32;
     pub fn arp_num_particles(ctx: *const ArpCtx) -> u32;
