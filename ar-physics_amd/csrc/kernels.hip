@@ -1395,6 +1395,19 @@ extern "C" __global__ void __launch_bounds__(256, ARP_NL_WAVES) k_fluid_nl_f_bat
     const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
     fluid_body<3, false, true>(a, bid, nb);
 }
+// a batched stage's list pair in one launch (scenes of one mobile and one boundary material):
+// a block whose scene's stage builds searches and builds its block of particles, one whose
+// stage reuses walks it; the body that does not apply returns at once.  Both bodies read the
+// decision from the previous stage's words, which neither writes, so they agree.
+extern "C" __global__ void __launch_bounds__(256, ARP_NL_WAVES) k_fluid_nlbw_f_batch(BatchArgs b) {
+    uint32_t bid, nb;
+    const FluidArgs& a = batch_scene<FluidArgs>(b, xcd_block(), bid, nb);
+    if (a.phys.nm <= (uint32_t)kMatLds)
+        fluid_body<2, true, false>(a, bid, nb);
+    else
+        fluid_body<2, false, false>(a, bid, nb);
+    fluid_body<3, false, true>(a, bid, nb);
+}
 // the batch walks' other form, a choice of the batch autotuner: the entries within h
 // buffered in LDS and drained with their positions loaded again (five waves per SIMD)
 extern "C" __global__ void __launch_bounds__(256, 5) k_fluid_nl_f_batch_buf(BatchArgs b) {

@@ -85,6 +85,7 @@ enum Kern {
     K_FLUID_NL_F_B_BUF,
     K_FLUID_NL_B_BUF,
     K_SOLID_ADJ_STRAIN,
+    K_FLUID_NLBW_F_B,
     K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_rowsort",
@@ -99,7 +100,8 @@ const char* kKernNames[K_COUNT] = {"k_key",     "k_scan",    "k_scatter",  "k_ro
                                    "k_fluid_adj_g", "k_fluid_adj_rec_g", "k_fluid_nlbr", "k_fluid_nlr",
                                    "k_fluid_adj_f", "k_fluid_adj_rec_f", "k_verlet_pre", "k_fluid_nl_f", "k_fluid_nlr_f",
                                    "k_spec_restore", "k_fluid_nl_f_batch", "k_strain_batch", "k_solid_batch",
-                                   "k_fluid_nl_f_batch_buf", "k_fluid_nl_batch_buf", "k_solid_adj_strain"};
+                                   "k_fluid_nl_f_batch_buf", "k_fluid_nl_batch_buf", "k_solid_adj_strain",
+                                   "k_fluid_nlbw_f_batch"};
 
 constexpr size_t kPad = 16;  // elements of slack after every per-particle array
 
@@ -3186,11 +3188,19 @@ int build_plan(arp_batch* b, float dt) {
             }
         }
         for (size_t j = 0; j < nl; j++) {
+            // a stage's list pair (build, then the fast-material walk with the same arguments)
+            // becomes one launch of the walk's size, unless the walk is the buffered form
+            bool pair = !b->walk_buf && j + 1 < nl && lists[0][j].k == K_FLUID_NLB && lists[0][j + 1].k == K_FLUID_NL_F;
+            for (size_t s = 0; pair && s < ns; s++) pair = lists[s][j].args == lists[s][j + 1].args;
+            if (pair) continue;  // (laid out with the walk, next)
+            const bool fused = !b->walk_buf && j > 0 && lists[0][j - 1].k == K_FLUID_NLB && lists[0][j].k == K_FLUID_NL_F &&
+                               std::all_of(lists.begin(), lists.end(),
+                                           [&](const std::vector<Launch>& l) { return l[j - 1].args == l[j].args; });
             uint32_t total = 0;
             for (size_t s = 0; s < ns; s++) total += lists[s][j].blocks;
             if (total == 0) continue;  // no scene takes part
             BLaunch B;
-            B.k = batch_kernel(lists[0][j].k);
+            B.k = fused ? K_FLUID_NLBW_F_B : batch_kernel(lists[0][j].k);
             if (b->walk_buf && B.k == K_FLUID_NL_F_B) B.k = K_FLUID_NL_F_B_BUF;
             if (b->walk_buf && B.k == K_FLUID_NL_B) B.k = K_FLUID_NL_B_BUF;
             if (B.k == K_COUNT) {
@@ -3417,8 +3427,8 @@ int arp_batch_tune(arp_batch* b, float dt, uint32_t trial_steps, uint32_t* chose
     }
     // the knobs: particles per row-sort workgroup, and the list walks' form (registers only,
     // four waves per SIMD; or the entries within h buffered in LDS, five waves per SIMD)
-    const uint32_t divs[] = {16, 32, 64, 128, 512};
-    constexpr int kDiv = 5, kCand = 2 * kDiv;
+    const uint32_t divs[] = {16, 32, 64, 128, 512, 2048};
+    constexpr int kDiv = 6, kCand = 2 * kDiv;
     hipEvent_t e0, e1;
     HIPCHK(L0, hipEventCreate(&e0));
     HIPCHK(L0, hipEventCreate(&e1));

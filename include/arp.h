@@ -328,7 +328,7 @@ int arp_batch_adjoint_record(arp_batch* batch, float dt, uint32_t nsteps, uint32
 int arp_batch_adjoint_backward(arp_batch* batch, const void* const* d_grad_out, void* const* d_grad_in,
                                float* const* grad_params);
 /* Autotuner: times trial_steps batch steps (after two untimed ones) for each candidate pair
- * of row-sort workgroup size (particles per workgroup: 16 .. 512) and list-walk form
+ * of row-sort workgroup size (particles per workgroup: 16 .. 2048) and list-walk form
  * (registers only, or the neighbours within h buffered in LDS), in two rounds, and keeps
  * the fastest (*chosen_div: the row-sort size); the scenes' states are restored after
  * every trial.  The results are the same for every choice. */

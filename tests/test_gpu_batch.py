@@ -131,7 +131,7 @@ def test_batch_tune_leaves_states(torch_cuda):
     t0 = [s.time for s in sims]
     div = B.tune(dt, 2)
     print("tuned row-sort workgroup size", div, flush=True)
-    assert div in (16, 32, 64, 128, 512)
+    assert div in (16, 32, 64, 128, 512, 2048)
     for k, s in enumerate(sims):
         assert _same(s.particles(), before[k])
         assert s.time == t0[k]
