@@ -54,21 +54,28 @@ __device__ __forceinline__ bool cell_of(const GridP& g, float x, float y, float 
 // Wave-aggregated atomic slot reservation: lanes with the same bin share one atomicAdd.
 // Every lane of the wave must call it (valid lanes reserve a slot).
 __device__ __forceinline__ uint32_t agg_rank(uint32_t* counters, uint32_t bin, bool valid) {
+    // the lanes of a wave are grouped by bin (a ballot per distinct bin, no memory access),
+    // then every group's first lane reserves the group's slots with one atomic, all groups
+    // at once: one round trip per wave, where reserving group after group waited for each
     unsigned long long active = __ballot(valid);
     const uint32_t lane = __lane_id();
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint32_t rank = 0;
+    uint32_t leader = lane, within = 0, count = 0;
     while (active) {
-        int leader = __ffsll(active) - 1;
-        uint32_t lb = __shfl(bin, leader);
-        unsigned long long m = __ballot(bin == lb) & active;
-        uint32_t base = 0;
-        if (lane == (uint32_t)leader) base = atomicAdd(&counters[lb], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if ((m >> lane) & 1ull) rank = base + (uint32_t)__popcll(m & lt);
+        const int l = __ffsll(active) - 1;
+        const uint32_t lb = __shfl(bin, l);
+        const unsigned long long m = __ballot(bin == lb) & active;
+        if ((m >> lane) & 1ull) {
+            leader = (uint32_t)l;
+            within = (uint32_t)__popcll(m & lt);
+        }
+        if (lane == (uint32_t)l) count = (uint32_t)__popcll(m);
         active &= ~m;
     }
-    return rank;
+    uint32_t base = 0;
+    if (valid && leader == lane) base = atomicAdd(&counters[bin], count);
+    base = __shfl(base, (int)leader);
+    return base + within;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {

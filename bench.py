@@ -427,7 +427,7 @@ def main():
     log("particles %d boundary %d" % (sc.n, sc.nb))
     sim = FluidSim(sc, integrator=args.integrator, timestep=dt)
     sim.set_neighbor_skin(args.skin)
-    if args.short_skin > 0:
+    if 0 < args.short_skin < args.skin:  # (two levels need a long list: --skin 0 searches at every stage)
         sim.set_neighbor_short_skin(args.short_skin)
     if args.max_age is not None:
         sim.set_neighbor_max_age(args.max_age)
@@ -570,7 +570,7 @@ def main():
             "alg_bytes_per_launch": bytes_launch,
         },
         "kernels": kern,
-        "neighbor_lists": {"skin": args.skin, "short_skin": args.short_skin, "stage_builds": nl1[0] - nl0[0],
+        "neighbor_lists": {"skin": args.skin, "short_skin": args.short_skin if 0 < args.short_skin < args.skin else 0.0, "stage_builds": nl1[0] - nl0[0],
                            "stage_reuses": nl1[1] - nl0[1], "short_list_refreshes": rf1 - rf0,
                            # timed steps issued as list walks alone, and those run again
                            "speculative_steps": sp1[0] - sp0[0], "speculation_aborts": sp1[1] - sp0[1]},
