@@ -311,7 +311,7 @@ def contact_loss(sc):
     return loss
 
 
-def time_ensemble(nscenes, steps, dt):
+def time_ensemble(nscenes, steps, dt, lists=None):
     """Independent fluid scenes (log-uniform 2,048-65,536 particles, alpha uniform in
     [0.01, 0.1], seeded) stepped as one batch (graph replay; list skins, row-sort workgroup
     size and list-walk form autotuned), against the same scenes stepped one after the
@@ -327,7 +327,13 @@ def time_ensemble(nscenes, steps, dt):
     sims = [FluidSim(sc, integrator="rk2", timestep=dt) for sc in scenes]
     npart = sum(sc.n for sc in scenes)
     B = Batch(sims, graph=True)
-    skin, short_skin = B.tune_lists(dt, 40)
+    if lists is None:
+        skin, short_skin = B.tune_lists(dt, 40)
+    else:  # fixed list skins (reproducible profiles)
+        skin, short_skin = lists
+        for s in sims:
+            s.set_neighbor_skin(skin)
+            s.set_neighbor_short_skin(short_skin)
     div = B.tune(dt, 4)
     _, walk_buffered = B.tuning()
     B.step(dt, 2)
@@ -382,6 +388,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256,
                     help="scenes of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--ensemble-lists", default="auto",
+                    help="list skins of the ensemble: auto (the list tuner) or SKIN,SHORT_SKIN")
     # (the list settings measured fastest on this workload: skins 0.2-0.5 with short skins
     # 0.03-0.06 gave 1.10e10-1.16e10 particle-steps/s, one level at 0.2 9.8e9)
     ap.add_argument("--skin", type=float, default=0.4,
@@ -529,7 +537,8 @@ def main():
         solid = {"cantilever_3d": time_solid_config("cantilever", c3, "rk4", 1e-5, 200, 200, 20, cantilever_loss(c3)),
                  "contact_3d": time_solid_config("contact", c4, "rk2", 2e-5, 1000, 200, 50, contact_loss(c4))}
     if args.ensemble > 0:
-        ens = time_ensemble(args.ensemble, args.ensemble_steps, dt)
+        lists = None if args.ensemble_lists == "auto" else tuple(float(v) for v in args.ensemble_lists.split(","))
+        ens = time_ensemble(args.ensemble, args.ensemble_steps, dt, lists)
     value = sc.n * args.steps / sec
     f = kern.get(fluid_kernel)
     bytes_launch = B_ALG_FLUID * sc.n
