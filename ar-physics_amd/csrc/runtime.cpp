@@ -2103,11 +2103,12 @@ int arp_set_integrator(arp_ctx* c, const ArpTableau* t, float timestep, uint32_t
 // rebuild chain.  The result is the one of issuing the chain at every stage.  The call
 // returns once its steps have run, so calls of fewer than kSpecMin steps (one frame at a
 // time, say) stay asynchronous and issue the chain.  The host waits on an event after every
-// kSpecE steps (an event between two kernels idles the GPU for about 6 us); a snapshot slot
-// is overwritten only by a later step, which returns at once when an earlier one aborted,
-// so the slot of an aborted step survives until its restore.
-constexpr uint32_t kSpecW = 8;
-constexpr uint32_t kSpecE = 4;
+// kSpecE steps (an event between two kernels idles the GPU for about 6 us, so groups of
+// eight steps: one such pause per 16 launches of a two-stage method); a snapshot slot is
+// overwritten only by a later step, which returns at once when an earlier one aborted, so
+// the slot of an aborted step survives until its restore.
+constexpr uint32_t kSpecW = 16;
+constexpr uint32_t kSpecE = 8;
 constexpr uint32_t kSpecMin = 8;
 static_assert(kSpecW % kSpecE == 0 && kSpecW / kSpecE < 4, "events in flight: at most three of four");
 
