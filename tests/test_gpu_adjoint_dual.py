@@ -46,10 +46,13 @@ def _flat(y):
     return np.concatenate([y["den"][:, None], y["pos"][:, :3], y["vel"][:, :3]], 1)
 
 
-def _gpu_grad(torch, sc, integ, dt, nsteps, K, g):
+def _gpu_grad(torch, sc, integ, dt, nsteps, K, g, short_skin=0.0):
     from arp.sim import FluidSim
 
     sim = FluidSim(sc, integrator=integ, timestep=dt)
+    if short_skin:
+        sim.set_neighbor_skin(0.3)
+        sim.set_neighbor_short_skin(short_skin)
     sim.upload_state(sc.particles)
     sim.adjoint_record(dt, nsteps, K)
     d_in = torch.zeros((sc.n, 12), dtype=torch.float32, device="cuda")
@@ -85,6 +88,23 @@ def _cantilever_order(order):
     return dataclasses.replace(sc, materials=mats)
 
 
+def _fluid_large():
+    """A viscous 24^3 block (13,824 particles, jittered, 1 % density noise, random velocities
+    of 0.5 m/s per axis), so that the short lists are refreshed within the recorded steps."""
+    import dataclasses
+
+    from arp import scene as S
+
+    sc = S.small_fluid_3d(n_side=24, box=(0.2, 0.2, 0.2))
+    rng = np.random.default_rng(9)
+    p = sc.particles.copy()
+    p["den"] = (1000.0 * (1.0 + 0.01 * rng.standard_normal(len(p)))).astype(np.float32)
+    p["vel"][:, :3] = (0.5 * rng.standard_normal((len(p), 3))).astype(np.float32)
+    mats = sc.materials.copy()
+    mats["visc"][1] = np.float32(2e-3)
+    return dataclasses.replace(sc, particles=p, materials=mats, interactions=S.resolve_interactions(mats, sc.h))
+
+
 def _plastic(seed):
     """The deformed plastic block (plastic flow active at every solid, |S| > A) with its
     stiffness and damping 1e5 times larger and relaxation time 4e3 s, stepped with dt 1e-7: the
@@ -104,6 +124,9 @@ def _plastic(seed):
 CASES = {
     "fluid_rk2_replay": (lambda: _scene(True), "rk2", 5e-5, 3, 2, {1: ("bulk_visc", "visc")}),
     "fluid_rk4": (lambda: _scene(False), "rk4", 5e-5, 2, 2, {1: ("bulk_visc",)}),
+    # 13,824 particles, 12 steps with a checkpoint every 4 (two segments replayed), on
+    # two-level kept lists (skins 0.3 / 0.03 h) recorded speculatively
+    "fluid_large_rk2_lists": (_fluid_large, "rk2", 5e-5, 12, 4, {1: ("bulk_visc", "visc")}, 0.03),
     "cantilever_rk2": (_cantilever, "rk2", 1e-5, 2, 1,
                        {1: ("normal_stiffness", "shear_stiffness", "normal_damp", "shear_damp")}),
     "contact_rk2": (_contact, "rk2", 1e-5, 2, 2, {1: ("normal_stiffness", "shear_stiffness"), 2: ("bulk_visc",)}),
@@ -128,7 +151,8 @@ PAIRS = [("vel", "pos"), ("den", "vel"), ("pos", "vel"), ("vel", "den"), ("vel",
 
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_adjoint_vs_exact_derivative(torch_cuda, oracle_mod, case):
-    make, integ, dt, nsteps, K, params = CASES[case]
+    make, integ, dt, nsteps, K, params = CASES[case][:6]
+    short_skin = CASES[case][6] if len(CASES[case]) > 6 else 0.0
     sc = make()
     tab = T.tableau(integ)
     o = oracle_mod.Oracle(sc)
@@ -138,7 +162,7 @@ def test_adjoint_vs_exact_derivative(torch_cuda, oracle_mod, case):
     for gf, uf in PAIRS:
         same = gf == uf
         g = _masked(rng, sc.n, gf, 1.0, 0 if same else None)
-        gi, gp = _gpu_grad(torch_cuda, sc, integ, dt, nsteps, K, g)
+        gi, gp = _gpu_grad(torch_cuda, sc, integ, dt, nsteps, K, g, short_skin)
         for rep in range(3):  # several directions for one gradient
             u = _masked(rng, sc.n, uf, SCALE[uf], 1 if same else None)
             y, yd = o.jvp(st, _tangent(o, st, u), tab, dt, nsteps)
