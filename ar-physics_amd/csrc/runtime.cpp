@@ -3273,6 +3273,10 @@ int join_streams(arp_batch* b, bool into_batch) {
         for (size_t s = 0; s < ns; s++) {
             arp_ctx* c = b->ctx[s];
             if (c->stream == L0->stream) continue;
+            // (a context whose stream has finished its work needs no event)
+            const hipError_t q = hipStreamQuery(c->stream);
+            if (q == hipSuccess) continue;
+            if (q != hipErrorNotReady) return bfail(b, ARP_E_HIP, "stream query failed: %s", hipGetErrorString(q));
             HIPCHK(L0, hipEventRecord(b->ev[s], c->stream));
             HIPCHK(L0, hipStreamWaitEvent(L0->stream, b->ev[s], 0));
         }
