@@ -213,7 +213,7 @@ def time_adjoint(sim, sc, p_start, dt, nsteps, K):
                 k_fluid_adj_avg_us=1e3 * ms / cnt if cnt else None, grad_finite=finite)
 
 
-def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss):
+def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss, rounds=4):
     """A configuration with solids at its full size: fwd_steps plain steps from the initial
     state (timed as a whole), then forward + adjoint of adj_steps steps with a checkpoint
     every K (replaying every segment but the last) against the same steps plain (median of
@@ -250,7 +250,7 @@ def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss):
         # one untimed round (it allocates the recording and the reverse-mode buffers), then
         # the median of three, as for the fluid column
         fwds, recs, bwds = [], [], []
-        for r in range(4):
+        for r in range(max(rounds, 1)):
             sim.upload_state(sc.particles)
             fwd, _ = timed(lambda: sim.step(dt, adj_steps))
             sim.upload_state(sc.particles)
@@ -260,7 +260,7 @@ def time_solid_config(name, sc, integ, dt, fwd_steps, adj_steps, K, loss):
             g[:, [0, 4, 5, 6, 8, 9, 10]] = torch.from_numpy(g7.astype(np.float32)).cuda()
             gin = torch.zeros_like(g)
             bwd, gp = timed(lambda: sim.adjoint_backward(g, gin, want_params=True))
-            if r:
+            if r or rounds <= 1:  # (a single round is timed, allocation included)
                 fwds.append(fwd)
                 recs.append(rec)
                 bwds.append(bwd)
@@ -403,6 +403,12 @@ def main():
                     help="also run the cantilever (640,000 solids: 200 RK4 steps; forward + adjoint with a "
                          "checkpoint every 20) and the fluid-ball contact (327,680 particles: 1000 RK2 steps; "
                          "forward + adjoint of 200 steps, a checkpoint every 50) at full size (0: skip)")
+    ap.add_argument("--solid-only", default="all", choices=["all", "cantilever", "contact"],
+                    help="which solid configurations --solid-configs runs")
+    ap.add_argument("--solid-steps", type=int, default=200,
+                    help="forward + adjoint steps of the solid configurations (200 as configured; fewer for profiles)")
+    ap.add_argument("--solid-rounds", type=int, default=4,
+                    help="forward + adjoint rounds of a solid configuration (the first untimed when more than one)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="steps of the recorded rollouts whose reverse sweep is timed, with a checkpoint "
                          "every steps/4 (replays) and every steps (0: skip)")
@@ -532,10 +538,16 @@ def main():
     sim.close()
     solid = None
     if args.solid_configs:
-        c3 = S.cantilever_3d()
-        c4 = S.contact_3d()
-        solid = {"cantilever_3d": time_solid_config("cantilever", c3, "rk4", 1e-5, 200, 200, 20, cantilever_loss(c3)),
-                 "contact_3d": time_solid_config("contact", c4, "rk2", 2e-5, 1000, 200, 50, contact_loss(c4))}
+        solid = {}
+        f = args.solid_steps / 200.0  # (the configurations' step counts, scaled for short profiles)
+        if args.solid_only in ("all", "cantilever"):
+            c3 = S.cantilever_3d()
+            solid["cantilever_3d"] = time_solid_config("cantilever", c3, "rk4", 1e-5, args.solid_steps, args.solid_steps,
+                                                       max(1, int(20 * f)), cantilever_loss(c3), args.solid_rounds)
+        if args.solid_only in ("all", "contact"):
+            c4 = S.contact_3d()
+            solid["contact_3d"] = time_solid_config("contact", c4, "rk2", 2e-5, int(1000 * f), args.solid_steps,
+                                                    max(1, int(50 * f)), contact_loss(c4), args.solid_rounds)
     if args.ensemble > 0:
         lists = None if args.ensemble_lists == "auto" else tuple(float(v) for v in args.ensemble_lists.split(","))
         ens = time_ensemble(args.ensemble, args.ensemble_steps, dt, lists)
