@@ -1734,11 +1734,15 @@ __device__ __forceinline__ M4 ld_m4(const float* base, uint32_t s) {
     }
     return r;
 }
+// A and F of a solid: three rows of 12 bytes (kRows3 floats per solid), so that the gathers
+// of a neighbour's matrix move 36 bytes, not three padded 16-byte rows (the solid kernels are
+// bound by the data their gathers return)
+constexpr uint32_t kRows3 = 9;
 __device__ __forceinline__ void store_rows3(float* dst, const M4& A) {
-    float4* d = reinterpret_cast<float4*>(dst);
-    d[0] = make_float4(A.m[0], A.m[4], A.m[8], 0.f);
-    d[1] = make_float4(A.m[1], A.m[5], A.m[9], 0.f);
-    d[2] = make_float4(A.m[2], A.m[6], A.m[10], 0.f);
+    float3* d = reinterpret_cast<float3*>(dst);
+    d[0] = make_float3(A.m[0], A.m[4], A.m[8]);
+    d[1] = make_float3(A.m[1], A.m[5], A.m[9]);
+    d[2] = make_float3(A.m[2], A.m[6], A.m[10]);
 }
 
 __device__ __forceinline__ bool in_grid(const GridP& g, float4 p) {
@@ -1899,8 +1903,8 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
     const M4 S = m4_add(m4_hooke(E, ks * M.lambda, ks * M.mu, dim), m4_hooke(Ed, M.lambda_d, M.mu_d, dim));
     const M4 P = m4_mul(F4, m4_trans(S));
     const M4 A = m4_mul(P, ld_m4(a.Linv, s));
-    store_rows3(a.A + 12 * (size_t)s, A);
-    store_rows3(a.Fm + 12 * (size_t)s, F4);
+    store_rows3(a.A + kRows3 * (size_t)s, A);
+    store_rows3(a.Fm + kRows3 * (size_t)s, F4);
     M4 rate = m4_zero();
     if (M.plastic) {
         // eq_plastic_strain_sq carries the integer factor (dim-1)/dim == 0 (solid.rs:212-215)
@@ -1953,22 +1957,23 @@ __device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uin
         const float4 xi = X[i], vi = XV[i];
         const float m1 = M.mass, d1 = xi.w, V1 = m1 / d1;
         const float4 Xs = REF[s];
-        const float4* A1 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s;
-        const float4* F1 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s;
+        const float3* A1 = reinterpret_cast<const float3*>(a.A) + 3 * (size_t)s;
+        const float3* F1 = reinterpret_cast<const float3*>(a.Fm) + 3 * (size_t)s;
         const bool valid_i = a.sflag[i] == 0u;
         uint32_t ci[3] = {0, 0, 0};
         cell_of(a.grid, xi.x, xi.y, xi.z, ci[0], ci[1], ci[2]);
         float fx = 0.f, fy = 0.f, fz = 0.f;
         const float4* EG = reinterpret_cast<const float4*>(a.eG);
-        const float4* AA = reinterpret_cast<const float4*>(a.A);
-        const float4* FF = reinterpret_cast<const float4*>(a.Fm);
+        const float3* AA = reinterpret_cast<const float3*>(a.A);
+        const float3* FF = reinterpret_cast<const float3*>(a.Fm);
         const uint32_t q0 = a.eOff[s], q1 = a.eOff[s + 1];
         // ARP_EL_GROUP entries at a time: each level of the gathers (entry -> storage index
         // -> state) is issued for the whole group before the next; the group's pairs are then
         // summed in entry order (a tail entry reloads the last one, unused)
         for (uint32_t qb = q0; qb < q1; qb += ARP_EL_GROUP) {
             uint32_t sq[ARP_EL_GROUP], jq[ARP_EL_GROUP];
-            float4 gq[ARP_EL_GROUP], xq[ARP_EL_GROUP], Aq[ARP_EL_GROUP][3];
+            float4 gq[ARP_EL_GROUP], xq[ARP_EL_GROUP];
+            float3 Aq[ARP_EL_GROUP][3];
 #pragma unroll
             for (int k = 0; k < ARP_EL_GROUP; k++) {
                 const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
@@ -2010,7 +2015,7 @@ __device__ __forceinline__ void solid_body(const SolidArgs& a, uint32_t bid, uin
                         const float dX[3] = {X2.x - Xs.x, X2.y - Xs.y, X2.z - Xs.z};
                         const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
                         const float l = sqrtf(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
-                        const float4* F2 = FF + 3 * (size_t)s2;
+                        const float3* F2 = FF + 3 * (size_t)s2;
                         float err[3];
                         for (int r = 0; r < 3; r++) {
                             const float4 fa = make_float4(0.5f * (F1[r].x + F2[r].x), 0.5f * (F1[r].y + F2[r].y),
@@ -2904,13 +2909,13 @@ __device__ __forceinline__ float3 contact_vjp(float ax, float ay, float az, floa
 struct HgOut {
     float f[3], gdx[3], gfa[9], gd1, gd2;
 };
-__device__ __forceinline__ void hourglass_vjp(const float4* F1, const float4* F2, const float dX[3], const float dx[3],
+__device__ __forceinline__ void hourglass_vjp(const float3* F1, const float3* F2, const float dX[3], const float dx[3],
                                               float m1, float m2, float d1, float d2, float W, float h, float ax,
                                               float ay, float az, HgOut& o) {
     const float EPS = 0.1f;  // EPSILON of the reference shaders (fluid.rs:9, solid.rs:9)
     float fa[9];
     for (int r = 0; r < 3; r++) {
-        const float4 x = F1[r], y = F2[r];
+        const float3 x = F1[r], y = F2[r];
         fa[3 * r + 0] = 0.5f * (x.x + y.x);
         fa[3 * r + 1] = 0.5f * (x.y + y.y);
         fa[3 * r + 2] = 0.5f * (x.z + y.z);
@@ -2972,21 +2977,22 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     const float4 wpi = ld_w(aa.Wp, i, ws), wvi = ld_w(aa.Wv, i, ws);
     const float ai[3] = {wvi.x / d1, wvi.y / d1, wvi.z / d1};
     const float zm = dim > 2 ? 1.0f : 0.0f;  // the elastic force drops z below 3-D
-    const float4* A1 = reinterpret_cast<const float4*>(a.A) + 3 * (size_t)s;
-    const float4* F1 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s;
+    const float3* A1 = reinterpret_cast<const float3*>(a.A) + 3 * (size_t)s;
+    const float3* F1 = reinterpret_cast<const float3*>(a.Fm) + 3 * (size_t)s;
     const float4 Xs = REF[s];
     const bool valid_i = a.sflag[i] == 0u;
     uint32_t ci[3] = {0, 0, 0};
     cell_of(a.grid, xi.x, xi.y, xi.z, ci[0], ci[1], ci[2]);
     float Ab[9] = {0}, Fhb[9] = {0}, xb[3] = {0, 0, 0}, rb = 0.0f, fs[3] = {0, 0, 0};
     const float4* EG = reinterpret_cast<const float4*>(a.eG);
-    const float4* AA = reinterpret_cast<const float4*>(a.A);
+    const float3* AA = reinterpret_cast<const float3*>(a.A);
     const float4* WVv = reinterpret_cast<const float4*>(aa.Wv);
     const uint32_t q0 = a.eOff[s], q1 = a.eOff[s + 1];
     // ARP_ADJ_GROUP entries at a time, each gather level issued for the group (as k_solid)
     for (uint32_t qb = q0; qb < q1; qb += ARP_ADJ_GROUP) {
         uint32_t sq[ARP_ADJ_GROUP], jq[ARP_ADJ_GROUP];
-        float4 gq[ARP_ADJ_GROUP], xq[ARP_ADJ_GROUP], wq[ARP_ADJ_GROUP], Aq[ARP_ADJ_GROUP][3];
+        float4 gq[ARP_ADJ_GROUP], xq[ARP_ADJ_GROUP], wq[ARP_ADJ_GROUP];
+        float3 Aq[ARP_ADJ_GROUP][3];
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
             const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
@@ -3011,11 +3017,11 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
         const uint32_t s2 = sq[k], j = jq[k];
         const float4 g = gq[k], xj = xq[k];
         const float dj = xj.w, V2 = m1 / dj, c = -V1 * V2;
-        const float4* A2 = Aq[k];
+        const float3* A2 = Aq[k];
         const float gg[3] = {g.x, g.y, g.z};
         float e[3];
         for (int r = 0; r < 3; r++) {
-            const float4 x = A1[r], y = A2[r];
+            const float3 x = A1[r], y = A2[r];
             e[r] = (x.x + y.x) * gg[0] + (x.y + y.y) * gg[1] + (x.z + y.z) * gg[2];
         }
         fs[0] += c * e[0];
@@ -3038,7 +3044,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
             for (int d = 0; d < 3; d++) adj &= (ci[d] + 1 >= cj[d]) && (cj[d] + 1 >= ci[d]);
             if (adj) {
                 const float4 X2 = REF[s2];
-                const float4* F2 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s2;
+                const float3* F2 = reinterpret_cast<const float3*>(a.Fm) + 3 * (size_t)s2;
                 const float dX[3] = {X2.x - Xs.x, X2.y - Xs.y, X2.z - Xs.z};
                 const float dx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
                 HgOut o = {};
@@ -3363,10 +3369,10 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj_strain(SolidAdjArg
         Edb.m[4 * q] = y.x, Edb.m[4 * q + 1] = y.y, Edb.m[4 * q + 2] = y.z, Edb.m[4 * q + 3] = y.w;
     }
     const float fd9[9] = {eb[8].x, eb[8].y, eb[8].z, eb[8].w, eb[9].x, eb[9].y, eb[9].z, eb[9].w, eb[10].x};
-    const float4* F1 = reinterpret_cast<const float4*>(a.Fm) + 3 * (size_t)s;
+    const float3* F1 = reinterpret_cast<const float3*>(a.Fm) + 3 * (size_t)s;
     M4 F4 = m4_zero(), Fd4 = m4_zero();
     for (int r = 0; r < 3; r++) {
-        const float4 row = F1[r];
+        const float3 row = F1[r];
         F4.m[r] = row.x, F4.m[4 + r] = row.y, F4.m[8 + r] = row.z;
         for (int c = 0; c < 3; c++) Fd4.m[4 * c + r] = fd9[3 * r + c];
     }

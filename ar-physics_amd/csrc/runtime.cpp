@@ -915,8 +915,8 @@ int alloc_state(arp_ctx* c) {
         }
         for (DBuf* b : {&c->Yeps, &c->keps, &c->ps_eps})
             if ((rc = dalloc(c, *b, 64 * ns))) return rc;
-        for (DBuf* b : {&c->A_d, &c->Fm_d})
-            if ((rc = dalloc(c, *b, 48 * ns))) return rc;
+        for (DBuf* b : {&c->A_d, &c->Fm_d})  // three rows of three floats per solid
+            if ((rc = dalloc(c, *b, 36 * ns))) return rc;
         // per grid row: the self-contact row test's displacement box (two float4)
         if ((rc = dalloc(c, c->rowBox, 32 * (size_t)std::max<uint32_t>(c->grid.nrows, 1)))) return rc;
     }
@@ -1393,7 +1393,7 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
         g.full = false;
         if (c->solids_active) {
             const size_t ns = std::max<uint32_t>(c->ns, 1);
-            if ((rc = dalloc(c, R.sA[s], 48 * ns)) || (rc = dalloc(c, R.sF[s], 48 * ns)) ||
+            if ((rc = dalloc(c, R.sA[s], 36 * ns)) || (rc = dalloc(c, R.sF[s], 36 * ns)) ||
                 (rc = dalloc(c, R.sBox[s], 32 * nr)))
                 return rc;
             if (any_plastic(c) && (rc = dalloc(c, R.sEps[s], 64 * ns))) return rc;
