@@ -369,6 +369,8 @@ struct SolidArgs {
     const ARP_G uint32_t* sOff;     // strain list: ns+1 offsets
     const ARP_G uint32_t* sPid;     //   neighbour particle id
     const ARP_G float* sG;          //   float4 (L^-1 gradW(dX) xyz, V2)
+    const ARP_G float* sG2;         //   float4 (L_s2^-1 gradW(X_s - X_s2) xyz, 0): the weight of this
+                                    //   solid in neighbour s2's F (the reverse pass; static)
     const ARP_G uint32_t* eOff;     // elastic-force list (same material): ns+1 offsets
     const ARP_G uint32_t* ePid;     //   neighbour particle id
     const ARP_G uint32_t* eSid;     //   neighbour solid id
@@ -381,8 +383,9 @@ struct SolidArgs {
     // the step (its rate is zero): its strain reads it here and its RK update of the plastic
     // strain is skipped (null: every solid updates it)
     const ARP_G float* epsY;
-    ARP_G float* A;                 // per solid: (P L^-1) rows, 3 float4
-    ARP_G float* Fm;                // per solid: F rows, 3 float4
+    ARP_G float* A;                 // per solid: (P L^-1), three rows of three floats
+    ARP_G float* Fm;                // per solid: F, three rows of three floats
+    ARP_G float* Fd;                // per solid: F', three rows of three floats (recorded stages; may be null)
     ARP_G float* keps;              // per solid: plastic strain rate (16)
     // current grid, for the hourglass stencil test and self contact
     const ARP_G uint32_t* perm;

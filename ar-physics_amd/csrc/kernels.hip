@@ -1905,6 +1905,7 @@ __device__ __forceinline__ void strain_body(const SolidArgs& a, uint32_t bid, ui
     const M4 A = m4_mul(P, ld_m4(a.Linv, s));
     store_rows3(a.A + kRows3 * (size_t)s, A);
     store_rows3(a.Fm + kRows3 * (size_t)s, F4);
+    if (a.Fd) store_rows3(a.Fd + kRows3 * (size_t)s, Fd4);  // (a recorded stage: for the reverse pass)
     M4 rate = m4_zero();
     if (M.plastic) {
         // eq_plastic_strain_sq carries the integer factor (dim-1)/dim == 0 (solid.rs:212-215)
@@ -3131,7 +3132,18 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj1(SolidAdjArgs aa) 
     const float4 vi = XVv[i];
     float F[9] = {0}, Fd[9] = {0};
     const float4* G = reinterpret_cast<const float4*>(a.sG);
-    const uint32_t p0 = a.sOff[s], p1 = a.sOff[s + 1];
+    // F and F' of the recorded stage as its k_strain formed them (the same sums in the same
+    // order, so the same values); without them, formed again from the strain neighbours
+    const uint32_t p0 = a.sOff[s], p1 = a.Fd ? p0 : a.sOff[s + 1];
+    if (a.Fd) {
+        const float3* Fr = reinterpret_cast<const float3*>(a.Fm) + 3 * (size_t)s;
+        const float3* Fdr = reinterpret_cast<const float3*>(a.Fd) + 3 * (size_t)s;
+        for (int r = 0; r < 3; r++) {
+            const float3 f = Fr[r], fd = Fdr[r];
+            F[3 * r] = f.x, F[3 * r + 1] = f.y, F[3 * r + 2] = f.z;
+            Fd[3 * r] = fd.x, Fd[3 * r + 1] = fd.y, Fd[3 * r + 2] = fd.z;
+        }
+    }
     for (uint32_t qb = p0; qb < p1; qb += ARP_ST_GROUP) {  // (grouped as in k_strain)
         uint32_t jj[ARP_ST_GROUP];
         float4 gq[ARP_ST_GROUP], xq[ARP_ST_GROUP], vq[ARP_ST_GROUP];
@@ -3399,12 +3411,10 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
     const uint32_t i = a.inv[p];
     if (!(a.mk[i] & kMkElastic)) return;
     const DMat& M = a.mats[a.mk[i] & kMatMask];
-    const float4* REF = reinterpret_cast<const float4*>(a.ref);
-    const float h = a.phys.h, hh = h * h, kn = a.phys.k;
     const float Vp = M.mass / M.start_den;  // this particle's volume in its neighbours' F
     const float* fb = aa.Fbar + kFbarStride * (size_t)s;
-    const float4 Xs = REF[s];
     float xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
+    const float4* G2 = reinterpret_cast<const float4*>(a.sG2);
     const float4* G = reinterpret_cast<const float4*>(a.sG);
     float f1[18];  // this solid's F, F' cotangents
     {
@@ -3420,21 +3430,18 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
     const uint32_t q0 = a.sOff[s], q1 = a.sOff[s + 1];
     for (uint32_t qb = q0; qb < q1; qb += ARP_ADJ_GROUP) {
         uint32_t sq[ARP_ADJ_GROUP];
-        float4 gq[ARP_ADJ_GROUP], Xq[ARP_ADJ_GROUP], Lq[ARP_ADJ_GROUP][3], Fq[ARP_ADJ_GROUP][5];
+        float4 gq[ARP_ADJ_GROUP], g2q[ARP_ADJ_GROUP], Fq[ARP_ADJ_GROUP][5];
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
             const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
             sq[k] = a.sPid[q];
             gq[k] = G[q];
+            g2q[k] = G2[q];
         }
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) sq[k] = a.sidByPid[sq[k]];
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
-            Xq[k] = REF[sq[k]];
-            const float4* L4 = reinterpret_cast<const float4*>(a.Linv) + 4 * (size_t)sq[k];
-#pragma unroll
-            for (int r = 0; r < 3; r++) Lq[k][r] = L4[r];
             const float4* F4 = reinterpret_cast<const float4*>(aa.Fbar + kFbarStride * (size_t)sq[k]);
 #pragma unroll
             for (int r = 0; r < 5; r++) Fq[k][r] = F4[r];
@@ -3454,17 +3461,8 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
                 vb[r] = live ? vb[r] - td : vb[r];
             }
             // the neighbour's F holds this particle with weight Vp L_s2^-1 gradW(X_s - X_s2)
-            const float4 X2 = Xq[k];
-            const float r4[4] = {Xs.x - X2.x, Xs.y - X2.y, Xs.z - X2.z, Xs.w - X2.w};
-            const float rr = r4[0] * r4[0] + r4[1] * r4[1] + r4[2] * r4[2] + r4[3] * r4[3];
-            const float diff = fmaxf(hh - rr, 0.0f);
-            const float gs = -6.0f * kn * diff * diff;
-            const float gw[3] = {gs * r4[0], gs * r4[1], gs * r4[2]};
-            // columns 0..2 of L_s2^-1 (column-major), rows 0..2
-            const float L2[12] = {Lq[k][0].x, Lq[k][0].y, Lq[k][0].z, Lq[k][0].w, Lq[k][1].x, Lq[k][1].y,
-                                  Lq[k][1].z, Lq[k][1].w, Lq[k][2].x, Lq[k][2].y, Lq[k][2].z, Lq[k][2].w};
-            float g2[3];
-            for (int r = 0; r < 3; r++) g2[r] = L2[r] * gw[0] + L2[4 + r] * gw[1] + L2[8 + r] * gw[2];
+            // (static: formed at upload)
+            const float g2[3] = {g2q[k].x, g2q[k].y, g2q[k].z};
             const float fb2[20] = {Fq[k][0].x, Fq[k][0].y, Fq[k][0].z, Fq[k][0].w, Fq[k][1].x, Fq[k][1].y, Fq[k][1].z,
                                    Fq[k][1].w, Fq[k][2].x, Fq[k][2].y, Fq[k][2].z, Fq[k][2].w, Fq[k][3].x, Fq[k][3].y,
                                    Fq[k][3].z, Fq[k][3].w, Fq[k][4].x, Fq[k][4].y, Fq[k][4].z, Fq[k][4].w};

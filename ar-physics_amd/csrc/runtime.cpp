@@ -146,7 +146,7 @@ struct RecStep {
     DBuf perm[ARP_MAX_STAGES], sflag[ARP_MAX_STAGES], rowOff[ARP_MAX_STAGES], rowStamp[ARP_MAX_STAGES];
     // solid stages: the strain kernel's A = P L^-1, F and self-contact row boxes, written by
     // the recorded stage itself (so that the reverse stage does not recompute them)
-    DBuf sA[ARP_MAX_STAGES], sF[ARP_MAX_STAGES], sBox[ARP_MAX_STAGES];
+    DBuf sA[ARP_MAX_STAGES], sF[ARP_MAX_STAGES], sFd[ARP_MAX_STAGES], sBox[ARP_MAX_STAGES];
     DBuf sEps[ARP_MAX_STAGES];  // each stage's input plastic strain (scenes with plastic flow)
     GridSlot slot[ARP_MAX_STAGES];
 };
@@ -218,7 +218,7 @@ struct arp_ctx {
     DBuf inv, sflag, KF[2];
     // solids: static reference-neighbour data and plastic-strain registers (solid-id order)
     bool solids_active = false;
-    DBuf part_d, ref_d, Linv_d, sOff_d, sPid_d, sG_d, eOff_d, ePid_d, eSid_d, eG_d;
+    DBuf part_d, ref_d, Linv_d, sOff_d, sPid_d, sG_d, sG2_d, eOff_d, ePid_d, eSid_d, eG_d;
     DBuf Yeps, Xeps[2], AccE, Ke[kMaxK], keps, A_d, Fm_d, ps_eps, rowBox;
     // per-stage API buffers
     DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys, saos_tmp;
@@ -308,7 +308,7 @@ struct arp_ctx {
             bufs.push_back(&K[k][1]);
             bufs.push_back(&Ke[k]);
         }
-        for (DBuf* b : {&sflag, &KF[0], &KF[1], &part_d, &ref_d, &Linv_d, &sOff_d, &sPid_d, &sG_d, &eOff_d,
+        for (DBuf* b : {&sflag, &KF[0], &KF[1], &part_d, &ref_d, &Linv_d, &sOff_d, &sPid_d, &sG_d, &sG2_d, &eOff_d,
                         &ePid_d, &eSid_d, &eG_d, &Yeps, &Xeps[0], &Xeps[1], &AccE, &keps, &A_d, &Fm_d, &ps_eps, &rowBox,
                         &saos_tmp})
             bufs.push_back(b);
@@ -603,6 +603,21 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
     }
     sOff[ns] = (uint32_t)sPid.size();
     eOff[ns] = (uint32_t)ePid.size();
+    // the reverse pass's weight of solid t in the F of each strain neighbour s2:
+    // L_s2^-1 gradW(X_t - X_s2), static like the reference positions and L^-1 (the operations
+    // of the kernel's gradient, in the same order)
+    std::vector<float> sG2(sG.size(), 0.0f);
+    for (uint32_t t = 0; t < ns; t++)
+        for (uint32_t q = sOff[t]; q < sOff[t + 1]; q++) {
+            const uint32_t sid2 = particles[sPid[q]].solid_id;
+            if (sid2 >= ns) continue;
+            const uint32_t s2 = rank[sid2];
+            float r4[4], gw[4];
+            for (int j = 0; j < 4; j++) r4[j] = ref[4 * (size_t)t + j] - ref[4 * (size_t)s2 + j];
+            gradw(r4, gw);
+            const float* L2 = &Linv[16 * (size_t)s2];
+            for (int r = 0; r < 3; r++) sG2[4 * (size_t)q + r] = L2[r] * gw[0] + L2[4 + r] * gw[1] + L2[8 + r] * gw[2];
+        }
     int rc;
     if ((rc = upload(c, c->part_d, part.data(), 4 * part.size()))) return rc;
     if ((rc = upload(c, c->ref_d, ref.data(), 4 * ref.size()))) return rc;
@@ -610,6 +625,7 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
     if ((rc = upload(c, c->sOff_d, sOff.data(), 4 * sOff.size()))) return rc;
     if ((rc = upload(c, c->sPid_d, sPid.data(), 4 * sPid.size()))) return rc;
     if ((rc = upload(c, c->sG_d, sG.data(), 4 * sG.size()))) return rc;
+    if ((rc = upload(c, c->sG2_d, sG2.data(), 4 * sG2.size()))) return rc;
     if ((rc = upload(c, c->eOff_d, eOff.data(), 4 * eOff.size()))) return rc;
     if ((rc = upload(c, c->ePid_d, ePid.data(), 4 * ePid.size()))) return rc;
     if ((rc = upload(c, c->eSid_d, eSid.data(), 4 * eSid.size()))) return rc;
@@ -989,6 +1005,7 @@ SolidArgs solid_base(arp_ctx* c) {
     a.sOff = P<uint32_t>(c->sOff_d);
     a.sPid = P<uint32_t>(c->sPid_d);
     a.sG = P<float>(c->sG_d);
+    a.sG2 = P<float>(c->sG2_d);
     a.eOff = P<uint32_t>(c->eOff_d);
     a.ePid = P<uint32_t>(c->ePid_d);
     a.eSid = P<uint32_t>(c->eSid_d);
@@ -1322,6 +1339,7 @@ int step_once(arp_ctx* c, float dt, RecStep* rec = nullptr, const SpecArg* sp = 
             if (rec) {  // the stage's strain results stay in the record
                 sa.A = P<float>(rec->sA[s]);
                 sa.Fm = P<float>(rec->sF[s]);
+                sa.Fd = P<float>(rec->sFd[s]);
                 sa.rowBox = P<float>(rec->sBox[s]);
             }
             sa.mode = 0;
@@ -1394,7 +1412,7 @@ int alloc_rec(arp_ctx* c, RecStep& R, int S) {
         if (c->solids_active) {
             const size_t ns = std::max<uint32_t>(c->ns, 1);
             if ((rc = dalloc(c, R.sA[s], 36 * ns)) || (rc = dalloc(c, R.sF[s], 36 * ns)) ||
-                (rc = dalloc(c, R.sBox[s], 32 * nr)))
+                (rc = dalloc(c, R.sFd[s], 36 * ns)) || (rc = dalloc(c, R.sBox[s], 32 * nr)))
                 return rc;
             if (any_plastic(c) && (rc = dalloc(c, R.sEps[s], 64 * ns))) return rc;
         }
@@ -1417,7 +1435,7 @@ void trim_rec(arp_ctx* c, size_t nrec, size_t nck) {
         for (int s = 0; s < ARP_MAX_STAGES; s++) free_buf(R.nbList[s]), free_buf(R.nbCount[s]);
         for (int s = 0; s < ARP_MAX_STAGES; s++)
             for (DBuf* b : {&R.Xp[s], &R.Xv[s], &R.perm[s], &R.sflag[s], &R.rowOff[s], &R.rowStamp[s], &R.sA[s],
-                            &R.sF[s], &R.sBox[s], &R.sEps[s]})
+                            &R.sF[s], &R.sFd[s], &R.sBox[s], &R.sEps[s]})
                 free_buf(*b);
     }
     for (size_t i = nck; i < c->ck.size(); i++)
@@ -1612,6 +1630,7 @@ int reverse_step(arp_ctx* c, RecStep& R, float dt) {
             // the stage's A, F and row boxes as its recorded forward pass left them
             sa.A = P<float>(R.sA[s]);
             sa.Fm = P<float>(R.sF[s]);
+            sa.Fd = P<float>(R.sFd[s]);
             sa.rowBox = P<float>(R.sBox[s]);
             SolidAdjArgs sj{};
             sj.f = sa;
