@@ -369,6 +369,7 @@ struct SolidArgs {
     const ARP_G uint32_t* sOff;     // strain list: ns+1 offsets
     const ARP_G uint32_t* sPid;     //   neighbour particle id
     const ARP_G float* sG;          //   float4 (L^-1 gradW(dX) xyz, V2)
+    const ARP_G uint32_t* sSid;     //   internal index of the entry's solid (static)
     const ARP_G float* sG2;         //   float4 (L_s2^-1 gradW(X_s - X_s2) xyz, 0): the weight of this
                                     //   solid in neighbour s2's F (the reverse pass; static)
     const ARP_G uint32_t* eOff;     // elastic-force list (same material): ns+1 offsets

@@ -3434,12 +3434,10 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
             const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
-            sq[k] = a.sPid[q];
+            sq[k] = a.sSid[q];  // (static: the entry's solid, no particle -> solid lookup)
             gq[k] = G[q];
             g2q[k] = G2[q];
         }
-#pragma unroll
-        for (int k = 0; k < ARP_ADJ_GROUP; k++) sq[k] = a.sidByPid[sq[k]];
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
             const float4* F4 = reinterpret_cast<const float4*>(aa.Fbar + kFbarStride * (size_t)sq[k]);

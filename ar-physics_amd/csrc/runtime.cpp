@@ -218,7 +218,7 @@ struct arp_ctx {
     DBuf inv, sflag, KF[2];
     // solids: static reference-neighbour data and plastic-strain registers (solid-id order)
     bool solids_active = false;
-    DBuf part_d, ref_d, Linv_d, sOff_d, sPid_d, sG_d, sG2_d, eOff_d, ePid_d, eSid_d, eG_d;
+    DBuf part_d, ref_d, Linv_d, sOff_d, sPid_d, sSid_d, sG_d, sG2_d, eOff_d, ePid_d, eSid_d, eG_d;
     DBuf Yeps, Xeps[2], AccE, Ke[kMaxK], keps, A_d, Fm_d, ps_eps, rowBox;
     // per-stage API buffers
     DBuf ps_mk[2], ps_pid[2], aos_tmp, ps_keys, saos_tmp;
@@ -308,7 +308,7 @@ struct arp_ctx {
             bufs.push_back(&K[k][1]);
             bufs.push_back(&Ke[k]);
         }
-        for (DBuf* b : {&sflag, &KF[0], &KF[1], &part_d, &ref_d, &Linv_d, &sOff_d, &sPid_d, &sG_d, &sG2_d, &eOff_d,
+        for (DBuf* b : {&sflag, &KF[0], &KF[1], &part_d, &ref_d, &Linv_d, &sOff_d, &sPid_d, &sSid_d, &sG_d, &sG2_d, &eOff_d,
                         &ePid_d, &eSid_d, &eG_d, &Yeps, &Xeps[0], &Xeps[1], &AccE, &keps, &A_d, &Fm_d, &ps_eps, &rowBox,
                         &saos_tmp})
             bufs.push_back(b);
@@ -607,9 +607,11 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
     // L_s2^-1 gradW(X_t - X_s2), static like the reference positions and L^-1 (the operations
     // of the kernel's gradient, in the same order)
     std::vector<float> sG2(sG.size(), 0.0f);
+    std::vector<uint32_t> sSid(sPid.size());  // (an entry without a solid: its own, weight 0)
     for (uint32_t t = 0; t < ns; t++)
         for (uint32_t q = sOff[t]; q < sOff[t + 1]; q++) {
             const uint32_t sid2 = particles[sPid[q]].solid_id;
+            sSid[q] = sid2 < ns ? rank[sid2] : t;
             if (sid2 >= ns) continue;
             const uint32_t s2 = rank[sid2];
             float r4[4], gw[4];
@@ -626,6 +628,7 @@ int build_solid_lists(arp_ctx* c, const ArpParticle* particles, uint32_t n, cons
     if ((rc = upload(c, c->sPid_d, sPid.data(), 4 * sPid.size()))) return rc;
     if ((rc = upload(c, c->sG_d, sG.data(), 4 * sG.size()))) return rc;
     if ((rc = upload(c, c->sG2_d, sG2.data(), 4 * sG2.size()))) return rc;
+    if ((rc = upload(c, c->sSid_d, sSid.data(), 4 * sSid.size()))) return rc;
     if ((rc = upload(c, c->eOff_d, eOff.data(), 4 * eOff.size()))) return rc;
     if ((rc = upload(c, c->ePid_d, ePid.data(), 4 * ePid.size()))) return rc;
     if ((rc = upload(c, c->eSid_d, eSid.data(), 4 * eSid.size()))) return rc;
@@ -1006,6 +1009,7 @@ SolidArgs solid_base(arp_ctx* c) {
     a.sPid = P<uint32_t>(c->sPid_d);
     a.sG = P<float>(c->sG_d);
     a.sG2 = P<float>(c->sG2_d);
+    a.sSid = P<uint32_t>(c->sSid_d);
     a.eOff = P<uint32_t>(c->eOff_d);
     a.ePid = P<uint32_t>(c->ePid_d);
     a.eSid = P<uint32_t>(c->eSid_d);
