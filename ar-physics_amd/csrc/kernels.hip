@@ -3430,7 +3430,8 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
     const uint32_t q0 = a.sOff[s], q1 = a.sOff[s + 1];
     for (uint32_t qb = q0; qb < q1; qb += ARP_ADJ_GROUP) {
         uint32_t sq[ARP_ADJ_GROUP];
-        float4 gq[ARP_ADJ_GROUP], g2q[ARP_ADJ_GROUP], Fq[ARP_ADJ_GROUP][5];
+        float4 gq[ARP_ADJ_GROUP], g2q[ARP_ADJ_GROUP], Fq[ARP_ADJ_GROUP][4];
+        float2 Ft[ARP_ADJ_GROUP];  // (floats 16, 17 of the 18 cotangents; 18, 19 are padding)
 #pragma unroll
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
             const uint32_t q = min(qb + (uint32_t)k, q1 - 1u);
@@ -3442,7 +3443,8 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
         for (int k = 0; k < ARP_ADJ_GROUP; k++) {
             const float4* F4 = reinterpret_cast<const float4*>(aa.Fbar + kFbarStride * (size_t)sq[k]);
 #pragma unroll
-            for (int r = 0; r < 5; r++) Fq[k][r] = F4[r];
+            for (int r = 0; r < 4; r++) Fq[k][r] = F4[r];
+            Ft[k] = reinterpret_cast<const float2*>(F4)[8];
         }
         // (every entry of the group is evaluated and a tail entry's sums are discarded by
         // selection, so that the group's loads stay ahead of the arithmetic)
@@ -3463,7 +3465,7 @@ extern "C" __global__ void __launch_bounds__(256) k_solid_adj2(SolidAdjArgs aa) 
             const float g2[3] = {g2q[k].x, g2q[k].y, g2q[k].z};
             const float fb2[20] = {Fq[k][0].x, Fq[k][0].y, Fq[k][0].z, Fq[k][0].w, Fq[k][1].x, Fq[k][1].y, Fq[k][1].z,
                                    Fq[k][1].w, Fq[k][2].x, Fq[k][2].y, Fq[k][2].z, Fq[k][2].w, Fq[k][3].x, Fq[k][3].y,
-                                   Fq[k][3].z, Fq[k][3].w, Fq[k][4].x, Fq[k][4].y, Fq[k][4].z, Fq[k][4].w};
+                                   Fq[k][3].z, Fq[k][3].w, Ft[k].x, Ft[k].y, 0.0f, 0.0f};
             for (int r = 0; r < 3; r++) {
                 const float t = Vp * (fb2[3 * r] * g2[0] + fb2[3 * r + 1] * g2[1] + fb2[3 * r + 2] * g2[2]);
                 const float td = Vp * (fb2[9 + 3 * r] * g2[0] + fb2[9 + 3 * r + 1] * g2[1] + fb2[9 + 3 * r + 2] * g2[2]);
